@@ -1,0 +1,856 @@
+// C ABI of the fastEDM prediction core: device context, edm_predict and edm_predict_batch.
+// The dispatcher picks the kernel route per task (register top-k for k <= 64, dense distances for
+// k = all), stages inputs into a grow-only device pool and copies the outputs back on one stream.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "../../include/fastedm_hip.h"
+#include "kernels_common.h"
+
+namespace fedm {
+
+template <int KCAP, int EMAX, bool GENERIC>
+__global__ void knn_slices_f64(LpParams P, int k, int slice_len, double* out_s, int* out_i, int* out_cnt);
+template <int EMAX, bool GENERIC>
+__global__ void dense_dist_f64(LpParams P, int t0, int nt, int slice_len, double* D, int* out_cnt);
+template <int KCAP>
+__global__ void finish_lists(FinishParams F, const double* ps, const int* pi, const int* pcnt, int S);
+__global__ void finish_dense(FinishParams F, const double* D, int t0, int nt, const int* pcnt, int S);
+template <int NMAX>
+__global__ void smap_f64(SmapParams S, const double* D, int t0, int nt);
+size_t smap_lds_bytes(int n);
+
+
+template <typename T>
+__global__ void fill_kernel(T* p, size_t n, T v)
+{
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+} // namespace fedm
+
+using namespace fedm;
+
+struct DevBuf
+{
+  void* p = nullptr;
+  size_t bytes = 0;
+};
+
+struct edm_ctx
+{
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[8] = {};
+  std::vector<DevBuf> bufs;
+  double timings[4] = { 0, 0, 0, 0 };
+  int last_error = 0;
+
+  void* get(int slot, size_t bytes)
+  {
+    if ((int)bufs.size() <= slot)
+      bufs.resize(slot + 1);
+    DevBuf& b = bufs[slot];
+    if (b.bytes < bytes) {
+      if (b.p)
+        hipFree(b.p);
+      b.p = nullptr;
+      b.bytes = 0;
+      size_t want = std::max(bytes, (size_t)256);
+      if (hipMalloc(&b.p, want) != hipSuccess) {
+        b.p = nullptr;
+        return nullptr;
+      }
+      b.bytes = want;
+    }
+    return b.p;
+  }
+};
+
+namespace fedm {
+// Device buffer slots of the context pool.
+enum
+{
+  B_M,
+  B_MP,
+  B_MT,
+  B_MTT,
+  B_MPPT,
+  B_MPAN,
+  B_MPPAN,
+  B_METRICS,
+  B_IDW,
+  B_THETAS,
+  B_PS,
+  B_PI,
+  B_PCNT,
+  B_NNI,
+  B_NNS,
+  B_NNK,
+  B_NNCNT,
+  B_PRED,
+  B_RC,
+  B_KUSED,
+  B_COEFFS,
+  B_NNOUT,
+  B_D,
+  B_MIX0,
+  B_MIX1,
+  B_MIX2,
+  B_MIX3,
+  B_MIX4,
+  B_MIX5,
+  B_NUM
+};
+} // namespace fedm
+
+#define HIP_TRY(x)                                                                                               \
+  do {                                                                                                           \
+    hipError_t e_ = (x);                                                                                         \
+    if (e_ != hipSuccess) {                                                                                      \
+      fprintf(stderr, "fastedm_hip: %s failed: %s (%s:%d)\n", #x, hipGetErrorString(e_), __FILE__, __LINE__);   \
+      return -(int)e_ - 1000;                                                                                    \
+    }                                                                                                            \
+  } while (0)
+
+template <typename T>
+static T* up(edm_ctx* c, int slot, const T* host, size_t n, int* err)
+{
+  if (host == nullptr || n == 0)
+    return nullptr;
+  T* d = (T*)c->get(slot, n * sizeof(T));
+  if (!d) {
+    *err = -2;
+    return nullptr;
+  }
+  if (hipMemcpyAsync(d, host, n * sizeof(T), hipMemcpyHostToDevice, c->stream) != hipSuccess)
+    *err = -3;
+  return d;
+}
+
+template <typename T>
+static void dfill(edm_ctx* c, T* p, size_t n, T v)
+{
+  if (!p || n == 0)
+    return;
+  int blocks = (int)std::min<size_t>(4096, (n + 255) / 256);
+  hipLaunchKernelGGL(fill_kernel<T>, dim3(blocks), dim3(256), 0, c->stream, p, n, v);
+}
+
+static int emax_for(int Ea)
+{
+  return Ea <= 8 ? 8 : Ea <= 16 ? 16 : Ea <= 32 ? 32 : Ea <= 64 ? 64 : -1;
+}
+static int kcap_for(int k)
+{
+  return k <= 4 ? 4 : k <= 8 ? 8 : k <= 16 ? 16 : k <= 32 ? 32 : k <= 64 ? 64 : -1;
+}
+
+template <int KCAP, int EMAX>
+static void launch_knn_e(bool generic, dim3 g, hipStream_t s, const LpParams& P, int k, int sl, double* ps, int* pi,
+                         int* pc)
+{
+  if (generic)
+    hipLaunchKernelGGL((knn_slices_f64<KCAP, EMAX, true>), g, dim3(256), 0, s, P, k, sl, ps, pi, pc);
+  else
+    hipLaunchKernelGGL((knn_slices_f64<KCAP, EMAX, false>), g, dim3(256), 0, s, P, k, sl, ps, pi, pc);
+}
+
+template <int KCAP>
+static void launch_knn_k(int emax, bool generic, dim3 g, hipStream_t s, const LpParams& P, int k, int sl, double* ps,
+                         int* pi, int* pc)
+{
+  switch (emax) {
+    case 8: launch_knn_e<KCAP, 8>(generic, g, s, P, k, sl, ps, pi, pc); break;
+    case 16: launch_knn_e<KCAP, 16>(generic, g, s, P, k, sl, ps, pi, pc); break;
+    case 32: launch_knn_e<KCAP, 32>(generic, g, s, P, k, sl, ps, pi, pc); break;
+    default: launch_knn_e<KCAP, 64>(generic, g, s, P, k, sl, ps, pi, pc); break;
+  }
+}
+
+static void launch_knn(int kcap, int emax, bool generic, dim3 g, hipStream_t s, const LpParams& P, int k, int sl,
+                       double* ps, int* pi, int* pc)
+{
+  switch (kcap) {
+    case 4: launch_knn_k<4>(emax, generic, g, s, P, k, sl, ps, pi, pc); break;
+    case 8: launch_knn_k<8>(emax, generic, g, s, P, k, sl, ps, pi, pc); break;
+    case 16: launch_knn_k<16>(emax, generic, g, s, P, k, sl, ps, pi, pc); break;
+    case 32: launch_knn_k<32>(emax, generic, g, s, P, k, sl, ps, pi, pc); break;
+    default: launch_knn_k<64>(emax, generic, g, s, P, k, sl, ps, pi, pc); break;
+  }
+}
+
+static void launch_dense(int emax, bool generic, dim3 g, hipStream_t s, const LpParams& P, int t0, int nt, int sl,
+                         double* D, int* pc)
+{
+#define FEDM_DENSE(EM)                                                                                           \
+  if (generic)                                                                                                   \
+    hipLaunchKernelGGL((dense_dist_f64<EM, true>), g, dim3(256), 0, s, P, t0, nt, sl, D, pc);                    \
+  else                                                                                                           \
+    hipLaunchKernelGGL((dense_dist_f64<EM, false>), g, dim3(256), 0, s, P, t0, nt, sl, D, pc);
+  switch (emax) {
+    case 8: FEDM_DENSE(8) break;
+    case 16: FEDM_DENSE(16) break;
+    case 32: FEDM_DENSE(32) break;
+    default: FEDM_DENSE(64) break;
+  }
+#undef FEDM_DENSE
+}
+
+static void launch_finish_lists(int kcap, dim3 g, hipStream_t s, const FinishParams& F, const double* ps, const int* pi,
+                                const int* pc, int S)
+{
+  switch (kcap) {
+    case 4: hipLaunchKernelGGL(finish_lists<4>, g, dim3(256), 0, s, F, ps, pi, pc, S); break;
+    case 8: hipLaunchKernelGGL(finish_lists<8>, g, dim3(256), 0, s, F, ps, pi, pc, S); break;
+    case 16: hipLaunchKernelGGL(finish_lists<16>, g, dim3(256), 0, s, F, ps, pi, pc, S); break;
+    case 32: hipLaunchKernelGGL(finish_lists<32>, g, dim3(256), 0, s, F, ps, pi, pc, S); break;
+    default: hipLaunchKernelGGL(finish_lists<64>, g, dim3(256), 0, s, F, ps, pi, pc, S); break;
+  }
+}
+
+static int launch_smap(int n, dim3 g, hipStream_t s, const SmapParams& S, const double* D, int t0, int nt)
+{
+  size_t lds = smap_lds_bytes(n);
+  if (n <= 8)
+    hipLaunchKernelGGL(smap_f64<8>, g, dim3(64), lds, s, S, D, t0, nt);
+  else if (n <= 16)
+    hipLaunchKernelGGL(smap_f64<16>, g, dim3(64), lds, s, S, D, t0, nt);
+  else if (n <= 32)
+    hipLaunchKernelGGL(smap_f64<32>, g, dim3(64), lds, s, S, D, t0, nt);
+  else if (n <= 64) {
+    static bool attr = [] {
+      hipFuncSetAttribute((const void*)smap_f64<64>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+      return true;
+    }();
+    (void)attr;
+    hipLaunchKernelGGL(smap_f64<64>, g, dim3(64), lds, s, S, D, t0, nt);
+  } else
+    return -22;
+  return 0;
+}
+
+extern "C" {
+
+int edm_ctx_create(edm_ctx** out, int device)
+{
+  *out = nullptr;
+  edm_ctx* c = new edm_ctx();
+  c->device = device;
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return -1;
+  }
+  for (auto& e : c->ev)
+    hipEventCreate(&e);
+  c->bufs.resize(B_NUM);
+  *out = c;
+  return 0;
+}
+
+void edm_ctx_destroy(edm_ctx* c)
+{
+  if (!c)
+    return;
+  hipSetDevice(c->device);
+  hipStreamSynchronize(c->stream);
+  for (auto& b : c->bufs)
+    if (b.p)
+      hipFree(b.p);
+  for (auto& e : c->ev)
+    hipEventDestroy(e);
+  hipStreamDestroy(c->stream);
+  delete c;
+}
+
+void edm_ctx_timings(edm_ctx* c, double* ms4)
+{
+  for (int i = 0; i < 4; i++)
+    ms4[i] = c->timings[i];
+}
+
+static float ev_ms(hipEvent_t a, hipEvent_t b)
+{
+  float ms = 0.f;
+  hipEventElapsedTime(&ms, a, b);
+  return ms;
+}
+
+static void host_fill_outputs(const edm_manifold_view* Mp, const edm_options* o, double* pred, int* rc,
+                              double* coeffs, int* k_used, int* nn_idx, int nn_cap, int Ea, int rcv)
+{
+  const size_t Np = Mp->num_points, T = o->num_thetas;
+  for (size_t i = 0; i < T * Np; i++) {
+    pred[i] = FEDM_MISSING_D;
+    rc[i] = rcv;
+  }
+  if (coeffs)
+    for (size_t i = 0; i < Np * (size_t)(Ea + 1); i++)
+      coeffs[i] = FEDM_MISSING_D;
+  for (size_t i = 0; i < Np; i++)
+    k_used[i] = FEDM_MISSING_I;
+  if (nn_idx)
+    for (size_t i = 0; i < Np * (size_t)nn_cap; i++)
+      nn_idx[i] = -1;
+}
+
+int edm_predict(edm_ctx* c, const edm_manifold_view* M, const edm_manifold_view* Mp, const edm_options* o,
+                double* pred, int* rc, double* coeffs, int* k_used, int* nn_idx, int nn_cap, int (*keep_going)(void))
+{
+  const int NL = M->num_points, NP = Mp->num_points, Ea = M->E_actual, T = o->num_thetas, n = Ea + 1;
+  if (!c || Ea != Mp->E_actual || T <= 0 || Ea <= 0)
+    return -22;
+  if (NP == 0)
+    return 0;
+  if (keep_going && !keep_going()) {
+    host_fill_outputs(Mp, o, pred, rc, coeffs, k_used, nn_idx, nn_cap, Ea, FEDM_UNKNOWN_ERROR);
+    for (int i = 0; i < NP; i++)
+      rc[i] = FEDM_BREAK_HIT;
+    return FEDM_BREAK_HIT;
+  }
+  const int emax = emax_for(Ea);
+  const int k = o->k;
+  const bool list_route = k > 0 && k <= 64;
+  if (emax < 0 || o->distance == FEDM_DIST_WASSERSTEIN || (!list_route && k > 0 && k > 64 && false)) {
+    host_fill_outputs(Mp, o, pred, rc, coeffs, k_used, nn_idx, nn_cap, Ea, FEDM_UNKNOWN_ERROR);
+    return -95; // not supported on this route
+  }
+  hipSetDevice(c->device);
+  hipStream_t s = c->stream;
+  int err = 0;
+
+  std::vector<int> metrics(Ea, 0);
+  if (o->metrics)
+    std::copy(o->metrics, o->metrics + Ea, metrics.begin());
+
+  LpParams P;
+  std::memset(&P, 0, sizeof(P));
+  P.M = up(c, B_M, M->data, (size_t)NL * Ea, &err);
+  P.Mp = up(c, B_MP, Mp->data, (size_t)NP * Ea, &err);
+  const double* dtargets = up(c, B_MT, M->targets, (size_t)NL, &err);
+  P.Mtt = up(c, B_MTT, M->target_times, (size_t)NL, &err);
+  P.Mppt = up(c, B_MPPT, Mp->point_times, (size_t)NP, &err);
+  if (o->panel_mode) {
+    P.Mpanel = up(c, B_MPAN, M->panel, (size_t)NL, &err);
+    P.Mppanel = up(c, B_MPPAN, Mp->panel, (size_t)NP, &err);
+  }
+  P.metrics = up(c, B_METRICS, metrics.data(), (size_t)Ea, &err);
+  if (o->id_weights && o->num_panels > 0)
+    P.id_weights = up(c, B_IDW, o->id_weights, (size_t)o->num_panels * o->num_panels, &err);
+  const double* dthetas = up(c, B_THETAS, o->thetas, (size_t)T, &err);
+  P.NL = NL;
+  P.NP = NP;
+  P.Ea = Ea;
+  P.num_panels = o->num_panels;
+  P.missing_distance = o->missing_distance;
+  P.idw = o->idw;
+  const bool lazy = o->low_memory != 0;
+  int f = 0;
+  if (o->has_missing || lazy)
+    f |= F_MISSING;
+  if (o->has_categorical || lazy)
+    f |= F_CATEGORICAL;
+  if (o->panel_mode && o->idw < 0)
+    f |= F_SKIP_OTHER;
+  if (o->use_only_past && !lazy)
+    f |= F_SKIP_FUTURE;
+  if (o->distance == FEDM_DIST_MAE)
+    f |= F_MAE;
+  if (o->panel_mode && o->idw > 0)
+    f |= F_IDW;
+  else if (o->panel_mode && P.id_weights)
+    f |= F_IDWEIGHTS;
+  P.flags = f;
+  const bool generic = f != 0;
+
+  double* dpred = (double*)c->get(B_PRED, sizeof(double) * (size_t)T * NP);
+  int* drc = (int*)c->get(B_RC, sizeof(int) * (size_t)T * NP);
+  int* dku = (int*)c->get(B_KUSED, sizeof(int) * (size_t)NP);
+  double* dco = coeffs ? (double*)c->get(B_COEFFS, sizeof(double) * (size_t)NP * n) : nullptr;
+  int* dnno = (nn_idx && nn_cap > 0) ? (int*)c->get(B_NNOUT, sizeof(int) * (size_t)NP * nn_cap) : nullptr;
+  int* nnk = (int*)c->get(B_NNK, sizeof(int) * (size_t)NP);
+  int* nncnt = (int*)c->get(B_NNCNT, sizeof(int) * (size_t)NP);
+  if (err || !dpred || !drc || !dku || !nnk || !nncnt || (coeffs && !dco) || (nn_idx && nn_cap > 0 && !dnno))
+    return err ? err : -2;
+  dfill(c, dpred, (size_t)T * NP, FEDM_MISSING_D);
+  dfill(c, drc, (size_t)T * NP, (int)FEDM_UNKNOWN_ERROR);
+  dfill(c, dku, (size_t)NP, (int)FEDM_MISSING_I);
+  dfill(c, dco, (size_t)NP * n, FEDM_MISSING_D);
+  dfill(c, dnno, (size_t)NP * nn_cap, -1);
+  dfill(c, nnk, (size_t)NP, 0);
+
+  FinishParams F;
+  std::memset(&F, 0, sizeof(F));
+  F.NP = NP;
+  F.NL = NL;
+  F.k = k;
+  F.T = T;
+  F.force_compute = o->force_compute;
+  F.algorithm = o->algorithm;
+  F.thetas = dthetas;
+  F.targets = dtargets;
+  F.pred = dpred;
+  F.rc = drc;
+  F.k_used = dku;
+  F.nn_k = nnk;
+  F.nn_cnt = nncnt;
+  F.nn_out = dnno;
+  F.nn_cap = dnno ? nn_cap : 0;
+
+  SmapParams Sm;
+  std::memset(&Sm, 0, sizeof(Sm));
+  Sm.NP = NP;
+  Sm.NL = NL;
+  Sm.Ea = Ea;
+  Sm.T = T;
+  Sm.M = P.M;
+  Sm.Mp = P.Mp;
+  Sm.targets = dtargets;
+  Sm.thetas = dthetas;
+  Sm.nn_k = nnk;
+  Sm.nn_cnt = nncnt;
+  Sm.pred = dpred;
+  Sm.rc = drc;
+  Sm.coeffs = (o->save_smap_coeffs && dco) ? dco : nullptr;
+
+  const bool smap = o->algorithm == FEDM_SMAP;
+  HIP_TRY(hipEventRecord(c->ev[0], s));
+
+  if (list_route) {
+    const int kcap = kcap_for(k);
+    const int bx = (NP + 255) / 256;
+    int S = std::max(1, std::min(MAX_SLICES, (1024 + bx - 1) / bx));
+    S = std::max(1, std::min(S, NL / 256));
+    const int sl = (NL + S - 1) / S;
+    double* ps = (double*)c->get(B_PS, sizeof(double) * (size_t)S * NP * kcap);
+    int* pi = (int*)c->get(B_PI, sizeof(int) * (size_t)S * NP * kcap);
+    int* pc = (int*)c->get(B_PCNT, sizeof(int) * (size_t)S * NP);
+    F.nn_i = (int*)c->get(B_NNI, sizeof(int) * (size_t)NP * kcap);
+    F.nn_s = (double*)c->get(B_NNS, sizeof(double) * (size_t)NP * kcap);
+    F.nn_stride = kcap;
+    if (!ps || !pi || !pc || !F.nn_i || !F.nn_s)
+      return -2;
+    launch_knn(kcap, emax, generic, dim3(bx, S), s, P, k, sl, ps, pi, pc);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(c->ev[1], s));
+    launch_finish_lists(kcap, dim3(bx), s, F, ps, pi, pc, S);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(c->ev[2], s));
+    if (smap) {
+      Sm.nn_i = F.nn_i;
+      Sm.nn_s = F.nn_s;
+      Sm.nn_stride = kcap;
+      int r = launch_smap(n, dim3(NP, T), s, Sm, nullptr, 0, NP);
+      if (r)
+        return r;
+      HIP_TRY(hipGetLastError());
+    }
+    HIP_TRY(hipEventRecord(c->ev[3], s));
+  } else {
+    // every valid neighbour (k < 0, or k larger than the register lists hold)
+    const size_t budget = (size_t)1 << 30;
+    int nt = (int)std::max<size_t>(1, std::min<size_t>((size_t)NP, budget / (sizeof(double) * (size_t)std::max(NL, 1))));
+    const int bxmax = (nt + 255) / 256;
+    int S = std::max(1, std::min(MAX_SLICES, (1024 + bxmax - 1) / bxmax));
+    S = std::max(1, std::min(S, NL / 256));
+    const int sl = (NL + S - 1) / S;
+    double* D = (double*)c->get(B_D, sizeof(double) * (size_t)nt * std::max(NL, 1));
+    int* pc = (int*)c->get(B_PCNT, sizeof(int) * (size_t)S * NP);
+    F.nn_i = nullptr;
+    F.nn_s = nullptr;
+    F.nn_stride = 0;
+    if (!D || !pc)
+      return -2;
+    HIP_TRY(hipEventRecord(c->ev[1], s));
+    for (int t0 = 0; t0 < NP; t0 += nt) {
+      const int cnt = std::min(nt, NP - t0);
+      launch_dense(emax, generic, dim3((cnt + 255) / 256, S), s, P, t0, cnt, sl, D, pc);
+      HIP_TRY(hipGetLastError());
+      hipLaunchKernelGGL(finish_dense, dim3((cnt + 255) / 256), dim3(256), 0, s, F, D, t0, cnt, pc, S);
+      HIP_TRY(hipGetLastError());
+      if (smap) {
+        int r = launch_smap(n, dim3(cnt, T), s, Sm, D, t0, cnt);
+        if (r)
+          return r;
+        HIP_TRY(hipGetLastError());
+      }
+      if (keep_going && !keep_going())
+        break;
+    }
+    HIP_TRY(hipEventRecord(c->ev[2], s));
+    HIP_TRY(hipEventRecord(c->ev[3], s));
+  }
+  HIP_TRY(hipMemcpyAsync(pred, dpred, sizeof(double) * (size_t)T * NP, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipMemcpyAsync(rc, drc, sizeof(int) * (size_t)T * NP, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipMemcpyAsync(k_used, dku, sizeof(int) * (size_t)NP, hipMemcpyDeviceToHost, s));
+  if (coeffs)
+    HIP_TRY(hipMemcpyAsync(coeffs, dco, sizeof(double) * (size_t)NP * n, hipMemcpyDeviceToHost, s));
+  if (dnno)
+    HIP_TRY(hipMemcpyAsync(nn_idx, dnno, sizeof(int) * (size_t)NP * nn_cap, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  c->timings[0] = ev_ms(c->ev[0], c->ev[1]);
+  c->timings[1] = ev_ms(c->ev[1], c->ev[2]);
+  c->timings[2] = ev_ms(c->ev[2], c->ev[3]);
+  c->timings[3] = ev_ms(c->ev[0], c->ev[3]);
+  if (keep_going && !keep_going()) {
+    for (int i = 0; i < NP; i++)
+      if (k_used[i] == FEDM_MISSING_I)
+        rc[i] = FEDM_BREAK_HIT;
+    return FEDM_BREAK_HIT;
+  }
+  return 0;
+}
+
+} // extern "C"
+
+// ------------------------------------------------------------------------------------------------
+// Batches of tasks over shared point tables.
+
+namespace fedm {
+struct BatchParams
+{
+  const double* L;
+  const float* L32;
+  const double* Ltgt;
+  const double* P;
+  const int* lib_begin;
+  const int* lib_count;
+  const int* lib_rows;
+  const int* pred_begin;
+  const int* pred_count;
+  const int* pred_rows;
+  const int* k;
+  const int64_t* out_begin;
+  const int2* items;
+  int num_items;
+  int* counter;
+  const double* thetas;
+  int T, Ea, force_compute;
+  double c1, c2, c3;
+  double* pred;
+  int* rc;
+  int* k_used;
+};
+template <int KCAP, int EMAX, bool MIXED>
+__global__ void batch_simplex(BatchParams B);
+} // namespace fedm
+
+struct edm_batch
+{
+  edm_ctx* ctx = nullptr;
+  bool device_route = false;
+  int kcap = 0, emax = 0, mixed = 0, T = 0, Ea = 0, tasks = 0;
+  int64_t total_out = 0, total_k = 0;
+  std::vector<void*> owned;
+  BatchParams B;
+  int grid = 0;
+  float last_ms = 0.f;
+  // host copies for the task-by-task route
+  edm_manifold_view Lv, Pv;
+  edm_options o;
+  std::vector<double> Ldata, Ltgt, Ltt, Pdata, Ppt, thetas;
+  std::vector<int> Lpan, Ppan, lb, lc, lr, pb, pc, pr, kk, metrics;
+  std::vector<int64_t> ob;
+  std::vector<double> hpred;
+  std::vector<int> hrc, hku;
+
+  template <typename T>
+  T* dev(const T* host, size_t n)
+  {
+    void* p = nullptr;
+    if (hipMalloc(&p, std::max<size_t>(n * sizeof(T), 256)) != hipSuccess)
+      return nullptr;
+    owned.push_back(p);
+    if (host && n && hipMemcpyAsync(p, host, n * sizeof(T), hipMemcpyHostToDevice, ctx->stream) != hipSuccess)
+      return nullptr;
+    return (T*)p;
+  }
+  ~edm_batch()
+  {
+    if (ctx)
+      hipStreamSynchronize(ctx->stream);
+    for (void* p : owned)
+      hipFree(p);
+  }
+};
+
+template <int KCAP, int EMAX>
+static const void* batch_kernel_e(bool mixed)
+{
+  return mixed ? (const void*)batch_simplex<KCAP, EMAX, true> : (const void*)batch_simplex<KCAP, EMAX, false>;
+}
+template <int KCAP>
+static const void* batch_kernel_k(int emax, bool mixed)
+{
+  switch (emax) {
+    case 4: return batch_kernel_e<KCAP, 4>(mixed);
+    case 8: return batch_kernel_e<KCAP, 8>(mixed);
+    case 16: return batch_kernel_e<KCAP, 16>(mixed);
+    default: return batch_kernel_e<KCAP, 32>(mixed);
+  }
+}
+static const void* batch_kernel(int kcap, int emax, bool mixed)
+{
+  switch (kcap) {
+    case 4: return batch_kernel_k<4>(emax, mixed);
+    case 8: return batch_kernel_k<8>(emax, mixed);
+    case 16: return batch_kernel_k<16>(emax, mixed);
+    default: return batch_kernel_k<32>(emax, mixed);
+  }
+}
+
+extern "C" {
+
+int edm_batch_create(edm_ctx* c, const edm_manifold_view* L, const edm_manifold_view* P, const edm_options* o,
+                     int num_tasks, const int* lib_begin, const int* lib_count, const int* lib_rows,
+                     const int* pred_begin, const int* pred_count, const int* pred_rows, const int* k,
+                     const int64_t* out_begin, edm_batch** out)
+{
+  *out = nullptr;
+  if (!c || num_tasks < 0 || L->E_actual != P->E_actual || o->num_thetas <= 0)
+    return -22;
+  hipSetDevice(c->device);
+  auto b = new edm_batch();
+  b->ctx = c;
+  b->tasks = num_tasks;
+  b->T = o->num_thetas;
+  b->Ea = L->E_actual;
+  const int Ea = b->Ea;
+  int kmax = 0, kmin = INT_MAX;
+  size_t lib_total = 0, pred_total = 0;
+  for (int t = 0; t < num_tasks; t++) {
+    kmax = std::max(kmax, k[t]);
+    kmin = std::min(kmin, k[t]);
+    lib_total = std::max(lib_total, (size_t)lib_begin[t] + lib_count[t]);
+    pred_total = std::max(pred_total, (size_t)pred_begin[t] + pred_count[t]);
+    b->total_out = std::max(b->total_out, out_begin[t] + (int64_t)b->T * pred_count[t]);
+  }
+  b->total_k = b->total_out / b->T;
+  // host copies (every route keeps them: they make the task-by-task route and fetch possible)
+  b->Lv = *L;
+  b->Pv = *P;
+  b->o = *o;
+  b->Ldata.assign(L->data, L->data + (size_t)L->num_points * Ea);
+  b->Ltgt.assign(L->targets, L->targets + L->num_points);
+  if (L->target_times)
+    b->Ltt.assign(L->target_times, L->target_times + L->num_points);
+  if (L->panel)
+    b->Lpan.assign(L->panel, L->panel + L->num_points);
+  b->Pdata.assign(P->data, P->data + (size_t)P->num_points * Ea);
+  if (P->point_times)
+    b->Ppt.assign(P->point_times, P->point_times + P->num_points);
+  if (P->panel)
+    b->Ppan.assign(P->panel, P->panel + P->num_points);
+  b->thetas.assign(o->thetas, o->thetas + o->num_thetas);
+  if (o->metrics)
+    b->metrics.assign(o->metrics, o->metrics + Ea);
+  b->lb.assign(lib_begin, lib_begin + num_tasks);
+  b->lc.assign(lib_count, lib_count + num_tasks);
+  b->lr.assign(lib_rows, lib_rows + lib_total);
+  b->pb.assign(pred_begin, pred_begin + num_tasks);
+  b->pc.assign(pred_count, pred_count + num_tasks);
+  b->pr.assign(pred_rows, pred_rows + pred_total);
+  b->kk.assign(k, k + num_tasks);
+  b->ob.assign(out_begin, out_begin + num_tasks);
+
+  const bool generic = o->has_missing || o->has_categorical || o->panel_mode || o->use_only_past || o->low_memory ||
+                       o->distance != FEDM_DIST_EUCLIDEAN;
+  const int emax = Ea <= 4 ? 4 : Ea <= 8 ? 8 : Ea <= 16 ? 16 : Ea <= 32 ? 32 : -1;
+  const int kcap = kmax <= 4 ? 4 : kmax <= 8 ? 8 : kmax <= 16 ? 16 : kmax <= 32 ? 32 : -1;
+  double A = 0;
+  for (double v : b->Ldata)
+    A = std::max(A, std::fabs(v));
+  for (double v : b->Pdata)
+    A = std::max(A, std::fabs(v));
+  b->device_route = o->algorithm == FEDM_SIMPLEX && !generic && emax > 0 && kcap > 0 && kmin >= 1 && A < 1e30;
+  if (!b->device_route) {
+    *out = b;
+    return 0;
+  }
+  b->kcap = kcap;
+  b->emax = emax;
+  b->mixed = o->precision == FEDM_MIXED;
+
+  BatchParams& B = b->B;
+  std::memset(&B, 0, sizeof(B));
+  B.L = b->dev(b->Ldata.data(), b->Ldata.size());
+  if (b->mixed) {
+    std::vector<float> l32(b->Ldata.begin(), b->Ldata.end());
+    B.L32 = b->dev(l32.data(), l32.size());
+    hipStreamSynchronize(c->stream);
+  }
+  B.Ltgt = b->dev(b->Ltgt.data(), b->Ltgt.size());
+  B.P = b->dev(b->Pdata.data(), b->Pdata.size());
+  B.lib_begin = b->dev(lib_begin, num_tasks);
+  B.lib_count = b->dev(lib_count, num_tasks);
+  B.lib_rows = b->dev(lib_rows, lib_total);
+  B.pred_begin = b->dev(pred_begin, num_tasks);
+  B.pred_count = b->dev(pred_count, num_tasks);
+  B.pred_rows = b->dev(pred_rows, pred_total);
+  B.k = b->dev(k, num_tasks);
+  B.out_begin = b->dev(out_begin, num_tasks);
+  // work queue: (task, first target) items, heaviest library first
+  std::vector<int> order(num_tasks);
+  for (int t = 0; t < num_tasks; t++)
+    order[t] = t;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int q) { return lib_count[a] > lib_count[q]; });
+  std::vector<int2> items;
+  for (int t : order)
+    for (int t0 = 0; t0 < pred_count[t]; t0 += 256)
+      items.push_back(make_int2(t, t0));
+  B.items = b->dev(items.data(), items.size());
+  B.num_items = (int)items.size();
+  B.counter = b->dev<int>(nullptr, 1);
+  B.thetas = b->dev(o->thetas, (size_t)o->num_thetas);
+  B.T = o->num_thetas;
+  B.Ea = Ea;
+  B.force_compute = o->force_compute;
+  const double u = 5.9604644775390625e-08; // 2^-24
+  const double Ap = A * (1 + u);
+  B.c1 = (4 * u * Ap + 4 * u * u * Ap) * std::sqrt((double)Ea);
+  B.c2 = 2.01 * u + 1.01 * Ea * u / (1 - Ea * u);
+  B.c3 = 4.04 * Ea * u * u * Ap * Ap;
+  B.pred = b->dev<double>(nullptr, (size_t)b->total_out);
+  B.rc = b->dev<int>(nullptr, (size_t)b->total_out);
+  B.k_used = b->dev<int>(nullptr, (size_t)b->total_k);
+  for (void* p : b->owned)
+    if (!p) {
+      delete b;
+      return -2;
+    }
+  int dev = 0, ncu = 0, per = 0;
+  hipGetDevice(&dev);
+  hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+  hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, batch_kernel(kcap, emax, b->mixed), 256, 0);
+  b->grid = std::max(1, std::min(B.num_items, ncu * std::max(per, 1)));
+  if (hipStreamSynchronize(c->stream) != hipSuccess) {
+    delete b;
+    return -3;
+  }
+  *out = b;
+  return 0;
+}
+
+int edm_batch_run(edm_batch* b)
+{
+  edm_ctx* c = b->ctx;
+  hipSetDevice(c->device);
+  if (!b->device_route) {
+    // task-by-task: gather each task's manifolds and predict them with edm_predict
+    const int Ea = b->Ea;
+    b->hpred.assign((size_t)b->total_out, FEDM_MISSING_D);
+    b->hrc.assign((size_t)b->total_out, FEDM_UNKNOWN_ERROR);
+    b->hku.assign((size_t)b->total_k, FEDM_MISSING_I);
+    for (int tk = 0; tk < b->tasks; tk++) {
+      const int nl = b->lc[tk], np = b->pc[tk];
+      std::vector<double> md((size_t)nl * Ea), mt(nl), mtt(nl), mpd((size_t)np * Ea), mppt(np);
+      std::vector<int> mpan, mppan;
+      for (int r = 0; r < nl; r++) {
+        const int row = b->lr[b->lb[tk] + r];
+        std::copy(b->Ldata.begin() + (size_t)row * Ea, b->Ldata.begin() + (size_t)(row + 1) * Ea,
+                  md.begin() + (size_t)r * Ea);
+        mt[r] = b->Ltgt[row];
+        mtt[r] = b->Ltt.empty() ? 0.0 : b->Ltt[row];
+        if (!b->Lpan.empty())
+          mpan.push_back(b->Lpan[row]);
+      }
+      for (int r = 0; r < np; r++) {
+        const int row = b->pr[b->pb[tk] + r];
+        std::copy(b->Pdata.begin() + (size_t)row * Ea, b->Pdata.begin() + (size_t)(row + 1) * Ea,
+                  mpd.begin() + (size_t)r * Ea);
+        mppt[r] = b->Ppt.empty() ? 0.0 : b->Ppt[row];
+        if (!b->Ppan.empty())
+          mppan.push_back(b->Ppan[row]);
+      }
+      edm_manifold_view M = b->Lv, Mp = b->Pv;
+      M.data = md.data();
+      M.targets = mt.data();
+      M.target_times = mtt.data();
+      M.point_times = nullptr;
+      M.panel = mpan.empty() ? nullptr : mpan.data();
+      M.num_points = nl;
+      Mp.data = mpd.data();
+      Mp.targets = nullptr;
+      Mp.point_times = mppt.data();
+      Mp.target_times = nullptr;
+      Mp.panel = mppan.empty() ? nullptr : mppan.data();
+      Mp.num_points = np;
+      edm_options to = b->o;
+      to.k = b->kk[tk];
+      to.thetas = b->thetas.data();
+      to.metrics = b->metrics.empty() ? nullptr : b->metrics.data();
+      const int64_t ob = b->ob[tk];
+      const int r = edm_predict(c, &M, &Mp, &to, b->hpred.data() + ob, b->hrc.data() + ob, nullptr,
+                                b->hku.data() + ob / b->T, nullptr, 0, nullptr);
+      if (r != 0)
+        return r;
+    }
+    return 0;
+  }
+  BatchParams& B = b->B;
+  hipStream_t s = c->stream;
+  dfill(c, B.counter, 1, 0);
+  dfill(c, B.pred, (size_t)b->total_out, FEDM_MISSING_D);
+  dfill(c, B.rc, (size_t)b->total_out, (int)FEDM_UNKNOWN_ERROR);
+  dfill(c, B.k_used, (size_t)b->total_k, (int)FEDM_MISSING_I);
+  HIP_TRY(hipEventRecord(c->ev[4], s));
+  const void* fn = batch_kernel(b->kcap, b->emax, b->mixed);
+  void* args[] = { &B };
+  HIP_TRY(hipLaunchKernel(fn, dim3(b->grid), dim3(256), args, 0, s));
+  HIP_TRY(hipEventRecord(c->ev[5], s));
+  HIP_TRY(hipEventSynchronize(c->ev[5]));
+  b->last_ms = ev_ms(c->ev[4], c->ev[5]);
+  c->timings[0] = b->last_ms;
+  c->timings[1] = 0;
+  c->timings[2] = 0;
+  c->timings[3] = b->last_ms;
+  return 0;
+}
+
+int edm_batch_fetch(edm_batch* b, double* pred, int* rc, int* k_used)
+{
+  edm_ctx* c = b->ctx;
+  hipSetDevice(c->device);
+  if (!b->device_route) {
+    std::copy(b->hpred.begin(), b->hpred.end(), pred);
+    std::copy(b->hrc.begin(), b->hrc.end(), rc);
+    std::copy(b->hku.begin(), b->hku.end(), k_used);
+    return 0;
+  }
+  HIP_TRY(hipMemcpyAsync(pred, b->B.pred, sizeof(double) * b->total_out, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipMemcpyAsync(rc, b->B.rc, sizeof(int) * b->total_out, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipMemcpyAsync(k_used, b->B.k_used, sizeof(int) * b->total_k, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+void edm_batch_destroy(edm_batch* b)
+{
+  delete b;
+}
+
+int edm_predict_batch(edm_ctx* c, const edm_manifold_view* L, const edm_manifold_view* P, const edm_options* o,
+                      int num_tasks, const int* lib_begin, const int* lib_count, const int* lib_rows,
+                      const int* pred_begin, const int* pred_count, const int* pred_rows, const int* k,
+                      const int64_t* out_begin, double* pred, int* rc, int* k_used, int (*keep_going)(void))
+{
+  if (keep_going && !keep_going())
+    return FEDM_BREAK_HIT;
+  edm_batch* b = nullptr;
+  int r = edm_batch_create(c, L, P, o, num_tasks, lib_begin, lib_count, lib_rows, pred_begin, pred_count, pred_rows,
+                           k, out_begin, &b);
+  if (r != 0)
+    return r;
+  r = edm_batch_run(b);
+  if (r == 0)
+    r = edm_batch_fetch(b, pred, rc, k_used);
+  edm_batch_destroy(b);
+  return r;
+}
+
+} // extern "C"

@@ -1,0 +1,98 @@
+// Shared device-side definitions of the fastEDM HIP kernels (gfx950).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <climits>
+#include <cstdint>
+
+namespace fedm {
+
+constexpr double MISSING_D = 1.0e+100;
+constexpr double INF_D = __builtin_huge_val();
+// A squared kth distance times this factor bounds every squared distance whose correctly rounded
+// square root can still be below the kth distance (margin 2^-49 >> the 2^-52 rounding).
+constexpr double SQRT_GUARD = 1.0 + 1.7763568394002505e-15;
+
+constexpr int LDS_CHUNK_DOUBLES = 4096; // 32 KiB library chunk per workgroup
+constexpr int LDS_CHUNK_MAXPTS = 1024;
+constexpr int MAX_SLICES = 64;
+
+enum
+{
+  RC_SUCCESS = 0,
+  RC_BREAK_HIT = 1,
+  RC_INVALID_ALGORITHM = 400,
+  RC_INSUFFICIENT_UNIQUE = 503,
+  RC_UNKNOWN_ERROR = 8000
+};
+enum
+{
+  ALG_SIMPLEX = 0,
+  ALG_SMAP = 1
+};
+
+// Distance option bits (src/distances.cpp:151-229 template switches and Options fields).
+enum
+{
+  F_MISSING = 1,
+  F_CATEGORICAL = 2,
+  F_SKIP_OTHER = 4,
+  F_SKIP_FUTURE = 8,
+  F_MAE = 16,
+  F_IDW = 32,
+  F_IDWEIGHTS = 64
+};
+
+struct LpParams
+{
+  const double* M;     // library points [NL x Ea]
+  const double* Mp;    // prediction points [NP x Ea]
+  const double* Mtt;   // library target times [NL] (skip-future only)
+  const double* Mppt;  // prediction point times [NP]
+  const int* Mpanel;   // [NL] or null
+  const int* Mppanel;  // [NP] or null
+  const int* metrics;  // [Ea]
+  const float* id_weights;
+  int NL, NP, Ea, num_panels;
+  double missing_distance, idw;
+  int flags;
+};
+
+struct FinishParams
+{
+  int NP, NL, k, T, force_compute, algorithm;
+  const double* thetas;  // [T]
+  const double* targets; // library targets [NL]
+  double* pred;          // [T x NP]
+  int* rc;               // [T x NP]
+  int* k_used;           // [NP]
+  int* nn_k;             // [NP]: >0 list length, -1 dense (all valid), 0 none
+  int* nn_cnt;           // [NP]: valid count on the dense route
+  int* nn_i;             // [NP x nn_stride]
+  double* nn_s;          // [NP x nn_stride]
+  int nn_stride;
+  int* nn_out;           // [NP x nn_cap] or null
+  int nn_cap;
+};
+
+struct SmapParams
+{
+  int NP, NL, Ea, T;
+  const double* M;
+  const double* Mp;
+  const double* targets;
+  const double* thetas;
+  const int* nn_k;
+  const int* nn_cnt;
+  const int* nn_i;
+  const double* nn_s;
+  int nn_stride;
+  double* pred;
+  int* rc;
+  double* coeffs; // [NP x (Ea+1)] or null
+};
+
+} // namespace fedm
+
+#include "jacobi_wave.h"

@@ -1,0 +1,620 @@
+// Plain (FP64) kernels of the fastEDM prediction core for gfx950 (MI355X).
+//
+// Semantics follow the reference CPU path (EDM-Developers/fastEDM-r):
+//   L^p distances        src/distances.cpp:151-229 (eager) and :69-149 (lazy/lowMemory)
+//   k-NN order           src/edm.cpp:652-681  — ascending (distance, library position)
+//   k policy             src/edm.cpp:593-621
+//   simplex              src/edm.cpp:707-735
+//   S-map                src/edm.cpp:737-814  — Gram X^T X, X^T y, then the JacobiSVD solve
+// Built with -ffp-contract=off: every FP64 multiply and add is rounded separately, in the same
+// order as the CPU path, so distances, neighbour order and S-map Gram matrices are bit-identical.
+#include "kernels_common.h"
+#include "fdlibm_exp.h"
+
+namespace fedm {
+
+// ----------------------------------------------------------------------------------------------
+// Pair distance.  The library point comes from LDS (xs), the target from registers (y).
+// Returns the accumulated distance, or MISSING_D when the pair is skipped or has a missing
+// coordinate without missingdistance (both are dropped by the caller's keep rule).
+template <int EMAX, bool GENERIC>
+__device__ __forceinline__ double pair_acc(const LpParams& P, const double* __restrict__ xs, const double (&y)[EMAX],
+                                           int i, int lib_panel, double lib_tt, int tgt_panel, double tgt_pt)
+{
+  double acc = 0.0;
+  if (GENERIC) {
+    const int f = P.flags;
+    if ((f & F_SKIP_OTHER) && lib_panel != tgt_panel)
+      return MISSING_D;
+    if ((f & F_SKIP_FUTURE) && lib_tt > tgt_pt)
+      return MISSING_D;
+    if (f & F_IDW)
+      acc += P.idw * (double)(lib_panel != tgt_panel);
+    else if (f & F_IDWEIGHTS)
+      acc += (double)P.id_weights[(size_t)lib_panel * P.num_panels + tgt_panel];
+    const bool mae = f & F_MAE;
+#pragma unroll
+    for (int j = 0; j < EMAX; j++) {
+      if (j < P.Ea) {
+        const double x = xs[j];
+        double d;
+        if ((f & F_MISSING) && (x == MISSING_D || y[j] == MISSING_D)) {
+          if (P.missing_distance == 0.0)
+            return MISSING_D;
+          d = P.missing_distance;
+        } else if ((f & F_CATEGORICAL) && P.metrics[j] == 1) {
+          d = (double)(x != y[j]);
+        } else {
+          d = x - y[j];
+        }
+        if (mae)
+          acc += fabs(d) / (double)P.Ea;
+        else
+          acc += d * d;
+      }
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < EMAX; j++) {
+      if (j < P.Ea) {
+        const double d = xs[j] - y[j];
+        acc += d * d;
+      }
+    }
+  }
+  return acc;
+}
+
+// Lexicographic (s, idx) order of the reference's k-NN (src/edm.cpp:663-668).
+__device__ __forceinline__ bool key_less(double s1, int i1, double s2, int i2)
+{
+  return s1 < s2 || (s1 == s2 && i1 < i2);
+}
+
+// Sorted insertion of (s, i) into a register-resident list of KCAP entries.
+template <int KCAP>
+__device__ __forceinline__ void list_insert(double (&ls)[KCAP], int (&li)[KCAP], double s, int i)
+{
+  double cs = s;
+  int ci = i;
+#pragma unroll
+  for (int j = 0; j < KCAP; j++) {
+    const bool take = key_less(cs, ci, ls[j], li[j]);
+    const double ts = take ? cs : ls[j];
+    const int ti = take ? ci : li[j];
+    cs = take ? ls[j] : cs;
+    ci = take ? li[j] : ci;
+    ls[j] = ts;
+    li[j] = ti;
+  }
+}
+
+template <int KCAP>
+__device__ __forceinline__ double list_at(const double (&ls)[KCAP], int k)
+{
+  double v = INF_D;
+#pragma unroll
+  for (int j = 0; j < KCAP; j++)
+    if (j == k - 1)
+      v = ls[j];
+  return v;
+}
+
+// Staging of one library chunk into LDS (row-major [CH][Ea]), plus panel/target-time columns.
+__device__ __forceinline__ void stage_chunk(const LpParams& P, double* lds_x, int* lds_panel, double* lds_tt, int c0,
+                                            int cn, bool generic)
+{
+  const int total = cn * P.Ea;
+  const double* src = P.M + (size_t)c0 * P.Ea;
+  for (int e = threadIdx.x; e < total; e += blockDim.x)
+    lds_x[e] = src[e];
+  if (generic) {
+    for (int e = threadIdx.x; e < cn; e += blockDim.x) {
+      lds_panel[e] = P.Mpanel ? P.Mpanel[c0 + e] : 0;
+      lds_tt[e] = P.Mtt ? P.Mtt[c0 + e] : 0.0;
+    }
+  }
+}
+
+// ----------------------------------------------------------------------------------------------
+// Distance + exact top-KCAP per (target, library slice).
+// grid = (ceil(NP / 256), S), block = 256: one thread per target, one library slice per blockIdx.y.
+// Writes the slice's list sorted by (s, position) and its count of valid pairs.
+template <int KCAP, int EMAX, bool GENERIC>
+__global__ __launch_bounds__(256) void knn_slices_f64(LpParams P, int k, int slice_len, double* __restrict__ out_s,
+                                                     int* __restrict__ out_i, int* __restrict__ out_cnt)
+{
+  __shared__ double lds_x[LDS_CHUNK_DOUBLES];
+  __shared__ int lds_panel[LDS_CHUNK_MAXPTS];
+  __shared__ double lds_tt[LDS_CHUNK_MAXPTS];
+
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool active = t < P.NP;
+  const int lo = blockIdx.y * slice_len;
+  const int hi = min(P.NL, lo + slice_len);
+
+  double y[EMAX];
+#pragma unroll
+  for (int j = 0; j < EMAX; j++)
+    y[j] = (active && j < P.Ea) ? P.Mp[(size_t)t * P.Ea + j] : 0.0;
+  const int tgt_panel = (GENERIC && active && P.Mppanel) ? P.Mppanel[t] : 0;
+  const double tgt_pt = (GENERIC && active && P.Mppt) ? P.Mppt[t] : 0.0;
+
+  double ls[KCAP];
+  int li[KCAP];
+#pragma unroll
+  for (int j = 0; j < KCAP; j++) {
+    ls[j] = INF_D;
+    li[j] = INT_MAX;
+  }
+  double s_kth = INF_D, thr = INF_D;
+  int cnt = 0;
+  const bool mae = GENERIC && (P.flags & F_MAE);
+  const int ch = min(LDS_CHUNK_MAXPTS, LDS_CHUNK_DOUBLES / P.Ea);
+
+  for (int c0 = lo; c0 < hi; c0 += ch) {
+    const int cn = min(ch, hi - c0);
+    __syncthreads();
+    stage_chunk(P, lds_x, lds_panel, lds_tt, c0, cn, GENERIC);
+    __syncthreads();
+    if (active) {
+      for (int c = 0; c < cn; c++) {
+        const double acc = pair_acc<EMAX, GENERIC>(P, lds_x + c * P.Ea, y, c0 + c, GENERIC ? lds_panel[c] : 0,
+                                                   GENERIC ? lds_tt[c] : 0.0, tgt_panel, tgt_pt);
+        if (acc != 0.0 && acc != MISSING_D) {
+          cnt++;
+          if (acc < thr) {
+            const double s = mae ? acc : sqrt(acc);
+            if (s < s_kth) {
+              list_insert<KCAP>(ls, li, s, c0 + c);
+              s_kth = list_at<KCAP>(ls, k);
+              thr = (s_kth == INF_D) ? INF_D : (mae ? s_kth : s_kth * s_kth * SQRT_GUARD);
+            }
+          }
+        }
+      }
+    }
+  }
+  if (active) {
+    const size_t base = ((size_t)blockIdx.y * P.NP + t) * KCAP;
+#pragma unroll
+    for (int j = 0; j < KCAP; j++) {
+      out_s[base + j] = ls[j];
+      out_i[base + j] = li[j];
+    }
+    out_cnt[(size_t)blockIdx.y * P.NP + t] = cnt;
+  }
+}
+
+// ----------------------------------------------------------------------------------------------
+// Dense distances (k = all): D[t][i] = s, or -1 where the pair is not a valid neighbour.
+template <int EMAX, bool GENERIC>
+__global__ __launch_bounds__(256) void dense_dist_f64(LpParams P, int t0, int nt, int slice_len, double* __restrict__ D,
+                                                     int* __restrict__ out_cnt)
+{
+  __shared__ double lds_x[LDS_CHUNK_DOUBLES];
+  __shared__ int lds_panel[LDS_CHUNK_MAXPTS];
+  __shared__ double lds_tt[LDS_CHUNK_MAXPTS];
+
+  const int tl = blockIdx.x * blockDim.x + threadIdx.x;
+  const int t = t0 + tl;
+  const bool active = tl < nt && t < P.NP;
+  const int lo = blockIdx.y * slice_len;
+  const int hi = min(P.NL, lo + slice_len);
+  double y[EMAX];
+#pragma unroll
+  for (int j = 0; j < EMAX; j++)
+    y[j] = (active && j < P.Ea) ? P.Mp[(size_t)t * P.Ea + j] : 0.0;
+  const int tgt_panel = (GENERIC && active && P.Mppanel) ? P.Mppanel[t] : 0;
+  const double tgt_pt = (GENERIC && active && P.Mppt) ? P.Mppt[t] : 0.0;
+  const bool mae = GENERIC && (P.flags & F_MAE);
+  const int ch = min(LDS_CHUNK_MAXPTS, LDS_CHUNK_DOUBLES / P.Ea);
+  int cnt = 0;
+  double* row = D + (size_t)tl * P.NL;
+  for (int c0 = lo; c0 < hi; c0 += ch) {
+    const int cn = min(ch, hi - c0);
+    __syncthreads();
+    stage_chunk(P, lds_x, lds_panel, lds_tt, c0, cn, GENERIC);
+    __syncthreads();
+    if (active) {
+      for (int c = 0; c < cn; c++) {
+        const double acc = pair_acc<EMAX, GENERIC>(P, lds_x + c * P.Ea, y, c0 + c, GENERIC ? lds_panel[c] : 0,
+                                                   GENERIC ? lds_tt[c] : 0.0, tgt_panel, tgt_pt);
+        double s = -1.0;
+        if (acc != 0.0 && acc != MISSING_D) {
+          cnt++;
+          s = mae ? acc : sqrt(acc);
+        }
+        row[c0 + c] = s;
+      }
+    }
+  }
+  if (active)
+    out_cnt[(size_t)blockIdx.y * P.NP + t] = cnt;
+}
+
+// ----------------------------------------------------------------------------------------------
+// Merge the per-slice lists, apply the k policy (src/edm.cpp:593-621), record the neighbour set,
+// and make the simplex predictions (src/edm.cpp:707-735).  One thread per target.
+// nn_k[t] = number of neighbours in nn_i/nn_s; -1 means "all valid, in library order, see D";
+// 0 means no prediction is made for this target.
+template <int KCAP>
+__global__ __launch_bounds__(256) void finish_lists(FinishParams F, const double* __restrict__ ps,
+                                                   const int* __restrict__ pi, const int* __restrict__ pcnt, int S)
+{
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= F.NP)
+    return;
+  int cnt = 0;
+  for (int s = 0; s < S; s++)
+    cnt += pcnt[(size_t)s * F.NP + t];
+
+  int k = F.k;
+  int* nk = F.nn_k + t;
+  if (k > cnt) {
+    if (F.force_compute) {
+      k = cnt;
+    } else {
+      F.rc[t] = RC_INSUFFICIENT_UNIQUE;
+      *nk = 0;
+      return;
+    }
+  }
+  if (k == 0 || cnt == 0) {
+    F.rc[t] = F.force_compute ? RC_SUCCESS : RC_INSUFFICIENT_UNIQUE;
+    *nk = 0;
+    return;
+  }
+  int* oi = F.nn_i + (size_t)t * F.nn_stride;
+  double* os = F.nn_s + (size_t)t * F.nn_stride;
+  if (k == cnt) {
+    // Every valid neighbour, in library order (src/edm.cpp:615-616). cnt <= KCAP here, so every
+    // slice list holds all of its valid pairs; slices are in library order already.
+    int o = 0;
+    for (int s = 0; s < S; s++) {
+      const int c = pcnt[(size_t)s * F.NP + t];
+      // a slice list is sorted by distance: re-sort its c entries by position
+      const size_t base = ((size_t)s * F.NP + t) * KCAP;
+      for (int a = 0; a < c; a++) {
+        oi[o + a] = pi[base + a];
+        os[o + a] = ps[base + a];
+      }
+      for (int a = 1; a < c; a++) {
+        const int ki = oi[o + a];
+        const double ks = os[o + a];
+        int b = a - 1;
+        while (b >= 0 && oi[o + b] > ki) {
+          oi[o + b + 1] = oi[o + b];
+          os[o + b + 1] = os[o + b];
+          b--;
+        }
+        oi[o + b + 1] = ki;
+        os[o + b + 1] = ks;
+      }
+      o += c;
+    }
+  } else {
+    // First k of the merged (s, position) order.
+    int head[MAX_SLICES];
+    for (int s = 0; s < S; s++)
+      head[s] = 0;
+    for (int o = 0; o < k; o++) {
+      int best = -1;
+      double bs = INF_D;
+      int bi = INT_MAX;
+      for (int s = 0; s < S; s++) {
+        const int h = head[s];
+        if (h >= KCAP)
+          continue;
+        const size_t base = ((size_t)s * F.NP + t) * KCAP + h;
+        const double vs = ps[base];
+        const int vi = pi[base];
+        if (vi != INT_MAX && key_less(vs, vi, bs, bi)) {
+          bs = vs;
+          bi = vi;
+          best = s;
+        }
+      }
+      if (best < 0) {
+        k = o;
+        break;
+      }
+      head[best]++;
+      oi[o] = bi;
+      os[o] = bs;
+    }
+  }
+  *nk = k;
+  F.k_used[t] = k;
+  if (F.nn_out) {
+    for (int j = 0; j < F.nn_cap; j++)
+      F.nn_out[(size_t)t * F.nn_cap + j] = j < k ? oi[j] : -1;
+  }
+  if (F.algorithm == ALG_SIMPLEX) {
+    double minDist = os[0];
+    for (int j = 1; j < k; j++)
+      minDist = fmin(minDist, os[j]);
+    for (int th = 0; th < F.T; th++) {
+      const double theta = F.thetas[th];
+      double sumw = 0.0;
+      for (int j = 0; j < k; j++)
+        sumw = sumw + fdlibm_exp(-theta * (os[j] / minDist));
+      double r = 0.0;
+      for (int j = 0; j < k; j++)
+        r = r + F.targets[oi[j]] * (fdlibm_exp(-theta * (os[j] / minDist)) / sumw);
+      F.pred[(size_t)th * F.NP + t] = r;
+      F.rc[(size_t)th * F.NP + t] = RC_SUCCESS;
+    }
+  } else if (F.algorithm != ALG_SMAP) {
+    F.rc[t] = RC_INVALID_ALGORITHM;
+    *nk = 0;
+  }
+}
+
+// Same policy for the dense (k = all) route: per-slice counts were written by dense_dist_f64.
+__global__ __launch_bounds__(256) void finish_dense(FinishParams F, const double* __restrict__ D, int t0, int nt,
+                                                   const int* __restrict__ pcnt, int S)
+{
+  const int tl = blockIdx.x * blockDim.x + threadIdx.x;
+  const int t = t0 + tl;
+  if (tl >= nt || t >= F.NP)
+    return;
+  int cnt = 0;
+  for (int s = 0; s < S; s++)
+    cnt += pcnt[(size_t)s * F.NP + t];
+  int k = F.k;
+  if (k > cnt) {
+    if (F.force_compute) {
+      k = cnt;
+    } else {
+      F.rc[t] = RC_INSUFFICIENT_UNIQUE;
+      F.nn_k[t] = 0;
+      return;
+    }
+  }
+  if (k == 0 || cnt == 0) {
+    F.rc[t] = F.force_compute ? RC_SUCCESS : RC_INSUFFICIENT_UNIQUE;
+    F.nn_k[t] = 0;
+    return;
+  }
+  if (k > 0 && k < cnt) {
+    // a selection wider than the register lists is not made on this route
+    F.rc[t] = RC_UNKNOWN_ERROR;
+    F.nn_k[t] = 0;
+    return;
+  }
+  // k < 0 or k == cnt: every valid neighbour in library order.
+  F.nn_k[t] = -1;
+  F.nn_cnt[t] = cnt;
+  F.k_used[t] = cnt;
+  const double* row = D + (size_t)tl * F.NL;
+  if (F.nn_out) {
+    int o = 0;
+    for (int i = 0; i < F.NL && o < F.nn_cap; i++)
+      if (row[i] >= 0.0)
+        F.nn_out[(size_t)t * F.nn_cap + o++] = i;
+    for (; o < F.nn_cap; o++)
+      F.nn_out[(size_t)t * F.nn_cap + o] = -1;
+  }
+  if (F.algorithm == ALG_SIMPLEX) {
+    double minDist = INF_D;
+    for (int i = 0; i < F.NL; i++)
+      if (row[i] >= 0.0)
+        minDist = fmin(minDist, row[i]);
+    for (int th = 0; th < F.T; th++) {
+      const double theta = F.thetas[th];
+      double sumw = 0.0;
+      for (int i = 0; i < F.NL; i++)
+        if (row[i] >= 0.0)
+          sumw = sumw + fdlibm_exp(-theta * (row[i] / minDist));
+      double r = 0.0;
+      for (int i = 0; i < F.NL; i++)
+        if (row[i] >= 0.0)
+          r = r + F.targets[i] * (fdlibm_exp(-theta * (row[i] / minDist)) / sumw);
+      F.pred[(size_t)th * F.NP + t] = r;
+      F.rc[(size_t)th * F.NP + t] = RC_SUCCESS;
+    }
+  } else if (F.algorithm != ALG_SMAP) {
+    F.rc[t] = RC_INVALID_ALGORITHM;
+    F.nn_k[t] = 0;
+  }
+}
+
+// ----------------------------------------------------------------------------------------------
+// S-map: one wavefront per (target, theta).  Gram matrix in FP64 in the CPU path's summation
+// order, then the JacobiSVD solve in LDS (jacobi_wave.h).
+template <int NMAX>
+__global__ __launch_bounds__(64) void smap_f64(SmapParams S, const double* __restrict__ D, int t0, int nt)
+{
+  constexpr int NTRI = NMAX * (NMAX + 1) / 2;
+  constexpr int EPL = (NTRI + NMAX + 63) / 64; // Gram + rhs entries per lane
+  extern __shared__ double smem[];
+  const int Ea = S.Ea, n = Ea + 1;
+  // dynamic LDS: Xc[64 x n] | yc[64] | cd[64] | W, U, V [n x n] | bvec, sv, tmp, xsol [n] | cidx[64]
+  double* Xc = smem;
+  double* yc = Xc + 64 * n;
+  double* cd = yc + 64;
+  double* W = cd + 64;
+  double* U = W + n * n;
+  double* V = U + n * n;
+  double* bvec = V + n * n;
+  double* sv = bvec + n;
+  double* tmp = sv + n;
+  double* xsol = tmp + n;
+  int* cidx = (int*)(xsol + n);
+
+  const int tl = blockIdx.x;
+  const int t = t0 + tl;
+  const int th = blockIdx.y;
+  const int lane = threadIdx.x;
+  if (tl >= nt || t >= S.NP)
+    return;
+  const int kk = S.nn_k[t];
+  if (kk == 0)
+    return;
+  const bool dense = kk < 0;
+  const int k = dense ? S.nn_cnt[t] : kk;
+  const double* drow = dense ? D + (size_t)tl * S.NL : nullptr;
+  const int* li = S.nn_i + (size_t)t * S.nn_stride;
+  const double* lsd = S.nn_s + (size_t)t * S.nn_stride;
+
+  // mean of the selected distances, summed in neighbour order (Eigen mean = sum / size)
+  double sum = 0.0;
+  if (lane == 0) {
+    if (dense) {
+      for (int i = 0; i < S.NL; i++)
+        if (drow[i] >= 0.0)
+          sum += drow[i];
+    } else {
+      for (int j = 0; j < k; j++)
+        sum += lsd[j];
+    }
+  }
+  sum = __shfl(sum, 0);
+  const double mean = sum / (double)k;
+  const double theta = S.thetas[th];
+
+  // this lane's Gram entries: e < ntri -> (p, q) of the upper triangle, then the rhs rows
+  const int ntri = n * (n + 1) / 2;
+  int ep[EPL], eq[EPL];
+  double acc[EPL];
+#pragma unroll
+  for (int m = 0; m < EPL; m++) {
+    const int e = lane + 64 * m;
+    acc[m] = 0.0;
+    ep[m] = -1;
+    eq[m] = -1;
+    if (e < ntri) {
+      int p = 0, rem = e;
+      while (rem >= n - p) {
+        rem -= n - p;
+        p++;
+      }
+      ep[m] = p;
+      eq[m] = p + rem;
+    } else if (e < ntri + n) {
+      ep[m] = e - ntri;
+      eq[m] = -2;
+    }
+  }
+
+  int pos = 0, done = 0;
+  while (done < k) {
+    int cnum = 0;
+    if (dense) {
+      // next run of valid entries of the distance row, in library order
+      while (cnum == 0 && pos < S.NL) {
+        const int i = pos + lane;
+        const bool v = i < S.NL && drow[i] >= 0.0;
+        const unsigned long long b = __ballot(v);
+        const int before = __popcll(b & ((1ull << lane) - 1ull));
+        if (v) {
+          cidx[before] = i;
+          cd[before] = drow[i];
+        }
+        cnum = __popcll(b);
+        pos += 64;
+      }
+    } else {
+      cnum = min(64, k - done);
+      if (lane < cnum) {
+        cidx[lane] = li[done + lane];
+        cd[lane] = lsd[done + lane];
+      }
+    }
+    if (cnum == 0)
+      break;
+    __syncthreads();
+    if (lane < cnum) {
+      const int ci = cidx[lane];
+      const double w = fdlibm_exp(-theta * (cd[lane] / mean));
+      const double* row = S.M + (size_t)ci * Ea;
+      Xc[lane * n] = w;
+      for (int j = 1; j < n; j++)
+        Xc[lane * n + j] = w * row[j - 1];
+      yc[lane] = w * S.targets[ci];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < EPL; m++) {
+      if (ep[m] >= 0) {
+        double a = acc[m];
+        if (eq[m] >= 0) {
+          for (int c = 0; c < cnum; c++)
+            a += Xc[c * n + ep[m]] * Xc[c * n + eq[m]];
+        } else {
+          for (int c = 0; c < cnum; c++)
+            a += Xc[c * n + ep[m]] * yc[c];
+        }
+        acc[m] = a;
+      }
+    }
+    __syncthreads();
+    done += cnum;
+  }
+#pragma unroll
+  for (int m = 0; m < EPL; m++) {
+    if (ep[m] >= 0) {
+      if (eq[m] >= 0) {
+        W[ep[m] * n + eq[m]] = acc[m];
+        W[eq[m] * n + ep[m]] = acc[m];
+      } else {
+        bvec[ep[m]] = acc[m];
+      }
+    }
+  }
+  __syncthreads();
+
+  jacobi_svd_solve_wave(n, W, U, V, bvec, sv, tmp, xsol, lane);
+
+  if (lane == 0) {
+    double r = xsol[0];
+    const double* yp = S.Mp + (size_t)t * Ea;
+    for (int j = 0; j < Ea; j++)
+      if (yp[j] != MISSING_D)
+        r += yp[j] * xsol[j + 1];
+    S.pred[(size_t)th * S.NP + t] = r;
+    S.rc[(size_t)th * S.NP + t] = RC_SUCCESS;
+  }
+  if (S.coeffs && th == S.T - 1 && lane < n) {
+    const double c = xsol[lane];
+    S.coeffs[(size_t)t * n + lane] = fabs(c) < 1.0e-11 ? MISSING_D : c;
+  }
+}
+
+// LDS bytes smap_f64 needs for an n x n system.
+size_t smap_lds_bytes(int n)
+{
+  return sizeof(double) * (size_t)(64 * n + 64 + 64 + 3 * n * n + 4 * n) + sizeof(int) * 64;
+}
+
+// ----------------------------------------------------------------------------------------------
+// Explicit instantiations used by the dispatcher.
+#define FEDM_KNN(KC, EM)                                                                                         \
+  template __global__ void knn_slices_f64<KC, EM, false>(LpParams, int, int, double*, int*, int*);               \
+  template __global__ void knn_slices_f64<KC, EM, true>(LpParams, int, int, double*, int*, int*);
+#define FEDM_KNN_E(KC) FEDM_KNN(KC, 8) FEDM_KNN(KC, 16) FEDM_KNN(KC, 32) FEDM_KNN(KC, 64)
+FEDM_KNN_E(4)
+FEDM_KNN_E(8)
+FEDM_KNN_E(16)
+FEDM_KNN_E(32)
+FEDM_KNN_E(64)
+template __global__ void dense_dist_f64<8, false>(LpParams, int, int, int, double*, int*);
+template __global__ void dense_dist_f64<16, false>(LpParams, int, int, int, double*, int*);
+template __global__ void dense_dist_f64<32, false>(LpParams, int, int, int, double*, int*);
+template __global__ void dense_dist_f64<64, false>(LpParams, int, int, int, double*, int*);
+template __global__ void dense_dist_f64<8, true>(LpParams, int, int, int, double*, int*);
+template __global__ void dense_dist_f64<16, true>(LpParams, int, int, int, double*, int*);
+template __global__ void dense_dist_f64<32, true>(LpParams, int, int, int, double*, int*);
+template __global__ void dense_dist_f64<64, true>(LpParams, int, int, int, double*, int*);
+template __global__ void finish_lists<4>(FinishParams, const double*, const int*, const int*, int);
+template __global__ void finish_lists<8>(FinishParams, const double*, const int*, const int*, int);
+template __global__ void finish_lists<16>(FinishParams, const double*, const int*, const int*, int);
+template __global__ void finish_lists<32>(FinishParams, const double*, const int*, const int*, int);
+template __global__ void finish_lists<64>(FinishParams, const double*, const int*, const int*, int);
+template __global__ void smap_f64<8>(SmapParams, const double*, int, int);
+template __global__ void smap_f64<16>(SmapParams, const double*, int, int);
+template __global__ void smap_f64<32>(SmapParams, const double*, int, int);
+template __global__ void smap_f64<64>(SmapParams, const double*, int, int);
+
+} // namespace fedm

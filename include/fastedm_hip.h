@@ -1,0 +1,210 @@
+/*
+ * fastedm_hip.h — C ABI of the MI355X prediction core of fastEDM (libfastedm_hip.so).
+ *
+ * Plain pointers and sizes only; nothing is thrown across this boundary.  Every entry point
+ * returns 0 on success, a positive EDM retcode (1 = BREAK_HIT) when interrupted through keep_going,
+ * or a negative value on a device or argument error (the per-target rc outputs are then 8000).
+ *
+ * Conventions shared with the reference (EDM-Developers/fastEDM-r):
+ *   missing value   FEDM_MISSING_D = 1e100        (src/manifold.h:6)
+ *   missing integer FEDM_MISSING_I = INT_MIN      (src/manifold.h:8)
+ *   retcodes        0 SUCCESS, 1 BREAK_HIT, 400 INVALID_ALGORITHM, 503 INSUFFICIENT_UNIQUE,
+ *                   8000 UNKNOWN_ERROR            (src/common.h:3-13)
+ */
+#ifndef FASTEDM_HIP_H
+#define FASTEDM_HIP_H
+
+#include <limits.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FEDM_MISSING_D 1.0e+100
+#define FEDM_MISSING_I INT_MIN
+
+enum
+{
+  FEDM_SUCCESS = 0,
+  FEDM_BREAK_HIT = 1,
+  FEDM_INVALID_ALGORITHM = 400,
+  FEDM_INSUFFICIENT_UNIQUE = 503,
+  FEDM_UNKNOWN_ERROR = 8000
+};
+enum
+{
+  FEDM_SIMPLEX = 0,
+  FEDM_SMAP = 1
+};
+enum
+{
+  FEDM_DIST_MAE = 0,
+  FEDM_DIST_EUCLIDEAN = 1,
+  FEDM_DIST_WASSERSTEIN = 2
+};
+enum
+{
+  FEDM_PLAIN = 0, /* FP64 throughout: predictions to 1e-10 of the CPU path, neighbour indices bit-exact */
+  FEDM_MIXED = 1, /* FP32 distance screening + FP64 re-rank; FP32 S-map Gram, FP64 solve */
+  FEDM_PLAIN_TASKWISE = 2 /* command level only: plain, one edm_predict per task (the drop-in seam) */
+};
+
+/* A manifold as the reference's Manifold exposes it (src/manifold.h:159-286). */
+typedef struct
+{
+  const double* data;         /* [num_points x E_actual] row-major */
+  const double* targets;      /* [num_points] */
+  const double* point_times;  /* [num_points] */
+  const double* target_times; /* [num_points] */
+  const int* panel;           /* [num_points] or NULL */
+  int num_points, E_x, E_dt, E_extras, E_lagged_extras, E_actual;
+} edm_manifold_view;
+
+/* The fields of the reference's Options (src/common.h:64-91) that the predictor reads. */
+typedef struct
+{
+  int algorithm;       /* FEDM_SIMPLEX / FEDM_SMAP; anything else gives rc INVALID_ALGORITHM */
+  int distance;        /* FEDM_DIST_* */
+  int k;               /* neighbours, already set by the task configurator (src/edm.cpp:155-163); <0 = all */
+  int num_thetas;
+  const double* thetas; /* [num_thetas] */
+  const int* metrics;   /* [E_actual] 0 = Diff, 1 = CheckSame (expanded, src/distances.cpp:29-67), or NULL */
+  double missing_distance;
+  int panel_mode;
+  double idw;              /* panel mismatch penalty; <0 = skip other panels */
+  int num_panels;          /* dimension of id_weights */
+  const float* id_weights; /* [num_panels x num_panels] keyed (library panel, prediction panel); panel ids
+                              must then lie in [0, num_panels). NULL = none */
+  int has_missing, has_categorical, use_only_past, force_compute, save_smap_coeffs, low_memory;
+  double aspect_ratio; /* Wasserstein time scaling */
+  int precision;       /* FEDM_PLAIN / FEDM_MIXED */
+  int nthreads;        /* unused by the GPU path */
+} edm_options;
+
+typedef struct edm_ctx edm_ctx;
+
+/* Device context: one HIP stream and a grow-only pool of device buffers reused across tasks.
+ * Replaces the reference's per-task af::setDevice + toGPU uploads (src/edm.cpp:320-332). */
+int edm_ctx_create(edm_ctx** ctx, int device);
+void edm_ctx_destroy(edm_ctx* ctx);
+
+/*
+ * edm_predict — every prediction of one task on the GPU.
+ * Replaces af_make_prediction (src/edm.cpp:1058-1061), called from edm_task (src/edm.cpp:401-408) in
+ * place of the per-target make_prediction loop (src/edm.cpp:367-428); semantics are those of the
+ * CPU path make_prediction (src/edm.cpp:565-814).
+ *   pred   [num_thetas x Mp->num_points] row-major (MISSING_D where no prediction)
+ *   rc     [num_thetas x Mp->num_points]
+ *   coeffs [Mp->num_points x (E_actual+1)] or NULL (S-map coefficients of the last theta)
+ *   k_used [Mp->num_points] (MISSING_I where no neighbours were used)
+ *   nn_idx [Mp->num_points x nn_cap] or NULL: the chosen library positions in the order the
+ *          predictor used them, -1 padded (not part of the reference's seam; used for parity checks)
+ *   keep_going: polled between stages, may be NULL.
+ */
+int edm_predict(edm_ctx* ctx, const edm_manifold_view* M, const edm_manifold_view* Mp, const edm_options* o,
+                double* pred, int* rc, double* coeffs, int* k_used, int* nn_idx, int nn_cap, int (*keep_going)(void));
+
+/*
+ * edm_predict_batch — many tasks that index one shared point table, in one persistent launch per
+ * stage (CCM library-size x replicate sweeps, E/theta grids).  Task t uses library rows
+ * lib_rows[lib_begin[t] .. lib_begin[t]+lib_count[t]) (ascending, rows of L) and prediction rows
+ * pred_rows[pred_begin[t] .. +pred_count[t]) (rows of P); ranges may be shared between tasks.
+ * The per-task results are those edm_predict gives for the manifolds these rows select
+ * (the reference's per-task edm_task, src/edm.cpp:285-548, run serially at src/edm.cpp:249-251).
+ *   k[t]: neighbours of task t (as edm_options.k); o->k is ignored.
+ *   pred/rc: task t writes [num_thetas x pred_count[t]] at out_begin[t]; k_used at kbegin[t]=out_begin[t]/num_thetas.
+ * The library table L carries the library-side targets/times; P the prediction-side points.
+ */
+int edm_predict_batch(edm_ctx* ctx, const edm_manifold_view* L, const edm_manifold_view* P, const edm_options* o,
+                      int num_tasks, const int* lib_begin, const int* lib_count, const int* lib_rows,
+                      const int* pred_begin, const int* pred_count, const int* pred_rows, const int* k,
+                      const int64_t* out_begin, double* pred, int* rc, int* k_used, int (*keep_going)(void));
+
+/* The same batch in three steps, so that the device-resident part can be timed or replayed:
+ * edm_batch_create uploads the tables, row lists and work queue once; edm_batch_run makes every
+ * prediction on the device (inputs resident in HBM, results left there); edm_batch_fetch copies
+ * pred/rc/k_used back in the layout of edm_predict_batch.  Arguments as edm_predict_batch. */
+typedef struct edm_batch edm_batch;
+int edm_batch_create(edm_ctx* ctx, const edm_manifold_view* L, const edm_manifold_view* P, const edm_options* o,
+                     int num_tasks, const int* lib_begin, const int* lib_count, const int* lib_rows,
+                     const int* pred_begin, const int* pred_count, const int* pred_rows, const int* k,
+                     const int64_t* out_begin, edm_batch** out);
+int edm_batch_run(edm_batch* b);
+int edm_batch_fetch(edm_batch* b, double* pred, int* rc, int* k_used);
+void edm_batch_destroy(edm_batch* b);
+
+/* Last stage timings of the most recent call on ctx (milliseconds, HIP events on the ctx stream):
+ * [0] distance/selection kernel, [1] merge/predict kernel, [2] S-map kernel, [3] whole device section. */
+void edm_ctx_timings(edm_ctx* ctx, double* ms4);
+
+/* ---------------------------------------------------------------------------------------------
+ * Command level: the reference's run_command (src/r.cpp:107-468), i.e. what the R function edm()
+ * (R/edm.R:232-393) calls, with the predictions made on the GPU.
+ */
+typedef struct
+{
+  int n;
+  const double* t;
+  const double* x;
+  const double* y;    /* xmap variable, or NULL for explore */
+  const double* co_x; /* copredict variable or NULL */
+  const int* panel;   /* or NULL */
+  int num_extras;
+  const double* extras; /* [num_extras x n] */
+  int num_E;
+  const int* E;
+  int tau;
+  int num_thetas;
+  const double* thetas;
+  int num_libraries;
+  const int* libraries;
+  int k, algorithm, num_reps, p, crossfold, full, shuffle, seed;
+  int save_final_predictions, save_final_copredictions, save_manifolds, save_smap_coeffs;
+  int dt, reldt;
+  double dt_weight;
+  int allow_missing;
+  double missing_distance, panel_weight;
+  const double* panel_weights; /* [P x P] row-major, or NULL */
+  int panel_weights_dim;
+  int num_threads, low_memory, predict_with_past;
+  int distance;        /* the R front end always passes FEDM_DIST_EUCLIDEAN (src/r.cpp:156) */
+  double aspect_ratio;
+  int precision;       /* FEDM_PLAIN / FEDM_MIXED */
+} fedm_command;
+
+typedef struct fedm_result fedm_result;
+typedef struct fedm_plan fedm_plan;
+
+/* The command in three steps: fedm_plan_create normalises the inputs, draws every library/prediction
+ * split, builds one point table per (E, copredict) and uploads it with the tasks' row lists as
+ * device batches; fedm_plan_run makes every prediction on the device (repeatable: inputs stay
+ * resident); fedm_plan_result copies the predictions back and computes rho/MAE per task as
+ * edm_task does (src/edm.cpp:440-467). */
+int fedm_plan_create(edm_ctx* ctx, const fedm_command* c, fedm_plan** out);
+int fedm_plan_run(fedm_plan* p);
+int fedm_plan_result(fedm_plan* p, fedm_result** out);
+int64_t fedm_plan_num_predictions(const fedm_plan* p);
+void fedm_plan_destroy(fedm_plan* p);
+
+/* fedm_plan_create + fedm_plan_run + fedm_plan_result. */
+
+int fedm_run_command(edm_ctx* ctx, const fedm_command* c, fedm_result** out);
+int fedm_result_rc(const fedm_result* r);
+int fedm_result_num_stats(const fedm_result* r, int copredict);
+void fedm_result_stats(const fedm_result* r, int copredict, int* E, int* library, double* theta, double* rho,
+                       double* mae);
+void fedm_result_k(const fedm_result* r, int* kmin, int* kmax);
+double fedm_result_scalar(const fedm_result* r, int which); /* 0 missingdistance, 1 dtWeight */
+/* which: 0 predictions, 1 copredictions, 2 coeffs; R's to_R_matrix layout (column-major). */
+int fedm_result_matrix_shape(const fedm_result* r, int which, int* rows, int* cols);
+void fedm_result_matrix(const fedm_result* r, int which, double* out);
+int fedm_result_num_manifolds(const fedm_result* r);
+void fedm_result_manifold_shape(const fedm_result* r, int i, int prediction_set, int* rows, int* cols);
+void fedm_result_manifold(const fedm_result* r, int i, int prediction_set, double* out);
+void fedm_result_free(fedm_result* r);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -8,6 +8,8 @@
  */
 #include "oracle.h"
 
+extern "C" double orc_weight_exp(double x);
+
 #include <algorithm>
 #include <atomic>
 #include <cfloat>
@@ -747,6 +749,57 @@ static void jacobi_solve(int n, const double* Ain, const double* b, double* x, d
     *rank_out = rank;
 }
 
+
+/* ---------------------------------------------------------------- weights' exp */
+
+// exp(x) by FDLIBM's published method (Sun e_exp.c): x = k ln2 + r with a two-part ln2, a degree-5
+// rational-polynomial exp(r), exact scaling by 2^k; within 1 ulp of exp.  The reference evaluates its
+// simplex weights with libm exp (src/edm.cpp:722) and its S-map weights with Eigen's array exp
+// (src/edm.cpp:745); both round within 1 ulp too, so the last bit of the weights is not pinned by the
+// reference.  The oracle and the HIP kernels evaluate this same expression, so their FP64 weights agree
+// bit for bit.
+static double weight_exp(double x)
+{
+  const double ln2HI = 6.93147180369123816490e-01, ln2LO = 1.90821492927058770002e-10,
+               invln2 = 1.44269504088896338700e+00;
+  const double P1 = 1.66666666666666019037e-01, P2 = -2.77777777770155933842e-03, P3 = 6.61375632143793436117e-05,
+               P4 = -1.65339022054652515390e-06, P5 = 4.13813679705723846039e-08;
+  if (x != x)
+    return x;
+  if (x > 709.782712893383973096)
+    return HUGE_VAL;
+  if (x < -745.13321910194110842)
+    return 0.0;
+  const double ax = std::fabs(x);
+  if (ax < 3.725290298461914e-09)
+    return 1.0 + x;
+  int k;
+  double hi, lo;
+  if (ax > 0.5 * 0.6931471805599453) {
+    if (ax < 1.5 * 0.6931471805599453) {
+      k = x < 0 ? -1 : 1;
+      hi = x - k * ln2HI;
+      lo = k * ln2LO;
+    } else {
+      k = (int)(invln2 * x + (x < 0 ? -0.5 : 0.5));
+      const double t = k;
+      hi = x - t * ln2HI;
+      lo = t * ln2LO;
+    }
+  } else {
+    k = 0;
+    hi = x;
+    lo = 0.0;
+  }
+  const double r = hi - lo;
+  const double t = r * r;
+  const double c = r - t * (P1 + t * (P2 + t * (P3 + t * (P4 + t * P5))));
+  if (k == 0)
+    return 1.0 - ((r * c) / (c - 2.0) - r);
+  const double y = 1.0 - ((lo - (r * c) / (2.0 - c)) - hi);
+  return std::ldexp(y, k);
+}
+
 /* ---------------------------------------------------------------- predictors (src/edm.cpp:565-814) */
 
 struct TaskOut
@@ -769,7 +822,7 @@ static void simplex(int Mp_i, int t, const orc_options& o, const orc_manifold_vi
   double sumw = 0.0;
   const double theta = o.thetas[t];
   for (int j = 0; j < k; j++) {
-    w[j] = std::exp(-theta * (nn.dists[j] / minDist));
+    w[j] = weight_exp(-theta * (nn.dists[j] / minDist));
     sumw = sumw + w[j];
   }
   double r = 0.0;
@@ -787,7 +840,7 @@ static void smap(int Mp_i, int t, const orc_options& o, const orc_manifold_view&
   double mean = mean_seq(nn.dists.data(), k);
   std::vector<double> w(k), X((size_t)k * n), y(k);
   for (int i = 0; i < k; i++)
-    w[i] = std::exp(-o.thetas[t] * (nn.dists[i] / mean));
+    w[i] = weight_exp(-o.thetas[t] * (nn.dists[i] / mean));
   for (int i = 0; i < k; i++) {
     const double* row = M.data + (size_t)nn.inds[i] * E;
     X[(size_t)i * n] = w[i];
@@ -1530,4 +1583,9 @@ double orc_mae(int n, const double* a, const double* b)
 {
   return mean_absolute_error(std::vector<double>(a, a + n), std::vector<double>(b, b + n));
 }
+}
+
+extern "C" double orc_weight_exp(double x)
+{
+  return orc::weight_exp(x);
 }

@@ -1,0 +1,65 @@
+"""HIP path (edm_predict through the C ABI) against the CPU oracle on seeded tasks.
+
+Bar: neighbour indices, k_used and retcodes bit-exact; predictions and S-map coefficients within
+|d| <= 1e-12 + 1e-10 |ref| (FP64 plain form)."""
+import zlib
+
+import numpy as np
+import pytest
+
+from oracle import oracle
+from tests.conftest import load_product
+from tests.tasks import assert_predictions_close, random_task
+
+pytestmark = pytest.mark.gpu
+
+CASES = {
+    "simplex_E3_k4": dict(NL=500, NP=300, Ea=3, k=4),
+    "simplex_duplicates": dict(NL=300, NP=200, Ea=2, k=3, duplicate_targets=True),
+    "simplex_integer_ties": dict(NL=400, NP=150, Ea=2, k=5, integer=True),
+    "simplex_all": dict(NL=300, NP=120, Ea=3, k=-1, thetas=(0.0, 1.0, 3.0)),
+    "simplex_k_exceeds_valid": dict(NL=6, NP=20, Ea=2, k=8),
+    "simplex_slices_merge": dict(NL=6000, NP=1500, Ea=4, k=7, thetas=(0.5, 1.0)),
+    "smap_all_thetas": dict(NL=600, NP=300, Ea=4, k=-1, algorithm=1, thetas=(0, .5, 1, 2, 4, 8), save_coeffs=True),
+    "smap_k6_coeffs": dict(NL=400, NP=200, Ea=5, k=6, algorithm=1, thetas=(0.2,), save_coeffs=True),
+    "missing_with_distance": dict(NL=300, NP=150, Ea=4, k=5, missing=0.05, missing_distance=0.3),
+    "missing_dropped": dict(NL=300, NP=150, Ea=4, k=5, missing=0.05),
+    "panels_skip_other": dict(NL=300, NP=150, Ea=3, k=4, panels=3, idw=-1.0),
+    "panels_idw": dict(NL=300, NP=150, Ea=3, k=4, panels=3, idw=0.7),
+    "panels_id_weights": dict(NL=300, NP=150, Ea=3, k=4, panels=4, id_weights=True),
+    "only_past": dict(NL=300, NP=150, Ea=3, k=4, past=True),
+    "low_memory_ignores_past": dict(NL=300, NP=150, Ea=3, k=4, past=True, low_memory=True),
+    "categorical": dict(NL=300, NP=150, Ea=3, k=4, categorical=True, integer=True),
+    "mae_distance": dict(NL=300, NP=150, Ea=3, k=4, mae=True),
+    "smap_E16_k18": dict(NL=2000, NP=300, Ea=16, k=18, algorithm=1, thetas=(0, 1, 3)),
+    "smap_E40_k42": dict(NL=1500, NP=120, Ea=40, k=42, algorithm=1, thetas=(2.0,), save_coeffs=True),
+    "simplex_E40_k41": dict(NL=1500, NP=200, Ea=40, k=41),
+    "invalid_algorithm": dict(NL=100, NP=50, Ea=2, k=3, algorithm=7),
+}
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_task_parity(name):
+    fe = load_product()
+    spec = CASES[name]
+    M, Mp, opts = random_task(zlib.crc32(name.encode()) % 1000, **spec)
+    cap = 64
+    print(f"case {name}: NL={spec['NL']} NP={spec['NP']} Ea={spec['Ea']} k={spec['k']}", flush=True)
+    rp, rr, rc_, rk, rn = oracle.predict(M, Mp, opts, nn_cap=cap)
+    gp, gr, gc, gk, gn, status = fe.predict(M, Mp, opts, nn_cap=cap)
+    assert status == 0
+    np.testing.assert_array_equal(gr, rr)
+    np.testing.assert_array_equal(gk, rk)
+    np.testing.assert_array_equal(gn, rn)
+    assert_predictions_close(gp, rp)
+    if opts.get("save_smap_coeffs"):
+        assert_predictions_close(gc, rc_)
+
+
+def test_keep_going_interrupts():
+    fe = load_product()
+    M, Mp, opts = random_task(5, NL=200, NP=100, Ea=3, k=4)
+    gp, gr, gc, gk, gn, status = fe.predict(M, Mp, opts, keep_going=lambda: 0)
+    assert status == 1
+    assert (gr[0] == 1).all()
+    assert (gp == 1.0e100).all()
