@@ -93,8 +93,8 @@ def cpu_baseline(t, x, y, kw, seconds):
         reps = min(100, max(reps + 1, int(reps * seconds / max(dt, 1e-3)) + 1))
     preds = reps * len(libs) * 19998
     return {"value": preds / dt, "unit": "predictions/s", "cores": cores, "kind": "port",
-            "sample": f"{reps} of 100 replicates x 40 libraries ({preds} predictions, {dt:.1f} s, "
-                      f"{cores} threads), rc={res['rc']}"}
+            "sample": f"the first {reps} of the 100 replicates x 40 libraries ({preds} predictions, {dt:.1f} s, "
+                      f"{cores} threads), rc={res['rc']}"}, res, reps
 
 
 def main():
@@ -119,6 +119,8 @@ def main():
     # batched mixed-precision route (the headline)
     print("building the batched mixed-precision plan", file=sys.stderr, flush=True)
     mixed = fe.Plan(t, x, y, precision=fe.MIXED, **kw)
+    if mixed.device_batches != 1:
+        raise SystemExit("the CCM sweep did not map to one persistent device launch")
     n_pred = mixed.num_predictions
     sec, kms = timed_steps(mixed, fe, args.steps, args.warmup)
     print(f"mixed: {sec * 1e3:.1f} ms/step, kernel {kms:.1f} ms", file=sys.stderr, flush=True)
@@ -167,7 +169,13 @@ def main():
 
     if not args.no_cpu:
         print("timing the CPU oracle sample", file=sys.stderr, flush=True)
-        out["cpu_baseline"] = cpu_baseline(t, x, y, kw, args.cpu_seconds)
+        out["cpu_baseline"], res_cpu, reps = cpu_baseline(t, x, y, kw, args.cpu_seconds)
+        n = len(res_cpu["stats"]["rho"])
+        out["parity"]["oracle_sample_tasks"] = n
+        out["parity"]["max_abs_d_rho_mixed_vs_oracle"] = float(
+            np.nanmax(np.abs(res_mixed["stats"]["rho"][:n] - res_cpu["stats"]["rho"])))
+        out["parity"]["max_abs_d_mae_mixed_vs_oracle"] = float(
+            np.nanmax(np.abs(res_mixed["stats"]["mae"][:n] - res_cpu["stats"]["mae"])))
     print(json.dumps(out))
 
 

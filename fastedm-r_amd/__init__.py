@@ -56,7 +56,7 @@ class Command(C.Structure):
 
 EXPORTS = ["edm_ctx_create", "edm_ctx_destroy", "edm_predict", "edm_predict_batch", "edm_ctx_timings",
            "edm_batch_create", "edm_batch_run", "edm_batch_fetch", "edm_batch_destroy", "fedm_plan_create",
-           "fedm_plan_run", "fedm_plan_result", "fedm_plan_num_predictions", "fedm_plan_destroy", "fedm_run_command", "fedm_result_rc", "fedm_result_num_stats", "fedm_result_stats", "fedm_result_k",
+           "fedm_plan_run", "fedm_plan_result", "fedm_plan_num_predictions", "fedm_plan_device_batches", "fedm_plan_destroy", "edm_batch_on_device", "fedm_run_command", "fedm_result_rc", "fedm_result_num_stats", "fedm_result_stats", "fedm_result_k",
            "fedm_result_scalar", "fedm_result_matrix_shape", "fedm_result_matrix", "fedm_result_num_manifolds",
            "fedm_result_manifold_shape", "fedm_result_manifold", "fedm_result_free"]
 
@@ -94,6 +94,7 @@ def load_library():
         L.fedm_plan_num_predictions.argtypes = [C.c_void_p]
         L.fedm_plan_num_predictions.restype = C.c_int64
         L.fedm_plan_destroy.argtypes = [C.c_void_p]
+        L.fedm_plan_device_batches.argtypes = [C.c_void_p]
         for name in ["fedm_result_rc", "fedm_result_num_stats", "fedm_result_stats", "fedm_result_k",
                      "fedm_result_scalar", "fedm_result_matrix_shape", "fedm_result_matrix",
                      "fedm_result_num_manifolds", "fedm_result_manifold_shape", "fedm_result_manifold",
@@ -419,6 +420,11 @@ class Plan:
     @property
     def num_predictions(self):
         return int(load_library().fedm_plan_num_predictions(self._h))
+
+    @property
+    def device_batches(self):
+        """Number of (E, copredict) groups that run as one persistent device launch."""
+        return int(load_library().fedm_plan_device_batches(self._h))
 
     def run(self):
         status = load_library().fedm_plan_run(self._h)

@@ -662,11 +662,23 @@ int edm_batch_create(edm_ctx* c, const edm_manifold_view* L, const edm_manifold_
                        o->distance != FEDM_DIST_EUCLIDEAN;
   const int emax = Ea <= 4 ? 4 : Ea <= 8 ? 8 : Ea <= 16 ? 16 : Ea <= 32 ? 32 : -1;
   const int kcap = kmax <= 4 ? 4 : kmax <= 8 ? 8 : kmax <= 16 ? 16 : kmax <= 32 ? 32 : -1;
+  // largest coordinate magnitude over the rows the tasks use (scales the FP32 screening bound)
   double A = 0;
-  for (double v : b->Ldata)
-    A = std::max(A, std::fabs(v));
-  for (double v : b->Pdata)
-    A = std::max(A, std::fabs(v));
+  {
+    std::vector<char> usedL(L->num_points, 0), usedP(P->num_points, 0);
+    for (size_t i = 0; i < lib_total; i++)
+      usedL[lib_rows[i]] = 1;
+    for (size_t i = 0; i < pred_total; i++)
+      usedP[pred_rows[i]] = 1;
+    for (int r = 0; r < L->num_points; r++)
+      if (usedL[r])
+        for (int j = 0; j < Ea; j++)
+          A = std::max(A, std::fabs(b->Ldata[(size_t)r * Ea + j]));
+    for (int r = 0; r < P->num_points; r++)
+      if (usedP[r])
+        for (int j = 0; j < Ea; j++)
+          A = std::max(A, std::fabs(b->Pdata[(size_t)r * Ea + j]));
+  }
   b->device_route = o->algorithm == FEDM_SIMPLEX && !generic && emax > 0 && kcap > 0 && kmin >= 1 && A < 1e30;
   if (!b->device_route) {
     *out = b;
@@ -827,6 +839,11 @@ int edm_batch_fetch(edm_batch* b, double* pred, int* rc, int* k_used)
   HIP_TRY(hipMemcpyAsync(k_used, b->B.k_used, sizeof(int) * b->total_k, hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
   return 0;
+}
+
+int edm_batch_on_device(const edm_batch* b)
+{
+  return b->device_route ? 1 : 0;
 }
 
 void edm_batch_destroy(edm_batch* b)

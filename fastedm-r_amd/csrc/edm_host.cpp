@@ -1002,6 +1002,14 @@ int64_t fedm_plan_num_predictions(const fedm_plan* P)
   return n;
 }
 
+int fedm_plan_device_batches(const fedm_plan* P)
+{
+  int n = 0;
+  for (const auto& g : P->groups)
+    n += g.batch != nullptr && edm_batch_on_device(g.batch);
+  return n;
+}
+
 void fedm_plan_destroy(fedm_plan* P)
 {
   delete P;

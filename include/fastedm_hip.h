@@ -133,6 +133,9 @@ int edm_batch_create(edm_ctx* ctx, const edm_manifold_view* L, const edm_manifol
 int edm_batch_run(edm_batch* b);
 int edm_batch_fetch(edm_batch* b, double* pred, int* rc, int* k_used);
 void edm_batch_destroy(edm_batch* b);
+/* 1 when the batch runs as one persistent device launch (simplex, Euclidean, no missing/panel
+ * options, E_actual <= 32, 1 <= k <= 32), 0 when it runs task by task through edm_predict. */
+int edm_batch_on_device(const edm_batch* b);
 
 /* Last stage timings of the most recent call on ctx (milliseconds, HIP events on the ctx stream):
  * [0] distance/selection kernel, [1] merge/predict kernel, [2] S-map kernel, [3] whole device section. */
@@ -185,6 +188,7 @@ int fedm_plan_create(edm_ctx* ctx, const fedm_command* c, fedm_plan** out);
 int fedm_plan_run(fedm_plan* p);
 int fedm_plan_result(fedm_plan* p, fedm_result** out);
 int64_t fedm_plan_num_predictions(const fedm_plan* p);
+int fedm_plan_device_batches(const fedm_plan* p); /* groups that run as one persistent launch */
 void fedm_plan_destroy(fedm_plan* p);
 
 /* fedm_plan_create + fedm_plan_run + fedm_plan_result. */

@@ -71,6 +71,9 @@ def lib():
         L.orc_mae.restype = C.c_double
         L.orc_emd.restype = C.c_double
         L.orc_mt64_stream.argtypes = [C.c_uint64, C.c_int, _dp]
+        L.orc_manifold.argtypes = [C.POINTER(Command), C.c_int, _ip, C.c_int, C.c_int, _dp, _dp, C.c_int]
+        L.orc_split_masks.argtypes = [C.c_int, _ip, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                      C.c_int, _ip, _ip, _ip, C.c_int]
         _lib = L
     return _lib
 

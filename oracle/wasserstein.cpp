@@ -59,8 +59,10 @@ double exact_emd(const double* C, int li, int lj)
           continue;
         for (int e = 0; e < (int)g[u].size(); e++) {
           const Edge& ed = g[u][e];
-          if (ed.cap > 0 && dist[u] + ed.cost < dist[ed.to] - 1e-15 * (1.0 + std::fabs(dist[ed.to]))) {
-            dist[ed.to] = dist[u] + ed.cost;
+          const double cand = dist[u] + ed.cost;
+          const double cur = dist[ed.to];
+          if (ed.cap > 0 && (std::isinf(cur) || cand < cur - 1e-15 * (1.0 + std::fabs(cur)))) {
+            dist[ed.to] = cand;
             pv[ed.to] = u;
             pe[ed.to] = e;
             upd = true;

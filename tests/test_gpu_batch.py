@@ -58,6 +58,7 @@ def test_ccm_command_plan_matches_oracle(precision):
     t = np.arange(1, 801, dtype=float)
     kw = dict(E=3, library=[10, 25, 60, 150, 400, 790], numReps=4, savePredictions=True)
     plan = fe.Plan(t, Y, X, precision=precision, **kw)
+    assert plan.device_batches == 1
     plan.run()
     plan.run()
     a = plan.result()

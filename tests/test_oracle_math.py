@@ -1,0 +1,256 @@
+"""Pins the CPU oracle where the reference's golden rho values leave it open: closed forms, brute
+force on tiny inputs, library routines (numpy/scipy) and the reference's own RNG stream.  Each check
+is chosen so that a plausible slip (a dropped term, a wrong index, a transposed operand, a missing
+sqrt) fails it."""
+import json
+import os
+
+import numpy as np
+import pytest
+import scipy.optimize
+
+from oracle import oracle
+from tests.conftest import GOLDEN
+
+MISSING = 1.0e100
+
+
+def test_mt64_matches_reference_stream():
+    """MtRng64::getReal2 (src/vendor/mersennetwister.h) compiled from the reference: tempered seed first,
+    then the MT19937-64 stream."""
+    ref = json.load(open(os.path.join(GOLDEN, "mt64_streams.json")))
+    for seed, vals in ref.items():
+        got = oracle.mt64_stream(int(seed), len(vals))
+        np.testing.assert_array_equal(got, np.asarray(vals))
+
+
+def brute_lp(M, Mp, i, mae=False, missing_distance=0.0, idw=0.0, lp=None, pp=None):
+    out = []
+    for j, x in enumerate(M):
+        acc = 0.0
+        if idw > 0:
+            acc += idw * (lp[j] != pp[i])
+        for a, b in zip(x, Mp[i]):
+            if a == MISSING or b == MISSING:
+                if missing_distance == 0:
+                    acc = None
+                    break
+                d = missing_distance
+            else:
+                d = a - b
+            acc += abs(d) / len(x) if mae else d * d
+        if acc is None or acc == 0:
+            continue
+        out.append((acc if mae else np.sqrt(acc), j))
+    return out
+
+
+@pytest.mark.parametrize("mae", [False, True])
+def test_knn_brute_force(mae):
+    rng = np.random.default_rng(3)
+    M = rng.integers(0, 3, (40, 2)).astype(float)  # many exact ties
+    Mp = rng.integers(0, 3, (15, 2)).astype(float)
+    k = 5
+    opts = {"algorithm": 0, "k": k, "thetas": [0.0], "distance": 0 if mae else 1}
+    pred, rc, _, ku, nn = oracle.predict({"data": M, "targets": np.arange(40.0)}, {"data": Mp}, opts, nn_cap=k)
+    for i in range(15):
+        cand = brute_lp(M, Mp, i, mae=mae)
+        order = sorted(cand)[:k]  # (distance, position) ascending
+        assert list(nn[i][:len(order)]) == [j for _, j in order]
+        # theta = 0: plain mean of the neighbours' targets (src/edm.cpp:721-730)
+        assert pred[0, i] == pytest.approx(np.mean([j for _, j in order]), rel=1e-14)
+
+
+def test_idw_penalty_sits_inside_the_sqrt():
+    M = np.array([[0.0], [3.0]])
+    Mp = np.array([[0.0]])
+    opts = {"algorithm": 0, "k": 2, "thetas": [1.0], "panel_mode": 1, "idw": 16.0}
+    Md = {"data": M, "targets": np.array([1.0, 2.0]), "panel": np.array([1, 0], np.int32)}
+    Mpd = {"data": Mp, "panel": np.array([0], np.int32)}
+    pred, rc, _, ku, nn = oracle.predict(Md, Mpd, opts, nn_cap=2)
+    # d0 = sqrt(16 + 0) = 4, d1 = sqrt(0 + 9) = 3: both kept (k == #valid -> library order)
+    w = np.exp(-1.0 * np.array([4.0, 3.0]) / 3.0)
+    assert pred[0, 0] == pytest.approx((1.0 * w[0] + 2.0 * w[1]) / w.sum(), rel=1e-14)
+
+
+def test_missing_coordinates():
+    M = np.array([[0.0, MISSING], [1.0, 1.0], [2.0, 2.0]])
+    Mp = np.array([[0.0, 0.0]])
+    for md, expect in [(0.0, [1, 2]), (0.5, [0, 1, 2])]:
+        opts = {"algorithm": 0, "k": 3, "thetas": [0.0], "has_missing": 1, "missing_distance": md}
+        _, _, _, ku, nn = oracle.predict({"data": M, "targets": np.zeros(3)}, {"data": Mp}, opts, nn_cap=3)
+        assert list(nn[0][:ku[0]]) == expect
+        assert brute_lp(M, Mp, 0, missing_distance=md)[0][1] == expect[0]
+
+
+def test_simplex_k1_is_nearest_target():
+    rng = np.random.default_rng(1)
+    M = rng.random((30, 3))
+    Mp = rng.random((10, 3))
+    t = rng.random(30)
+    pred, *_ = oracle.predict({"data": M, "targets": t}, {"data": Mp}, {"algorithm": 0, "k": 1, "thetas": [2.0]})
+    nearest = [np.argmin(((M - p) ** 2).sum(1)) for p in Mp]
+    np.testing.assert_array_equal(pred[0], t[nearest])
+
+
+def test_smap_theta0_all_is_global_ols():
+    """theta = 0, k = all: every weight is 1, so S-map is ordinary least squares on [1, X]."""
+    rng = np.random.default_rng(7)
+    M = rng.random((60, 3))
+    t = 0.3 + M @ np.array([1.0, -2.0, 0.5]) + 0.01 * rng.standard_normal(60)
+    Mp = rng.random((8, 3))
+    pred, rc, coeffs, ku, _ = oracle.predict({"data": M, "targets": t}, {"data": Mp},
+                                             {"algorithm": 1, "k": -1, "thetas": [0.0], "save_smap_coeffs": 1})
+    X = np.hstack([np.ones((60, 1)), M])
+    beta = np.linalg.lstsq(X, t, rcond=None)[0]
+    np.testing.assert_allclose(pred[0], np.hstack([np.ones((8, 1)), Mp]) @ beta, rtol=1e-10)
+    np.testing.assert_allclose(coeffs[0], beta, rtol=1e-9)
+
+
+def test_smap_rank_deficient_gives_minimum_norm():
+    """A duplicated column makes X^T X singular: the JacobiSVD solve with Eigen's cutoff returns the
+    minimum-norm solution, i.e. pinv(X^T X) X^T y."""
+    rng = np.random.default_rng(11)
+    a = rng.random(40)
+    M = np.stack([a, a, rng.random(40)], 1)
+    t = 1.0 + 2.0 * a + rng.random(40) * 0.1
+    Mp = rng.random((5, 3))
+    pred, rc, coeffs, *_ = oracle.predict({"data": M, "targets": t}, {"data": Mp},
+                                          {"algorithm": 1, "k": -1, "thetas": [0.0], "save_smap_coeffs": 1})
+    X = np.hstack([np.ones((40, 1)), M])
+    A = X.T @ X
+    beta = np.linalg.pinv(A, rcond=4 * np.finfo(float).eps) @ (X.T @ t)
+    np.testing.assert_allclose(coeffs[0], beta, rtol=1e-7, atol=1e-9)
+    assert coeffs[0][1] == pytest.approx(coeffs[0][2], rel=1e-9)  # the two equal columns share weight
+
+
+def test_jacobi_svd_singular_values_and_solve():
+    rng = np.random.default_rng(5)
+    for n in [2, 5, 17, 41]:
+        X = rng.standard_normal((n + 3, n))
+        A = X.T @ X
+        b = rng.standard_normal(n)
+        x, sv, rank = oracle.jacobi_solve(A, b)
+        np.testing.assert_allclose(sv, np.linalg.svd(A, compute_uv=False), rtol=1e-10)
+        assert rank == n
+        np.testing.assert_allclose(A @ x, b, rtol=1e-8, atol=1e-10)
+    # cutoff: singular values below s0 * n * eps are dropped
+    A = np.diag([1.0, 1e-20, 0.5])
+    x, sv, rank = oracle.jacobi_solve(A, np.array([1.0, 1.0, 1.0]))
+    assert rank == 2
+    np.testing.assert_allclose(x, [1.0, 0.0, 2.0])
+
+
+def test_correlation_and_mae_against_numpy():
+    rng = np.random.default_rng(2)
+    a, b = rng.random(300), rng.random(300)
+    assert oracle.lib().orc_correlation(300, oracle.dptr(a), oracle.dptr(b)) == pytest.approx(
+        np.corrcoef(a, b)[0, 1], rel=1e-12)
+    assert oracle.lib().orc_mae(300, oracle.dptr(a), oracle.dptr(b)) == pytest.approx(np.mean(np.abs(a - b)),
+                                                                                    rel=1e-12)
+    c = np.full(10, 0.5)
+    assert oracle.lib().orc_correlation(10, oracle.dptr(c), oracle.dptr(a[:10].copy())) == MISSING
+
+
+def test_emd_against_assignment_and_linprog():
+    rng = np.random.default_rng(9)
+    for n in [2, 3, 5]:
+        C = rng.random((n, n))
+        r, c = scipy.optimize.linear_sum_assignment(C)
+        assert oracle.emd(C) == pytest.approx(C[r, c].sum() / n, rel=1e-12)
+    for li, lj in [(2, 3), (4, 3), (1, 5)]:
+        C = rng.random((li, lj))
+        A_eq = []
+        for i in range(li):
+            row = np.zeros(li * lj)
+            row[i * lj:(i + 1) * lj] = 1
+            A_eq.append(row)
+        for j in range(lj):
+            row = np.zeros(li * lj)
+            row[j::lj] = 1
+            A_eq.append(row)
+        b_eq = [1 / li] * li + [1 / lj] * lj
+        lp = scipy.optimize.linprog(C.ravel(), A_eq=np.array(A_eq), b_eq=b_eq, bounds=(0, None))
+        assert oracle.emd(C) == pytest.approx(lp.fun, rel=1e-9)
+
+
+def command(t, x, **kw):
+    c, keep, _ = oracle.make_command(t, x, **kw)
+    return c, keep
+
+
+def test_manifold_lags_and_targets_by_hand():
+    t = np.arange(1.0, 11.0)
+    x = 10.0 * np.arange(1.0, 11.0)
+    c, keep = command(t, x, E=3, p=1)
+    n = 10
+    f = np.ones(n, np.int32)
+    out = np.zeros(n * 3)
+    tg = np.zeros(n)
+    m = oracle.lib().orc_manifold(oracle.C.byref(c), 3, oracle.iptr(f), 1, 0, oracle.dptr(out), oracle.dptr(tg), n)
+    pts = out.reshape(n, 3)
+    assert m == 10
+    np.testing.assert_array_equal(pts[5], [60.0, 50.0, 40.0])  # lag j of row i is row i - j
+    assert pts[0][1] == MISSING and pts[1][2] == MISSING
+    np.testing.assert_array_equal(tg[:9], x[1:])  # target p = 1 step ahead
+    assert tg[9] == MISSING
+
+
+def test_manifold_panel_boundary_and_gap():
+    t = np.array([1.0, 2, 3, 4, 1, 2, 3, 5])
+    x = np.arange(1.0, 9.0)
+    panel = np.array([0, 0, 0, 0, 1, 1, 1, 1])
+    c, keep = command(t, x, E=2, panel=panel, p=1)
+    f = np.ones(8, np.int32)
+    out = np.zeros(16)
+    tg = np.zeros(8)
+    oracle.lib().orc_manifold(oracle.C.byref(c), 2, oracle.iptr(f), 1, 0, oracle.dptr(out), oracle.dptr(tg), 8)
+    pts = out.reshape(8, 2)
+    assert pts[4][1] == MISSING  # first row of panel 1 has no lag across the boundary
+    assert pts[7][1] == MISSING  # t=5 has no t=4 observation in its panel
+    assert tg[3] == MISSING and tg[6] == MISSING  # no target across the boundary or the gap
+
+
+def test_dt_columns_constant_on_regular_grid():
+    t = np.arange(1.0, 21.0) * 2.0
+    x = np.sin(np.arange(20.0)) + 2.0
+    c, keep = command(t, x, E=3, dt=True, dtWeight=1.0, p=1)
+    f = np.ones(20, np.int32)
+    out = np.zeros(20 * 6)
+    tg = np.zeros(20)
+    oracle.lib().orc_manifold(oracle.C.byref(c), 3, oracle.iptr(f), 1, 0, oracle.dptr(out), oracle.dptr(tg), 20)
+    pts = out.reshape(20, 6)
+    np.testing.assert_array_equal(pts[10][3:], [2.0, 2.0, 2.0])  # t_target - t_now, then t_prev - t_lag
+
+
+def test_fixed_size_library_draws_exactly_library_rows():
+    n = 300
+    usable = np.ones(n, np.int32)
+    usable[:5] = 0
+    libs = np.array([10, 50, 290], np.int32)
+    lib_out = np.zeros(3 * 2 * n, np.int32)
+    pred_out = np.zeros(3 * 2 * n, np.int32)
+    ntask = oracle.lib().orc_split_masks(n, oracle.iptr(usable), 0, 0, 1, 0, 0, 2, 1, 3, oracle.iptr(libs),
+                                         oracle.iptr(lib_out), oracle.iptr(pred_out), 6)
+    assert ntask == 6
+    lib_out = lib_out.reshape(6, n)
+    pred_out = pred_out.reshape(6, n)
+    assert list(lib_out.sum(1)) == [10, 50, 290, 10, 50, 290]
+    assert (pred_out == usable).all()
+    assert not lib_out[:, :5].any()
+    # the draw is the library smallest of the uniforms (src/library_prediction_split.h:236-263)
+    u = oracle.mt64_stream(0, 295)
+    np.testing.assert_array_equal(np.nonzero(lib_out[0][5:])[0], np.sort(np.argsort(u)[:10]))
+
+
+def test_crossfold_groups_partition_usable_rows():
+    n = 103
+    usable = np.ones(n, np.int32)
+    lib_out = np.zeros(5 * n, np.int32)
+    pred_out = np.zeros(5 * n, np.int32)
+    oracle.lib().orc_split_masks(n, oracle.iptr(usable), 1, 0, 0, 5, 0, 1, 1, 0, None, oracle.iptr(lib_out),
+                                 oracle.iptr(pred_out), 5)
+    pred_out = pred_out.reshape(5, n)
+    lib_out = lib_out.reshape(5, n)
+    assert (pred_out.sum(0) == 1).all()  # every usable row is predicted exactly once
+    assert ((lib_out + pred_out) == 1).all()
