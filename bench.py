@@ -27,6 +27,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "simplex/S-map predictions/sec, batched-mixed vs plain GPU, at max|d yhat|<=1e-6"
 FP32_PEAK_TFLOPS = 157.3
+FP64_PEAK_TFLOPS = 78.6
 
 
 def load_product():
@@ -67,15 +68,51 @@ def pair_count(libs, reps, n_pred=19998):
 
 
 def timed_steps(plan, fe, steps, warmup):
+    """Wall time per step and the mean HIP-event stage times [batch, rank build, rank scan, direct] (ms)."""
     for _ in range(warmup):
         plan.run()
-    kernel_ms = []
+    stages = []
     t0 = time.perf_counter()
     for _ in range(steps):
         plan.run()
-        kernel_ms.append(fe.timings()[0])
+        stages.append(fe.timings())
     wall = time.perf_counter() - t0
-    return wall / steps, float(np.mean(kernel_ms))
+    return wall / steps, np.mean(np.asarray(stages), axis=0)
+
+
+HBM_PEAK_GBS = 8000.0
+N_TARGETS = 19998
+K = 4
+
+
+def roofline(st, libs, reps):
+    """Roofline of the stage that dominates the device time, with that stage's algorithmic work.
+
+    direct     (batch_simplex): 3 E_a = 9 flops per library-target pair scanned (FP64 for short libraries);
+    rank build (rank_build):    9 flops per pool-target pair and histogram/filter pass (2 passes);
+    rank scan  (rank_scan):     HBM bytes of the ranked prefixes a task reads: the expected scan length
+                                k * pool / L entries of 4 B plus k entries of 8 B per (task, target)."""
+    pool = N_TARGETS
+    lthr = max(2.0 * np.sqrt(K * pool), 8.0 * (K + 8))
+    direct_libs = libs[libs < lthr]
+    rank_libs = libs[libs >= lthr]
+    work = {
+        # libraries below the rank threshold are short: the direct kernel runs its FP64 variant
+        "direct": ("mfma", 9.0 * reps * float(np.sum(direct_libs)) * N_TARGETS, "TFLOP/s", FP64_PEAK_TFLOPS,
+                   "batch_simplex<4,4,false>"),
+        "rank_build": ("mfma", 2 * 9.0 * pool * N_TARGETS, "TFLOP/s", FP32_PEAK_TFLOPS, "rank_build<4>"),
+        "rank_scan": ("hbm", reps * float(np.sum(K * pool / rank_libs * 4.0 + K * 8.0)) * N_TARGETS, "GB/s",
+                      HBM_PEAK_GBS, "rank_scan<4,4>"),
+    }
+    times = {"rank_build": st[1], "rank_scan": st[2], "direct": st[3]}
+    name = max(times, key=lambda n: times[n])
+    bound, amount, unit, peak, kernel = work[name]
+    ms = float(times[name])
+    achieved = amount / (ms * 1e-3) / (1e12 if unit == "TFLOP/s" else 1e9)
+    return {"bound": bound, "achieved": achieved, "peak": peak, "unit": unit, "frac": achieved / peak,
+            "traffic": None, "kernel": kernel, "kernel_ms": ms,
+            "algorithmic_work_per_launch": amount,
+            "engine": {"direct": "fp64 VALU", "rank_build": "fp32 VALU", "rank_scan": "HBM3E"}[name]}
 
 
 def cpu_baseline(t, x, y, kw, seconds):
@@ -122,8 +159,10 @@ def main():
     if mixed.device_batches != 1:
         raise SystemExit("the CCM sweep did not map to one persistent device launch")
     n_pred = mixed.num_predictions
-    sec, kms = timed_steps(mixed, fe, args.steps, args.warmup)
-    print(f"mixed: {sec * 1e3:.1f} ms/step, kernel {kms:.1f} ms", file=sys.stderr, flush=True)
+    sec, st = timed_steps(mixed, fe, args.steps, args.warmup)
+    kms = float(st[0])
+    print(f"mixed: {sec * 1e3:.1f} ms/step, device {st[0]:.1f} ms (rank build {st[1]:.1f}, rank scan {st[2]:.1f}, "
+          f"direct {st[3]:.1f})", file=sys.stderr, flush=True)
     res_mixed = mixed.result()
 
     out = {"metric": METRIC, "value": n_pred / sec, "unit": "predictions/s", "n_gpus": 1, "steps": args.steps,
@@ -132,20 +171,18 @@ def main():
            "config": {"workload": "ccm_sweep_coupled_logistic_T20000_E3_40lib_10to19000",
                       "replicates": args.reps, "tasks": int(args.reps * len(libs)), "predictions_per_step": n_pred,
                       "library_target_pairs_per_step": pairs, "k": 4, "algorithm": "simplex",
-                      "form": "batched persistent launch, FP32 screening + FP64 re-rank"}}
+                      "form": "batched: rank-table route (FP32-screened exact ranking + bitmask scan) for "
+                              "libraries >= 2 sqrt(k n), direct exact FP64 kernel for shorter ones"}}
 
-    flops = 3 * 3 * pairs
-    achieved = flops / (kms * 1e-3) / 1e12
-    out["roofline"] = {"bound": "mfma", "engine": "fp32 VALU (the FP32 vector and FP32 matrix peaks coincide)",
-                       "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                       "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None,
-                       "kernel": "batch_simplex<4,4,true>", "kernel_ms": kms,
-                       "algorithmic_flops_per_launch": flops}
+    out["stages_ms"] = {"device_total": float(st[0]), "rank_build": float(st[1]), "rank_scan": float(st[2]),
+                        "direct": float(st[3])}
+    out["roofline"] = roofline(st, libs, args.reps)
 
     # plain forms and parity
     print("building the plain plans", file=sys.stderr, flush=True)
     pb = fe.Plan(t, x, y, precision=fe.PLAIN, **kw)
-    sec_pb, kms_pb = timed_steps(pb, fe, 1, 0)
+    sec_pb, st_pb = timed_steps(pb, fe, 1, 0)
+    kms_pb = float(st_pb[0])
     res_pb = pb.result()
     out["plain_batched"] = {"value": n_pred / sec_pb, "ms_per_step": sec_pb * 1e3, "kernel_ms": kms_pb}
     d_rho = np.nanmax(np.abs(res_mixed["stats"]["rho"] - res_pb["stats"]["rho"]))

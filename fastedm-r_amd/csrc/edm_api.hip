@@ -4,7 +4,10 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
+#include <cstdlib>
+#include <map>
 #include <cstring>
 #include <vector>
 
@@ -538,6 +541,55 @@ struct BatchParams
 };
 template <int KCAP, int EMAX, bool MIXED>
 __global__ void batch_simplex(BatchParams B);
+
+constexpr int RANK_STORE = 1024;
+constexpr int RANK_ZCAP = 8;
+struct RankParams
+{
+  const double* L;
+  const float* L32;
+  const double* P;
+  const int* pool;
+  int npool;
+  const int* pred;
+  int np, Ea;
+  double c1, c2, c3;
+  int* rank_row;
+  double* rank_s;
+  int* rank_len;
+  int* zrow;
+  int* zlen;
+};
+struct ScanParams
+{
+  const double* L;
+  const double* Ltgt;
+  const double* P;
+  const int* pred;
+  int np, Ea, T, force_compute, mask_words;
+  const unsigned* masks;
+  const int* lib_begin;
+  const int* lib_count;
+  const int* lib_rows;
+  const int* k;
+  const int64_t* out_begin;
+  const int2* items;
+  int num_items;
+  int* counter;
+  const int* rank_row;
+  const double* rank_s;
+  const int* rank_len;
+  const int* zrow;
+  const int* zlen;
+  const double* thetas;
+  double* pred_out;
+  int* rc;
+  int* k_used;
+};
+template <int EMAX>
+__global__ void rank_build(RankParams R);
+template <int KCAP, int EMAX>
+__global__ void rank_scan(ScanParams S);
 } // namespace fedm
 
 struct edm_batch
@@ -549,6 +601,14 @@ struct edm_batch
   std::vector<void*> owned;
   BatchParams B;
   int grid = 0;
+  // rank-table route (one shared prediction list whose tasks draw libraries from one pool)
+  bool rank_route = false;
+  bool direct_mixed = false; // FP32 screening in the direct kernel (pays off only for long scans)
+  RankParams R;
+  ScanParams S;
+  int scan_grid = 0;
+  size_t scan_lds = 0;
+  int rank_tasks = 0;
   float last_ms = 0.f;
   // host copies for the task-by-task route
   edm_manifold_view Lv, Pv;
@@ -601,6 +661,35 @@ static const void* batch_kernel(int kcap, int emax, bool mixed)
     case 8: return batch_kernel_k<8>(emax, mixed);
     case 16: return batch_kernel_k<16>(emax, mixed);
     default: return batch_kernel_k<32>(emax, mixed);
+  }
+}
+
+static const void* rank_build_kernel(int emax)
+{
+  switch (emax) {
+    case 4: return (const void*)rank_build<4>;
+    case 8: return (const void*)rank_build<8>;
+    case 16: return (const void*)rank_build<16>;
+    default: return (const void*)rank_build<32>;
+  }
+}
+template <int KCAP>
+static const void* rank_scan_k(int emax)
+{
+  switch (emax) {
+    case 4: return (const void*)rank_scan<KCAP, 4>;
+    case 8: return (const void*)rank_scan<KCAP, 8>;
+    case 16: return (const void*)rank_scan<KCAP, 16>;
+    default: return (const void*)rank_scan<KCAP, 32>;
+  }
+}
+static const void* rank_scan_kernel(int kcap, int emax)
+{
+  switch (kcap) {
+    case 4: return rank_scan_k<4>(emax);
+    case 8: return rank_scan_k<8>(emax);
+    case 16: return rank_scan_k<16>(emax);
+    default: return rank_scan_k<32>(emax);
   }
 }
 
@@ -687,11 +776,13 @@ int edm_batch_create(edm_ctx* c, const edm_manifold_view* L, const edm_manifold_
   b->kcap = kcap;
   b->emax = emax;
   b->mixed = o->precision == FEDM_MIXED;
+  b->direct_mixed = b->mixed;
 
   BatchParams& B = b->B;
   std::memset(&B, 0, sizeof(B));
   B.L = b->dev(b->Ldata.data(), b->Ldata.size());
   if (b->mixed) {
+    // FP32 copy for screening; rows outside every task hold MISSING and saturate to infinity
     std::vector<float> l32(b->Ldata.begin(), b->Ldata.end());
     B.L32 = b->dev(l32.data(), l32.size());
     hipStreamSynchronize(c->stream);
@@ -706,17 +797,61 @@ int edm_batch_create(edm_ctx* c, const edm_manifold_view* L, const edm_manifold_
   B.pred_rows = b->dev(pred_rows, pred_total);
   B.k = b->dev(k, num_tasks);
   B.out_begin = b->dev(out_begin, num_tasks);
-  // work queue: (task, first target) items, heaviest library first
+  // Rank-table route: the largest set of tasks sharing one prediction list, when their libraries are
+  // large enough that scanning a ranked prefix beats scanning the library (k * pool / L < L).
+  std::vector<char> is_rank(num_tasks, 0);
+  int rank_pb = -1, rank_pc = 0;
+  std::vector<int> pool;
+  {
+    std::map<std::pair<int, int>, std::vector<int>> by_list;
+    for (int t = 0; t < num_tasks; t++)
+      by_list[{ pred_begin[t], pred_count[t] }].push_back(t);
+    size_t best = 0;
+    std::pair<int, int> key{ -1, 0 };
+    for (auto& kv : by_list)
+      if (kv.second.size() > best) {
+        best = kv.second.size();
+        key = kv.first;
+      }
+    const auto& members = by_list[key];
+    std::vector<char> inpool(L->num_points, 0);
+    for (int t : members)
+      for (int i = 0; i < lib_count[t]; i++)
+        inpool[lib_rows[lib_begin[t] + i]] = 1;
+    for (int r = 0; r < L->num_points; r++)
+      if (inpool[r])
+        pool.push_back(r);
+    const double npool = (double)pool.size();
+    const int lthr = (int)std::max(2.0 * std::sqrt((double)std::max(kmax, 1) * npool), 8.0 * (kmax + RANK_ZCAP));
+    double work = 0;
+    int nrank = 0;
+    for (int t : members)
+      if (lib_count[t] >= lthr) {
+        work += lib_count[t];
+        nrank++;
+      }
+    const char* env = getenv("FEDM_RANK_ROUTE");
+    const bool allowed = !(env && env[0] == '0');
+    if (allowed && nrank > 0 && work >= 4.0 * npool && npool >= 4096 && L->num_points <= (1 << 18) &&
+        key.second > 0) {
+      b->rank_route = true;
+      rank_pb = key.first;
+      rank_pc = key.second;
+      for (int t : members)
+        if (lib_count[t] >= lthr)
+          is_rank[t] = 1;
+      b->rank_tasks = nrank;
+    }
+  }
+  // work queues: (task, first target) items, heaviest library first
   std::vector<int> order(num_tasks);
   for (int t = 0; t < num_tasks; t++)
     order[t] = t;
   std::stable_sort(order.begin(), order.end(), [&](int a, int q) { return lib_count[a] > lib_count[q]; });
-  std::vector<int2> items;
+  std::vector<int2> items, ritems;
   for (int t : order)
     for (int t0 = 0; t0 < pred_count[t]; t0 += 256)
-      items.push_back(make_int2(t, t0));
-  B.items = b->dev(items.data(), items.size());
-  B.num_items = (int)items.size();
+      (is_rank[t] ? ritems : items).push_back(make_int2(t, t0));
   B.counter = b->dev<int>(nullptr, 1);
   B.thetas = b->dev(o->thetas, (size_t)o->num_thetas);
   B.T = o->num_thetas;
@@ -730,6 +865,82 @@ int edm_batch_create(edm_ctx* c, const edm_manifold_view* L, const edm_manifold_
   B.pred = b->dev<double>(nullptr, (size_t)b->total_out);
   B.rc = b->dev<int>(nullptr, (size_t)b->total_out);
   B.k_used = b->dev<int>(nullptr, (size_t)b->total_k);
+  if (b->rank_route) {
+    if (!b->mixed) {
+      // the table build screens in FP32; the plain form keeps its FP64-only route
+      b->rank_route = false;
+      items.insert(items.end(), ritems.begin(), ritems.end());
+      ritems.clear();
+    }
+  }
+  B.items = b->dev(items.data(), items.size());
+  B.num_items = (int)items.size();
+  {
+    // FP32 screening needs long scans: with short libraries nearly every pair reaches the FP64 path
+    int lmax = 0;
+    for (const int2& it : items)
+      lmax = std::max(lmax, lib_count[it.x]);
+    b->direct_mixed = b->mixed && lmax >= 2048;
+  }
+  if (b->rank_route) {
+    RankParams& R = b->R;
+    std::memset(&R, 0, sizeof(R));
+    R.L = B.L;
+    R.L32 = B.L32;
+    R.P = B.P;
+    R.pool = b->dev(pool.data(), pool.size());
+    R.npool = (int)pool.size();
+    R.pred = B.pred_rows + rank_pb;
+    R.np = rank_pc;
+    R.Ea = Ea;
+    R.c1 = B.c1;
+    R.c2 = B.c2;
+    R.c3 = B.c3;
+    R.rank_row = b->dev<int>(nullptr, (size_t)RANK_STORE * rank_pc);
+    R.rank_s = b->dev<double>(nullptr, (size_t)RANK_STORE * rank_pc);
+    R.rank_len = b->dev<int>(nullptr, (size_t)rank_pc);
+    R.zrow = b->dev<int>(nullptr, (size_t)RANK_ZCAP * rank_pc);
+    R.zlen = b->dev<int>(nullptr, (size_t)rank_pc);
+    ScanParams& S = b->S;
+    std::memset(&S, 0, sizeof(S));
+    S.L = B.L;
+    S.Ltgt = B.Ltgt;
+    S.P = B.P;
+    S.pred = R.pred;
+    S.np = rank_pc;
+    S.Ea = Ea;
+    S.T = B.T;
+    S.force_compute = B.force_compute;
+    S.mask_words = (L->num_points + 31) / 32;
+    // library membership bitmasks of the rank-route tasks, indexed by task
+    std::vector<unsigned> masks((size_t)num_tasks * S.mask_words, 0u);
+    for (int t = 0; t < num_tasks; t++)
+      if (is_rank[t])
+        for (int i = 0; i < lib_count[t]; i++) {
+          const int r = lib_rows[lib_begin[t] + i];
+          masks[(size_t)t * S.mask_words + (r >> 5)] |= 1u << (r & 31);
+        }
+    S.masks = b->dev(masks.data(), masks.size());
+    S.lib_begin = B.lib_begin;
+    S.lib_count = B.lib_count;
+    S.lib_rows = B.lib_rows;
+    S.k = B.k;
+    S.out_begin = B.out_begin;
+    S.items = b->dev(ritems.data(), ritems.size());
+    S.num_items = (int)ritems.size();
+    S.counter = b->dev<int>(nullptr, 1);
+    S.rank_row = R.rank_row;
+    S.rank_s = R.rank_s;
+    S.rank_len = R.rank_len;
+    S.zrow = R.zrow;
+    S.zlen = R.zlen;
+    S.thetas = B.thetas;
+    S.pred_out = B.pred;
+    S.rc = B.rc;
+    S.k_used = B.k_used;
+    b->scan_lds = sizeof(unsigned) * (size_t)S.mask_words;
+    hipStreamSynchronize(c->stream);
+  }
   for (void* p : b->owned)
     if (!p) {
       delete b;
@@ -738,8 +949,13 @@ int edm_batch_create(edm_ctx* c, const edm_manifold_view* L, const edm_manifold_
   int dev = 0, ncu = 0, per = 0;
   hipGetDevice(&dev);
   hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
-  hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, batch_kernel(kcap, emax, b->mixed), 256, 0);
+  hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, batch_kernel(kcap, emax, b->direct_mixed), 256, 0);
   b->grid = std::max(1, std::min(B.num_items, ncu * std::max(per, 1)));
+  if (b->rank_route) {
+    int sper = 0;
+    hipOccupancyMaxActiveBlocksPerMultiprocessor(&sper, rank_scan_kernel(kcap, emax), 256, b->scan_lds);
+    b->scan_grid = std::max(1, std::min(b->S.num_items, ncu * std::max(sper, 1)));
+  }
   if (hipStreamSynchronize(c->stream) != hipSuccess) {
     delete b;
     return -3;
@@ -811,16 +1027,31 @@ int edm_batch_run(edm_batch* b)
   dfill(c, B.rc, (size_t)b->total_out, (int)FEDM_UNKNOWN_ERROR);
   dfill(c, B.k_used, (size_t)b->total_k, (int)FEDM_MISSING_I);
   HIP_TRY(hipEventRecord(c->ev[4], s));
-  const void* fn = batch_kernel(b->kcap, b->emax, b->mixed);
+  if (b->rank_route) {
+    dfill(c, b->S.counter, 1, 0);
+    void* rargs[] = { &b->R };
+    HIP_TRY(hipLaunchKernel(rank_build_kernel(b->emax), dim3(b->R.np), dim3(256), rargs, 0, s));
+    HIP_TRY(hipEventRecord(c->ev[6], s));
+    void* sargs[] = { &b->S };
+    if (b->S.num_items > 0)
+      HIP_TRY(hipLaunchKernel(rank_scan_kernel(b->kcap, b->emax), dim3(b->scan_grid), dim3(256), sargs,
+                              b->scan_lds, s));
+  } else {
+    HIP_TRY(hipEventRecord(c->ev[6], s));
+  }
+  HIP_TRY(hipEventRecord(c->ev[7], s));
+  const void* fn = batch_kernel(b->kcap, b->emax, b->direct_mixed);
   void* args[] = { &B };
-  HIP_TRY(hipLaunchKernel(fn, dim3(b->grid), dim3(256), args, 0, s));
+  if (B.num_items > 0)
+    HIP_TRY(hipLaunchKernel(fn, dim3(b->grid), dim3(256), args, 0, s));
   HIP_TRY(hipEventRecord(c->ev[5], s));
   HIP_TRY(hipEventSynchronize(c->ev[5]));
   b->last_ms = ev_ms(c->ev[4], c->ev[5]);
+  // [0] whole batch, [1] rank-table build, [2] rank scan, [3] direct kernel
   c->timings[0] = b->last_ms;
-  c->timings[1] = 0;
-  c->timings[2] = 0;
-  c->timings[3] = b->last_ms;
+  c->timings[1] = ev_ms(c->ev[4], c->ev[6]);
+  c->timings[2] = ev_ms(c->ev[6], c->ev[7]);
+  c->timings[3] = ev_ms(c->ev[7], c->ev[5]);
   return 0;
 }
 

@@ -27,15 +27,17 @@ def ccm_like(seed, n, Ea, sizes, reps, integer=False):
 
 
 @pytest.mark.parametrize("precision", [0, 1], ids=["plain", "mixed"])
-@pytest.mark.parametrize("case", ["E3_k4", "E5_k6_integer", "E8_k9_thetas", "tiny_libraries"])
+@pytest.mark.parametrize("case", ["E3_k4", "E5_k6_integer", "E8_k9_thetas", "tiny_libraries", "E3_k4_long"])
 def test_batch_matches_oracle(precision, case):
     fe = load_product()
     spec = {"E3_k4": (3, 4, [10, 37, 150, 900], 3, False, (1.0,)),
             "E5_k6_integer": (5, 6, [12, 80, 400], 2, True, (1.0,)),
             "E8_k9_thetas": (8, 9, [20, 300, 1200], 2, False, (0.0, 1.0, 4.0)),
-            "tiny_libraries": (2, 3, [1, 2, 3, 4, 6], 2, False, (1.0,))}[case]
+            "tiny_libraries": (2, 3, [1, 2, 3, 4, 6], 2, False, (1.0,)),
+            "E3_k4_long": (3, 4, [2100, 2900], 2, False, (1.0,))}[case]  # FP32-screened direct kernel
     Ea, k, sizes, reps, integer, thetas = spec
-    L, libs, preds = ccm_like(zlib.crc32(case.encode()) % 997, 1500, Ea, sizes, reps, integer)
+    n = 3000 if case == "E3_k4_long" else 1500
+    L, libs, preds = ccm_like(zlib.crc32(case.encode()) % 997, n, Ea, sizes, reps, integer)
     opts = {"algorithm": 0, "k": k, "thetas": np.asarray(thetas), "precision": precision}
     print(f"case {case} precision {precision}: {len(libs)} tasks", flush=True)
     got = fe.predict_batch(L, L, opts, libs, preds, [k] * len(libs))
@@ -71,3 +73,42 @@ def test_ccm_command_plan_matches_oracle(precision):
     assert np.array_equal(np.isnan(x), np.isnan(y))
     ok = ~np.isnan(y)
     assert np.all(np.abs(x[ok] - y[ok]) <= 1e-12 + 1e-10 * np.abs(y[ok]))
+
+
+@pytest.mark.parametrize("case", ["continuous", "integer_ties_duplicates"])
+def test_rank_route_matches_oracle(case):
+    """Large libraries sharing one prediction list take the rank-table route (mixed precision)."""
+    fe = load_product()
+    integer = case != "continuous"
+    n, Ea = (6000, 3) if not integer else (5000, 2)
+    L, libs, preds = ccm_like(zlib.crc32(case.encode()) % 991, n, Ea, [40, 700, 2600, n - 100], 4, integer)
+    if integer:
+        L["data"] = np.floor(L["data"] * 2.0)  # values 0..9: many duplicate points and exact ties
+    opts = {"algorithm": 0, "k": 4, "thetas": np.asarray([1.0, 3.0]), "precision": 1}
+    print(f"case {case}: {len(libs)} tasks over {n} rows", flush=True)
+    got = fe.predict_batch(L, L, opts, libs, preds, [4] * len(libs))
+    assert fe.timings()[1] > 0.0  # the rank table was built on this call
+    for t, (lib, pr) in enumerate(zip(libs, preds)):
+        M = {"data": L["data"][lib], "targets": L["targets"][lib]}
+        Mp = {"data": L["data"][pr]}
+        rp, rr, _, rk, _ = oracle.predict(M, Mp, dict(opts, k=4, nthreads=16))
+        gp, gr, gk = got[t]
+        np.testing.assert_array_equal(gr, rr)
+        np.testing.assert_array_equal(gk, rk)
+        assert_predictions_close(gp, rp)
+
+
+def test_ccm_plan_rank_route_matches_oracle():
+    fe = load_product()
+    from tests.scenarios import logistic_map
+    X, Y = logistic_map(6000)
+    t = np.arange(1, 6001, dtype=float)
+    kw = dict(E=3, library=[20, 300, 1500, 5900], numReps=3)
+    plan = fe.Plan(t, Y, X, precision=1, **kw)
+    assert plan.device_batches == 1
+    plan.run()
+    a = plan.result()
+    b = oracle.edm(t, Y, X, numThreads=16, **kw)
+    assert a["rc"] == b["rc"] and (a["kMin"], a["kMax"]) == (b["kMin"], b["kMax"])
+    np.testing.assert_array_equal(a["stats"]["rho"], b["stats"]["rho"])
+    np.testing.assert_array_equal(a["stats"]["mae"], b["stats"]["mae"])
