@@ -85,6 +85,14 @@ N_TARGETS = 19998
 K = 4
 
 
+def rank_threshold(k, npool):
+    import ctypes
+    fe = sys.modules["fastedm_r_amd"]
+    f = fe.load_library().fedm_rank_threshold
+    f.argtypes = [ctypes.c_int, ctypes.c_int]
+    return f(k, npool)
+
+
 def roofline(st, libs, reps):
     """Roofline of the stage that dominates the device time, with that stage's algorithmic work.
 
@@ -93,7 +101,7 @@ def roofline(st, libs, reps):
     rank scan  (rank_scan):     HBM bytes of the ranked prefixes a task reads: the expected scan length
                                 k * pool / L entries of 4 B plus k entries of 8 B per (task, target)."""
     pool = N_TARGETS
-    lthr = max(2.0 * np.sqrt(K * pool), 8.0 * (K + 8))
+    lthr = rank_threshold(K, pool)
     direct_libs = libs[libs < lthr]
     rank_libs = libs[libs >= lthr]
     work = {

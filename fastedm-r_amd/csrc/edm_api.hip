@@ -664,6 +664,21 @@ static const void* batch_kernel(int kcap, int emax, bool mixed)
   }
 }
 
+// Smallest library that takes the rank route: scanning a ranked prefix costs about k * pool / L
+// entries per target, the direct scan L pairs; FEDM_RANK_LTHR_SCALE overrides the scale (tuning).
+static int rank_threshold(int kmax, int npool)
+{
+  double scale = 1.5;
+  if (const char* e = getenv("FEDM_RANK_LTHR_SCALE"))
+    scale = atof(e);
+  return (int)std::max(scale * std::sqrt((double)std::max(kmax, 1) * npool), 2.0 * (kmax + RANK_ZCAP));
+}
+
+extern "C" int fedm_rank_threshold(int kmax, int npool)
+{
+  return rank_threshold(kmax, npool);
+}
+
 static const void* rank_build_kernel(int emax)
 {
   switch (emax) {
@@ -822,7 +837,7 @@ int edm_batch_create(edm_ctx* c, const edm_manifold_view* L, const edm_manifold_
       if (inpool[r])
         pool.push_back(r);
     const double npool = (double)pool.size();
-    const int lthr = (int)std::max(2.0 * std::sqrt((double)std::max(kmax, 1) * npool), 8.0 * (kmax + RANK_ZCAP));
+    const int lthr = rank_threshold(kmax, (int)pool.size());
     double work = 0;
     int nrank = 0;
     for (int t : members)

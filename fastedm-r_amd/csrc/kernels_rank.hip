@@ -328,10 +328,32 @@ __global__ __launch_bounds__(256) void rank_scan(ScanParams S)
     bool direct = zl < 0 || k >= cnt;
     if (!direct) {
       const int len = S.rank_len[t];
-      for (int j = 0; j < len && found < k; j++) {
-        const int row = S.rank_row[(size_t)j * S.np + t];
+      const int* rr = S.rank_row + t;
+      const size_t st = (size_t)S.np;
+      // four prefix entries per iteration keep several loads in flight
+      int j = 0;
+      for (; j + 4 <= len && found < k; j += 4) {
+        const int r0 = rr[(size_t)j * st], r1 = rr[(size_t)(j + 1) * st], r2 = rr[(size_t)(j + 2) * st],
+                  r3 = rr[(size_t)(j + 3) * st];
+        const int rows4[4] = { r0, r1, r2, r3 };
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+          if (found < k && inlib(rows4[u])) {
+            const double s = S.rank_s[(size_t)(j + u) * st + t];
+#pragma unroll
+            for (int q = 0; q < KCAP; q++)
+              if (q == found) {
+                ls[q] = s;
+                li[q] = rows4[u];
+              }
+            found++;
+          }
+        }
+      }
+      for (; j < len && found < k; j++) {
+        const int row = rr[(size_t)j * st];
         if (inlib(row)) {
-          const double s = S.rank_s[(size_t)j * S.np + t];
+          const double s = S.rank_s[(size_t)j * st + t];
 #pragma unroll
           for (int q = 0; q < KCAP; q++)
             if (q == found) {

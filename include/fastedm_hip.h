@@ -137,6 +137,10 @@ void edm_batch_destroy(edm_batch* b);
  * options, E_actual <= 32, 1 <= k <= 32), 0 when it runs task by task through edm_predict. */
 int edm_batch_on_device(const edm_batch* b);
 
+/* Smallest library size that a batch sends to the rank-table route for k neighbours over a pool of
+ * npool rows (scale from FEDM_RANK_LTHR_SCALE, default 1.5 * sqrt(k * npool)). */
+int fedm_rank_threshold(int kmax, int npool);
+
 /* Last stage timings of the most recent call on ctx (milliseconds, HIP events on the ctx stream):
  * [0] distance/selection kernel, [1] merge/predict kernel, [2] S-map kernel, [3] whole device section. */
 void edm_ctx_timings(edm_ctx* ctx, double* ms4);

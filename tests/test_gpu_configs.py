@@ -77,6 +77,7 @@ def test_cfg4_panels_smap_dt():
     t0 = time.perf_counter()
     a = fe.edm(t, x, **kw)
     gpu_s = time.perf_counter() - t0
+    stages = fe.timings()
     assert a["rc"] == 0
     # manifolds against the oracle's, whole and bit for bit
     c, keep, _ = oracle.make_command(t, x, panel=panel, E=8, dt=True, theta=[0, 1, 3], algorithm="smap")
@@ -110,7 +111,8 @@ def test_cfg4_panels_smap_dt():
     rp, rr, _, rk, _ = oracle.predict(Md, Mpd, opts)
     assert (rk == 18).all()
     d = close(a["predictions"][pred_rows[pick], 0], rp[2])
-    print(f"cfg4: {len(lib_rows)} library x {len(pred_rows)} targets, E_a=16, k=18, 3 thetas: GPU {gpu_s:.2f} s; "
+    print(f"cfg4: {len(lib_rows)} library x {len(pred_rows)} targets, E_a=16, k=18, 3 thetas: GPU {gpu_s:.2f} s "
+          f"(device: select {stages[0]:.1f} ms, merge {stages[1]:.1f} ms, S-map {stages[2]:.1f} ms); "
           f"sampled max |d yhat| {d:.2e}; rho {np.round(a['stats']['rho'], 5)}")
 
 
@@ -122,6 +124,7 @@ def test_cfg5_lorenz96_smap():
     t0 = time.perf_counter()
     gp, gr, gc, gk, gn, status = fe.predict(lib, pred, opts)
     gpu_s = time.perf_counter() - t0
+    stages = fe.timings()
     assert status == 0 and (gr == 0).all() and (gk == 42).all()
     pick = sample_rows(pred["data"].shape[0], 120, 5)
     rp, rr, rc_, rk, _ = oracle.predict(lib, {"data": pred["data"][pick]}, opts)
@@ -129,5 +132,6 @@ def test_cfg5_lorenz96_smap():
     dc = close(np.where(gc[pick] == 1e100, np.nan, gc[pick]), np.where(rc_ == 1e100, np.nan, rc_))
     yt = pred["targets"]
     rho = np.corrcoef(gp[0], yt)[0, 1]
-    print(f"cfg5: {lib['data'].shape[0]} library x {pred['data'].shape[0]} targets, E_a=40, k=42: GPU {gpu_s:.2f} s; "
+    print(f"cfg5: {lib['data'].shape[0]} library x {pred['data'].shape[0]} targets, E_a=40, k=42: GPU {gpu_s:.2f} s "
+          f"(device: select {stages[0]:.1f} ms, merge {stages[1]:.1f} ms, S-map {stages[2]:.1f} ms); "
           f"sampled max |d yhat| {d:.2e}, max |d coeff| {dc:.2e}; rho {rho:.5f}")
