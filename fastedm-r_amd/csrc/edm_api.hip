@@ -27,6 +27,23 @@ template <int NMAX>
 __global__ void smap_f64(SmapParams S, const double* D, int t0, int nt);
 size_t smap_lds_bytes(int n);
 
+struct MfmaParams
+{
+  const float* M32;
+  const float* Mn;
+  const float* P32;
+  const float* Pn;
+  const double* M;
+  const double* Mp;
+  const double* c1t;
+  const double* c3t;
+  double c2;
+  int NL, NP, Ea;
+};
+template <int KCAP, int KP>
+__global__ void knn_mfma(MfmaParams P, int k, int slice_len, double* out_s, int* out_i, int* out_cnt);
+__global__ void mfma_layout_probe(float* out);
+
 
 template <typename T>
 __global__ void fill_kernel(T* p, size_t n, T v)
@@ -185,6 +202,30 @@ static void launch_knn(int kcap, int emax, bool generic, dim3 g, hipStream_t s, 
     case 16: launch_knn_k<16>(emax, generic, g, s, P, k, sl, ps, pi, pc); break;
     case 32: launch_knn_k<32>(emax, generic, g, s, P, k, sl, ps, pi, pc); break;
     default: launch_knn_k<64>(emax, generic, g, s, P, k, sl, ps, pi, pc); break;
+  }
+}
+
+template <int KCAP>
+static void launch_mfma_k(int kp, dim3 g, hipStream_t s, const MfmaParams& P, int k, int sl, double* ps, int* pi,
+                          int* pc)
+{
+  if (kp == 16)
+    hipLaunchKernelGGL((knn_mfma<KCAP, 16>), g, dim3(256), 0, s, P, k, sl, ps, pi, pc);
+  else if (kp == 32)
+    hipLaunchKernelGGL((knn_mfma<KCAP, 32>), g, dim3(256), 0, s, P, k, sl, ps, pi, pc);
+  else
+    hipLaunchKernelGGL((knn_mfma<KCAP, 64>), g, dim3(256), 0, s, P, k, sl, ps, pi, pc);
+}
+
+static void launch_mfma(int kcap, int kp, dim3 g, hipStream_t s, const MfmaParams& P, int k, int sl, double* ps,
+                        int* pi, int* pc)
+{
+  switch (kcap) {
+    case 4: launch_mfma_k<4>(kp, g, s, P, k, sl, ps, pi, pc); break;
+    case 8: launch_mfma_k<8>(kp, g, s, P, k, sl, ps, pi, pc); break;
+    case 16: launch_mfma_k<16>(kp, g, s, P, k, sl, ps, pi, pc); break;
+    case 32: launch_mfma_k<32>(kp, g, s, P, k, sl, ps, pi, pc); break;
+    default: launch_mfma_k<64>(kp, g, s, P, k, sl, ps, pi, pc); break;
   }
 }
 
@@ -424,7 +465,101 @@ int edm_predict(edm_ctx* c, const edm_manifold_view* M, const edm_manifold_view*
   const bool smap = o->algorithm == FEDM_SMAP;
   HIP_TRY(hipEventRecord(c->ev[0], s));
 
-  if (list_route) {
+  // matrix-core screening pays once a point has a few dozen coordinates
+  double amax = 0.0;
+  const bool want_mfma = list_route && o->precision == FEDM_MIXED && !generic && Ea >= 8;
+  if (want_mfma) {
+    for (size_t i = 0; i < (size_t)NL * Ea; i++)
+      amax = std::max(amax, std::fabs(M->data[i]));
+    for (size_t i = 0; i < (size_t)NP * Ea; i++)
+      amax = std::max(amax, std::fabs(Mp->data[i]));
+  }
+  const bool mfma = want_mfma && amax < 1e30;
+  if (list_route && mfma) {
+    const int kcap = kcap_for(k);
+    const int kp = Ea <= 16 ? 16 : Ea <= 32 ? 32 : 64;
+    const int bx = (NP + 63) / 64;
+    int S = std::max(1, std::min(MAX_SLICES, (2048 + bx - 1) / bx));
+    S = std::max(1, std::min(S, NL / 256));
+    const int sl = (NL + S - 1) / S;
+    // centred FP32 operands, their FP32 norms and the per-target screening bound
+    std::vector<double> ctr(Ea, 0.0);
+    for (int j = 0; j < NL; j++)
+      for (int d = 0; d < Ea; d++)
+        ctr[d] += M->data[(size_t)j * Ea + d];
+    for (int d = 0; d < Ea; d++)
+      ctr[d] /= std::max(NL, 1);
+    std::vector<float> m32((size_t)NL * kp, 0.f), mn(NL), p32((size_t)NP * kp, 0.f), pn(NP);
+    std::vector<double> c1t(NP), c3t(NP);
+    double R = 0.0;
+    for (int j = 0; j < NL; j++) {
+      double r2 = 0.0;
+      float n2 = 0.f;
+      for (int d = 0; d < Ea; d++) {
+        const double v = M->data[(size_t)j * Ea + d] - ctr[d];
+        const float f = (float)v;
+        m32[(size_t)j * kp + d] = f;
+        n2 += f * f;
+        r2 += v * v;
+      }
+      mn[j] = n2;
+      R = std::max(R, std::sqrt(r2));
+    }
+    const double u = 5.9604644775390625e-08;
+    for (int i = 0; i < NP; i++) {
+      double r2 = 0.0;
+      float n2 = 0.f;
+      for (int d = 0; d < Ea; d++) {
+        const double v = Mp->data[(size_t)i * Ea + d] - ctr[d];
+        const float f = (float)v;
+        p32[(size_t)i * kp + d] = f;
+        n2 += f * f;
+        r2 += v * v;
+      }
+      pn[i] = n2;
+      const double rr = (R + std::sqrt(r2)) * (1.0 + 1e-6);
+      c1t[i] = 2.02 * u * rr;
+      c3t[i] = ((kp + 3) * 1.02 * u + u * u) * rr * rr;
+    }
+    MfmaParams MP;
+    MP.M32 = up(c, B_MIX0, m32.data(), m32.size(), &err);
+    MP.Mn = up(c, B_MIX1, mn.data(), mn.size(), &err);
+    MP.P32 = up(c, B_MIX2, p32.data(), p32.size(), &err);
+    MP.Pn = up(c, B_MIX3, pn.data(), pn.size(), &err);
+    MP.c1t = up(c, B_MIX4, c1t.data(), c1t.size(), &err);
+    MP.c3t = up(c, B_MIX5, c3t.data(), c3t.size(), &err);
+    MP.c2 = 4.0 * u;
+    MP.M = P.M;
+    MP.Mp = P.Mp;
+    MP.NL = NL;
+    MP.NP = NP;
+    MP.Ea = Ea;
+    double* ps = (double*)c->get(B_PS, sizeof(double) * (size_t)S * NP * kcap);
+    int* pi = (int*)c->get(B_PI, sizeof(int) * (size_t)S * NP * kcap);
+    int* pc = (int*)c->get(B_PCNT, sizeof(int) * (size_t)S * NP);
+    F.nn_i = (int*)c->get(B_NNI, sizeof(int) * (size_t)NP * kcap);
+    F.nn_s = (double*)c->get(B_NNS, sizeof(double) * (size_t)NP * kcap);
+    F.nn_stride = kcap;
+    if (err || !ps || !pi || !pc || !F.nn_i || !F.nn_s)
+      return err ? err : -2;
+    HIP_TRY(hipEventRecord(c->ev[0], s));
+    launch_mfma(kcap, kp, dim3(bx, S), s, MP, k, sl, ps, pi, pc);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(c->ev[1], s));
+    launch_finish_lists(kcap, dim3((NP + 255) / 256), s, F, ps, pi, pc, S);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(c->ev[2], s));
+    if (smap) {
+      Sm.nn_i = F.nn_i;
+      Sm.nn_s = F.nn_s;
+      Sm.nn_stride = kcap;
+      int r = launch_smap(n, dim3(NP, T), s, Sm, nullptr, 0, NP);
+      if (r)
+        return r;
+      HIP_TRY(hipGetLastError());
+    }
+    HIP_TRY(hipEventRecord(c->ev[3], s));
+  } else if (list_route) {
     const int kcap = kcap_for(k);
     const int bx = (NP + 255) / 256;
     int S = std::max(1, std::min(MAX_SLICES, (1024 + bx - 1) / bx));
@@ -672,6 +807,17 @@ static int rank_threshold(int kmax, int npool)
   if (const char* e = getenv("FEDM_RANK_LTHR_SCALE"))
     scale = atof(e);
   return (int)std::max(scale * std::sqrt((double)std::max(kmax, 1) * npool), 2.0 * (kmax + RANK_ZCAP));
+}
+
+extern "C" int edm_selftest_mfma(float* out256)
+{
+  float* d = nullptr;
+  if (hipMalloc(&d, 256 * sizeof(float)) != hipSuccess)
+    return -2;
+  hipLaunchKernelGGL(mfma_layout_probe, dim3(1), dim3(64), 0, 0, d);
+  hipError_t e = hipMemcpy(out256, d, 256 * sizeof(float), hipMemcpyDeviceToHost);
+  hipFree(d);
+  return e == hipSuccess ? 0 : -3;
 }
 
 extern "C" int fedm_rank_threshold(int kmax, int npool)

@@ -1,0 +1,217 @@
+// Mixed-precision k-NN for one task: FP32 matrix-core distance tiles with an exact FP64 re-rank.
+//
+// Distances are screened with the expansion ||x||^2 + ||y||^2 - 2 x.y on centred FP32 coordinates;
+// the dot products of a 16-target x 16-library tile come from v_mfma_f32_16x16x4f32 (KP/4
+// instructions per tile).  A pair is a candidate when its FP32 distance is within a rigorous bound of
+// the target's current k-th distance (bound: FP32 rounding of the centred inputs, of the norms and dot
+// products, and of the final combination).  Candidates are queued per target in LDS and recomputed in
+// FP64 in the reference's operation order (src/distances.cpp:186-216) by the target's owner lane,
+// which keeps the exact (sqrt distance, position) list (src/edm.cpp:652-681).  The per-slice lists have
+// the layout of knn_slices_f64, so the merge, simplex and S-map stages are shared with the plain route.
+#include "kernels_common.h"
+
+namespace fedm {
+
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+struct MfmaParams
+{
+  const float* M32;   // [NL x KP] centred FP32 library rows, zero padded
+  const float* Mn;    // [NL] FP32 squared norms of those rows
+  const float* P32;   // [NP x KP] centred FP32 targets
+  const float* Pn;    // [NP]
+  const double* M;    // FP64 library [NL x Ea]
+  const double* Mp;   // FP64 targets [NP x Ea]
+  const double* c1t;  // [NP] bound: coefficient of sqrt(D)
+  const double* c3t;  // [NP] bound: constant term
+  double c2;          // bound: coefficient of D
+  int NL, NP, Ea;
+};
+
+__device__ __forceinline__ float up_f32(double x)
+{
+  float f = (float)x;
+  if ((double)f < x)
+    f = __int_as_float(__float_as_int(f) + 1);
+  return f;
+}
+
+constexpr int MCH = 64; // library points per LDS chunk (four 16-point tiles)
+
+template <int KCAP, int KP>
+__global__ __launch_bounds__(256) void knn_mfma(MfmaParams P, int k, int slice_len, double* __restrict__ out_s,
+                                               int* __restrict__ out_i, int* __restrict__ out_cnt)
+{
+  __shared__ float Bs[MCH * KP];
+  __shared__ float Bn[MCH];
+  __shared__ float thr_s[64];
+  __shared__ int qcnt[64];
+  __shared__ unsigned char queue[64 * MCH];
+
+  const int lane = threadIdx.x & 63;
+  const int w = threadIdx.x >> 6;
+  const int g = lane >> 4;  // lane group: supplies K index g of each 4-wide MFMA step
+  const int l16 = lane & 15;
+  const int tbase = blockIdx.x * 64 + w * 16;
+  const int lo = blockIdx.y * slice_len;
+  const int hi = min(P.NL, lo + slice_len);
+
+  // A operand: target row (tbase + l16), coordinates 16q + 4g .. +3
+  v4f a[KP / 16];
+  {
+    const int t = tbase + l16;
+#pragma unroll
+    for (int q = 0; q < KP / 16; q++) {
+      if (t < P.NP)
+        a[q] = *(const v4f*)(P.P32 + (size_t)t * KP + 16 * q + 4 * g);
+      else
+        a[q] = v4f{ 0.f, 0.f, 0.f, 0.f };
+    }
+  }
+  // output rows of this lane: targets tbase + 4g + r
+  float ny[4];
+#pragma unroll
+  for (int r = 0; r < 4; r++) {
+    const int t = tbase + 4 * g + r;
+    ny[r] = t < P.NP ? P.Pn[t] : 0.f;
+  }
+  // owner lanes 0..15 keep the exact lists of targets tbase + lane
+  const bool owner = lane < 16 && (tbase + lane) < P.NP;
+  const int ot = tbase + lane;
+  double ls[KCAP];
+  int li[KCAP];
+#pragma unroll
+  for (int j = 0; j < KCAP; j++) {
+    ls[j] = INF_D;
+    li[j] = INT_MAX;
+  }
+  double s_kth = INF_D, thr = INF_D;
+  int zeros = 0;
+  const double c1 = owner ? P.c1t[ot] : 0.0, c3 = owner ? P.c3t[ot] : 0.0;
+  if (lane < 16) {
+    thr_s[w * 16 + lane] = __builtin_huge_valf();
+    qcnt[w * 16 + lane] = 0;
+  }
+
+  for (int c0 = lo; c0 < hi; c0 += MCH) {
+    const int cn = min(MCH, hi - c0);
+    __syncthreads();
+    for (int e = threadIdx.x; e < MCH * KP; e += blockDim.x) {
+      const int j = e / KP;
+      Bs[e] = j < cn ? P.M32[(size_t)(c0 + j) * KP + (e - j * KP)] : 0.f;
+    }
+    for (int e = threadIdx.x; e < MCH; e += blockDim.x)
+      Bn[e] = e < cn ? P.Mn[c0 + e] : 0.f;
+    __syncthreads();
+    float th[4];
+#pragma unroll
+    for (int r = 0; r < 4; r++)
+      th[r] = thr_s[w * 16 + 4 * g + r];
+#pragma unroll
+    for (int jt = 0; jt < MCH / 16; jt++) {
+      const int j = jt * 16 + l16;
+      v4f acc = { 0.f, 0.f, 0.f, 0.f };
+#pragma unroll
+      for (int q = 0; q < KP / 16; q++) {
+        const v4f b = *(const v4f*)(Bs + j * KP + 16 * q + 4 * g);
+        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a[q][0], b[0], acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a[q][1], b[1], acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a[q][2], b[2], acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a[q][3], b[3], acc, 0, 0, 0);
+      }
+      if (j < cn) {
+        const float nx = Bn[j];
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          const float d2 = __builtin_fmaf(-2.0f, acc[r], ny[r] + nx);
+          const int ti = 4 * g + r;
+          if (d2 <= th[r] && tbase + ti < P.NP) {
+            const int slot = atomicAdd(&qcnt[w * 16 + ti], 1);
+            queue[(w * 16 + ti) * MCH + slot] = (unsigned char)j;
+          }
+        }
+      }
+    }
+    __syncthreads();
+    // owner lanes: exact FP64 recomputation of their queued candidates
+    if (owner) {
+      const int nq = qcnt[w * 16 + lane];
+      const double* y = P.Mp + (size_t)ot * P.Ea;
+      for (int e = 0; e < nq; e++) {
+        const int j = queue[(w * 16 + lane) * MCH + e];
+        const double* x = P.M + (size_t)(c0 + j) * P.Ea;
+        double acc = 0.0;
+        for (int d = 0; d < P.Ea; d++) {
+          const double u = x[d] - y[d];
+          acc += u * u;
+        }
+        if (acc == 0.0 || acc == MISSING_D) {
+          zeros++;
+          continue;
+        }
+        if (acc < thr) {
+          const double s = sqrt(acc);
+          if (s < s_kth) {
+            double cs = s;
+            int ci = c0 + j;
+#pragma unroll
+            for (int q = 0; q < KCAP; q++) {
+              const bool take = cs < ls[q] || (cs == ls[q] && ci < li[q]);
+              const double ts = take ? cs : ls[q];
+              const int tv = take ? ci : li[q];
+              cs = take ? ls[q] : cs;
+              ci = take ? li[q] : ci;
+              ls[q] = ts;
+              li[q] = tv;
+            }
+            double v = INF_D;
+#pragma unroll
+            for (int q = 0; q < KCAP; q++)
+              if (q == k - 1)
+                v = ls[q];
+            s_kth = v;
+            if (s_kth != INF_D)
+              thr = s_kth * s_kth * SQRT_GUARD;
+          }
+        }
+      }
+      qcnt[w * 16 + lane] = 0;
+      thr_s[w * 16 + lane] =
+        thr == INF_D ? __builtin_huge_valf() : up_f32(thr + 2.0 * (c1 * sqrt(thr) + P.c2 * thr + c3));
+    }
+  }
+  if (owner) {
+    const size_t base = ((size_t)blockIdx.y * P.NP + ot) * KCAP;
+#pragma unroll
+    for (int j = 0; j < KCAP; j++) {
+      out_s[base + j] = ls[j];
+      out_i[base + j] = li[j];
+    }
+    out_cnt[(size_t)blockIdx.y * P.NP + ot] = (hi > lo ? hi - lo : 0) - zeros;
+  }
+}
+
+// Lane-layout self-test of v_mfma_f32_16x16x4f32 as knn_mfma uses it: A[i][k] = i + 1 for k = 0,
+// B[k][j] = 100 (j + 1) for k = 0, zero elsewhere; out[lane * 4 + r] must be D[4 g + r][lane % 16].
+__global__ void mfma_layout_probe(float* out)
+{
+  const int lane = threadIdx.x;
+  const int g = lane >> 4, l16 = lane & 15;
+  const float av = g == 0 ? (float)(l16 + 1) : 0.f;
+  const float bv = g == 0 ? 100.f * (float)(l16 + 1) : 0.f;
+  v4f acc = { 0.f, 0.f, 0.f, 0.f };
+  acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, acc, 0, 0, 0);
+#pragma unroll
+  for (int r = 0; r < 4; r++)
+    out[lane * 4 + r] = acc[r];
+}
+
+#define FEDM_MFMA(KC, KPv) template __global__ void knn_mfma<KC, KPv>(MfmaParams, int, int, double*, int*, int*);
+#define FEDM_MFMA_K(KC) FEDM_MFMA(KC, 16) FEDM_MFMA(KC, 32) FEDM_MFMA(KC, 64)
+FEDM_MFMA_K(4)
+FEDM_MFMA_K(8)
+FEDM_MFMA_K(16)
+FEDM_MFMA_K(32)
+FEDM_MFMA_K(64)
+
+} // namespace fedm

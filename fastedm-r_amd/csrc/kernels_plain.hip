@@ -8,6 +8,8 @@
 //   S-map                src/edm.cpp:737-814  — Gram X^T X, X^T y, then the JacobiSVD solve
 // Built with -ffp-contract=off: every FP64 multiply and add is rounded separately, in the same
 // order as the CPU path, so distances, neighbour order and S-map Gram matrices are bit-identical.
+#include <algorithm>
+
 #include "kernels_common.h"
 #include "fdlibm_exp.h"
 
@@ -430,18 +432,20 @@ __global__ __launch_bounds__(64) void smap_f64(SmapParams S, const double* __res
   constexpr int EPL = (NTRI + NMAX + 63) / 64; // Gram + rhs entries per lane
   extern __shared__ double smem[];
   const int Ea = S.Ea, n = Ea + 1;
-  // dynamic LDS: Xc[64 x n] | yc[64] | cd[64] | W, U, V [n x n] | bvec, sv, tmp, xsol [n] | cidx[64]
-  double* Xc = smem;
-  double* yc = Xc + 64 * n;
-  double* cd = yc + 64;
-  double* W = cd + 64;
-  double* U = W + n * n;
-  double* V = U + n * n;
-  double* bvec = V + n * n;
+  // dynamic LDS: [ bvec, sv, tmp, xsol (n each) | cd[64] | cidx[64] ] then one region used first for the
+  // Gram stage (Xc[64 x n], yc[64]) and afterwards for the SVD (W, U, V [n x n])
+  double* bvec = smem;
   double* sv = bvec + n;
   double* tmp = sv + n;
   double* xsol = tmp + n;
-  int* cidx = (int*)(xsol + n);
+  double* cd = xsol + n;
+  int* cidx = (int*)(cd + 64);
+  double* region = (double*)(cidx + 64);
+  double* Xc = region;
+  double* yc = Xc + 64 * n;
+  double* W = region;
+  double* U = W + n * n;
+  double* V = U + n * n;
 
   const int tl = blockIdx.x;
   const int t = t0 + tl;
@@ -585,7 +589,8 @@ __global__ __launch_bounds__(64) void smap_f64(SmapParams S, const double* __res
 // LDS bytes smap_f64 needs for an n x n system.
 size_t smap_lds_bytes(int n)
 {
-  return sizeof(double) * (size_t)(64 * n + 64 + 64 + 3 * n * n + 4 * n) + sizeof(int) * 64;
+  const size_t region = std::max((size_t)(64 * n + 64), (size_t)(3 * n * n));
+  return sizeof(double) * (4 * (size_t)n + 64 + region) + sizeof(int) * 64;
 }
 
 // ----------------------------------------------------------------------------------------------

@@ -141,6 +141,11 @@ int edm_batch_on_device(const edm_batch* b);
  * npool rows (scale from FEDM_RANK_LTHR_SCALE, default 1.5 * sqrt(k * npool)). */
 int fedm_rank_threshold(int kmax, int npool);
 
+/* Diagnostic: the lane layout of the FP32 matrix-core instruction the mixed route uses (a 16x16x4
+ * product with A[i][0] = i+1, B[0][j] = 100 (j+1)); out256[lane*4 + r] is D[4 (lane/16) + r][lane%16]
+ * on a conforming device.  Returns 0 when the probe ran. */
+int edm_selftest_mfma(float* out256);
+
 /* Last stage timings of the most recent call on ctx (milliseconds, HIP events on the ctx stream):
  * [0] distance/selection kernel, [1] merge/predict kernel, [2] S-map kernel, [3] whole device section. */
 void edm_ctx_timings(edm_ctx* ctx, double* ms4);

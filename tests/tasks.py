@@ -57,5 +57,5 @@ def assert_predictions_close(gpu, ref, rtol=1e-10, atol=1e-12):
     d = np.abs(gpu[~gm] - ref[~rm])
     tol = atol + rtol * np.abs(ref[~rm])
     bad = d > tol
-    assert not bad.any(), f"{bad.sum()} predictions differ, max |d| {d.max():.3e}"
+    assert not bad.any(), f"{bad.sum()} values differ, max |d| {d.max():.3e}"
     return float(d.max()) if d.size else 0.0

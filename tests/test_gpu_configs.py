@@ -2,7 +2,8 @@
 
 cfg1 and cfg2 are compared whole (stats and predictions).  cfg4 (512 panels, S-map, dt, E=8) and
 cfg5 (Lorenz-96, E_a = 40, k = 42) run at full size on the GPU; the oracle checks the manifolds whole
-and the predictions of a sample of targets against the full library.  Bar: predictions within
+and the predictions of a sample of targets against the full library.  Both precision forms are held to
+the same bar (the mixed form changes only the neighbour screening): predictions within
 1e-12 + 1e-10 |ref|; rho and MAE likewise; manifolds and k bit-exact."""
 import time
 
@@ -18,12 +19,13 @@ pytestmark = pytest.mark.gpu
 REL, ABS = 1e-10, 1e-12
 
 
-def close(a, b):
+def close(a, b, rel=REL, abs_=ABS):
+    """Max |a - b| after checking |a - b| <= abs_ + rel |b| and the NA pattern."""
     a, b = np.asarray(a, float), np.asarray(b, float)
     assert np.array_equal(np.isnan(a), np.isnan(b))
     ok = ~np.isnan(b)
     d = np.abs(a[ok] - b[ok])
-    assert np.all(d <= ABS + REL * np.abs(b[ok])), float(d.max())
+    assert np.all(d <= abs_ + rel * np.abs(b[ok])), float(d.max())
     return float(d.max()) if d.size else 0.0
 
 
@@ -69,11 +71,12 @@ def sample_rows(n, m, seed):
     return np.sort(np.random.default_rng(seed).choice(n, size=min(m, n), replace=False))
 
 
-def test_cfg4_panels_smap_dt():
+@pytest.mark.parametrize("precision", [0, 1], ids=["plain", "mixed"])
+def test_cfg4_panels_smap_dt(precision):
     fe = load_product()
     t, x, panel = configs.cfg4_panels()
     kw = dict(panel=panel, E=8, dt=True, theta=[0, 1, 3], algorithm="smap", savePredictions=True,
-              saveManifolds=True)
+              saveManifolds=True, precision=precision)
     t0 = time.perf_counter()
     a = fe.edm(t, x, **kw)
     gpu_s = time.perf_counter() - t0
@@ -111,16 +114,17 @@ def test_cfg4_panels_smap_dt():
     rp, rr, _, rk, _ = oracle.predict(Md, Mpd, opts)
     assert (rk == 18).all()
     d = close(a["predictions"][pred_rows[pick], 0], rp[2])
-    print(f"cfg4: {len(lib_rows)} library x {len(pred_rows)} targets, E_a=16, k=18, 3 thetas: GPU {gpu_s:.2f} s "
+    print(f"cfg4 precision={precision}: {len(lib_rows)} library x {len(pred_rows)} targets, E_a=16, k=18, 3 thetas: GPU {gpu_s:.2f} s "
           f"(device: select {stages[0]:.1f} ms, merge {stages[1]:.1f} ms, S-map {stages[2]:.1f} ms); "
           f"sampled max |d yhat| {d:.2e}; rho {np.round(a['stats']['rho'], 5)}")
 
 
-def test_cfg5_lorenz96_smap():
+@pytest.mark.parametrize("precision", [0, 1], ids=["plain", "mixed"])
+def test_cfg5_lorenz96_smap(precision):
     fe = load_product()
     series = configs.lorenz96()
     lib, pred = configs.cfg5_manifold(series)
-    opts = {"algorithm": 1, "k": 42, "thetas": [2.0], "save_smap_coeffs": 1, "nthreads": 16}
+    opts = {"algorithm": 1, "k": 42, "thetas": [2.0], "save_smap_coeffs": 1, "nthreads": 16, "precision": precision}
     t0 = time.perf_counter()
     gp, gr, gc, gk, gn, status = fe.predict(lib, pred, opts)
     gpu_s = time.perf_counter() - t0
@@ -132,6 +136,6 @@ def test_cfg5_lorenz96_smap():
     dc = close(np.where(gc[pick] == 1e100, np.nan, gc[pick]), np.where(rc_ == 1e100, np.nan, rc_))
     yt = pred["targets"]
     rho = np.corrcoef(gp[0], yt)[0, 1]
-    print(f"cfg5: {lib['data'].shape[0]} library x {pred['data'].shape[0]} targets, E_a=40, k=42: GPU {gpu_s:.2f} s "
+    print(f"cfg5 precision={precision}: {lib['data'].shape[0]} library x {pred['data'].shape[0]} targets, E_a=40, k=42: GPU {gpu_s:.2f} s "
           f"(device: select {stages[0]:.1f} ms, merge {stages[1]:.1f} ms, S-map {stages[2]:.1f} ms); "
           f"sampled max |d yhat| {d:.2e}, max |d coeff| {dc:.2e}; rho {rho:.5f}")
