@@ -23,6 +23,8 @@ __global__ void dense_dist_f64(LpParams P, int t0, int nt, int slice_len, double
 template <int KCAP>
 __global__ void finish_lists(FinishParams F, const double* ps, const int* pi, const int* pcnt, int S);
 __global__ void finish_dense(FinishParams F, const double* D, int t0, int nt, const int* pcnt, int S);
+__global__ void select_wide(FinishParams F, const double* D, int t0, int nt, unsigned long long* gkey, int* gidx,
+                            int gstride);
 template <int NMAX>
 __global__ void smap_f64(SmapParams S, const double* D, int t0, int nt);
 size_t smap_lds_bytes(int n);
@@ -119,6 +121,8 @@ enum
   B_COEFFS,
   B_NNOUT,
   B_D,
+  B_WKEY,
+  B_WIDX,
   B_MIX0,
   B_MIX1,
   B_MIX2,
@@ -360,7 +364,7 @@ int edm_predict(edm_ctx* c, const edm_manifold_view* M, const edm_manifold_view*
   const int emax = emax_for(Ea);
   const int k = o->k;
   const bool list_route = k > 0 && k <= 64;
-  if (emax < 0 || o->distance == FEDM_DIST_WASSERSTEIN || (!list_route && k > 0 && k > 64 && false)) {
+  if (emax < 0 || o->distance == FEDM_DIST_WASSERSTEIN) {
     host_fill_outputs(Mp, o, pred, rc, coeffs, k_used, nn_idx, nn_cap, Ea, FEDM_UNKNOWN_ERROR);
     return -95; // not supported on this route
   }
@@ -590,20 +594,33 @@ int edm_predict(edm_ctx* c, const edm_manifold_view* M, const edm_manifold_view*
     }
     HIP_TRY(hipEventRecord(c->ev[3], s));
   } else {
-    // every valid neighbour (k < 0, or k larger than the register lists hold)
+    // dense distance rows: every valid neighbour (k < 0), or an exact selection wider than the
+    // register lists (k > 64) made from the rows by select_wide
+    const bool wide = k > 64;
+    const int n2 = wide ? (int)std::min<long long>(1ll << 30, 1ll << (64 - __builtin_clzll((unsigned long long)k - 1))) : 0;
+    const int gstride = wide && n2 > 4096 ? n2 : 0;
+    const size_t per_target = sizeof(double) * (size_t)std::max(NL, 1) +
+                              (wide ? (sizeof(int) + sizeof(double)) * (size_t)k : 0) +
+                              (sizeof(unsigned long long) + sizeof(int)) * (size_t)gstride;
     const size_t budget = (size_t)1 << 30;
-    int nt = (int)std::max<size_t>(1, std::min<size_t>((size_t)NP, budget / (sizeof(double) * (size_t)std::max(NL, 1))));
+    int nt = (int)std::max<size_t>(1, std::min<size_t>((size_t)NP, budget / per_target));
     const int bxmax = (nt + 255) / 256;
     int S = std::max(1, std::min(MAX_SLICES, (1024 + bxmax - 1) / bxmax));
     S = std::max(1, std::min(S, NL / 256));
     const int sl = (NL + S - 1) / S;
     double* D = (double*)c->get(B_D, sizeof(double) * (size_t)nt * std::max(NL, 1));
     int* pc = (int*)c->get(B_PCNT, sizeof(int) * (size_t)S * NP);
-    F.nn_i = nullptr;
-    F.nn_s = nullptr;
-    F.nn_stride = 0;
-    if (!D || !pc)
+    F.nn_i = wide ? (int*)c->get(B_NNI, sizeof(int) * (size_t)nt * k) : nullptr;
+    F.nn_s = wide ? (double*)c->get(B_NNS, sizeof(double) * (size_t)nt * k) : nullptr;
+    F.nn_stride = wide ? k : 0;
+    unsigned long long* gkey =
+      gstride ? (unsigned long long*)c->get(B_WKEY, sizeof(unsigned long long) * (size_t)nt * gstride) : nullptr;
+    int* gidx = gstride ? (int*)c->get(B_WIDX, sizeof(int) * (size_t)nt * gstride) : nullptr;
+    if (!D || !pc || (wide && (!F.nn_i || !F.nn_s)) || (gstride && (!gkey || !gidx)))
       return -2;
+    Sm.nn_i = F.nn_i;
+    Sm.nn_s = F.nn_s;
+    Sm.nn_stride = F.nn_stride;
     HIP_TRY(hipEventRecord(c->ev[1], s));
     for (int t0 = 0; t0 < NP; t0 += nt) {
       const int cnt = std::min(nt, NP - t0);
@@ -611,6 +628,11 @@ int edm_predict(edm_ctx* c, const edm_manifold_view* M, const edm_manifold_view*
       HIP_TRY(hipGetLastError());
       hipLaunchKernelGGL(finish_dense, dim3((cnt + 255) / 256), dim3(256), 0, s, F, D, t0, cnt, pc, S);
       HIP_TRY(hipGetLastError());
+      if (wide) {
+        hipLaunchKernelGGL(select_wide, dim3(cnt), dim3(256), 0, s, F, D, t0, cnt, gkey, gidx, gstride);
+        HIP_TRY(hipGetLastError());
+      }
+      Sm.nn_t0 = t0;
       if (smap) {
         int r = launch_smap(n, dim3(cnt, T), s, Sm, D, t0, cnt);
         if (r)

@@ -85,9 +85,10 @@ struct SmapParams
   const double* thetas;
   const int* nn_k;
   const int* nn_cnt;
-  const int* nn_i;
+  const int* nn_i;   // list of target t at row t - nn_t0
   const double* nn_s;
   int nn_stride;
+  int nn_t0;
   double* pred;
   int* rc;
   double* coeffs; // [NP x (Ea+1)] or null

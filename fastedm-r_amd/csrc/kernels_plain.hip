@@ -353,7 +353,8 @@ __global__ __launch_bounds__(256) void finish_lists(FinishParams F, const double
   }
 }
 
-// Same policy for the dense (k = all) route: per-slice counts were written by dense_dist_f64.
+// Same policy for the dense route (k < 0, or k wider than the register lists): per-slice counts were
+// written by dense_dist_f64.
 __global__ __launch_bounds__(256) void finish_dense(FinishParams F, const double* __restrict__ D, int t0, int nt,
                                                    const int* __restrict__ pcnt, int S)
 {
@@ -380,9 +381,13 @@ __global__ __launch_bounds__(256) void finish_dense(FinishParams F, const double
     return;
   }
   if (k > 0 && k < cnt) {
-    // a selection wider than the register lists is not made on this route
-    F.rc[t] = RC_UNKNOWN_ERROR;
-    F.nn_k[t] = 0;
+    // a selection wider than the register lists: select_wide makes it (and the simplex prediction);
+    // the neighbour set is recorded for any algorithm, as the CPU path selects before it dispatches
+    F.nn_cnt[t] = cnt;
+    F.k_used[t] = k;
+    F.nn_k[t] = k;
+    if (F.algorithm != ALG_SIMPLEX && F.algorithm != ALG_SMAP)
+      F.rc[t] = RC_INVALID_ALGORITHM;
     return;
   }
   // k < 0 or k == cnt: every valid neighbour in library order.
@@ -459,8 +464,8 @@ __global__ __launch_bounds__(64) void smap_f64(SmapParams S, const double* __res
   const bool dense = kk < 0;
   const int k = dense ? S.nn_cnt[t] : kk;
   const double* drow = dense ? D + (size_t)tl * S.NL : nullptr;
-  const int* li = S.nn_i + (size_t)t * S.nn_stride;
-  const double* lsd = S.nn_s + (size_t)t * S.nn_stride;
+  const int* li = S.nn_i + (size_t)(t - S.nn_t0) * S.nn_stride;
+  const double* lsd = S.nn_s + (size_t)(t - S.nn_t0) * S.nn_stride;
 
   // mean of the selected distances, summed in neighbour order (Eigen mean = sum / size)
   double sum = 0.0;

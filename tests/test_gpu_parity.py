@@ -35,6 +35,15 @@ CASES = {
     "smap_E40_k42": dict(NL=1500, NP=120, Ea=40, k=42, algorithm=1, thetas=(2.0,), save_coeffs=True),
     "simplex_E40_k41": dict(NL=1500, NP=200, Ea=40, k=41),
     "invalid_algorithm": dict(NL=100, NP=50, Ea=2, k=3, algorithm=7),
+    # k wider than the register lists: radix select + sort on the dense rows
+    "simplex_wide_k100": dict(NL=3000, NP=200, Ea=3, k=100, thetas=(0.5, 2.0)),
+    "simplex_wide_ties_k90": dict(NL=2000, NP=150, Ea=2, k=90, integer=True),
+    "simplex_wide_duplicates_k80": dict(NL=2000, NP=150, Ea=3, k=80, duplicate_targets=True),
+    "simplex_wide_global_sort_k5000": dict(NL=9000, NP=60, Ea=3, k=5000),
+    "smap_wide_k150": dict(NL=2000, NP=150, Ea=5, k=150, algorithm=1, thetas=(0, 1, 3), save_coeffs=True),
+    "wide_missing_panels_k120": dict(NL=2000, NP=100, Ea=4, k=120, missing=0.05, missing_distance=0.3, panels=3,
+                                     idw=0.5),
+    "wide_invalid_algorithm": dict(NL=300, NP=50, Ea=2, k=70, algorithm=7),
 }
 
 
@@ -43,7 +52,7 @@ def test_task_parity(name):
     fe = load_product()
     spec = CASES[name]
     M, Mp, opts = random_task(zlib.crc32(name.encode()) % 1000, **spec)
-    cap = 64
+    cap = max(64, spec["k"])  # the whole neighbour list
     print(f"case {name}: NL={spec['NL']} NP={spec['NP']} Ea={spec['Ea']} k={spec['k']}", flush=True)
     rp, rr, rc_, rk, rn = oracle.predict(M, Mp, opts, nn_cap=cap)
     gp, gr, gc, gk, gn, status = fe.predict(M, Mp, opts, nn_cap=cap)
