@@ -23,6 +23,7 @@ __global__ void dense_dist_f64(LpParams P, int t0, int nt, int slice_len, double
 template <int KCAP>
 __global__ void finish_lists(FinishParams F, const double* ps, const int* pi, const int* pcnt, int S);
 __global__ void finish_dense(FinishParams F, const double* D, int t0, int nt, const int* pcnt, int S);
+__global__ void wasserstein_dense(WassParams P, int t0, int nt, int slice_len, double* D, int* out_cnt);
 __global__ void select_wide(FinishParams F, const double* D, int t0, int nt, unsigned long long* gkey, int* gidx,
                             int gstride);
 template <int NMAX>
@@ -363,8 +364,18 @@ int edm_predict(edm_ctx* c, const edm_manifold_view* M, const edm_manifold_view*
   }
   const int emax = emax_for(Ea);
   const int k = o->k;
-  const bool list_route = k > 0 && k <= 64;
-  if (emax < 0 || o->distance == FEDM_DIST_WASSERSTEIN) {
+  const bool wass = o->distance == FEDM_DIST_WASSERSTEIN;
+  const bool list_route = !wass && k > 0 && k <= 64;
+  // Wasserstein point layout [x lags | dt lags | lagged extras | unlagged extras] (src/distances.cpp:318-326)
+  const int wE = M->E_x;
+  const int wrows = wE > 0 ? 1 + (M->E_dt > 0) + M->E_lagged_extras / wE : 0;
+  const int wunl = M->E_extras - M->E_lagged_extras;
+  bool wass_ok = wE >= 1 && wE <= 32 && (M->E_dt == 0 || M->E_dt == wE) && M->E_lagged_extras % wE == 0 &&
+                 wunl >= 0 && wrows * wE + wunl == Ea && o->missing_distance >= 0.0;
+  if (wass && wass_ok && o->panel_mode && o->idw <= 0 && o->id_weights)
+    for (int a = 0; a < o->num_panels * o->num_panels; a++)
+      wass_ok = wass_ok && o->id_weights[a] >= 0.0f;
+  if (emax < 0 || (wass && !wass_ok)) {
     host_fill_outputs(Mp, o, pred, rc, coeffs, k_used, nn_idx, nn_cap, Ea, FEDM_UNKNOWN_ERROR);
     return -95; // not supported on this route
   }
@@ -372,9 +383,11 @@ int edm_predict(edm_ctx* c, const edm_manifold_view* M, const edm_manifold_view*
   hipStream_t s = c->stream;
   int err = 0;
 
-  std::vector<int> metrics(Ea, 0);
+  // metrics: one per coordinate for L^p, one per variable for Wasserstein (src/distances.cpp:29-67)
+  const int nmet = wass ? wrows + wunl : Ea;
+  std::vector<int> metrics(nmet, 0);
   if (o->metrics)
-    std::copy(o->metrics, o->metrics + Ea, metrics.begin());
+    std::copy(o->metrics, o->metrics + nmet, metrics.begin());
 
   LpParams P;
   std::memset(&P, 0, sizeof(P));
@@ -387,7 +400,7 @@ int edm_predict(edm_ctx* c, const edm_manifold_view* M, const edm_manifold_view*
     P.Mpanel = up(c, B_MPAN, M->panel, (size_t)NL, &err);
     P.Mppanel = up(c, B_MPPAN, Mp->panel, (size_t)NP, &err);
   }
-  P.metrics = up(c, B_METRICS, metrics.data(), (size_t)Ea, &err);
+  P.metrics = up(c, B_METRICS, metrics.data(), (size_t)nmet, &err);
   if (o->id_weights && o->num_panels > 0)
     P.id_weights = up(c, B_IDW, o->id_weights, (size_t)o->num_panels * o->num_panels, &err);
   const double* dthetas = up(c, B_THETAS, o->thetas, (size_t)T, &err);
@@ -415,6 +428,30 @@ int edm_predict(edm_ctx* c, const edm_manifold_view* M, const edm_manifold_view*
     f |= F_IDWEIGHTS;
   P.flags = f;
   const bool generic = f != 0;
+
+  WassParams W;
+  std::memset(&W, 0, sizeof(W));
+  if (wass) {
+    W.M = P.M;
+    W.Mp = P.Mp;
+    W.Mpanel = P.Mpanel;
+    W.Mppanel = P.Mppanel;
+    W.metrics = P.metrics;
+    W.id_weights = P.id_weights;
+    W.NL = NL;
+    W.NP = NP;
+    W.Ea = Ea;
+    W.E = wE;
+    W.E_dt = M->E_dt;
+    W.rows = wrows;
+    W.n_unlagged = wunl;
+    W.num_panels = o->num_panels;
+    W.missing_distance = o->missing_distance;
+    W.idw = o->idw;
+    W.aspect_ratio = o->aspect_ratio;
+    W.panel_term = !o->panel_mode ? 0 : o->idw > 0 ? 1 : P.id_weights ? 2 : 0;
+    W.skip_other = o->panel_mode && o->idw < 0;
+  }
 
   double* dpred = (double*)c->get(B_PRED, sizeof(double) * (size_t)T * NP);
   int* drc = (int*)c->get(B_RC, sizeof(int) * (size_t)T * NP);
@@ -596,7 +633,7 @@ int edm_predict(edm_ctx* c, const edm_manifold_view* M, const edm_manifold_view*
   } else {
     // dense distance rows: every valid neighbour (k < 0), or an exact selection wider than the
     // register lists (k > 64) made from the rows by select_wide
-    const bool wide = k > 64;
+    const bool wide = wass ? k > 0 : k > 64;
     const int n2 = wide ? (int)std::min<long long>(1ll << 30, 1ll << (64 - __builtin_clzll((unsigned long long)k - 1))) : 0;
     const int gstride = wide && n2 > 4096 ? n2 : 0;
     const size_t per_target = sizeof(double) * (size_t)std::max(NL, 1) +
@@ -604,9 +641,9 @@ int edm_predict(edm_ctx* c, const edm_manifold_view* M, const edm_manifold_view*
                               (sizeof(unsigned long long) + sizeof(int)) * (size_t)gstride;
     const size_t budget = (size_t)1 << 30;
     int nt = (int)std::max<size_t>(1, std::min<size_t>((size_t)NP, budget / per_target));
-    const int bxmax = (nt + 255) / 256;
+    const int bxmax = wass ? nt : (nt + 255) / 256;
     int S = std::max(1, std::min(MAX_SLICES, (1024 + bxmax - 1) / bxmax));
-    S = std::max(1, std::min(S, NL / 256));
+    S = std::max(1, std::min(S, NL / (wass ? 64 : 256)));
     const int sl = (NL + S - 1) / S;
     double* D = (double*)c->get(B_D, sizeof(double) * (size_t)nt * std::max(NL, 1));
     int* pc = (int*)c->get(B_PCNT, sizeof(int) * (size_t)S * NP);
@@ -624,7 +661,10 @@ int edm_predict(edm_ctx* c, const edm_manifold_view* M, const edm_manifold_view*
     HIP_TRY(hipEventRecord(c->ev[1], s));
     for (int t0 = 0; t0 < NP; t0 += nt) {
       const int cnt = std::min(nt, NP - t0);
-      launch_dense(emax, generic, dim3((cnt + 255) / 256, S), s, P, t0, cnt, sl, D, pc);
+      if (wass)
+        hipLaunchKernelGGL(wasserstein_dense, dim3(cnt, S), dim3(256), 0, s, W, t0, cnt, sl, D, pc);
+      else
+        launch_dense(emax, generic, dim3((cnt + 255) / 256, S), s, P, t0, cnt, sl, D, pc);
       HIP_TRY(hipGetLastError());
       hipLaunchKernelGGL(finish_dense, dim3((cnt + 255) / 256), dim3(256), 0, s, F, D, t0, cnt, pc, S);
       HIP_TRY(hipGetLastError());

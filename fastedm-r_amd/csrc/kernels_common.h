@@ -59,6 +59,21 @@ struct LpParams
   int flags;
 };
 
+// Wasserstein distances (src/distances.cpp:296-527).
+struct WassParams
+{
+  const double* M;        // [NL x Ea]
+  const double* Mp;       // [NP x Ea]
+  const int* Mpanel;      // [NL] or null
+  const int* Mppanel;     // [NP] or null
+  const int* metrics;     // one per variable: x, dt (when E_dt > 0), then the extras
+  const float* id_weights;
+  int NL, NP, Ea, E, E_dt, rows, n_unlagged, num_panels;
+  double missing_distance, idw, aspect_ratio;
+  int panel_term;         // 0 none, 1 idw penalty, 2 id_weights table
+  int skip_other;
+};
+
 struct FinishParams
 {
   int NP, NL, k, T, force_compute, algorithm;

@@ -69,7 +69,9 @@ typedef struct
   int k;               /* neighbours, already set by the task configurator (src/edm.cpp:155-163); <0 = all */
   int num_thetas;
   const double* thetas; /* [num_thetas] */
-  const int* metrics;   /* [E_actual] 0 = Diff, 1 = CheckSame (expanded, src/distances.cpp:29-67), or NULL */
+  const int* metrics;   /* 0 = Diff, 1 = CheckSame, expanded as src/distances.cpp:29-67 does, or NULL:
+                           L^p: one per coordinate [E_actual]; Wasserstein: one per variable
+                           [1 + (E_dt > 0) + number of extras] */
   double missing_distance;
   int panel_mode;
   double idw;              /* panel mismatch penalty; <0 = skip other panels */

@@ -1425,6 +1425,17 @@ int orc_predict(const orc_manifold_view* M, const orc_manifold_view* Mp, const o
   return 0;
 }
 
+int orc_distances(const orc_manifold_view* M, const orc_manifold_view* Mp, const orc_options* o, int Mp_i, int* inds,
+                  double* dists)
+{
+  Pairs pot = o->distance == 2 ? wasserstein_distances(Mp_i, *o, *M, *Mp) : lp_distances(Mp_i, *o, *M, *Mp);
+  for (size_t a = 0; a < pot.inds.size(); a++) {
+    inds[a] = pot.inds[a];
+    dists[a] = pot.dists[a];
+  }
+  return (int)pot.inds.size();
+}
+
 void* orc_run_command(const orc_command* c)
 {
   return run_command(*c);

@@ -86,6 +86,10 @@ int orc_predict(const orc_manifold_view* M, const orc_manifold_view* Mp, const o
                 double* coeffs, int* k_used, int* nn_idx, int nn_cap);
 
 /* Command-level entry: the reference run_command (src/r.cpp:107-468). Returns an opaque result. */
+/* Potential neighbours of target Mp_i before the k selection (src/distances.cpp:151-268, 490-527):
+ * library indices and distances, in library order; returns their count (buffers hold M->num_points). */
+int orc_distances(const orc_manifold_view* M, const orc_manifold_view* Mp, const orc_options* o, int Mp_i, int* inds,
+                  double* dists);
 void* orc_run_command(const orc_command* c);
 int orc_result_rc(void* r);
 int orc_result_num_stats(void* r, int copredict);

@@ -302,6 +302,19 @@ def predict(M, Mp, opts, nn_cap=0):
             nn[:Np * nn_cap].reshape(Np, nn_cap) if nn_cap else None)
 
 
+def distances(M, Mp, opts, i):
+    """Potential neighbours of target i before the k selection: (library indices, distances)."""
+    L = lib()
+    keep = []
+    mv, mpv = manifold_view(M, keep), manifold_view(Mp, keep)
+    o = options_struct(opts, keep)
+    n = M["data"].shape[0]
+    inds = np.zeros(max(n, 1), np.int32)
+    d = np.zeros(max(n, 1))
+    c = L.orc_distances(C.byref(mv), C.byref(mpv), C.byref(o), i, iptr(inds), dptr(d))
+    return inds[:c], d[:c]
+
+
 def manifold_view(m, keep):
     v = ManifoldView()
     data = np.ascontiguousarray(m["data"], np.float64)

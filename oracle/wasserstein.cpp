@@ -84,14 +84,17 @@ double exact_emd(const double* C, int li, int lj)
     }
     need -= f;
   }
-  // Flow on each (i,j) arc, summed as flow_ij * C_ij with flow in units of 1/(li*lj).
+  // Flow on each (i,j) arc in units of 1/(li*lj), summed as flow_ij * C_ij in the reference's arc order:
+  // EMD_wrap walks the arcs from the last id (i*lj + j) down to 0 (src/vendor/EMD_wrapper.h:106-113,
+  // src/vendor/full_bipartitegraph.h:98-100).
+  std::vector<long> flow((size_t)li * lj, 0);
   for (int i = 0; i < li; i++)
     for (const Edge& ed : g[i])
-      if (ed.to >= li && ed.to < li + lj) {
-        long flow = (long)li * lj - ed.cap;
-        if (flow > 0)
-          total += ((double)flow / ((double)li * lj)) * ed.cost;
-      }
+      if (ed.to >= li && ed.to < li + lj)
+        flow[(size_t)i * lj + (ed.to - li)] = (long)li * lj - ed.cap;
+  for (long a = (long)li * lj - 1; a >= 0; a--)
+    if (flow[a] > 0)
+      total += ((double)flow[a] / ((double)li * lj)) * C[a];
   return total;
 }
 

@@ -59,3 +59,51 @@ def assert_predictions_close(gpu, ref, rtol=1e-10, atol=1e-12):
     bad = d > tol
     assert not bad.any(), f"{bad.sum()} values differ, max |d| {d.max():.3e}"
     return float(d.max()) if d.size else 0.0
+
+
+def wasserstein_task(seed, NL, NP, E, k, dt=False, lagged=0, unlagged=0, missing=0.0, missing_distance=0.0,
+                     categorical=False, panels=0, idw=0.0, id_weights=False, algorithm=0, thetas=(1.0,),
+                     aspect_ratio=1.0, save_coeffs=False):
+    """A task with the Wasserstein point layout [x lags | dt lags | lagged extras | unlagged extras]
+    (src/distances.cpp:318-326) and its per-variable metrics (src/distances.cpp:37-47)."""
+    rng = np.random.default_rng(seed)
+    rows = 1 + int(dt) + lagged
+    Ea = rows * E + unlagged
+
+    def points(n):
+        P = np.empty((n, Ea))
+        P[:, :E] = rng.random((n, E))
+        r = 1
+        if dt:
+            P[:, E:2 * E] = rng.integers(1, 4, (n, E)).astype(np.float64)
+            r = 2
+        for _ in range(lagged):
+            P[:, r * E:(r + 1) * E] = (rng.integers(0, 3, (n, E)).astype(np.float64) if categorical
+                                       else rng.random((n, E)))
+            r += 1
+        P[:, rows * E:] = (rng.integers(0, 2, (n, unlagged)).astype(np.float64) if categorical
+                           else rng.random((n, unlagged)))
+        if missing > 0:
+            P[rng.random(P.shape) < missing] = MISSING_D
+        return P
+
+    M, Mp = points(NL), points(NP)
+    dims = {"E_x": E, "E_dt": E if dt else 0, "E_extras": lagged * E + unlagged, "E_lagged_extras": lagged * E}
+    Md = dict(data=M, targets=rng.random(NL) * 2 - 0.5, **dims)
+    Mpd = dict(data=Mp, targets=rng.random(NP), **dims)
+    metrics = np.zeros(rows + unlagged, np.int32)
+    if categorical:
+        metrics[1 + int(dt):] = 1
+    opts = {"algorithm": algorithm, "distance": 2, "k": k, "thetas": np.asarray(thetas, float),
+            "missing_distance": missing_distance, "has_missing": int(missing > 0), "metrics": metrics,
+            "has_categorical": int(categorical), "aspect_ratio": aspect_ratio, "save_smap_coeffs": int(save_coeffs),
+            "force_compute": 1}
+    if panels:
+        Md["panel"] = rng.integers(0, panels, NL).astype(np.int32)
+        Mpd["panel"] = rng.integers(0, panels, NP).astype(np.int32)
+        opts["panel_mode"] = 1
+        opts["idw"] = idw
+        if id_weights:
+            opts["id_weights"] = rng.random((panels, panels)).astype(np.float32)
+            opts["idw"] = 0.0
+    return Md, Mpd, opts

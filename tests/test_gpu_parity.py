@@ -9,7 +9,7 @@ import pytest
 
 from oracle import oracle
 from tests.conftest import load_product
-from tests.tasks import assert_predictions_close, random_task
+from tests.tasks import assert_predictions_close, random_task, wasserstein_task
 
 pytestmark = pytest.mark.gpu
 
@@ -54,6 +54,44 @@ def test_task_parity(name):
     M, Mp, opts = random_task(zlib.crc32(name.encode()) % 1000, **spec)
     cap = max(64, spec["k"])  # the whole neighbour list
     print(f"case {name}: NL={spec['NL']} NP={spec['NP']} Ea={spec['Ea']} k={spec['k']}", flush=True)
+    rp, rr, rc_, rk, rn = oracle.predict(M, Mp, opts, nn_cap=cap)
+    gp, gr, gc, gk, gn, status = fe.predict(M, Mp, opts, nn_cap=cap)
+    assert status == 0
+    np.testing.assert_array_equal(gr, rr)
+    np.testing.assert_array_equal(gk, rk)
+    np.testing.assert_array_equal(gn, rn)
+    assert_predictions_close(gp, rp)
+    if opts.get("save_smap_coeffs"):
+        assert_predictions_close(gc, rc_)
+
+
+WASS_CASES = {
+    "wass_E5_k4": dict(NL=400, NP=100, E=5, k=4),
+    "wass_dt_aspect": dict(NL=300, NP=80, E=6, k=5, dt=True, aspect_ratio=0.7, thetas=(0.5, 2.0)),
+    "wass_skip_missing": dict(NL=300, NP=80, E=6, k=4, missing=0.15),
+    "wass_missing_distance": dict(NL=300, NP=80, E=6, k=4, missing=0.15, missing_distance=0.4),
+    "wass_extras_categorical": dict(NL=300, NP=80, E=4, k=6, lagged=1, unlagged=2, categorical=True),
+    "wass_panels_idw": dict(NL=300, NP=80, E=5, k=4, panels=3, idw=0.3),
+    "wass_panels_skip_other": dict(NL=300, NP=80, E=5, k=4, panels=3, idw=-1.0),
+    "wass_panels_id_weights": dict(NL=300, NP=80, E=5, k=4, panels=3, id_weights=True),
+    "wass_smap_all": dict(NL=200, NP=60, E=4, k=-1, algorithm=1, thetas=(0.0, 1.0, 2.0), save_coeffs=True),
+    "wass_smap_k8": dict(NL=300, NP=60, E=5, k=8, algorithm=1, thetas=(1.0,), dt=True),
+    "wass_simplex_all": dict(NL=200, NP=60, E=4, k=-1, thetas=(0.0, 1.0)),
+    "wass_wide_k80": dict(NL=400, NP=40, E=5, k=80),
+    "wass_E16": dict(NL=200, NP=40, E=16, k=5),
+    "wass_E32": dict(NL=60, NP=10, E=32, k=3),
+}
+
+
+@pytest.mark.parametrize("name", sorted(WASS_CASES))
+def test_wasserstein_parity(name):
+    """Wasserstein distances (exact transport per pair on the GPU) against the oracle's exact solver:
+    neighbour sets bit-exact, predictions within the plain bar."""
+    fe = load_product()
+    spec = WASS_CASES[name]
+    M, Mp, opts = wasserstein_task(zlib.crc32(name.encode()) % 1000, **spec)
+    cap = max(64, spec["k"])
+    print(f"case {name}: NL={spec['NL']} NP={spec['NP']} E={spec['E']} k={spec['k']}", flush=True)
     rp, rr, rc_, rk, rn = oracle.predict(M, Mp, opts, nn_cap=cap)
     gp, gr, gc, gk, gn, status = fe.predict(M, Mp, opts, nn_cap=cap)
     assert status == 0

@@ -11,6 +11,7 @@ import scipy.optimize
 
 from oracle import oracle
 from tests.conftest import GOLDEN
+from tests.tasks import wasserstein_task
 
 MISSING = 1.0e100
 
@@ -172,6 +173,95 @@ def test_emd_against_assignment_and_linprog():
         b_eq = [1 / li] * li + [1 / lj] * lj
         lp = scipy.optimize.linprog(C.ravel(), A_eq=np.array(A_eq), b_eq=b_eq, bounds=(0, None))
         assert oracle.emd(C) == pytest.approx(lp.fun, rel=1e-9)
+
+
+def test_wasserstein_lags_is_sorted_matching():
+    """Lags only: the cost |x_a - y_b| on the line makes the EMD with uniform weights the 1-D W1 distance
+    of the two empirical distributions (scipy.stats.wasserstein_distance, sorted matching)."""
+    from scipy.stats import wasserstein_distance
+    M, Mp, o = wasserstein_task(1, 40, 6, 7, 4)
+    for i in range(6):
+        inds, d = oracle.distances(M, Mp, o, i)
+        ref = np.array([wasserstein_distance(M["data"][j], Mp["data"][i]) for j in range(40)])
+        np.testing.assert_array_equal(inds, np.arange(40))
+        np.testing.assert_allclose(d, ref, rtol=1e-12)
+
+
+def test_wasserstein_skipped_missing_columns():
+    """missingdistance == 0: columns with a MISSING entry are dropped, so len_i != len_j; the value is
+    the W1 distance of the remaining values (unequal sample sizes)."""
+    from scipy.stats import wasserstein_distance
+    M, Mp, o = wasserstein_task(2, 60, 6, 6, 4, missing=0.25)
+    for i in range(6):
+        inds, d = oracle.distances(M, Mp, o, i)
+        y = Mp["data"][i][Mp["data"][i] != MISSING]
+        exp_i, exp_d = [], []
+        for j in range(60):
+            x = M["data"][j][M["data"][j] != MISSING]
+            if x.size and y.size:
+                exp_i.append(j)
+                exp_d.append(wasserstein_distance(x, y))
+        np.testing.assert_array_equal(inds, exp_i)
+        np.testing.assert_allclose(d, exp_d, rtol=1e-12)
+
+
+def test_wasserstein_constant_terms():
+    """Unlagged extras, the panel penalty and a missingdistance column add constants: uniform marginals
+    carry unit mass, so each constant shifts the EMD by itself."""
+    from scipy.stats import wasserstein_distance
+    E = 5
+    M, Mp, o = wasserstein_task(3, 50, 5, E, 4, unlagged=2, panels=2, idw=0.35)
+    X, Y = M["data"], Mp["data"]
+    for i in range(5):
+        inds, d = oracle.distances(M, Mp, o, i)
+        ref = np.array([wasserstein_distance(X[j, :E], Y[i, :E]) + np.abs(X[j, E:] - Y[i, E:]).sum()
+                        + 0.35 * (M["panel"][j] != Mp["panel"][i]) for j in range(50)])
+        np.testing.assert_allclose(d, ref, rtol=1e-12)
+    # a MISSING column with missingdistance m: that column costs m against every column, so the optimum
+    # pairs it with the column that is left over
+    M, Mp, o = wasserstein_task(4, 1, 1, 2, 1, missing_distance=0.8)
+    M["data"][0] = [0.1, MISSING]
+    Mp["data"][0] = [0.3, 0.9]
+    inds, d = oracle.distances(M, Mp, o, 0)
+    C = np.array([[0.2, 0.8], [0.8, 0.8]])
+    r, c = scipy.optimize.linear_sum_assignment(C)
+    assert d[0] == pytest.approx(C[r, c].sum() / 2, rel=1e-14)
+
+
+def test_wasserstein_categorical_row_and_skip_other():
+    """A CheckSame extra row adds [a != b] to cell (a, b); with idw < 0 other panels are skipped."""
+    E = 3
+    M, Mp, o = wasserstein_task(5, 30, 4, E, 4, lagged=1, categorical=True, panels=3, idw=-1.0)
+    X, Y = M["data"], Mp["data"]
+    for i in range(4):
+        inds, d = oracle.distances(M, Mp, o, i)
+        same = np.nonzero(M["panel"] == Mp["panel"][i])[0]
+        exp_i, exp_d = [], []
+        for j in same:
+            Cm = np.abs(X[j, :E][:, None] - Y[i, :E][None, :]) + (X[j, E:2 * E][:, None] != Y[i, E:2 * E][None, :])
+            r, c = scipy.optimize.linear_sum_assignment(Cm)
+            v = Cm[r, c].sum() / E
+            if v != 0:
+                exp_i.append(j)
+                exp_d.append(v)
+        np.testing.assert_array_equal(inds, exp_i)
+        np.testing.assert_allclose(d, exp_d, rtol=1e-12)
+
+
+def test_wasserstein_dt_row_scaled_by_aspect():
+    """dt row: cell (a, b) adds gamma |dt_a - dt_b|, gamma = aspect (max x - min x + 1e-6)/(max dt + 1e-6)
+    of the library point (src/distances.cpp:344-366)."""
+    E = 4
+    M, Mp, o = wasserstein_task(6, 20, 3, E, 4, dt=True, aspect_ratio=0.6)
+    X, Y = M["data"], Mp["data"]
+    for i in range(3):
+        _, d = oracle.distances(M, Mp, o, i)
+        for j in range(20):
+            x, tx, y, ty = X[j, :E], X[j, E:], Y[i, :E], Y[i, E:]
+            g = 0.6 * (x.max() - x.min() + 1e-6) / (tx.max() + 1e-6)
+            Cm = np.abs(x[:, None] - y[None, :]) + g * np.abs(tx[:, None] - ty[None, :])
+            r, c = scipy.optimize.linear_sum_assignment(Cm)
+            assert d[j] == pytest.approx(Cm[r, c].sum() / E, rel=1e-12)
 
 
 def command(t, x, **kw):
