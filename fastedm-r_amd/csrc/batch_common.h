@@ -1,0 +1,83 @@
+// Parameter blocks of the batched launches (direct kernel, rank-table build and scan), shared by the
+// host dispatcher and the kernels.
+#pragma once
+#include "kernels_common.h"
+
+namespace fedm {
+
+struct BatchParams
+{
+  const double* L;      // library-side point table [NLrows x Ea] (FP64)
+  const float* L32;     // FP32 copy of L (MIXED)
+  const double* Ltgt;   // library-side targets [NLrows]
+  const double* P;      // prediction-side point table [NProws x Ea]
+  const int* lib_begin; // [tasks]
+  const int* lib_count;
+  const int* lib_rows;
+  const int* pred_begin;
+  const int* pred_count;
+  const int* pred_rows;
+  const int* k;          // [tasks]
+  const int64_t* out_begin;
+  const int2* items;     // (task, first target)
+  int num_items;
+  int* counter;          // work-queue head
+  const double* thetas;
+  int T, Ea, force_compute;
+  double c1, c2, c3;     // mixed-precision error bound B(D) = c1 sqrt(D) + c2 D + c3
+  double* pred;
+  int* rc;
+  int* k_used;
+  int2* redo;            // (task, target) whose squared-distance ranking may differ from the sqrt ranking
+  int* redo_count;       // entries appended (may exceed redo_cap: then every target is redone)
+  int redo_cap;
+};
+
+constexpr int RANK_STORE = 1024; // longest stored prefix
+constexpr int RANK_ZCAP = 8;     // rows at distance 0 kept per target
+
+struct RankParams
+{
+  const double* L;   // point table [rows x Ea] FP64
+  const float* L32;  // FP32 copy
+  const double* P;   // prediction-side table
+  const int* pool;   // pool rows (ascending) [npool]
+  int npool;
+  const int* pred;   // prediction rows [np]
+  int np, Ea;
+  double c1, c2, c3; // FP32 screening bound
+  int* rank_row;     // [RANK_STORE x np] table rows, target-minor
+  double* rank_s;    // [RANK_STORE x np]
+  int* rank_len;     // [np]
+  int* zrow;         // [RANK_ZCAP x np]
+  int* zlen;         // [np]: count, or -1 when more than RANK_ZCAP rows are at distance 0
+};
+
+struct ScanParams
+{
+  const double* L;       // point table FP64
+  const double* Ltgt;    // library-side targets by row
+  const double* P;
+  const int* pred;       // prediction rows of the shared list
+  int np, Ea, T, force_compute, mask_words;
+  const unsigned* masks; // [tasks x mask_words] library membership by table row
+  const int* lib_begin;
+  const int* lib_count;
+  const int* lib_rows;
+  const int* k;
+  const int64_t* out_begin;
+  const int2* items;     // (task, first target)
+  int num_items;
+  int* counter;
+  const int* rank_row;
+  const double* rank_s;
+  const int* rank_len;
+  const int* zrow;
+  const int* zlen;
+  const double* thetas;
+  double* pred_out;
+  int* rc;
+  int* k_used;
+};
+
+} // namespace fedm

@@ -12,7 +12,7 @@
 #include <vector>
 
 #include "../../include/fastedm_hip.h"
-#include "kernels_common.h"
+#include "batch_common.h"
 
 namespace fedm {
 
@@ -73,6 +73,7 @@ struct edm_ctx
   std::vector<DevBuf> bufs;
   double timings[4] = { 0, 0, 0, 0 };
   int last_error = 0;
+  int last_redo = 0;
 
   void* get(int slot, size_t bytes)
   {
@@ -321,6 +322,11 @@ void edm_ctx_timings(edm_ctx* c, double* ms4)
 {
   for (int i = 0; i < 4; i++)
     ms4[i] = c->timings[i];
+}
+
+int edm_ctx_exact_redos(const edm_ctx* c)
+{
+  return c->last_redo;
 }
 
 static float ev_ms(hipEvent_t a, hipEvent_t b)
@@ -712,77 +718,11 @@ int edm_predict(edm_ctx* c, const edm_manifold_view* M, const edm_manifold_view*
 // Batches of tasks over shared point tables.
 
 namespace fedm {
-struct BatchParams
-{
-  const double* L;
-  const float* L32;
-  const double* Ltgt;
-  const double* P;
-  const int* lib_begin;
-  const int* lib_count;
-  const int* lib_rows;
-  const int* pred_begin;
-  const int* pred_count;
-  const int* pred_rows;
-  const int* k;
-  const int64_t* out_begin;
-  const int2* items;
-  int num_items;
-  int* counter;
-  const double* thetas;
-  int T, Ea, force_compute;
-  double c1, c2, c3;
-  double* pred;
-  int* rc;
-  int* k_used;
-};
 template <int KCAP, int EMAX, bool MIXED>
 __global__ void batch_simplex(BatchParams B);
+template <int KCAP>
+__global__ void batch_redo(BatchParams B, int all);
 
-constexpr int RANK_STORE = 1024;
-constexpr int RANK_ZCAP = 8;
-struct RankParams
-{
-  const double* L;
-  const float* L32;
-  const double* P;
-  const int* pool;
-  int npool;
-  const int* pred;
-  int np, Ea;
-  double c1, c2, c3;
-  int* rank_row;
-  double* rank_s;
-  int* rank_len;
-  int* zrow;
-  int* zlen;
-};
-struct ScanParams
-{
-  const double* L;
-  const double* Ltgt;
-  const double* P;
-  const int* pred;
-  int np, Ea, T, force_compute, mask_words;
-  const unsigned* masks;
-  const int* lib_begin;
-  const int* lib_count;
-  const int* lib_rows;
-  const int* k;
-  const int64_t* out_begin;
-  const int2* items;
-  int num_items;
-  int* counter;
-  const int* rank_row;
-  const double* rank_s;
-  const int* rank_len;
-  const int* zrow;
-  const int* zlen;
-  const double* thetas;
-  double* pred_out;
-  int* rc;
-  int* k_used;
-};
 template <int EMAX>
 __global__ void rank_build(RankParams R);
 template <int KCAP, int EMAX>
@@ -807,6 +747,7 @@ struct edm_batch
   size_t scan_lds = 0;
   int rank_tasks = 0;
   float last_ms = 0.f;
+  int redo_last = 0; // targets handed to the exact selection in the last run
   // host copies for the task-by-task route
   edm_manifold_view Lv, Pv;
   edm_options o;
@@ -849,6 +790,15 @@ static const void* batch_kernel_k(int emax, bool mixed)
     case 8: return batch_kernel_e<KCAP, 8>(mixed);
     case 16: return batch_kernel_e<KCAP, 16>(mixed);
     default: return batch_kernel_e<KCAP, 32>(mixed);
+  }
+}
+static const void* batch_redo_kernel(int kcap)
+{
+  switch (kcap) {
+    case 4: return (const void*)batch_redo<4>;
+    case 8: return (const void*)batch_redo<8>;
+    case 16: return (const void*)batch_redo<16>;
+    default: return (const void*)batch_redo<32>;
   }
 }
 static const void* batch_kernel(int kcap, int emax, bool mixed)
@@ -1076,6 +1026,10 @@ int edm_batch_create(edm_ctx* c, const edm_manifold_view* L, const edm_manifold_
     for (int t0 = 0; t0 < pred_count[t]; t0 += 256)
       (is_rank[t] ? ritems : items).push_back(make_int2(t, t0));
   B.counter = b->dev<int>(nullptr, 1);
+  // targets whose squared-distance ranking may differ from the sqrt ranking (rare; see kernels_batch.hip)
+  B.redo_cap = (int)std::min<int64_t>(b->total_k, 1 << 16);
+  B.redo = b->dev<int2>(nullptr, (size_t)std::max(B.redo_cap, 1));
+  B.redo_count = b->dev<int>(nullptr, 1);
   B.thetas = b->dev(o->thetas, (size_t)o->num_thetas);
   B.T = o->num_thetas;
   B.Ea = Ea;
@@ -1265,8 +1219,23 @@ int edm_batch_run(edm_batch* b)
   HIP_TRY(hipEventRecord(c->ev[7], s));
   const void* fn = batch_kernel(b->kcap, b->emax, b->direct_mixed);
   void* args[] = { &B };
-  if (B.num_items > 0)
+  if (B.num_items > 0) {
+    dfill(c, B.redo_count, 1, 0);
     HIP_TRY(hipLaunchKernel(fn, dim3(b->grid), dim3(256), args, 0, s));
+    int all = 0;
+    void* rargs[] = { &B, &all };
+    HIP_TRY(hipLaunchKernel(batch_redo_kernel(b->kcap), dim3(64), dim3(256), rargs, 0, s));
+    int handed = 0;
+    HIP_TRY(hipMemcpyAsync(&handed, B.redo_count, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    if (handed > B.redo_cap) {
+      // more hand-overs than the list holds: repeat every target with the exact selection
+      all = 1;
+      HIP_TRY(hipLaunchKernel(batch_redo_kernel(b->kcap), dim3(b->grid), dim3(256), rargs, 0, s));
+    }
+    b->redo_last = handed;
+    c->last_redo = handed;
+  }
   HIP_TRY(hipEventRecord(c->ev[5], s));
   HIP_TRY(hipEventSynchronize(c->ev[5]));
   b->last_ms = ev_ms(c->ev[4], c->ev[5]);

@@ -11,7 +11,7 @@
 // rank_scan: a task's k nearest neighbours of a target are the first k rows of the target's ranked
 //   prefix that belong to the task's library (a bitmask in LDS).  Targets whose prefix runs out, or
 //   whose library is all-valid (k >= #valid), fall back to a direct exact scan of the library.
-#include "kernels_common.h"
+#include "batch_common.h"
 #include "fdlibm_exp.h"
 
 namespace fedm {
@@ -20,26 +20,6 @@ constexpr int RANK_BINS = 2048;   // FP32 bit-pattern bins: 8 exponent + 3 manti
 constexpr int RANK_TARGET = 600;  // rows wanted below the histogram cut
 constexpr int RANK_CAP = 1536;    // LDS candidate capacity
 constexpr int RANK_SORT = 2048;   // bitonic sort width (power of two >= RANK_CAP)
-constexpr int RANK_STORE = 1024;  // longest stored prefix
-constexpr int RANK_ZCAP = 8;      // rows at distance 0 kept per target
-
-struct RankParams
-{
-  const double* L;   // point table [rows x Ea] FP64
-  const float* L32;  // FP32 copy
-  const double* P;   // prediction-side table
-  const int* pool;   // pool rows (ascending) [npool]
-  int npool;
-  const int* pred;   // prediction rows [np]
-  int np, Ea;
-  double c1, c2, c3; // FP32 screening bound
-  int* rank_row;     // [RANK_STORE x np] table rows, target-minor
-  double* rank_s;    // [RANK_STORE x np]
-  int* rank_len;     // [np]
-  int* zrow;         // [RANK_ZCAP x np]
-  int* zlen;         // [np]: count, or -1 when more than RANK_ZCAP rows are at distance 0
-};
-
 __device__ __forceinline__ float up_float(double x)
 {
   float f = (float)x;
@@ -239,33 +219,6 @@ __global__ __launch_bounds__(256) void rank_build(RankParams R)
   if (threadIdx.x < RANK_ZCAP && threadIdx.x < nzero)
     R.zrow[(size_t)threadIdx.x * R.np + t] = zr[threadIdx.x];
 }
-
-struct ScanParams
-{
-  const double* L;       // point table FP64
-  const double* Ltgt;    // library-side targets by row
-  const double* P;
-  const int* pred;       // prediction rows of the shared list
-  int np, Ea, T, force_compute, mask_words;
-  const unsigned* masks; // [tasks x mask_words] library membership by table row
-  const int* lib_begin;
-  const int* lib_count;
-  const int* lib_rows;
-  const int* k;
-  const int64_t* out_begin;
-  const int2* items;     // (task, first target)
-  int num_items;
-  int* counter;
-  const int* rank_row;
-  const double* rank_s;
-  const int* rank_len;
-  const int* zrow;
-  const int* zlen;
-  const double* thetas;
-  double* pred_out;
-  int* rc;
-  int* k_used;
-};
 
 template <int KCAP>
 __device__ __forceinline__ void insert_sorted(double (&ls)[KCAP], int (&li)[KCAP], double s, int i)

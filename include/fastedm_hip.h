@@ -152,6 +152,11 @@ int edm_selftest_mfma(float* out256);
  * [0] distance/selection kernel, [1] merge/predict kernel, [2] S-map kernel, [3] whole device section. */
 void edm_ctx_timings(edm_ctx* ctx, double* ms4);
 
+/* Diagnostic: how many (task, target) pairs the last batched run on ctx re-selected with the
+ * sqrt-ordered list because two different squared distances near the k-th place could share a
+ * rounded square root (no reference counterpart; the results do not depend on it). */
+int edm_ctx_exact_redos(const edm_ctx* ctx);
+
 /* ---------------------------------------------------------------------------------------------
  * Command level: the reference's run_command (src/r.cpp:107-468), i.e. what the R function edm()
  * (R/edm.R:232-393) calls, with the predictions made on the GPU.

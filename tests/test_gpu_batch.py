@@ -52,6 +52,39 @@ def test_batch_matches_oracle(precision, case):
 
 
 @pytest.mark.parametrize("precision", [0, 1], ids=["plain", "mixed"])
+def test_batch_sqrt_collisions(precision):
+    """Squared distances 1 and 1 + 2^-52 share the rounded square root 1.0: the reference orders them
+    by library position, not by squared distance.  Groups of points around each target put such
+    pairs at the k-th place in both orders of position; the batched route must hand those targets to
+    its sqrt-ordered selection and agree with the oracle."""
+    fe = load_product()
+    offs = np.array([(1.0, 0.0), (1.0, 2.0 ** -26), (-1.0, 0.0), (0.5, 0.0), (2.0, 0.0), (3.0, 0.0),
+                     (0.0, 1.0 + 2.0 ** -52), (0.0, -1.0)])
+    centers = np.array([(10.0 * i, 20.0 * (i % 3)) for i in range(12)])
+    L = {"data": np.concatenate([c + offs for c in centers]), "targets": np.arange(96, dtype=float) / 7.0}
+    P = {"data": centers.copy()}
+    rng = np.random.default_rng(11)
+    libs, preds, ks = [], [], []
+    for k in (1, 2, 3, 4, 5):
+        for rep in range(4):
+            order = np.concatenate([g * 8 + rng.permutation(8) for g in range(12)])
+            libs.append(order.astype(np.int32))
+            preds.append(np.arange(12))
+            ks.append(k)
+    opts = {"algorithm": 0, "k": 1, "thetas": np.asarray([1.0, 3.0]), "precision": precision}
+    got = fe.predict_batch(L, P, opts, libs, preds, ks)
+    assert fe.exact_redos() > 0
+    for t, (lib, pr) in enumerate(zip(libs, preds)):
+        M = {"data": L["data"][lib], "targets": L["targets"][lib]}
+        Mp = {"data": P["data"][pr]}
+        rp, rr, _, rk, _ = oracle.predict(M, Mp, dict(opts, k=ks[t]))
+        gp, gr, gk = got[t]
+        np.testing.assert_array_equal(gr, rr)
+        np.testing.assert_array_equal(gk, rk)
+        assert_predictions_close(gp, rp)
+
+
+@pytest.mark.parametrize("precision", [0, 1], ids=["plain", "mixed"])
 def test_ccm_command_plan_matches_oracle(precision):
     """edm(t, y, x, library=..., numReps=...) through the plan route vs the oracle's run_command."""
     fe = load_product()
