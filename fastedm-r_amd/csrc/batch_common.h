@@ -46,11 +46,12 @@ struct RankParams
   const int* pred;   // prediction rows [np]
   int np, Ea;
   double c1, c2, c3; // FP32 screening bound
-  int* rank_row;     // [RANK_STORE x np] table rows, target-minor
-  double* rank_s;    // [RANK_STORE x np]
+  int* rank_row;     // entry j of target t at t * tstride + j * jstride (table rows)
+  double* rank_s;    // same layout
   int* rank_len;     // [np]
   int* zrow;         // [RANK_ZCAP x np]
   int* zlen;         // [np]: count, or -1 when more than RANK_ZCAP rows are at distance 0
+  int tstride, jstride;
 };
 
 struct ScanParams
@@ -74,10 +75,21 @@ struct ScanParams
   const int* rank_len;
   const int* zrow;
   const int* zlen;
+  int tstride, jstride;  // rank table layout
   const double* thetas;
   double* pred_out;
   int* rc;
   int* k_used;
+  // one pass per target for every rank task (rank_scan_multi)
+  const unsigned* member;    // [table rows x words]: bit b of word w set when rank task 32 w + b has the row
+  int words;
+  int parts;                 // wavefronts per target, each with a contiguous range of task words
+  const int* rank_task;      // [num_rank] task of each rank task
+  int num_rank;
+  double* stage;             // [T x np x num_rank] predictions, target-major (rank_scatter moves them)
+  int2* redo;                // (task, target) pairs left to the exact selection
+  int* redo_count;
+  int redo_cap;
 };
 
 } // namespace fedm

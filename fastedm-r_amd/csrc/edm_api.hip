@@ -721,10 +721,13 @@ namespace fedm {
 template <int KCAP, int EMAX, bool MIXED>
 __global__ void batch_simplex(BatchParams B);
 template <int KCAP>
-__global__ void batch_redo(BatchParams B, int all);
+__global__ void batch_redo(BatchParams B, const int2* items, int num_items);
 
 template <int EMAX>
 __global__ void rank_build(RankParams R);
+template <int KCAP, int WPL>
+__global__ void rank_scan_multi(ScanParams S);
+__global__ void rank_scatter(ScanParams S);
 template <int KCAP, int EMAX>
 __global__ void rank_scan(ScanParams S);
 } // namespace fedm
@@ -746,6 +749,8 @@ struct edm_batch
   int scan_grid = 0;
   size_t scan_lds = 0;
   int rank_tasks = 0;
+  bool multi_scan = false; // one pass per target over every rank task (rank_scan_multi)
+  size_t multi_lds = 0;
   float last_ms = 0.f;
   int redo_last = 0; // targets handed to the exact selection in the last run
   // host copies for the task-by-task route
@@ -790,6 +795,21 @@ static const void* batch_kernel_k(int emax, bool mixed)
     case 8: return batch_kernel_e<KCAP, 8>(mixed);
     case 16: return batch_kernel_e<KCAP, 16>(mixed);
     default: return batch_kernel_e<KCAP, 32>(mixed);
+  }
+}
+template <int KCAP>
+static const void* rank_scan_multi_k(int words)
+{
+  return words <= 64 ? (const void*)rank_scan_multi<KCAP, 1>
+                     : words <= 128 ? (const void*)rank_scan_multi<KCAP, 2> : (const void*)rank_scan_multi<KCAP, 4>;
+}
+static const void* rank_scan_multi_kernel(int kcap, int words)
+{
+  switch (kcap) {
+    case 4: return rank_scan_multi_k<4>(words);
+    case 8: return rank_scan_multi_k<8>(words);
+    case 16: return rank_scan_multi_k<16>(words);
+    default: return rank_scan_multi_k<32>(words);
   }
 }
 static const void* batch_redo_kernel(int kcap)
@@ -998,10 +1018,19 @@ int edm_batch_create(edm_ctx* c, const edm_manifold_view* L, const edm_manifold_
     const int lthr = rank_threshold(kmax, (int)pool.size());
     double work = 0;
     int nrank = 0;
+    // library positions must follow table rows (ties between equal distances go by position)
+    auto ascending = [&](int t) {
+      for (int i = 1; i < lib_count[t]; i++)
+        if (lib_rows[lib_begin[t] + i] <= lib_rows[lib_begin[t] + i - 1])
+          return false;
+      return true;
+    };
+    std::vector<int> eligible;
     for (int t : members)
-      if (lib_count[t] >= lthr) {
+      if (lib_count[t] >= lthr && ascending(t)) {
         work += lib_count[t];
         nrank++;
+        eligible.push_back(t);
       }
     const char* env = getenv("FEDM_RANK_ROUTE");
     const bool allowed = !(env && env[0] == '0');
@@ -1010,9 +1039,8 @@ int edm_batch_create(edm_ctx* c, const edm_manifold_view* L, const edm_manifold_
       b->rank_route = true;
       rank_pb = key.first;
       rank_pc = key.second;
-      for (int t : members)
-        if (lib_count[t] >= lthr)
-          is_rank[t] = 1;
+      for (int t : eligible)
+        is_rank[t] = 1;
       b->rank_tasks = nrank;
     }
   }
@@ -1073,6 +1101,25 @@ int edm_batch_create(edm_ctx* c, const edm_manifold_view* L, const edm_manifold_
     R.c1 = B.c1;
     R.c2 = B.c2;
     R.c3 = B.c3;
+    // one pass per target over every rank task needs a hit list of KCAP prefix positions per
+    // (target, task); beyond 4 GiB of it the per-(task, target) scan is used
+    std::vector<int> rank_task;
+    for (int t = 0; t < num_tasks; t++)
+      if (is_rank[t])
+        rank_task.push_back(t);
+    // per wavefront: task words and KCAP hit positions per task (LDS); the task words of a target are
+    // split over `parts` wavefronts so that one holds at most 12 KiB.  The predictions are staged
+    // target-major ([T x targets x rank tasks] doubles, at most 4 GiB).
+    const size_t words = (rank_task.size() + 31) / 32;
+    const size_t per_word = sizeof(unsigned) * (1 + 16 * (size_t)kcap);
+    const size_t stage_bytes = sizeof(double) * (size_t)B.T * rank_pc * rank_task.size();
+    int parts = (int)std::max<size_t>(1, (words * per_word + 12 * 1024 - 1) / (12 * 1024));
+    parts = (int)std::min<size_t>(parts, words);
+    const size_t wp = (words + parts - 1) / parts;
+    const size_t wave_lds = wp * per_word;
+    b->multi_scan = rank_task.size() <= 8192 && wave_lds * 4 <= 128 * 1024 && stage_bytes <= ((size_t)4 << 30);
+    R.tstride = b->multi_scan ? RANK_STORE : 1;
+    R.jstride = b->multi_scan ? 1 : rank_pc;
     R.rank_row = b->dev<int>(nullptr, (size_t)RANK_STORE * rank_pc);
     R.rank_s = b->dev<double>(nullptr, (size_t)RANK_STORE * rank_pc);
     R.rank_len = b->dev<int>(nullptr, (size_t)rank_pc);
@@ -1111,11 +1158,36 @@ int edm_batch_create(edm_ctx* c, const edm_manifold_view* L, const edm_manifold_
     S.rank_len = R.rank_len;
     S.zrow = R.zrow;
     S.zlen = R.zlen;
+    S.tstride = R.tstride;
+    S.jstride = R.jstride;
     S.thetas = B.thetas;
     S.pred_out = B.pred;
     S.rc = B.rc;
     S.k_used = B.k_used;
+    S.redo = B.redo;
+    S.redo_count = B.redo_count;
+    S.redo_cap = B.redo_cap;
     b->scan_lds = sizeof(unsigned) * (size_t)S.mask_words;
+    if (b->multi_scan) {
+      // membership of table rows in the rank tasks' libraries, one bit per rank task
+      S.num_rank = (int)rank_task.size();
+      S.words = (S.num_rank + 31) / 32;
+      S.parts = parts;
+      S.stage = b->dev<double>(nullptr, stage_bytes / sizeof(double));
+      std::vector<unsigned> member((size_t)L->num_points * S.words, 0u);
+      for (int r = 0; r < S.num_rank; r++) {
+        const int t = rank_task[r];
+        for (int i = 0; i < lib_count[t]; i++)
+          member[(size_t)lib_rows[lib_begin[t] + i] * S.words + (r >> 5)] |= 1u << (r & 31);
+      }
+      S.member = b->dev(member.data(), member.size());
+      S.rank_task = b->dev(rank_task.data(), rank_task.size());
+      b->multi_lds = wave_lds * 4; // 4 wavefronts per block
+      if (b->multi_lds > 48 * 1024)
+        hipFuncSetAttribute(rank_scan_multi_kernel(kcap, S.words), hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)b->multi_lds);
+      hipStreamSynchronize(c->stream);
+    }
     hipStreamSynchronize(c->stream);
   }
   for (void* p : b->owned)
@@ -1203,6 +1275,7 @@ int edm_batch_run(edm_batch* b)
   dfill(c, B.pred, (size_t)b->total_out, FEDM_MISSING_D);
   dfill(c, B.rc, (size_t)b->total_out, (int)FEDM_UNKNOWN_ERROR);
   dfill(c, B.k_used, (size_t)b->total_k, (int)FEDM_MISSING_I);
+  dfill(c, B.redo_count, 1, 0);
   HIP_TRY(hipEventRecord(c->ev[4], s));
   if (b->rank_route) {
     dfill(c, b->S.counter, 1, 0);
@@ -1210,7 +1283,13 @@ int edm_batch_run(edm_batch* b)
     HIP_TRY(hipLaunchKernel(rank_build_kernel(b->emax), dim3(b->R.np), dim3(256), rargs, 0, s));
     HIP_TRY(hipEventRecord(c->ev[6], s));
     void* sargs[] = { &b->S };
-    if (b->S.num_items > 0)
+    if (b->multi_scan) {
+      HIP_TRY(hipLaunchKernel(rank_scan_multi_kernel(b->kcap, b->S.words), dim3(((size_t)b->S.np * b->S.parts + 3) / 4),
+                              dim3(256), sargs, b->multi_lds, s));
+      HIP_TRY(hipLaunchKernel((const void*)rank_scatter, dim3((b->S.num_rank + 63) / 64, (b->S.np + 63) / 64, b->S.T),
+                              dim3(256), sargs, 0, s));
+    }
+    else if (b->S.num_items > 0)
       HIP_TRY(hipLaunchKernel(rank_scan_kernel(b->kcap, b->emax), dim3(b->scan_grid), dim3(256), sargs,
                               b->scan_lds, s));
   } else {
@@ -1219,19 +1298,26 @@ int edm_batch_run(edm_batch* b)
   HIP_TRY(hipEventRecord(c->ev[7], s));
   const void* fn = batch_kernel(b->kcap, b->emax, b->direct_mixed);
   void* args[] = { &B };
-  if (B.num_items > 0) {
-    dfill(c, B.redo_count, 1, 0);
+  if (B.num_items > 0)
     HIP_TRY(hipLaunchKernel(fn, dim3(b->grid), dim3(256), args, 0, s));
-    int all = 0;
-    void* rargs[] = { &B, &all };
+  {
+    // targets handed over by the direct kernel or the rank scan: exact selection
+    const int2* none = nullptr;
+    int zero = 0;
+    void* rargs[] = { &B, &none, &zero };
     HIP_TRY(hipLaunchKernel(batch_redo_kernel(b->kcap), dim3(64), dim3(256), rargs, 0, s));
     int handed = 0;
     HIP_TRY(hipMemcpyAsync(&handed, B.redo_count, sizeof(int), hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
     if (handed > B.redo_cap) {
       // more hand-overs than the list holds: repeat every target with the exact selection
-      all = 1;
-      HIP_TRY(hipLaunchKernel(batch_redo_kernel(b->kcap), dim3(b->grid), dim3(256), rargs, 0, s));
+      const int2* items[2] = { B.items, b->rank_route ? b->S.items : nullptr };
+      int counts[2] = { B.num_items, b->rank_route ? b->S.num_items : 0 };
+      for (int q = 0; q < 2; q++)
+        if (counts[q] > 0) {
+          void* aargs[] = { &B, &items[q], &counts[q] };
+          HIP_TRY(hipLaunchKernel(batch_redo_kernel(b->kcap), dim3(b->grid), dim3(256), aargs, 0, s));
+        }
     }
     b->redo_last = handed;
     c->last_redo = handed;

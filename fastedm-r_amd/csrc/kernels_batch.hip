@@ -343,14 +343,14 @@ __device__ void exact_target(const BatchParams& B, int tk, int t)
   finish_target<KCAP>(B, tk, t, np, lrows, nl, zeros, k, ls, li);
 }
 
-// all == 0: the (task, target) pairs batch_simplex handed over (at most redo_cap of them);
-// all != 0: every target of every item (used when more than redo_cap were handed over).
+// items == null: the (task, target) pairs handed over (at most redo_cap of them);
+// otherwise every target of the given (task, first target) items (used when more were handed over).
 template <int KCAP>
-__global__ __launch_bounds__(256) void batch_redo(BatchParams B, int all)
+__global__ __launch_bounds__(256) void batch_redo(BatchParams B, const int2* items, int num_items)
 {
-  if (all) {
-    for (int item = blockIdx.x; item < B.num_items; item += gridDim.x) {
-      const int2 it = B.items[item];
+  if (items) {
+    for (int item = blockIdx.x; item < num_items; item += gridDim.x) {
+      const int2 it = items[item];
       const int t = it.y + threadIdx.x;
       if (t < B.pred_count[it.x])
         exact_target<KCAP>(B, it.x, t);
@@ -369,7 +369,7 @@ __global__ __launch_bounds__(256) void batch_redo(BatchParams B, int all)
   template __global__ void batch_simplex<KC, EM, true>(BatchParams);
 #define FEDM_BATCH_E(KC)                                                                                         \
   FEDM_BATCH(KC, 4) FEDM_BATCH(KC, 8) FEDM_BATCH(KC, 16) FEDM_BATCH(KC, 32)                                      \
-  template __global__ void batch_redo<KC>(BatchParams, int);
+  template __global__ void batch_redo<KC>(BatchParams, const int2*, int);
 FEDM_BATCH_E(4)
 FEDM_BATCH_E(8)
 FEDM_BATCH_E(16)

@@ -209,8 +209,8 @@ __global__ __launch_bounds__(256) void rank_build(RankParams R)
   __syncthreads();
   kept = ncand;
   for (int c = threadIdx.x; c < kept; c += blockDim.x) {
-    R.rank_row[(size_t)c * R.np + t] = cq[c];
-    R.rank_s[(size_t)c * R.np + t] = cs[c];
+    R.rank_row[(size_t)t * R.tstride + (size_t)c * R.jstride] = cq[c];
+    R.rank_s[(size_t)t * R.tstride + (size_t)c * R.jstride] = cs[c];
   }
   if (threadIdx.x == 0) {
     R.rank_len[t] = kept;
@@ -281,8 +281,9 @@ __global__ __launch_bounds__(256) void rank_scan(ScanParams S)
     bool direct = zl < 0 || k >= cnt;
     if (!direct) {
       const int len = S.rank_len[t];
-      const int* rr = S.rank_row + t;
-      const size_t st = (size_t)S.np;
+      const int* rr = S.rank_row + (size_t)t * S.tstride;
+      const double* rs = S.rank_s + (size_t)t * S.tstride;
+      const size_t st = (size_t)S.jstride;
       // four prefix entries per iteration keep several loads in flight
       int j = 0;
       for (; j + 4 <= len && found < k; j += 4) {
@@ -292,7 +293,7 @@ __global__ __launch_bounds__(256) void rank_scan(ScanParams S)
 #pragma unroll
         for (int u = 0; u < 4; u++) {
           if (found < k && inlib(rows4[u])) {
-            const double s = S.rank_s[(size_t)(j + u) * st + t];
+            const double s = rs[(size_t)(j + u) * st];
 #pragma unroll
             for (int q = 0; q < KCAP; q++)
               if (q == found) {
@@ -306,7 +307,7 @@ __global__ __launch_bounds__(256) void rank_scan(ScanParams S)
       for (; j < len && found < k; j++) {
         const int row = rr[(size_t)j * st];
         if (inlib(row)) {
-          const double s = S.rank_s[(size_t)j * st + t];
+          const double s = rs[(size_t)j * st];
 #pragma unroll
           for (int q = 0; q < KCAP; q++)
             if (q == found) {
@@ -425,6 +426,274 @@ FEDM_RANK_B(4)
 FEDM_RANK_B(8)
 FEDM_RANK_B(16)
 FEDM_RANK_B(32)
+// One pass per target over its ranked prefix for every rank task at once.  A wavefront owns one
+// target; lane l owns the task words l, l + 64, ... of the membership bitmask (WPL words per lane), so
+// each task is handled by one lane only.  For prefix entry j (table row r, in (sqrt distance, row)
+// order) the lane ANDs row r's membership words with its still-searching tasks; every task hit records
+// j, and a task's k-th hit completes it.  Rank tasks have more library rows than k + RANK_ZCAP, so
+// k < #valid and the k-NN list is the first k members.  The predictions (src/edm.cpp:707-735) are
+// made after the scan, one task per lane at a time.  Tasks still searching when the prefix ends, and
+// every task of a target without an exact prefix, go to the exact selection.
+__device__ __forceinline__ void hand_over(const ScanParams& S, int task, int t)
+{
+  const int r = atomicAdd(S.redo_count, 1);
+  if (r < S.redo_cap)
+    S.redo[r] = make_int2(task, t);
+}
+
+// Makes one lane's LDS and global writes visible to the other lanes of its wavefront.
+__device__ __forceinline__ void ms_wave_sync()
+{
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+constexpr int MS_WAVES = 4;
+
+// S.parts wavefronts share a target, each taking a contiguous range of task words.  Per lane and task
+// word the hit counts of its 32 tasks are kept as NP bit planes in registers (bit b of plane p = bit p of
+// task b's count), so that counting the hits of one prefix entry and finding the tasks it completes are
+// a few bitwise operations; only the prefix position of each hit is stored one by one.  Dynamic LDS per
+// wavefront: task words [wp] (unsigned, the still-searching tasks after the scan), then per task KCAP
+// prefix positions ([wp * 32 * KCAP] unsigned short), wp = ceil(words / parts).  The predictions go to
+// S.stage, target-major, and rank_scatter moves them to the tasks' outputs.
+template <int KCAP>
+struct Planes
+{
+  static constexpr int NP = KCAP < 4 ? 2 : KCAP < 8 ? 3 : KCAP < 16 ? 4 : KCAP < 32 ? 5 : 6; // counts 0..KCAP
+};
+
+template <int KCAP, int WPL>
+__global__ __launch_bounds__(64 * MS_WAVES) void rank_scan_multi(ScanParams S)
+{
+  constexpr int NP = Planes<KCAP>::NP;
+  extern __shared__ unsigned ms_lds[];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int gw = blockIdx.x * MS_WAVES + w;
+  const int t = gw / S.parts;
+  if (t >= S.np)
+    return;
+  const int wp = (S.words + S.parts - 1) / S.parts;
+  const int w0 = (gw % S.parts) * wp;                // first task word of this wavefront
+  const int W = max(0, min(S.words - w0, wp));       // its task words
+  const int tau_lo = w0 * 32;                        // first task of this wavefront
+  const int ntask = max(0, min(S.num_rank - tau_lo, W * 32));
+  unsigned* act_sh = ms_lds + (size_t)w * (wp + wp * 16 * KCAP);
+  unsigned short* hits = (unsigned short*)(act_sh + wp);
+
+  unsigned act[WPL], cnt[WPL][NP], km[WPL][NP];
+  int live = 0;
+#pragma unroll
+  for (int q = 0; q < WPL; q++) {
+    const int wd = lane + 64 * q;
+    act[q] = 0u;
+#pragma unroll
+    for (int p = 0; p < NP; p++) {
+      cnt[q][p] = 0u;
+      km[q][p] = 0u;
+    }
+    if (wd < W) {
+      const int lo = wd * 32;
+      const int nb = min(32, ntask - lo);
+      act[q] = nb >= 32 ? ~0u : ((1u << nb) - 1u);
+      for (int b = 0; b < nb; b++) {
+        const unsigned km1 = (unsigned)(S.k[S.rank_task[tau_lo + lo + b]] - 1); // completing count
+#pragma unroll
+        for (int p = 0; p < NP; p++)
+          km[q][p] |= ((km1 >> p) & 1u) << b;
+      }
+      live += nb;
+    }
+  }
+  const int len = S.rank_len[t];
+  if (S.zlen[t] < 0 || len == 0) {
+#pragma unroll
+    for (int q = 0; q < WPL; q++)
+      for (unsigned m = act[q]; m; m &= m - 1)
+        hand_over(S, S.rank_task[tau_lo + (lane + 64 * q) * 32 + __ffs(m) - 1], t);
+    return;
+  }
+  const int* rr = S.rank_row + (size_t)t * S.tstride;
+  constexpr int G = 8; // prefix entries whose membership words are loaded together
+  for (int c0 = 0; c0 < len; c0 += 64) {
+    // stop once every task of every lane has its k neighbours
+    if (!__ballot(live > 0))
+      break;
+    const int n = min(64, len - c0);
+    const int row_l = lane < n ? rr[(size_t)(c0 + lane) * S.jstride] : 0;
+    unsigned mg[G][WPL];
+    auto load_group = [&](int g0) {
+#pragma unroll
+      for (int u = 0; u < G; u++) {
+        const int i = g0 + u;
+        const unsigned* mrow = S.member + (size_t)__shfl(row_l, i < n ? i : 0) * S.words + w0;
+#pragma unroll
+        for (int q = 0; q < WPL; q++)
+          mg[u][q] = (i < n && lane + 64 * q < W) ? mrow[lane + 64 * q] : 0u;
+      }
+    };
+    load_group(0);
+    for (int g0 = 0; g0 < n; g0 += G) {
+      unsigned cur[G][WPL];
+#pragma unroll
+      for (int u = 0; u < G; u++)
+#pragma unroll
+        for (int q = 0; q < WPL; q++)
+          cur[u][q] = mg[u][q];
+      if (g0 + G < n)
+        load_group(g0 + G);
+#pragma unroll
+      for (int u = 0; u < G; u++) {
+        const unsigned short pos = (unsigned short)(c0 + g0 + u);
+#pragma unroll
+        for (int q = 0; q < WPL; q++) {
+          const unsigned m = cur[u][q] & act[q];
+          if (!m)
+            continue;
+          const int base = (lane + 64 * q) * 32;
+          // record this entry as hit number count(b) of each task b in m
+          for (unsigned mm = m; mm; mm &= mm - 1) {
+            const int b = __ffs(mm) - 1;
+            int c = 0;
+#pragma unroll
+            for (int p = 0; p < NP; p++)
+              c |= (int)((cnt[q][p] >> b) & 1u) << p;
+            hits[(size_t)(base + b) * KCAP + c] = pos;
+          }
+          // tasks of m whose count was k - 1 are complete; the others count one more hit
+          unsigned eq = ~0u;
+#pragma unroll
+          for (int p = 0; p < NP; p++)
+            eq &= ~(cnt[q][p] ^ km[q][p]);
+          const unsigned done = m & eq;
+          unsigned carry = m;
+#pragma unroll
+          for (int p = 0; p < NP; p++) {
+            const unsigned nc = cnt[q][p] & carry;
+            cnt[q][p] ^= carry;
+            carry = nc;
+          }
+          act[q] &= ~done;
+          live -= __popc(done);
+        }
+      }
+    }
+  }
+  // publish the still-searching tasks, then predict the completed ones, U tasks per lane at a time
+#pragma unroll
+  for (int q = 0; q < WPL; q++)
+    if (lane + 64 * q < W)
+      act_sh[lane + 64 * q] = act[q];
+  ms_wave_sync();
+  constexpr int U = KCAP <= 4 ? 4 : KCAP <= 8 ? 2 : 1;
+  const double* rs = S.rank_s + (size_t)t * S.tstride;
+  for (int tau0 = lane * U; tau0 < ntask; tau0 += 64 * U) {
+    int task[U], kq[U];
+    bool open_[U];
+    int row[U][KCAP];
+    double ls[U][KCAP], tg[U][KCAP];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int tau = tau0 + u;
+      const bool in = tau < ntask;
+      task[u] = in ? S.rank_task[tau_lo + tau] : 0;
+      open_[u] = in && ((act_sh[tau >> 5] >> (tau & 31)) & 1u);
+      kq[u] = (in && !open_[u]) ? S.k[task[u]] : 0;
+#pragma unroll
+      for (int j = 0; j < KCAP; j++)
+        row[u][j] = j < kq[u] ? hits[(size_t)tau * KCAP + j] : 0;
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++)
+#pragma unroll
+      for (int j = 0; j < KCAP; j++) {
+        const size_t e = (size_t)row[u][j] * S.jstride;
+        ls[u][j] = j < kq[u] ? rs[e] : 0.0;
+        row[u][j] = j < kq[u] ? rr[e] : 0;
+      }
+#pragma unroll
+    for (int u = 0; u < U; u++)
+#pragma unroll
+      for (int j = 0; j < KCAP; j++)
+        tg[u][j] = j < kq[u] ? S.Ltgt[row[u][j]] : 0.0;
+    // hand-overs, then the U predictions side by side (independent exp and divide chains)
+    double minDist[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      if (tau0 + u < ntask && open_[u])
+        hand_over(S, task[u], t); // prefix exhausted before k members
+      minDist[u] = INF_D;
+#pragma unroll
+      for (int j = 0; j < KCAP; j++)
+        if (j < kq[u])
+          minDist[u] = fmin(minDist[u], ls[u][j]);
+    }
+    for (int th = 0; th < S.T; th++) {
+      const double theta = S.thetas[th];
+      double w[U][KCAP], sumw[U], r[U];
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        sumw[u] = 0.0;
+#pragma unroll
+        for (int j = 0; j < KCAP; j++)
+          if (j < kq[u]) {
+            w[u][j] = fdlibm_exp(-theta * (ls[u][j] / minDist[u]));
+            sumw[u] = sumw[u] + w[u][j];
+          }
+      }
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        r[u] = 0.0;
+#pragma unroll
+        for (int j = 0; j < KCAP; j++)
+          if (j < kq[u])
+            r[u] = r[u] + tg[u][j] * (w[u][j] / sumw[u]);
+      }
+      double* st = S.stage + ((size_t)th * S.np + t) * S.num_rank + tau_lo;
+#pragma unroll
+      for (int u = 0; u < U; u++)
+        if (tau0 + u < ntask)
+          st[tau0 + u] = r[u];
+    }
+  }
+}
+
+// Stage [theta][target][rank task] -> the tasks' outputs (prediction, rc SUCCESS, k used), 64 x 64
+// tiles through LDS so that both sides are read and written in whole rows.  Targets handed over to the
+// exact selection are overwritten afterwards by it.
+__global__ __launch_bounds__(256) void rank_scatter(ScanParams S)
+{
+  __shared__ double tile[64][65];
+  const int tau0 = blockIdx.x * 64, t0 = blockIdx.y * 64, th = blockIdx.z;
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  for (int r = ty; r < 64; r += 4) {
+    const int t = t0 + r, tau = tau0 + tx;
+    tile[r][tx] = (t < S.np && tau < S.num_rank) ? S.stage[((size_t)th * S.np + t) * S.num_rank + tau] : 0.0;
+  }
+  __syncthreads();
+  for (int r = ty; r < 64; r += 4) {
+    const int tau = tau0 + r, t = t0 + tx;
+    if (tau < S.num_rank && t < S.np) {
+      const int task = S.rank_task[tau];
+      const int64_t ob = S.out_begin[task];
+      S.pred_out[ob + (int64_t)th * S.np + t] = tile[tx][r];
+      S.rc[ob + (int64_t)th * S.np + t] = RC_SUCCESS;
+      if (th == 0)
+        S.k_used[ob / S.T + t] = S.k[task];
+    }
+  }
+}
+
+#define FEDM_RANK_M(KC)                                                                                          \
+  template __global__ void rank_scan_multi<KC, 1>(ScanParams);                                                   \
+  template __global__ void rank_scan_multi<KC, 2>(ScanParams);                                                   \
+  template __global__ void rank_scan_multi<KC, 4>(ScanParams);
+FEDM_RANK_M(4)
+FEDM_RANK_M(8)
+FEDM_RANK_M(16)
+FEDM_RANK_M(32)
+
 #define FEDM_RANK_S(KC, EM) template __global__ void rank_scan<KC, EM>(ScanParams);
 #define FEDM_RANK_SE(KC) FEDM_RANK_S(KC, 4) FEDM_RANK_S(KC, 8) FEDM_RANK_S(KC, 16) FEDM_RANK_S(KC, 32)
 FEDM_RANK_SE(4)
