@@ -26,7 +26,6 @@ __global__ void finish_dense(FinishParams F, const double* D, int t0, int nt, co
 __global__ void wasserstein_dense(WassParams P, int t0, int nt, int slice_len, double* D, int* out_cnt);
 __global__ void select_wide(FinishParams F, const double* D, int t0, int nt, unsigned long long* gkey, int* gidx,
                             int gstride);
-template <int NMAX>
 __global__ void smap_f64(SmapParams S, const double* D, int t0, int nt);
 size_t smap_lds_bytes(int n);
 
@@ -266,22 +265,14 @@ static void launch_finish_lists(int kcap, dim3 g, hipStream_t s, const FinishPar
 
 static int launch_smap(int n, dim3 g, hipStream_t s, const SmapParams& S, const double* D, int t0, int nt)
 {
-  size_t lds = smap_lds_bytes(n);
-  if (n <= 8)
-    hipLaunchKernelGGL(smap_f64<8>, g, dim3(64), lds, s, S, D, t0, nt);
-  else if (n <= 16)
-    hipLaunchKernelGGL(smap_f64<16>, g, dim3(64), lds, s, S, D, t0, nt);
-  else if (n <= 32)
-    hipLaunchKernelGGL(smap_f64<32>, g, dim3(64), lds, s, S, D, t0, nt);
-  else if (n <= 64) {
-    static bool attr = [] {
-      hipFuncSetAttribute((const void*)smap_f64<64>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-      return true;
-    }();
-    (void)attr;
-    hipLaunchKernelGGL(smap_f64<64>, g, dim3(64), lds, s, S, D, t0, nt);
-  } else
+  if (n > 65)
     return -22;
+  static bool attr = [] {
+    hipFuncSetAttribute((const void*)smap_f64, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    return true;
+  }();
+  (void)attr;
+  hipLaunchKernelGGL(smap_f64, g, dim3(64), smap_lds_bytes(n), s, S, D, t0, nt);
   return 0;
 }
 

@@ -428,29 +428,32 @@ __global__ __launch_bounds__(256) void finish_dense(FinishParams F, const double
 }
 
 // ----------------------------------------------------------------------------------------------
-// S-map: one wavefront per (target, theta).  Gram matrix in FP64 in the CPU path's summation
-// order, then the JacobiSVD solve in LDS (jacobi_wave.h).
-template <int NMAX>
+// S-map: one wavefront per (target, theta).  Gram matrix in FP64 in the CPU path's summation order
+// (each entry accumulated neighbour by neighbour, in neighbour order), then the JacobiSVD solve in LDS
+// (jacobi_wave.h).
+constexpr int SM_CH = 32; // neighbours staged per Gram chunk
+
 __global__ __launch_bounds__(64) void smap_f64(SmapParams S, const double* __restrict__ D, int t0, int nt)
 {
-  constexpr int NTRI = NMAX * (NMAX + 1) / 2;
-  constexpr int EPL = (NTRI + NMAX + 63) / 64; // Gram + rhs entries per lane
   extern __shared__ double smem[];
   const int Ea = S.Ea, n = Ea + 1;
-  // dynamic LDS: [ bvec, sv, tmp, xsol (n each) | cd[64] | cidx[64] ] then one region used first for the
-  // Gram stage (Xc[64 x n], yc[64]) and afterwards for the SVD (W, U, V [n x n])
+  const int ntri = n * (n + 1) / 2, nacc = ntri + n;
+  // dynamic LDS: bvec, sv, tmp, xsol [n each] | cd [SM_CH] | acc [ntri + n]: the upper triangle of X^T X
+  // and X^T y, packed | W [n x n] | one region used first for the Gram staging (Xc [SM_CH x n],
+  // yc [SM_CH]) and then for U, V [n x n] | cidx [SM_CH] (ints)
   double* bvec = smem;
   double* sv = bvec + n;
   double* tmp = sv + n;
   double* xsol = tmp + n;
   double* cd = xsol + n;
-  int* cidx = (int*)(cd + 64);
-  double* region = (double*)(cidx + 64);
+  double* acc = cd + SM_CH;
+  double* W = acc + nacc;
+  double* region = W + n * n;
   double* Xc = region;
-  double* yc = Xc + 64 * n;
-  double* W = region;
-  double* U = W + n * n;
+  double* yc = Xc + SM_CH * n;
+  double* U = region;
   double* V = U + n * n;
+  int* cidx = (int*)(region + max(SM_CH * n + SM_CH, 2 * n * n));
 
   const int tl = blockIdx.x;
   const int t = t0 + tl;
@@ -483,29 +486,8 @@ __global__ __launch_bounds__(64) void smap_f64(SmapParams S, const double* __res
   const double mean = sum / (double)k;
   const double theta = S.thetas[th];
 
-  // this lane's Gram entries: e < ntri -> (p, q) of the upper triangle, then the rhs rows
-  const int ntri = n * (n + 1) / 2;
-  int ep[EPL], eq[EPL];
-  double acc[EPL];
-#pragma unroll
-  for (int m = 0; m < EPL; m++) {
-    const int e = lane + 64 * m;
-    acc[m] = 0.0;
-    ep[m] = -1;
-    eq[m] = -1;
-    if (e < ntri) {
-      int p = 0, rem = e;
-      while (rem >= n - p) {
-        rem -= n - p;
-        p++;
-      }
-      ep[m] = p;
-      eq[m] = p + rem;
-    } else if (e < ntri + n) {
-      ep[m] = e - ntri;
-      eq[m] = -2;
-    }
-  }
+  for (int e = lane; e < nacc; e += 64)
+    acc[e] = 0.0;
 
   int pos = 0, done = 0;
   while (done < k) {
@@ -514,7 +496,7 @@ __global__ __launch_bounds__(64) void smap_f64(SmapParams S, const double* __res
       // next run of valid entries of the distance row, in library order
       while (cnum == 0 && pos < S.NL) {
         const int i = pos + lane;
-        const bool v = i < S.NL && drow[i] >= 0.0;
+        const bool v = lane < SM_CH && i < S.NL && drow[i] >= 0.0;
         const unsigned long long b = __ballot(v);
         const int before = __popcll(b & ((1ull << lane) - 1ull));
         if (v) {
@@ -522,10 +504,10 @@ __global__ __launch_bounds__(64) void smap_f64(SmapParams S, const double* __res
           cd[before] = drow[i];
         }
         cnum = __popcll(b);
-        pos += 64;
+        pos += SM_CH;
       }
     } else {
-      cnum = min(64, k - done);
+      cnum = min(SM_CH, k - done);
       if (lane < cnum) {
         cidx[lane] = li[done + lane];
         cd[lane] = lsd[done + lane];
@@ -544,32 +526,39 @@ __global__ __launch_bounds__(64) void smap_f64(SmapParams S, const double* __res
       yc[lane] = w * S.targets[ci];
     }
     __syncthreads();
-#pragma unroll
-    for (int m = 0; m < EPL; m++) {
-      if (ep[m] >= 0) {
-        double a = acc[m];
-        if (eq[m] >= 0) {
-          for (int c = 0; c < cnum; c++)
-            a += Xc[c * n + ep[m]] * Xc[c * n + eq[m]];
-        } else {
-          for (int c = 0; c < cnum; c++)
-            a += Xc[c * n + ep[m]] * yc[c];
+    for (int e = lane; e < nacc; e += 64) {
+      double a = acc[e];
+      if (e < ntri) {
+        int p = 0, rem = e;
+        while (rem >= n - p) {
+          rem -= n - p;
+          p++;
         }
-        acc[m] = a;
+        const int q = p + rem;
+        for (int c = 0; c < cnum; c++)
+          a += Xc[c * n + p] * Xc[c * n + q];
+      } else {
+        const int p = e - ntri;
+        for (int c = 0; c < cnum; c++)
+          a += Xc[c * n + p] * yc[c];
       }
+      acc[e] = a;
     }
     __syncthreads();
     done += cnum;
   }
-#pragma unroll
-  for (int m = 0; m < EPL; m++) {
-    if (ep[m] >= 0) {
-      if (eq[m] >= 0) {
-        W[ep[m] * n + eq[m]] = acc[m];
-        W[eq[m] * n + ep[m]] = acc[m];
-      } else {
-        bvec[ep[m]] = acc[m];
+  for (int e = lane; e < nacc; e += 64) {
+    if (e < ntri) {
+      int p = 0, rem = e;
+      while (rem >= n - p) {
+        rem -= n - p;
+        p++;
       }
+      const int q = p + rem;
+      W[p * n + q] = acc[e];
+      W[q * n + p] = acc[e];
+    } else {
+      bvec[e - ntri] = acc[e];
     }
   }
   __syncthreads();
@@ -594,8 +583,9 @@ __global__ __launch_bounds__(64) void smap_f64(SmapParams S, const double* __res
 // LDS bytes smap_f64 needs for an n x n system.
 size_t smap_lds_bytes(int n)
 {
-  const size_t region = std::max((size_t)(64 * n + 64), (size_t)(3 * n * n));
-  return sizeof(double) * (4 * (size_t)n + 64 + region) + sizeof(int) * 64;
+  const size_t nacc = (size_t)n * (n + 1) / 2 + n;
+  const size_t region = std::max((size_t)(SM_CH * n + SM_CH), (size_t)(2 * n * n));
+  return sizeof(double) * (4 * (size_t)n + SM_CH + nacc + (size_t)n * n + region) + sizeof(int) * SM_CH;
 }
 
 // ----------------------------------------------------------------------------------------------
@@ -622,9 +612,6 @@ template __global__ void finish_lists<8>(FinishParams, const double*, const int*
 template __global__ void finish_lists<16>(FinishParams, const double*, const int*, const int*, int);
 template __global__ void finish_lists<32>(FinishParams, const double*, const int*, const int*, int);
 template __global__ void finish_lists<64>(FinishParams, const double*, const int*, const int*, int);
-template __global__ void smap_f64<8>(SmapParams, const double*, int, int);
-template __global__ void smap_f64<16>(SmapParams, const double*, int, int);
-template __global__ void smap_f64<32>(SmapParams, const double*, int, int);
-template __global__ void smap_f64<64>(SmapParams, const double*, int, int);
+
 
 } // namespace fedm
