@@ -6,7 +6,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libfastedm_hip.so")
-SOURCES = ["edm_api.hip", "kernels_plain.hip", "kernels_batch.hip", "kernels_rank.hip", "kernels_mfma.hip", "kernels_select.hip", "kernels_wasserstein.hip", "edm_host.cpp"]
+SOURCES = ["edm_api.hip", "kernels_plain.hip", "kernels_batch.hip", "kernels_rank.hip", "kernels_mfma.hip", "kernels_select.hip", "kernels_wasserstein.hip", "kernels_smap.hip", "edm_host.cpp"]
 # -ffp-contract=off: FP64 products and sums are rounded one by one, as on the CPU path.
 # packed-fp32-ops disabled: no v_pk_* FP32 instructions are generated.
 FLAGS = ["-O3", "-std=c++17", "-fPIC", "-shared", "--offload-arch=gfx950", "-ffp-contract=off",

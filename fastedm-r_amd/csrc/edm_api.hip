@@ -26,7 +26,7 @@ __global__ void finish_dense(FinishParams F, const double* D, int t0, int nt, co
 __global__ void wasserstein_dense(WassParams P, int t0, int nt, int slice_len, double* D, int* out_cnt);
 __global__ void select_wide(FinishParams F, const double* D, int t0, int nt, unsigned long long* gkey, int* gidx,
                             int gstride);
-__global__ void smap_f64(SmapParams S, const double* D, int t0, int nt);
+__global__ void smap_multi(SmapParams S, const double* D, int t0, int nt);
 size_t smap_lds_bytes(int n);
 
 struct MfmaParams
@@ -263,16 +263,26 @@ static void launch_finish_lists(int kcap, dim3 g, hipStream_t s, const FinishPar
   }
 }
 
-static int launch_smap(int n, dim3 g, hipStream_t s, const SmapParams& S, const double* D, int t0, int nt)
+// S-map systems of targets [t0, t0 + nt) at every theta: a persistent grid of one-wavefront blocks.
+static int launch_smap(edm_ctx* c, int n, hipStream_t s, const SmapParams& S, const double* D, int t0, int nt)
 {
   if (n > 65)
     return -22;
-  static bool attr = [] {
-    hipFuncSetAttribute((const void*)smap_f64, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-    return true;
+  static int per_cu = [] {
+    hipFuncSetAttribute((const void*)smap_multi, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    return 0;
   }();
-  (void)attr;
-  hipLaunchKernelGGL(smap_f64, g, dim3(64), smap_lds_bytes(n), s, S, D, t0, nt);
+  (void)per_cu;
+  const size_t lds = smap_lds_bytes(n);
+  const int G = std::max(1, std::min(8, 64 / n));
+  const long long groups = ((long long)nt * S.T + G - 1) / G;
+  int dev = 0, ncu = 0, occ = 0;
+  hipGetDevice(&dev);
+  hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+  hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)smap_multi, 64, lds);
+  const int grid = (int)std::max<long long>(1, std::min<long long>(groups, (long long)ncu * std::max(occ, 1)));
+  (void)c;
+  hipLaunchKernelGGL(smap_multi, dim3(grid), dim3(64), lds, s, S, D, t0, nt);
   return 0;
 }
 
@@ -591,7 +601,7 @@ int edm_predict(edm_ctx* c, const edm_manifold_view* M, const edm_manifold_view*
       Sm.nn_i = F.nn_i;
       Sm.nn_s = F.nn_s;
       Sm.nn_stride = kcap;
-      int r = launch_smap(n, dim3(NP, T), s, Sm, nullptr, 0, NP);
+      int r = launch_smap(c, n, s, Sm, nullptr, 0, NP);
       if (r)
         return r;
       HIP_TRY(hipGetLastError());
@@ -621,7 +631,7 @@ int edm_predict(edm_ctx* c, const edm_manifold_view* M, const edm_manifold_view*
       Sm.nn_i = F.nn_i;
       Sm.nn_s = F.nn_s;
       Sm.nn_stride = kcap;
-      int r = launch_smap(n, dim3(NP, T), s, Sm, nullptr, 0, NP);
+      int r = launch_smap(c, n, s, Sm, nullptr, 0, NP);
       if (r)
         return r;
       HIP_TRY(hipGetLastError());
@@ -671,7 +681,7 @@ int edm_predict(edm_ctx* c, const edm_manifold_view* M, const edm_manifold_view*
       }
       Sm.nn_t0 = t0;
       if (smap) {
-        int r = launch_smap(n, dim3(cnt, T), s, Sm, D, t0, cnt);
+        int r = launch_smap(c, n, s, Sm, D, t0, cnt);
         if (r)
           return r;
         HIP_TRY(hipGetLastError());

@@ -428,166 +428,6 @@ __global__ __launch_bounds__(256) void finish_dense(FinishParams F, const double
 }
 
 // ----------------------------------------------------------------------------------------------
-// S-map: one wavefront per (target, theta).  Gram matrix in FP64 in the CPU path's summation order
-// (each entry accumulated neighbour by neighbour, in neighbour order), then the JacobiSVD solve in LDS
-// (jacobi_wave.h).
-constexpr int SM_CH = 32; // neighbours staged per Gram chunk
-
-__global__ __launch_bounds__(64) void smap_f64(SmapParams S, const double* __restrict__ D, int t0, int nt)
-{
-  extern __shared__ double smem[];
-  const int Ea = S.Ea, n = Ea + 1;
-  const int ntri = n * (n + 1) / 2, nacc = ntri + n;
-  // dynamic LDS: bvec, sv, tmp, xsol [n each] | cd [SM_CH] | acc [ntri + n]: the upper triangle of X^T X
-  // and X^T y, packed | W [n x n] | one region used first for the Gram staging (Xc [SM_CH x n],
-  // yc [SM_CH]) and then for U, V [n x n] | cidx [SM_CH] (ints)
-  double* bvec = smem;
-  double* sv = bvec + n;
-  double* tmp = sv + n;
-  double* xsol = tmp + n;
-  double* cd = xsol + n;
-  double* acc = cd + SM_CH;
-  double* W = acc + nacc;
-  double* region = W + n * n;
-  double* Xc = region;
-  double* yc = Xc + SM_CH * n;
-  double* U = region;
-  double* V = U + n * n;
-  int* cidx = (int*)(region + max(SM_CH * n + SM_CH, 2 * n * n));
-
-  const int tl = blockIdx.x;
-  const int t = t0 + tl;
-  const int th = blockIdx.y;
-  const int lane = threadIdx.x;
-  if (tl >= nt || t >= S.NP)
-    return;
-  const int kk = S.nn_k[t];
-  if (kk == 0)
-    return;
-  const bool dense = kk < 0;
-  const int k = dense ? S.nn_cnt[t] : kk;
-  const double* drow = dense ? D + (size_t)tl * S.NL : nullptr;
-  const int* li = S.nn_i + (size_t)(t - S.nn_t0) * S.nn_stride;
-  const double* lsd = S.nn_s + (size_t)(t - S.nn_t0) * S.nn_stride;
-
-  // mean of the selected distances, summed in neighbour order (Eigen mean = sum / size)
-  double sum = 0.0;
-  if (lane == 0) {
-    if (dense) {
-      for (int i = 0; i < S.NL; i++)
-        if (drow[i] >= 0.0)
-          sum += drow[i];
-    } else {
-      for (int j = 0; j < k; j++)
-        sum += lsd[j];
-    }
-  }
-  sum = __shfl(sum, 0);
-  const double mean = sum / (double)k;
-  const double theta = S.thetas[th];
-
-  for (int e = lane; e < nacc; e += 64)
-    acc[e] = 0.0;
-
-  int pos = 0, done = 0;
-  while (done < k) {
-    int cnum = 0;
-    if (dense) {
-      // next run of valid entries of the distance row, in library order
-      while (cnum == 0 && pos < S.NL) {
-        const int i = pos + lane;
-        const bool v = lane < SM_CH && i < S.NL && drow[i] >= 0.0;
-        const unsigned long long b = __ballot(v);
-        const int before = __popcll(b & ((1ull << lane) - 1ull));
-        if (v) {
-          cidx[before] = i;
-          cd[before] = drow[i];
-        }
-        cnum = __popcll(b);
-        pos += SM_CH;
-      }
-    } else {
-      cnum = min(SM_CH, k - done);
-      if (lane < cnum) {
-        cidx[lane] = li[done + lane];
-        cd[lane] = lsd[done + lane];
-      }
-    }
-    if (cnum == 0)
-      break;
-    __syncthreads();
-    if (lane < cnum) {
-      const int ci = cidx[lane];
-      const double w = fdlibm_exp(-theta * (cd[lane] / mean));
-      const double* row = S.M + (size_t)ci * Ea;
-      Xc[lane * n] = w;
-      for (int j = 1; j < n; j++)
-        Xc[lane * n + j] = w * row[j - 1];
-      yc[lane] = w * S.targets[ci];
-    }
-    __syncthreads();
-    for (int e = lane; e < nacc; e += 64) {
-      double a = acc[e];
-      if (e < ntri) {
-        int p = 0, rem = e;
-        while (rem >= n - p) {
-          rem -= n - p;
-          p++;
-        }
-        const int q = p + rem;
-        for (int c = 0; c < cnum; c++)
-          a += Xc[c * n + p] * Xc[c * n + q];
-      } else {
-        const int p = e - ntri;
-        for (int c = 0; c < cnum; c++)
-          a += Xc[c * n + p] * yc[c];
-      }
-      acc[e] = a;
-    }
-    __syncthreads();
-    done += cnum;
-  }
-  for (int e = lane; e < nacc; e += 64) {
-    if (e < ntri) {
-      int p = 0, rem = e;
-      while (rem >= n - p) {
-        rem -= n - p;
-        p++;
-      }
-      const int q = p + rem;
-      W[p * n + q] = acc[e];
-      W[q * n + p] = acc[e];
-    } else {
-      bvec[e - ntri] = acc[e];
-    }
-  }
-  __syncthreads();
-
-  jacobi_svd_solve_wave(n, W, U, V, bvec, sv, tmp, xsol, lane);
-
-  if (lane == 0) {
-    double r = xsol[0];
-    const double* yp = S.Mp + (size_t)t * Ea;
-    for (int j = 0; j < Ea; j++)
-      if (yp[j] != MISSING_D)
-        r += yp[j] * xsol[j + 1];
-    S.pred[(size_t)th * S.NP + t] = r;
-    S.rc[(size_t)th * S.NP + t] = RC_SUCCESS;
-  }
-  if (S.coeffs && th == S.T - 1 && lane < n) {
-    const double c = xsol[lane];
-    S.coeffs[(size_t)t * n + lane] = fabs(c) < 1.0e-11 ? MISSING_D : c;
-  }
-}
-
-// LDS bytes smap_f64 needs for an n x n system.
-size_t smap_lds_bytes(int n)
-{
-  const size_t nacc = (size_t)n * (n + 1) / 2 + n;
-  const size_t region = std::max((size_t)(SM_CH * n + SM_CH), (size_t)(2 * n * n));
-  return sizeof(double) * (4 * (size_t)n + SM_CH + nacc + (size_t)n * n + region) + sizeof(int) * SM_CH;
-}
-
 // ----------------------------------------------------------------------------------------------
 // Explicit instantiations used by the dispatcher.
 #define FEDM_KNN(KC, EM)                                                                                         \
