@@ -36,17 +36,19 @@ __device__ __forceinline__ float up_f32(double x)
   return f;
 }
 
-constexpr int MCH = 64; // library points per LDS chunk (four 16-point tiles)
-
 template <int KCAP, int KP>
 __global__ __launch_bounds__(256) void knn_mfma(MfmaParams P, int k, int slice_len, double* __restrict__ out_s,
                                                int* __restrict__ out_i, int* __restrict__ out_cnt)
 {
+  constexpr int MCH = KP == 64 ? 64 : 128;  // library points per LDS chunk (16-point tiles)
+  constexpr int F4 = MCH * KP / 4 / 256;    // float4 of the next chunk each thread prefetches
   __shared__ float Bs[MCH * KP];
   __shared__ float Bn[MCH];
   __shared__ float thr_s[64];
   __shared__ int qcnt[64];
   __shared__ unsigned char queue[64 * MCH];
+  __shared__ double Ls[64 * KCAP]; // exact lists of the block's targets, by (sqrt distance, position)
+  __shared__ int Li[64 * KCAP];
 
   const int lane = threadIdx.x & 63;
   const int w = threadIdx.x >> 6;
@@ -75,16 +77,16 @@ __global__ __launch_bounds__(256) void knn_mfma(MfmaParams P, int k, int slice_l
     const int t = tbase + 4 * g + r;
     ny[r] = t < P.NP ? P.Pn[t] : 0.f;
   }
-  // owner lanes 0..15 keep the exact lists of targets tbase + lane
+  // owner lanes 0..15 keep the exact lists of targets tbase + lane (LDS rows)
   const bool owner = lane < 16 && (tbase + lane) < P.NP;
   const int ot = tbase + lane;
-  double ls[KCAP];
-  int li[KCAP];
-#pragma unroll
-  for (int j = 0; j < KCAP; j++) {
-    ls[j] = INF_D;
-    li[j] = INT_MAX;
-  }
+  double* ls = Ls + (w * 16 + (lane & 15)) * KCAP;
+  int* li = Li + (w * 16 + (lane & 15)) * KCAP;
+  if (lane < 16)
+    for (int j = 0; j < KCAP; j++) {
+      ls[j] = INF_D;
+      li[j] = INT_MAX;
+    }
   double s_kth = INF_D, thr = INF_D;
   int zeros = 0;
   const double c1 = owner ? P.c1t[ot] : 0.0, c3 = owner ? P.c3t[ot] : 0.0;
@@ -93,21 +95,38 @@ __global__ __launch_bounds__(256) void knn_mfma(MfmaParams P, int k, int slice_l
     qcnt[w * 16 + lane] = 0;
   }
 
+  // the next chunk's FP32 rows and norms, loaded while the current chunk is processed
+  v4f nb[F4];
+  float nn = 0.f;
+  auto prefetch = [&](int c0) {
+    const int cn = min(MCH, hi - c0);
+#pragma unroll
+    for (int v = 0; v < F4; v++) {
+      const int E = threadIdx.x + 256 * v;
+      const int j = E / (KP / 4), sub = E - j * (KP / 4);
+      nb[v] = j < cn ? *(const v4f*)(P.M32 + (size_t)(c0 + j) * KP + 4 * sub) : v4f{ 0.f, 0.f, 0.f, 0.f };
+    }
+    nn = (threadIdx.x < MCH && (int)threadIdx.x < cn) ? P.Mn[c0 + threadIdx.x] : 0.f;
+  };
+  if (lo < hi)
+    prefetch(lo);
+
   for (int c0 = lo; c0 < hi; c0 += MCH) {
     const int cn = min(MCH, hi - c0);
     __syncthreads();
-    for (int e = threadIdx.x; e < MCH * KP; e += blockDim.x) {
-      const int j = e / KP;
-      Bs[e] = j < cn ? P.M32[(size_t)(c0 + j) * KP + (e - j * KP)] : 0.f;
-    }
-    for (int e = threadIdx.x; e < MCH; e += blockDim.x)
-      Bn[e] = e < cn ? P.Mn[c0 + e] : 0.f;
+#pragma unroll
+    for (int v = 0; v < F4; v++)
+      *(v4f*)(Bs + 4 * (threadIdx.x + 256 * v)) = nb[v];
+    if (threadIdx.x < MCH)
+      Bn[threadIdx.x] = nn;
+    if (c0 + MCH < hi)
+      prefetch(c0 + MCH);
     __syncthreads();
     float th[4];
 #pragma unroll
     for (int r = 0; r < 4; r++)
       th[r] = thr_s[w * 16 + 4 * g + r];
-#pragma unroll
+#pragma unroll 2
     for (int jt = 0; jt < MCH / 16; jt++) {
       const int j = jt * 16 + l16;
       v4f acc = { 0.f, 0.f, 0.f, 0.f };
@@ -141,9 +160,20 @@ __global__ __launch_bounds__(256) void knn_mfma(MfmaParams P, int k, int slice_l
         const int j = queue[(w * 16 + lane) * MCH + e];
         const double* x = P.M + (size_t)(c0 + j) * P.Ea;
         double acc = 0.0;
-        for (int d = 0; d < P.Ea; d++) {
-          const double u = x[d] - y[d];
-          acc += u * u;
+        // four coordinates per step: their loads are in flight together, the sum stays in order
+        for (int d0 = 0; d0 < P.Ea; d0 += 4) {
+          double xv[4], yv[4];
+#pragma unroll
+          for (int u = 0; u < 4; u++) {
+            xv[u] = d0 + u < P.Ea ? x[d0 + u] : 0.0;
+            yv[u] = d0 + u < P.Ea ? y[d0 + u] : 0.0;
+          }
+#pragma unroll
+          for (int u = 0; u < 4; u++)
+            if (d0 + u < P.Ea) {
+              const double dd = xv[u] - yv[u];
+              acc += dd * dd;
+            }
         }
         if (acc == 0.0 || acc == MISSING_D) {
           zeros++;
@@ -152,24 +182,17 @@ __global__ __launch_bounds__(256) void knn_mfma(MfmaParams P, int k, int slice_l
         if (acc < thr) {
           const double s = sqrt(acc);
           if (s < s_kth) {
-            double cs = s;
-            int ci = c0 + j;
-#pragma unroll
-            for (int q = 0; q < KCAP; q++) {
-              const bool take = cs < ls[q] || (cs == ls[q] && ci < li[q]);
-              const double ts = take ? cs : ls[q];
-              const int tv = take ? ci : li[q];
-              cs = take ? ls[q] : cs;
-              ci = take ? li[q] : ci;
-              ls[q] = ts;
-              li[q] = tv;
+            // sorted insertion; the last entry drops out
+            const int ci = c0 + j;
+            int q = KCAP - 1;
+            while (q > 0 && (s < ls[q - 1] || (s == ls[q - 1] && ci < li[q - 1]))) {
+              ls[q] = ls[q - 1];
+              li[q] = li[q - 1];
+              q--;
             }
-            double v = INF_D;
-#pragma unroll
-            for (int q = 0; q < KCAP; q++)
-              if (q == k - 1)
-                v = ls[q];
-            s_kth = v;
+            ls[q] = s;
+            li[q] = ci;
+            s_kth = ls[k - 1];
             if (s_kth != INF_D)
               thr = s_kth * s_kth * SQRT_GUARD;
           }
