@@ -98,8 +98,9 @@ def roofline(st, libs, reps):
 
     direct     (batch_simplex): 3 E_a = 9 flops per library-target pair scanned (FP64 for short libraries);
     rank build (rank_build):    9 flops per pool-target pair and histogram/filter pass (2 passes);
-    rank scan  (rank_scan):     HBM bytes of the ranked prefixes a task reads: the expected scan length
-                                k * pool / L entries of 4 B plus k entries of 8 B per (task, target)."""
+    rank scan  (rank_scan_multi + rank_scatter): HBM bytes of the ranked-prefix entries the tasks' k-NN
+                                selections need (k pool / L entries of 4 B plus k entries of 8 B per (task,
+                                target)) and of their outputs (16 B per prediction)."""
     pool = N_TARGETS
     lthr = rank_threshold(K, pool)
     direct_libs = libs[libs < lthr]
@@ -109,8 +110,8 @@ def roofline(st, libs, reps):
         "direct": ("mfma", 9.0 * reps * float(np.sum(direct_libs)) * N_TARGETS, "TFLOP/s", FP64_PEAK_TFLOPS,
                    "batch_simplex<4,4,false>"),
         "rank_build": ("mfma", 2 * 9.0 * pool * N_TARGETS, "TFLOP/s", FP32_PEAK_TFLOPS, "rank_build<4>"),
-        "rank_scan": ("hbm", reps * float(np.sum(K * pool / rank_libs * 4.0 + K * 8.0)) * N_TARGETS, "GB/s",
-                      HBM_PEAK_GBS, "rank_scan<4,4>"),
+        "rank_scan": ("hbm", reps * float(np.sum(K * pool / rank_libs * 4.0 + K * 8.0 + 16.0)) * N_TARGETS, "GB/s",
+                      HBM_PEAK_GBS, "rank_scan_multi<4,1> + rank_scatter"),
     }
     times = {"rank_build": st[1], "rank_scan": st[2], "direct": st[3]}
     name = max(times, key=lambda n: times[n])
@@ -121,6 +122,30 @@ def roofline(st, libs, reps):
             "traffic": None, "kernel": kernel, "kernel_ms": ms,
             "algorithmic_work_per_launch": amount,
             "engine": {"direct": "fp64 VALU", "rank_build": "fp32 VALU", "rank_scan": "HBM3E"}[name]}
+
+
+def panel_smap_leg(fe):
+    """Secondary line: S-map on 512 logistic-map panels (E = 8 with dt, E_a = 16, k = 18, theta 0, 1, 3;
+    265 665 library rows x 265 665 targets), mixed against plain, device stage times of one call each."""
+    from tests import configs
+    t, x, panel = configs.cfg4_panels()
+    kw = dict(panel=panel, E=8, dt=True, theta=[0, 1, 3], algorithm="smap", savePredictions=True)
+    out = {"workload": "panel_smap_512_logistic_panels_E8_dt_k18_3thetas", "predictions": None}
+    res = {}
+    for name, prec in (("mixed", fe.MIXED), ("plain", fe.PLAIN)):
+        t0 = time.perf_counter()
+        r = fe.edm(t, x, precision=prec, **kw)
+        wall = time.perf_counter() - t0
+        st = fe.timings()
+        res[name] = r
+        out[name] = {"device_ms": float(st[3]), "select_ms": float(st[0]), "merge_ms": float(st[1]),
+                     "smap_ms": float(st[2]), "wall_s": wall}
+    out["predictions"] = int(np.sum(~np.isnan(res["plain"]["predictions"]))) * 3
+    out["device_ratio_mixed_vs_plain"] = out["plain"]["device_ms"] / out["mixed"]["device_ms"]
+    a, b = res["mixed"]["predictions"], res["plain"]["predictions"]
+    ok = ~np.isnan(b)
+    out["max_abs_d_yhat_mixed_vs_plain"] = float(np.max(np.abs(a[ok] - b[ok]))) if ok.any() else 0.0
+    return out
 
 
 def cpu_baseline(t, x, y, kw, seconds):
@@ -151,6 +176,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-panel", action="store_true", help="skip the panel S-map line")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported"}))
@@ -179,8 +205,9 @@ def main():
            "config": {"workload": "ccm_sweep_coupled_logistic_T20000_E3_40lib_10to19000",
                       "replicates": args.reps, "tasks": int(args.reps * len(libs)), "predictions_per_step": n_pred,
                       "library_target_pairs_per_step": pairs, "k": 4, "algorithm": "simplex",
-                      "form": "batched: rank-table route (FP32-screened exact ranking + bitmask scan) for "
-                              "libraries >= 2 sqrt(k n), direct exact FP64 kernel for shorter ones"}}
+                      "form": "batched: rank-table route (FP32-screened exact ranking, one pass per target "
+                              "over every long-library task) for libraries >= 1.5 sqrt(k n), direct exact FP64 "
+                              "kernel for shorter ones"}}
 
     out["stages_ms"] = {"device_total": float(st[0]), "rank_build": float(st[1]), "rank_scan": float(st[2]),
                         "direct": float(st[3])}
@@ -211,6 +238,10 @@ def main():
         pt.close()
     pb.close()
     mixed.close()
+
+    if not args.no_panel:
+        print("panel S-map line (mixed and plain)", file=sys.stderr, flush=True)
+        out["panel_smap"] = panel_smap_leg(fe)
 
     if not args.no_cpu:
         print("timing the CPU oracle sample", file=sys.stderr, flush=True)

@@ -319,24 +319,42 @@ __device__ void exact_target(const BatchParams& B, int tk, int t)
   }
   double s_kth = INF_D, thr = INF_D;
   int zeros = 0;
-  for (int c = 0; c < nl; c++) {
-    const double* xr = B.L + (size_t)lrows[c] * Ea;
-    double acc = 0.0;
+  // RB library rows at a time, so that their loads are in flight together; each pair's sum keeps
+  // the coordinate order and the pairs are ranked in position order
+  constexpr int RB = KCAP >= 32 ? 2 : KCAP >= 16 ? 4 : 8;
+  for (int c0 = 0; c0 < nl; c0 += RB) {
+    const double* xr[RB];
+    double acc[RB];
+#pragma unroll
+    for (int u = 0; u < RB; u++) {
+      xr[u] = B.L + (size_t)lrows[c0 + u < nl ? c0 + u : c0] * Ea;
+      acc[u] = 0.0;
+    }
     for (int j = 0; j < Ea; j++) {
-      const double d = xr[j] - yp[j];
-      acc += d * d;
+      const double yj = yp[j];
+#pragma unroll
+      for (int u = 0; u < RB; u++) {
+        const double d = xr[u][j] - yj;
+        acc[u] += d * d;
+      }
     }
-    if (acc == 0.0 || acc == MISSING_D) {
-      zeros++;
-      continue;
-    }
-    if (acc < thr) {
-      const double s = sqrt(acc);
-      if (s < s_kth) {
-        insert_sqrt<KCAP>(ls, li, s, c);
-        s_kth = kth_of<KCAP>(ls, k);
-        if (s_kth != INF_D)
-          thr = s_kth * s_kth * SQRT_GUARD;
+#pragma unroll
+    for (int u = 0; u < RB; u++) {
+      const int c = c0 + u;
+      if (c >= nl)
+        break;
+      if (acc[u] == 0.0 || acc[u] == MISSING_D) {
+        zeros++;
+        continue;
+      }
+      if (acc[u] < thr) {
+        const double s = sqrt(acc[u]);
+        if (s < s_kth) {
+          insert_sqrt<KCAP>(ls, li, s, c);
+          s_kth = kth_of<KCAP>(ls, k);
+          if (s_kth != INF_D)
+            thr = s_kth * s_kth * SQRT_GUARD;
+        }
       }
     }
   }
