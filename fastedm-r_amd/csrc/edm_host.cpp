@@ -523,6 +523,7 @@ struct fedm_plan
   int rc0 = 0;     // 8000 when no row is usable (src/r.cpp:260-266)
   bool copred_mode = false;
   int save_manifolds = 0, save_smap_coeffs = 0;
+  std::vector<int> extra_metrics; // per extra: 0 Diff, 1 CheckSame (empty: all Diff)
   struct Task
   {
     int E, k, library, mask; // mask: index into masks (prediction rows)
@@ -556,6 +557,17 @@ struct fedm_plan
         edm_batch_destroy(g.batch);
   }
 };
+
+// Arguments of the task grid (src/edm.cpp:90-215).
+struct TaskGrid
+{
+  std::vector<int> Es, libraries;
+  std::vector<bool> usable;
+  int k, num_reps, crossfold;
+  bool explore, full, shuffle, save_predictions, save_copredictions, copredict;
+  int seed;
+};
+static int plan_tasks(edm_ctx* ctx, fedm_plan* P, const TaskGrid& grid);
 
 static int plan_command(edm_ctx* ctx, const fedm_command* c, fedm_plan* P)
 {
@@ -667,23 +679,41 @@ static int plan_command(edm_ctx* ctx, const fedm_command* c, fedm_plan* P)
   std::vector<int> Es(c->E, c->E + c->num_E);
   const int maxE = Es.back();
   const auto usable = s.usable(maxE, false);
+  std::vector<int> libraries(c->libraries, c->libraries + c->num_libraries);
+  const int nu = (int)std::count(usable.begin(), usable.end(), true);
+  if (!explore && libraries.empty() && nu > 0)
+    libraries = { nu };
+  TaskGrid grid{ Es, libraries, usable, c->k, c->num_reps, c->crossfold, explore, c->full != 0, c->shuffle != 0,
+                 c->save_final_predictions != 0, c->save_final_copredictions != 0, c->co_x != nullptr, c->seed };
+  return plan_tasks(ctx, P, grid);
+}
+
+// The task grid of launch_tasks (src/edm.cpp:90-215) over a prepared series and options; tasks keep
+// the reference's order for the results.  Groups of tasks sharing (E, copredict) get one point table
+// and one batched launch.
+static int plan_tasks(edm_ctx* ctx, fedm_plan* P, const TaskGrid& grid)
+{
+  Series& s = P->s;
+  edm_options& o = P->o;
+  const int n = s.n();
+  const std::vector<int>& Es = grid.Es;
+  const std::vector<int>& libraries = grid.libraries;
+  const std::vector<bool>& usable = grid.usable;
+  const bool explore = grid.explore;
+  const int maxE = Es.back();
   const int nu = (int)std::count(usable.begin(), usable.end(), true);
   if (nu == 0) {
     P->rc0 = 8000;
     return 0;
   }
-  std::vector<int> libraries(c->libraries, c->libraries + c->num_libraries);
-  if (!explore && libraries.empty())
-    libraries = { nu };
-  P->copred_mode = c->co_x != nullptr;
+  P->copred_mode = grid.copredict;
   std::vector<bool> cousable;
   if (P->copred_mode)
     cousable = s.usable(maxE, true);
 
-  // task grid (src/edm.cpp:90-215); tasks keep the reference's order for the results
-  Splits sp(explore, c->full != 0, c->shuffle != 0, c->crossfold, usable, c->seed);
+  Splits sp(explore, grid.full, grid.shuffle, grid.crossfold, usable, grid.seed);
   const int numLibraries = explore ? 1 : (int)libraries.size();
-  const int iters = c->num_reps > c->crossfold ? c->num_reps : c->crossfold;
+  const int iters = grid.num_reps > grid.crossfold ? grid.num_reps : grid.crossfold;
   auto mask_id = [&](const std::vector<bool>& m) {
     for (size_t q = P->pred_masks.size(); q-- > 0;)
       if (P->pred_masks[q] == m)
@@ -708,15 +738,16 @@ static int plan_command(edm_ctx* ctx, const fedm_command* c, fedm_plan* P)
           fresh = true;
         const int library = explore ? libSize : libraries[l];
         int kAdj;
-        if (c->k > 0)
-          kAdj = c->k;
-        else if (c->k < 0)
+        if (grid.k > 0)
+          kAdj = grid.k;
+        else if (grid.k < 0)
           kAdj = -1;
         else
-          kAdj = std::min(s.E_actual(E) + 1 + (c->algorithm == FEDM_SMAP), library);
+          kAdj = std::min(s.E_actual(E) + 1 + (o.algorithm == FEDM_SMAP), library);
         const bool last = (E == maxE) && (l + 1 == numLibraries);
-        const bool savePred = c->save_final_predictions && (iter == c->num_reps || (explore && c->crossfold > 0)) && last;
-        const bool saveCo = c->save_final_copredictions && (iter == c->num_reps || (explore && c->crossfold > 0)) && last;
+        const bool finalRep = iter == grid.num_reps || (explore && grid.crossfold > 0);
+        const bool savePred = grid.save_predictions && finalRep && last;
+        const bool saveCo = grid.save_copredictions && finalRep && last;
         if (fresh) {
           sp.draw_split(library, iter);
           fresh = false;
@@ -804,7 +835,18 @@ static int plan_command(edm_ctx* ctx, const fedm_command* c, fedm_plan* P)
       g.out_begin.push_back(g.total_out);
       g.total_out += (int64_t)T * P->pred_rows[tk.mask].size();
     }
+    // expanded metrics (src/distances.cpp:29-67): x, dt and lagged coordinates Diff, each extra its own
     std::vector<int> metrics(Ea, 0);
+    if (!P->extra_metrics.empty()) {
+      if (o.distance == FEDM_DIST_WASSERSTEIN) {
+        const int rows = 1 + (s.E_dt(g.E) > 0);
+        for (int e = 0; e < s.E_extras() && rows + e < Ea; e++)
+          metrics[rows + e] = P->extra_metrics[e];
+      } else {
+        for (int e = 0; e < s.E_extras(); e++)
+          metrics[g.E + s.E_dt(g.E) + e] = P->extra_metrics[e];
+      }
+    }
     edm_options go = o;
     go.metrics = metrics.data();
     edm_manifold_view Lv{ g.table.data(), g.targets.data(), g.ptimes.data(), g.ttimes.data(),
@@ -1025,6 +1067,78 @@ int fedm_run_command(edm_ctx* ctx, const fedm_command* c, fedm_result** out)
   if (r == 0)
     r = fedm_plan_result(P, out);
   fedm_plan_destroy(P);
+  return r;
+}
+
+int fedm_run_task_group(edm_ctx* ctx, const fedm_task_group* g, fedm_result** out)
+{
+  *out = nullptr;
+  if (!ctx || !g || g->n <= 0 || g->num_E <= 0 || g->num_extras_lagged != 0 || !g->t || !g->x || !g->obs ||
+      !g->usable || (g->num_extras > 0 && !g->extras))
+    return -22;
+  auto P = std::make_unique<fedm_plan>();
+  P->ctx = ctx;
+  const int n = g->n;
+  Series& s = P->s;
+  s.t.assign(g->t, g->t + n);
+  s.x.assign(g->x, g->x + n);
+  if (g->xmap) {
+    s.xmap.assign(g->xmap, g->xmap + n);
+    s.has_xmap = true;
+  }
+  if (g->co_x)
+    s.co_x.assign(g->co_x, g->co_x + n);
+  for (int e = 0; e < g->num_extras; e++)
+    s.extras.emplace_back(g->extras + (size_t)e * n, g->extras + (size_t)(e + 1) * n);
+  s.obs.assign(g->obs, g->obs + n);
+  if (g->panel) {
+    s.panel.assign(g->panel, g->panel + n);
+    s.has_panel = true;
+  }
+  s.tau = g->tau;
+  s.p = g->p;
+  s.dt = g->dt != 0;
+  s.reldt = g->reldt != 0;
+  s.w_dt = g->dt_weight;
+  s.allow_missing = g->missing_distance != 0.0;
+
+  edm_options& o = P->o;
+  std::memset(&o, 0, sizeof(o));
+  o.algorithm = g->algorithm;
+  o.distance = g->distance;
+  o.missing_distance = g->missing_distance;
+  o.has_missing = s.allow_missing;
+  o.force_compute = 1;
+  o.save_smap_coeffs = g->save_smap_coeffs;
+  o.aspect_ratio = g->aspect_ratio;
+  o.precision = g->precision;
+  P->thetas = g->num_thetas > 0 ? std::vector<double>(g->thetas, g->thetas + g->num_thetas) : std::vector<double>{ 1.0 };
+  o.num_thetas = (int)P->thetas.size();
+  o.thetas = P->thetas.data();
+  o.idw = g->idw;
+  o.panel_mode = g->panel_mode && g->panel != nullptr;
+  P->save_smap_coeffs = g->save_smap_coeffs;
+  if (g->extra_metrics) {
+    P->extra_metrics.assign(g->extra_metrics, g->extra_metrics + g->num_extras);
+    for (int m : P->extra_metrics)
+      o.has_categorical |= m != 0;
+  }
+
+  std::vector<bool> usable(n);
+  for (int i = 0; i < n; i++)
+    usable[i] = g->usable[i] != 0;
+  std::vector<int> libraries(g->libraries, g->libraries + g->num_libraries);
+  const int nu = (int)std::count(usable.begin(), usable.end(), true);
+  if (!g->explore && libraries.empty() && nu > 0)
+    libraries = { nu };
+  TaskGrid grid{ std::vector<int>(g->E, g->E + g->num_E), libraries, usable, g->k, g->num_reps, g->crossfold,
+                 g->explore != 0, g->full != 0, g->shuffle != 0, g->save_final_predictions != 0,
+                 g->save_final_copredictions != 0, g->copredict_mode != 0 && g->co_x != nullptr, g->seed };
+  int r = plan_tasks(ctx, P.get(), grid);
+  if (r == 0)
+    r = fedm_plan_run(P.get());
+  if (r == 0)
+    r = fedm_plan_result(P.get(), out);
   return r;
 }
 

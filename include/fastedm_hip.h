@@ -210,6 +210,54 @@ void fedm_plan_destroy(fedm_plan* p);
 /* fedm_plan_create + fedm_plan_run + fedm_plan_result. */
 
 int fedm_run_command(edm_ctx* ctx, const fedm_command* c, fedm_result** out);
+
+/* ---------------------------------------------------------------------------------------------
+ * A task group as the reference's saveInputs dump records it (src/r.cpp:275-309): the manifold
+ * generator's state after its normalisation (src/manifold.cpp:364-402), the options that the dump
+ * keeps (src/common.cpp:62-106) and the task-grid arguments of launch_tasks (src/edm.cpp:90-215).
+ * The dump does not keep the splitter seed (replays pass it), allowMissing (taken as
+ * missing_distance != 0: the R front end sets a positive default whenever it allows missing values,
+ * and otherwise usable points hold no missing coordinate), lowMemory and predictWithPast (taken as
+ * off) or panel weight matrices (only idw applies).
+ */
+typedef struct
+{
+  int n;
+  const double* t;          /* _t */
+  const double* x;          /* _x */
+  const double* xmap;       /* _xmap (when _xmap_mode), or NULL */
+  const double* co_x;       /* _co_x (copredict mode), or NULL */
+  const int* obs;           /* _observation_number */
+  const int* panel;         /* _panelIDs (when _panel_mode), or NULL */
+  const double* extras;     /* _extras, [num_extras x n] */
+  int num_extras, num_extras_lagged; /* lagged extras are not supported (-22) */
+  int dt, reldt;            /* _dt, _reldt */
+  double dt_weight;         /* _dtWeight */
+  int tau, p;               /* _tau, _p */
+  int algorithm;            /* opts.algorithm: FEDM_SIMPLEX / FEDM_SMAP */
+  int distance;             /* opts.distance: FEDM_DIST_* */
+  const double* thetas;     /* opts.thetas */
+  int num_thetas;
+  double missing_distance;  /* opts.missingdistance */
+  int panel_mode;           /* opts.panelMode */
+  double idw;               /* opts.idw */
+  const int* extra_metrics; /* opts.metrics, one per extra: 0 Diff, 1 CheckSame; or NULL */
+  double aspect_ratio;      /* opts.aspectRatio */
+  int save_smap_coeffs;     /* saveSMAPCoeffs */
+  const int* E;             /* Es */
+  int num_E;
+  const int* libraries;
+  int num_libraries;
+  int k, num_reps, crossfold, explore, full, shuffle;
+  int save_final_predictions, save_final_copredictions, copredict_mode;
+  const int* usable;        /* [n] */
+  int seed;
+  int precision;            /* FEDM_PLAIN / FEDM_MIXED / FEDM_PLAIN_TASKWISE */
+} fedm_task_group;
+
+/* Runs one dumped task group as launch_tasks would (no reference counterpart: the reference only
+ * writes the dumps); results as fedm_run_command's. */
+int fedm_run_task_group(edm_ctx* ctx, const fedm_task_group* g, fedm_result** out);
 int fedm_result_rc(const fedm_result* r);
 int fedm_result_num_stats(const fedm_result* r, int copredict);
 void fedm_result_stats(const fedm_result* r, int copredict, int* E, int* library, double* theta, double* rho,

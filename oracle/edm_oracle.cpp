@@ -1565,6 +1565,39 @@ int orc_usable(const orc_command* c, int maxE, int copred, int* usable_out)
   return s;
 }
 
+int orc_generator_state(const orc_command* c, int maxE, double* t, double* x, double* xmap, double* co_x,
+                        double* extras, int* obs, int* flags, double* dt_weight, int* usable_out)
+{
+  Generator g;
+  bool explore;
+  build_generator(*c, g, explore);
+  const int n = c->n;
+  for (int i = 0; i < n; i++) {
+    t[i] = g.t[i];
+    x[i] = g.x[i];
+    obs[i] = g.obs[i];
+    if (xmap)
+      xmap[i] = g.xmap_mode ? g.xmap[i] : 0.0;
+    if (co_x)
+      co_x[i] = g.co_x.empty() ? 0.0 : g.co_x[i];
+  }
+  for (int e = 0; e < g.num_extras && extras; e++)
+    for (int i = 0; i < n; i++)
+      extras[(size_t)e * n + i] = g.extras[e][i];
+  flags[0] = g.dt;
+  flags[1] = g.reldt;
+  flags[2] = g.xmap_mode;
+  flags[3] = g.panel_mode;
+  *dt_weight = g.dtWeight;
+  auto u = g.generate_usable(maxE, false);
+  int s = 0;
+  for (int i = 0; i < n; i++) {
+    usable_out[i] = u[i];
+    s += u[i];
+  }
+  return s;
+}
+
 int orc_manifold(const orc_command* c, int E, const int* filter, int ps, int copred, double* out, double* targets,
                  int max_points)
 {

@@ -114,6 +114,11 @@ int orc_split_masks(int n, const int* usable, int explore, int full, int shuffle
 int orc_manifold(const orc_command* c, int E, const int* filter, int prediction_set, int copredict, double* out,
                  double* targets, int max_points);
 int orc_usable(const orc_command* c, int maxE, int copredict, int* usable_out);
+/* The normalised generator state (src/manifold.h:38-127): series after replace_nan, extras, observation
+ * numbers, flags {dt, reldt, xmap mode, panel mode}, the dt weight and the usable rows for maxE; returns
+ * the usable count.  xmap, co_x and extras may be NULL. */
+int orc_generator_state(const orc_command* c, int maxE, double* t, double* x, double* xmap, double* co_x,
+                        double* extras, int* obs, int* flags, double* dt_weight, int* usable_out);
 /* Eigen-JacobiSVD-style solve of (X^T X) ics = X^T y for a dense n x n system: A row-major. */
 void orc_jacobi_solve(int n, const double* A, const double* b, double* x, double* singular_values, int* rank);
 double orc_correlation(int n, const double* a, const double* b);

@@ -74,6 +74,8 @@ def lib():
         L.orc_manifold.argtypes = [C.POINTER(Command), C.c_int, _ip, C.c_int, C.c_int, _dp, _dp, C.c_int]
         L.orc_split_masks.argtypes = [C.c_int, _ip, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                                       C.c_int, _ip, _ip, _ip, C.c_int]
+        L.orc_generator_state.argtypes = [C.POINTER(Command), C.c_int, _dp, _dp, _dp, _dp, _dp, _ip, _ip, _dp, _ip]
+        L.orc_generator_state.restype = C.c_int
         _lib = L
     return _lib
 
@@ -282,6 +284,29 @@ def edm(t, x, y=None, **kw):
     return collect_result(L, "orc_", r, kw.get("copredict") is not None and len(kw.get("copredict")) > 0,
                           {"allowMissing": kw.get("allowMissing", False),
                            "dt": kw.get("dt", False) or kw.get("reldt", False)})
+
+
+def generator_state(t, x, y=None, **kw):
+    """The manifold generator's state for an edm() call, as the reference's saveInputs dump keeps it
+    (src/manifold.cpp:364-382, src/r.cpp:160-262): normalised series, extras, observation numbers, flags,
+    dt weight and the usable rows for the last E."""
+    c, keep, explore = make_command(t, x, y, **kw)
+    L = lib()
+    n = c.n
+    maxE = int(c.E[c.num_E - 1])  # src/r.cpp:256: the last E
+    ne = c.num_extras
+    tt, xx, xm, cx = np.zeros(n), np.zeros(n), np.zeros(n), np.zeros(n)
+    ex = np.zeros(max(ne, 1) * n)
+    obs = np.zeros(n, np.int32)
+    flags = np.zeros(4, np.int32)
+    w = C.c_double()
+    usable = np.zeros(n, np.int32)
+    L.orc_generator_state(C.byref(c), maxE, dptr(tt), dptr(xx), dptr(xm), dptr(cx), dptr(ex), iptr(obs),
+                          iptr(flags), C.byref(w), iptr(usable))
+    return {"t": tt, "x": xx, "xmap": xm if flags[2] else None, "co_x": cx if c.co_x else None,
+            "extras": ex[:ne * n].reshape(ne, n), "obs": obs, "dt": bool(flags[0]), "reldt": bool(flags[1]),
+            "panel_mode": bool(flags[3]), "dt_weight": w.value, "usable": usable, "explore": explore,
+            "num_usable": int(usable.sum()), "command": c, "keep": keep}
 
 
 def predict(M, Mp, opts, nn_cap=0):
