@@ -42,6 +42,7 @@ struct RankParams
   const float* L32;  // FP32 copy
   const double* P;   // prediction-side table
   const int* pool;   // pool rows (ascending) [npool]
+  const float* pool32; // FP32 coordinates of the pool rows, in pool order, zero padded to EMAX [npool x EMAX]
   int npool;
   const int* pred;   // prediction rows [np]
   int np, Ea;

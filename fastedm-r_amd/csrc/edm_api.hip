@@ -719,7 +719,7 @@ int edm_predict(edm_ctx* c, const edm_manifold_view* M, const edm_manifold_view*
 // Batches of tasks over shared point tables.
 
 namespace fedm {
-template <int KCAP, int EMAX, bool MIXED>
+template <int KCAP, int EMAX>
 __global__ void batch_simplex(BatchParams B);
 template <int KCAP>
 __global__ void batch_redo(BatchParams B, const int2* items, int num_items);
@@ -744,7 +744,6 @@ struct edm_batch
   int grid = 0;
   // rank-table route (one shared prediction list whose tasks draw libraries from one pool)
   bool rank_route = false;
-  bool direct_mixed = false; // FP32 screening in the direct kernel (pays off only for long scans)
   RankParams R;
   ScanParams S;
   int scan_grid = 0;
@@ -783,19 +782,16 @@ struct edm_batch
   }
 };
 
-template <int KCAP, int EMAX>
-static const void* batch_kernel_e(bool mixed)
-{
-  return mixed ? (const void*)batch_simplex<KCAP, EMAX, true> : (const void*)batch_simplex<KCAP, EMAX, false>;
-}
 template <int KCAP>
-static const void* batch_kernel_k(int emax, bool mixed)
+static const void* batch_kernel_k(int demax)
 {
-  switch (emax) {
-    case 4: return batch_kernel_e<KCAP, 4>(mixed);
-    case 8: return batch_kernel_e<KCAP, 8>(mixed);
-    case 16: return batch_kernel_e<KCAP, 16>(mixed);
-    default: return batch_kernel_e<KCAP, 32>(mixed);
+  switch (demax) {
+    case 2: return (const void*)batch_simplex<KCAP, 2>;
+    case 3: return (const void*)batch_simplex<KCAP, 3>;
+    case 4: return (const void*)batch_simplex<KCAP, 4>;
+    case 8: return (const void*)batch_simplex<KCAP, 8>;
+    case 16: return (const void*)batch_simplex<KCAP, 16>;
+    default: return (const void*)batch_simplex<KCAP, 32>;
   }
 }
 template <int KCAP>
@@ -822,13 +818,18 @@ static const void* batch_redo_kernel(int kcap)
     default: return (const void*)batch_redo<32>;
   }
 }
-static const void* batch_kernel(int kcap, int emax, bool mixed)
+// The direct kernel's padded dimension: exact for E_actual 2 and 3, else the next of 4, 8, 16, 32.
+static int direct_emax(int Ea)
+{
+  return Ea <= 2 ? 2 : Ea == 3 ? 3 : Ea <= 4 ? 4 : Ea <= 8 ? 8 : Ea <= 16 ? 16 : 32;
+}
+static const void* batch_kernel(int kcap, int Ea)
 {
   switch (kcap) {
-    case 4: return batch_kernel_k<4>(emax, mixed);
-    case 8: return batch_kernel_k<8>(emax, mixed);
-    case 16: return batch_kernel_k<16>(emax, mixed);
-    default: return batch_kernel_k<32>(emax, mixed);
+    case 4: return batch_kernel_k<4>(direct_emax(Ea));
+    case 8: return batch_kernel_k<8>(direct_emax(Ea));
+    case 16: return batch_kernel_k<16>(direct_emax(Ea));
+    default: return batch_kernel_k<32>(direct_emax(Ea));
   }
 }
 
@@ -970,7 +971,6 @@ int edm_batch_create(edm_ctx* c, const edm_manifold_view* L, const edm_manifold_
   b->kcap = kcap;
   b->emax = emax;
   b->mixed = o->precision == FEDM_MIXED;
-  b->direct_mixed = b->mixed;
 
   BatchParams& B = b->B;
   std::memset(&B, 0, sizeof(B));
@@ -1081,13 +1081,6 @@ int edm_batch_create(edm_ctx* c, const edm_manifold_view* L, const edm_manifold_
   }
   B.items = b->dev(items.data(), items.size());
   B.num_items = (int)items.size();
-  {
-    // FP32 screening needs long scans: with short libraries nearly every pair reaches the FP64 path
-    int lmax = 0;
-    for (const int2& it : items)
-      lmax = std::max(lmax, lib_count[it.x]);
-    b->direct_mixed = b->mixed && lmax >= 2048;
-  }
   if (b->rank_route) {
     RankParams& R = b->R;
     std::memset(&R, 0, sizeof(R));
@@ -1095,6 +1088,15 @@ int edm_batch_create(edm_ctx* c, const edm_manifold_view* L, const edm_manifold_
     R.L32 = B.L32;
     R.P = B.P;
     R.pool = b->dev(pool.data(), pool.size());
+    {
+      // the pool rows' FP32 coordinates in pool order, padded to the kernel's width (16-byte rows)
+      std::vector<float> p32(pool.size() * (size_t)emax, 0.0f);
+      for (size_t q = 0; q < pool.size(); q++)
+        for (int j = 0; j < Ea; j++)
+          p32[q * emax + j] = (float)b->Ldata[(size_t)pool[q] * Ea + j];
+      R.pool32 = b->dev(p32.data(), p32.size());
+      hipStreamSynchronize(c->stream);
+    }
     R.npool = (int)pool.size();
     R.pred = B.pred_rows + rank_pb;
     R.np = rank_pc;
@@ -1199,7 +1201,7 @@ int edm_batch_create(edm_ctx* c, const edm_manifold_view* L, const edm_manifold_
   int dev = 0, ncu = 0, per = 0;
   hipGetDevice(&dev);
   hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
-  hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, batch_kernel(kcap, emax, b->direct_mixed), 256, 0);
+  hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, batch_kernel(kcap, Ea), 256, 0);
   b->grid = std::max(1, std::min(B.num_items, ncu * std::max(per, 1)));
   if (b->rank_route) {
     int sper = 0;
@@ -1297,7 +1299,7 @@ int edm_batch_run(edm_batch* b)
     HIP_TRY(hipEventRecord(c->ev[6], s));
   }
   HIP_TRY(hipEventRecord(c->ev[7], s));
-  const void* fn = batch_kernel(b->kcap, b->emax, b->direct_mixed);
+  const void* fn = batch_kernel(b->kcap, b->Ea);
   void* args[] = { &B };
   if (B.num_items > 0)
     HIP_TRY(hipLaunchKernel(fn, dim3(b->grid), dim3(256), args, 0, s));

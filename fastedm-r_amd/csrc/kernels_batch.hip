@@ -2,13 +2,13 @@
 // replicate sweeps, E/theta grids): one persistent launch drains a work queue of (task, block of 256
 // targets) items, heaviest libraries first.
 //
-// Per pair the library point is gathered through the task's row list into LDS, zero padded to EMAX
+// The library points are gathered through the task's row list into LDS in FP64, zero padded to EMAX
 // coordinates (a zero coordinate adds exactly 0 to the sum, so the padded sum is the CPU path's sum,
-// src/distances.cpp:186-216).  Two precisions:
-//  * PLAIN  — FP64 distance;
-//  * MIXED  — FP32 screening: the FP32 distance d32 is compared against the current k-th squared
-//    distance plus a rigorous bound on |d32 - d64| (FP32 rounding of the inputs, of the differences
-//    and of the FMA accumulation); only pairs that could enter the k-NN list are recomputed in FP64.
+// src/distances.cpp:186-216).  Both precision forms use this kernel: FP64 arithmetic runs at the FP32
+// rate on CDNA4 vector units, so FP32 screening saves nothing here.  The cost lies in the list
+// updates, which diverge across a wavefront: each thread screens 32 rows against its threshold
+// first and then walks only the rows that passed, so a wavefront runs as many updates as its busiest
+// thread needs rather than one per row in which any thread updates.
 //
 // The reference orders neighbours by (sqrt distance, library position) (src/edm.cpp:652-681).  The
 // scan keeps the k smallest by (squared distance, position) instead — no square root per candidate,
@@ -151,151 +151,172 @@ __device__ void finish_target(const BatchParams& B, int tk, int t, int np, const
   }
 }
 
-constexpr int BCH = 512; // library points per LDS chunk
+// FP64 squared distance of one LDS row (EMAX coordinates, zero padded) to y, in coordinate order.
+template <int EMAX>
+__device__ __forceinline__ double row_dist(const double* x, const double (&y)[EMAX])
+{
+  double acc = 0.0;
+#pragma unroll
+  for (int j = 0; j < EMAX; j++) {
+    const double d = x[j] - y[j];
+    acc += d * d;
+  }
+  return acc;
+}
 
-template <int KCAP, int EMAX, bool MIXED>
+constexpr int SUB = 32; // rows screened together against the threshold they start with
+
+template <int KCAP, int EMAX>
 __global__ __launch_bounds__(256) void batch_simplex(BatchParams B)
 {
-  __shared__ double lds64[MIXED ? 1 : BCH * EMAX];
-  __shared__ float lds32[MIXED ? BCH * EMAX : 1];
-  __shared__ int lds_row[BCH];
+  // library rows per LDS chunk (a multiple of SUB); rows past the chunk's end hold +inf and never pass
+  constexpr int BCH = EMAX <= 4 ? 512 : 4096 / EMAX;
+  __shared__ double lds64[BCH * EMAX];
   __shared__ int item_sh;
 
   const int Ea = B.Ea;
-  for (;;) {
+  if (threadIdx.x == 0)
+    item_sh = atomicAdd(B.counter, 1);
+  __syncthreads();
+  int item = item_sh;
+  while (item < B.num_items) {
     __syncthreads();
+    // the next item is claimed now, so that the claim's latency overlaps this item's work
     if (threadIdx.x == 0)
       item_sh = atomicAdd(B.counter, 1);
-    __syncthreads();
-    const int item = item_sh;
-    if (item >= B.num_items)
-      break;
-    const int2 it = B.items[item];
-    const int tk = it.x;
-    const int np = B.pred_count[tk];
-    const int nl = B.lib_count[tk];
-    const int t = it.y + threadIdx.x;
-    const bool active = t < np;
-    const int trow = active ? B.pred_rows[B.pred_begin[tk] + t] : 0;
-    const int* lrows = B.lib_rows + B.lib_begin[tk];
-    const int k = B.k[tk];
+    do {
+      const int2 it = B.items[item];
+      const int tk = it.x;
+      const int np = B.pred_count[tk];
+      const int nl = B.lib_count[tk];
+      const int t = it.y + threadIdx.x;
+      const bool active = t < np;
+      const int trow = active ? B.pred_rows[B.pred_begin[tk] + t] : 0;
+      const int* lrows = B.lib_rows + B.lib_begin[tk];
+      const int k = B.k[tk];
 
-    double y[EMAX];
-    float y32[MIXED ? EMAX : 1];
+      double y[EMAX];
 #pragma unroll
-    for (int j = 0; j < EMAX; j++) {
-      y[j] = (active && j < Ea) ? B.P[(size_t)trow * Ea + j] : 0.0;
-      if (MIXED)
-        y32[j] = (float)y[j];
-    }
-    double la[KCAP];
-    int li[KCAP];
+      for (int j = 0; j < EMAX; j++)
+        y[j] = (active && j < Ea) ? B.P[(size_t)trow * Ea + j] : 0.0;
+      double la[KCAP];
+      int li[KCAP];
 #pragma unroll
-    for (int j = 0; j < KCAP; j++) {
-      la[j] = INF_D;
-      li[j] = INT_MAX;
-    }
-    double kth = INF_D, thr_g = INF_D;
-    float thr32 = __builtin_huge_valf();
-    int zeros = 0;
-    bool near = false;
-
-    for (int c0 = 0; c0 < nl; c0 += BCH) {
-      const int cn = min(BCH, nl - c0);
-      __syncthreads();
-      for (int e = threadIdx.x; e < cn; e += blockDim.x)
-        lds_row[e] = lrows[c0 + e];
-      __syncthreads();
-      for (int e = threadIdx.x; e < cn * EMAX; e += blockDim.x) {
-        const int c = e / EMAX, j = e % EMAX;
-        if (MIXED)
-          lds32[e] = j < Ea ? B.L32[(size_t)lds_row[c] * Ea + j] : 0.0f;
-        else
-          lds64[e] = j < Ea ? B.L[(size_t)lds_row[c] * Ea + j] : 0.0;
+      for (int j = 0; j < KCAP; j++) {
+        la[j] = INF_D;
+        li[j] = INT_MAX;
       }
-      __syncthreads();
+      double kth = INF_D, thr_g = INF_D;
+      int zeros = 0;
+      bool near = false;
+
+      for (int c0 = 0; c0 < nl; c0 += BCH) {
+        const int cn = min(BCH, nl - c0);
+        const int cpad = (cn + SUB - 1) / SUB * SUB;
+        __syncthreads();
+        for (int c = threadIdx.x; c < cpad; c += blockDim.x) {
+          if (c < cn) {
+            const double* src = B.L + (size_t)lrows[c0 + c] * Ea;
+#pragma unroll
+            for (int j = 0; j < EMAX; j++)
+              lds64[c * EMAX + j] = j < Ea ? src[j] : 0.0;
+          } else {
+#pragma unroll
+            for (int j = 0; j < EMAX; j++)
+              lds64[c * EMAX + j] = j == 0 ? INF_D : 0.0;
+          }
+        }
+        __syncthreads();
+        if (!active)
+          continue;
+        for (int s0 = 0; s0 < cpad; s0 += SUB) {
+          // Screen SUB rows against a threshold, then walk the rows that pass in position order with
+          // the list as it stands.  Any threshold at or above (final k-th) (1 + 2^-49) gives the same
+          // list, zero count and hand-over decision as walking every row: the rows it drops can
+          // neither enter the final list nor lie just above its k-th entry.  The current threshold is
+          // one (it only falls); for the first rows, before it is finite, the KCAP-th smallest nonzero
+          // distance among them is another.  Rows at distance 0 always pass (every bound is > 0).
+          unsigned mask = 0;
+          if (KCAP * 4 <= SUB && c0 == 0 && s0 == 0) {
+            double dr[SUB], m[KCAP];
+#pragma unroll
+            for (int j = 0; j < KCAP; j++)
+              m[j] = INF_D;
+#pragma unroll
+            for (int r = 0; r < SUB; r++) {
+              dr[r] = row_dist<EMAX>(lds64 + r * EMAX, y);
+              double v = dr[r] == 0.0 ? INF_D : dr[r];
+#pragma unroll
+              for (int j = 0; j < KCAP; j++) {
+                const double lo = fmin(m[j], v);
+                v = fmax(m[j], v);
+                m[j] = lo;
+              }
+            }
+            const double t0 = m[KCAP - 1] == INF_D ? INF_D : m[KCAP - 1] * SQRT_GUARD;
+#pragma unroll
+            for (int r = 0; r < SUB; r++)
+              mask |= dr[r] < t0 ? 1u << r : 0u;
+          } else {
+#pragma unroll
+            for (int r = 0; r < SUB; r++)
+              mask |= row_dist<EMAX>(lds64 + (s0 + r) * EMAX, y) < thr_g ? 1u << r : 0u;
+          }
+          // the passing rows in position order, each against the list as it stands
+          while (mask) {
+            const int r = __builtin_ctz(mask);
+            mask &= mask - 1;
+            const double acc = row_dist<EMAX>(lds64 + (s0 + r) * EMAX, y);
+            // keep rule; a sum equal to MISSING cannot occur on this route (coordinates below 1e30)
+            const bool drop = acc == 0.0 || acc == MISSING_D;
+            zeros += drop ? 1 : 0;
+            // kth < acc < kth (1 + 2^-49): its square root may equal the k-th's
+            near = near | (!drop & (acc > kth) & (acc < thr_g));
+            // the update runs on every lane (an infinite key changes nothing), without branches
+            const double out = kth; // leaves the k-th place when the row enters
+            insert_acc<KCAP>(la, li, (!drop & (acc < kth)) ? acc : INF_D, c0 + s0 + r);
+            kth = k == KCAP ? la[KCAP - 1] : kth_of<KCAP>(la, k);
+            near = near | ((out > kth) & (out < kth * SQRT_GUARD));
+            thr_g = kth * SQRT_GUARD;
+          }
+        }
+      }
       if (!active)
         continue;
-      for (int c = 0; c < cn; c++) {
-        double acc = 0.0;
-        if (MIXED) {
-          float d32 = 0.0f;
+      if (near) {
+        const int r = atomicAdd(B.redo_count, 1);
+        if (r < B.redo_cap)
+          B.redo[r] = make_int2(tk, t);
+        continue;
+      }
+      double ls[KCAP];
+      bool tie = false;
 #pragma unroll
-          for (int j = 0; j < EMAX; j++) {
-            const float d = lds32[c * EMAX + j] - y32[j];
-            d32 = __builtin_fmaf(d, d, d32);
-          }
-          if (d32 > thr32)
-            continue;
-          const double* xr = B.L + (size_t)lds_row[c] * Ea;
+      for (int j = 0; j < KCAP; j++)
+        ls[j] = la[j] == INF_D ? INF_D : sqrt(la[j]);
 #pragma unroll
-          for (int j = 0; j < EMAX; j++)
-            if (j < Ea) {
-              const double d = xr[j] - y[j];
-              acc += d * d;
-            }
-        } else {
+      for (int j = 0; j + 1 < KCAP; j++)
+        tie = tie || (ls[j] == ls[j + 1] && li[j] > li[j + 1]);
+      if (tie) {
+        // equal square roots of different squared distances: order those by position
 #pragma unroll
-          for (int j = 0; j < EMAX; j++) {
-            const double d = lds64[c * EMAX + j] - y[j];
-            acc += d * d;
-          }
-        }
-        if (acc == 0.0 || acc == MISSING_D) {
-          zeros++;
-          continue;
-        }
-        if (acc < thr_g) {
-          if (acc < kth) {
-            const double out = kth; // leaves the k-th place
-            insert_acc<KCAP>(la, li, acc, c0 + c);
-            kth = kth_of<KCAP>(la, k);
-            if (out > kth && out < kth * SQRT_GUARD)
-              near = true;
-            if (kth != INF_D) {
-              thr_g = kth * SQRT_GUARD;
-              if (MIXED)
-                thr32 = float_up(thr_g + 2.0 * (B.c1 * sqrt(thr_g) + B.c2 * thr_g + B.c3));
-            }
-          } else if (acc > kth) {
-            near = true; // kth < acc < kth (1 + 2^-49): its square root may equal the k-th's
+        for (int pass = 0; pass < KCAP; pass++) {
+#pragma unroll
+          for (int j = pass & 1; j + 1 < KCAP; j += 2) {
+            const bool sw = ls[j + 1] < ls[j] || (ls[j + 1] == ls[j] && li[j + 1] < li[j]);
+            const double a = ls[j], b = ls[j + 1];
+            const int ia = li[j], ib = li[j + 1];
+            ls[j] = sw ? b : a;
+            ls[j + 1] = sw ? a : b;
+            li[j] = sw ? ib : ia;
+            li[j + 1] = sw ? ia : ib;
           }
         }
       }
-    }
-    if (!active)
-      continue;
-    if (near) {
-      const int r = atomicAdd(B.redo_count, 1);
-      if (r < B.redo_cap)
-        B.redo[r] = make_int2(tk, t);
-      continue;
-    }
-    double ls[KCAP];
-    bool tie = false;
-#pragma unroll
-    for (int j = 0; j < KCAP; j++)
-      ls[j] = la[j] == INF_D ? INF_D : sqrt(la[j]);
-#pragma unroll
-    for (int j = 0; j + 1 < KCAP; j++)
-      tie = tie || (ls[j] == ls[j + 1] && li[j] > li[j + 1]);
-    if (tie) {
-      // equal square roots of different squared distances: order those by position
-#pragma unroll
-      for (int pass = 0; pass < KCAP; pass++) {
-#pragma unroll
-        for (int j = pass & 1; j + 1 < KCAP; j += 2) {
-          const bool sw = ls[j + 1] < ls[j] || (ls[j + 1] == ls[j] && li[j + 1] < li[j]);
-          const double a = ls[j], b = ls[j + 1];
-          const int ia = li[j], ib = li[j + 1];
-          ls[j] = sw ? b : a;
-          ls[j + 1] = sw ? a : b;
-          li[j] = sw ? ib : ia;
-          li[j + 1] = sw ? ia : ib;
-        }
-      }
-    }
-    finish_target<KCAP>(B, tk, t, np, lrows, nl, zeros, k, ls, li);
+      finish_target<KCAP>(B, tk, t, np, lrows, nl, zeros, k, ls, li);
+    } while (0);
+    __syncthreads();
+    item = item_sh;
   }
 }
 
@@ -382,11 +403,9 @@ __global__ __launch_bounds__(256) void batch_redo(BatchParams B, const int2* ite
   }
 }
 
-#define FEDM_BATCH(KC, EM)                                                                                       \
-  template __global__ void batch_simplex<KC, EM, false>(BatchParams);                                            \
-  template __global__ void batch_simplex<KC, EM, true>(BatchParams);
+#define FEDM_BATCH(KC, EM) template __global__ void batch_simplex<KC, EM>(BatchParams);
 #define FEDM_BATCH_E(KC)                                                                                         \
-  FEDM_BATCH(KC, 4) FEDM_BATCH(KC, 8) FEDM_BATCH(KC, 16) FEDM_BATCH(KC, 32)                                      \
+  FEDM_BATCH(KC, 2) FEDM_BATCH(KC, 3) FEDM_BATCH(KC, 4) FEDM_BATCH(KC, 8) FEDM_BATCH(KC, 16) FEDM_BATCH(KC, 32)  \
   template __global__ void batch_redo<KC>(BatchParams, const int2*, int);
 FEDM_BATCH_E(4)
 FEDM_BATCH_E(8)

@@ -50,17 +50,23 @@ __global__ __launch_bounds__(256) void rank_build(RankParams R)
     y[j] = j < Ea ? R.P[(size_t)trow * Ea + j] : 0.0;
     y32[j] = (float)y[j];
   }
-  auto d32_of = [&](int row) {
-    const float* x = R.L32 + (size_t)row * Ea;
+  // FP32 squared distance of pool entry q (padded coordinates are 0 on both sides and add 0)
+  auto d32_of = [&](int q) {
+    typedef float v4f __attribute__((ext_vector_type(4)));
+    const v4f* x = (const v4f*)(R.pool32 + (size_t)q * EMAX);
     float d = 0.0f;
 #pragma unroll
-    for (int j = 0; j < EMAX; j++)
-      if (j < Ea) {
-        const float u = x[j] - y32[j];
-        d = __builtin_fmaf(u, u, d);
+    for (int j4 = 0; j4 < EMAX / 4; j4++) {
+      const v4f v = x[j4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const float w = v[u] - y32[4 * j4 + u];
+        d = __builtin_fmaf(w, w, d);
       }
+    }
     return d;
   };
+  constexpr int UNR = 4; // pool entries per thread whose loads are in flight together
   auto d64_of = [&](int row) {
     const double* x = R.L + (size_t)row * Ea;
     double acc = 0.0;
@@ -77,9 +83,17 @@ __global__ __launch_bounds__(256) void rank_build(RankParams R)
   for (int b = threadIdx.x; b < RANK_BINS; b += blockDim.x)
     hist[b] = 0;
   __syncthreads();
-  for (int q = threadIdx.x; q < R.npool; q += blockDim.x) {
-    const float d = d32_of(R.pool[q]);
-    atomicAdd(&hist[min(RANK_BINS - 1, (int)(__float_as_uint(d) >> 20))], 1u);
+  for (int q0 = threadIdx.x; q0 < R.npool; q0 += UNR * blockDim.x) {
+    float d[UNR];
+#pragma unroll
+    for (int u = 0; u < UNR; u++) {
+      const int q = q0 + u * blockDim.x;
+      d[u] = q < R.npool ? d32_of(q) : 0.0f;
+    }
+#pragma unroll
+    for (int u = 0; u < UNR; u++)
+      if (q0 + u * (int)blockDim.x < R.npool)
+        atomicAdd(&hist[min(RANK_BINS - 1, (int)(__float_as_uint(d[u]) >> 20))], 1u);
   }
   __syncthreads();
   if (threadIdx.x == 0) {
@@ -108,23 +122,32 @@ __global__ __launch_bounds__(256) void rank_build(RankParams R)
       overflow = 0;
     }
     __syncthreads();
-    for (int q = threadIdx.x; q < R.npool; q += blockDim.x) {
-      const int row = R.pool[q];
-      const float d = d32_of(row);
-      if (!(d <= T1))
-        continue;
-      const double acc = d64_of(row);
-      if (acc == 0.0) {
-        const int z = atomicAdd(&nzero, 1);
-        if (z < RANK_ZCAP)
-          zr[z] = row;
-      } else if (acc <= tau) {
-        const int c = atomicAdd(&ncand, 1);
-        if (c < RANK_CAP) {
-          cs[c] = sqrt(acc);
-          cq[c] = row;
-        } else {
-          overflow = 1;
+    for (int q0 = threadIdx.x; q0 < R.npool; q0 += UNR * blockDim.x) {
+      float d[UNR];
+#pragma unroll
+      for (int u = 0; u < UNR; u++) {
+        const int q = q0 + u * blockDim.x;
+        d[u] = q < R.npool ? d32_of(q) : __builtin_huge_valf();
+      }
+#pragma unroll
+      for (int u = 0; u < UNR; u++) {
+        const int q = q0 + u * blockDim.x;
+        if (q >= R.npool || !(d[u] <= T1))
+          continue;
+        const int row = R.pool[q];
+        const double acc = d64_of(row);
+        if (acc == 0.0) {
+          const int z = atomicAdd(&nzero, 1);
+          if (z < RANK_ZCAP)
+            zr[z] = row;
+        } else if (acc <= tau) {
+          const int c = atomicAdd(&ncand, 1);
+          if (c < RANK_CAP) {
+            cs[c] = sqrt(acc);
+            cq[c] = row;
+          } else {
+            overflow = 1;
+          }
         }
       }
     }
