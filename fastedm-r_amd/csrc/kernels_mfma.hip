@@ -36,8 +36,43 @@ __device__ __forceinline__ float up_f32(double x)
   return f;
 }
 
+// FP64 squared distance in the reference's order (src/distances.cpp:186-216); the loads of up to 16
+// coordinates of both rows (8 for wider points, to bound the registers) are issued together.
+template <int KP>
+__device__ __forceinline__ double dist_f64(const double* x, const double* y, int Ea)
+{
+  constexpr int LB = KP <= 16 ? KP : 8;
+  double acc = 0.0;
+#pragma unroll 1
+  for (int d0 = 0; d0 < KP; d0 += LB) {
+    double xv[LB], yv[LB];
+#pragma unroll
+    for (int u = 0; u < LB; u++) {
+      xv[u] = d0 + u < Ea ? x[d0 + u] : 0.0;
+      yv[u] = d0 + u < Ea ? y[d0 + u] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < LB; u++)
+      if (d0 + u < Ea) {
+        const double dd = xv[u] - yv[u];
+        acc += dd * dd;
+      }
+  }
+  return acc;
+}
+
+// Makes one lane's LDS writes visible to the other lanes of its wavefront.
+__device__ __forceinline__ void lds_wave_sync()
+{
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+constexpr int PAR = 4; // queued candidates per target recomputed by the whole wavefront at once
+
 template <int KCAP, int KP>
-__global__ __launch_bounds__(256) void knn_mfma(MfmaParams P, int k, int slice_len, double* __restrict__ out_s,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void knn_mfma(MfmaParams P, int k, int slice_len, double* __restrict__ out_s,
                                                int* __restrict__ out_i, int* __restrict__ out_cnt)
 {
   constexpr int MCH = KP == 64 ? 64 : 128;  // library points per LDS chunk (16-point tiles)
@@ -49,6 +84,7 @@ __global__ __launch_bounds__(256) void knn_mfma(MfmaParams P, int k, int slice_l
   __shared__ unsigned char queue[64 * MCH];
   __shared__ double Ls[64 * KCAP]; // exact lists of the block's targets, by (sqrt distance, position)
   __shared__ int Li[64 * KCAP];
+  __shared__ double accq[64 * PAR]; // FP64 distances of each target's first PAR queued candidates
 
   const int lane = threadIdx.x & 63;
   const int w = threadIdx.x >> 6;
@@ -126,55 +162,73 @@ __global__ __launch_bounds__(256) void knn_mfma(MfmaParams P, int k, int slice_l
 #pragma unroll
     for (int r = 0; r < 4; r++)
       th[r] = thr_s[w * 16 + 4 * g + r];
-#pragma unroll 2
-    for (int jt = 0; jt < MCH / 16; jt++) {
-      const int j = jt * 16 + l16;
-      v4f acc = { 0.f, 0.f, 0.f, 0.f };
+    bool tok[4]; // this lane's output rows are real targets
 #pragma unroll
-      for (int q = 0; q < KP / 16; q++) {
-        const v4f b = *(const v4f*)(Bs + j * KP + 16 * q + 4 * g);
-        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a[q][0], b[0], acc, 0, 0, 0);
-        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a[q][1], b[1], acc, 0, 0, 0);
-        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a[q][2], b[2], acc, 0, 0, 0);
-        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a[q][3], b[3], acc, 0, 0, 0);
-      }
-      if (j < cn) {
+    for (int r = 0; r < 4; r++)
+      tok[r] = tbase + 4 * g + r < P.NP;
+    // TB tiles at a time: their operand loads and their independent MFMA chains are issued together,
+    // then the 4 TB results of each lane are screened without branches; only a wavefront in which
+    // some pair passes takes the (divergent) queue appends
+    constexpr int TB = KP == 64 ? 2 : 4;
+    for (int jt0 = 0; jt0 < MCH / 16; jt0 += TB) {
+      v4f b[TB][KP / 16];
+#pragma unroll
+      for (int u = 0; u < TB; u++)
+#pragma unroll
+        for (int q = 0; q < KP / 16; q++)
+          b[u][q] = *(const v4f*)(Bs + ((jt0 + u) * 16 + l16) * KP + 16 * q + 4 * g);
+      v4f acc[TB];
+#pragma unroll
+      for (int u = 0; u < TB; u++)
+        acc[u] = v4f{ 0.f, 0.f, 0.f, 0.f };
+#pragma unroll
+      for (int q = 0; q < KP / 16; q++)
+#pragma unroll
+        for (int kk = 0; kk < 4; kk++)
+#pragma unroll
+          for (int u = 0; u < TB; u++)
+            acc[u] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[q][kk], b[u][q][kk], acc[u], 0, 0, 0);
+      unsigned cm = 0; // bit 4 u + r: tile u, output row r passes
+#pragma unroll
+      for (int u = 0; u < TB; u++) {
+        const int j = (jt0 + u) * 16 + l16;
         const float nx = Bn[j];
 #pragma unroll
         for (int r = 0; r < 4; r++) {
-          const float d2 = __builtin_fmaf(-2.0f, acc[r], ny[r] + nx);
-          const int ti = 4 * g + r;
-          if (d2 <= th[r] && tbase + ti < P.NP) {
-            const int slot = atomicAdd(&qcnt[w * 16 + ti], 1);
-            queue[(w * 16 + ti) * MCH + slot] = (unsigned char)j;
-          }
+          const float d2 = __builtin_fmaf(-2.0f, acc[u][r], ny[r] + nx);
+          cm |= ((d2 <= th[r]) & tok[r] & (j < cn)) ? 1u << (4 * u + r) : 0u;
+        }
+      }
+      if (__ballot(cm != 0)) {
+        for (unsigned m = cm; m; m &= m - 1) {
+          const int bit = __builtin_ctz(m);
+          const int ti = 4 * g + (bit & 3);
+          const int j = (jt0 + (bit >> 2)) * 16 + l16;
+          const int slot = atomicAdd(&qcnt[w * 16 + ti], 1);
+          queue[(w * 16 + ti) * MCH + slot] = (unsigned char)j;
         }
       }
     }
     __syncthreads();
-    // owner lanes: exact FP64 recomputation of their queued candidates
+    // exact FP64 recomputation: lane (target lane % 16, slot lane / 16) takes queued candidate `slot`
+    // of its target, so the first PAR candidates of all 16 targets cost one round of loads
+    {
+      const int ti = lane & 15, sl = lane >> 4;
+      const int qi = w * 16 + ti;
+      if (tbase + ti < P.NP && sl < qcnt[qi]) {
+        const int j = queue[qi * MCH + sl];
+        accq[qi * PAR + sl] = dist_f64<KP>(P.M + (size_t)(c0 + j) * P.Ea, P.Mp + (size_t)(tbase + ti) * P.Ea, P.Ea);
+      }
+    }
+    lds_wave_sync();
+    // owner lanes: the candidates in queue order (later ones recomputed here), into the exact list
     if (owner) {
       const int nq = qcnt[w * 16 + lane];
       const double* y = P.Mp + (size_t)ot * P.Ea;
       for (int e = 0; e < nq; e++) {
         const int j = queue[(w * 16 + lane) * MCH + e];
-        const double* x = P.M + (size_t)(c0 + j) * P.Ea;
-        double acc = 0.0;
-        // four coordinates per step: their loads are in flight together, the sum stays in order
-        for (int d0 = 0; d0 < P.Ea; d0 += 4) {
-          double xv[4], yv[4];
-#pragma unroll
-          for (int u = 0; u < 4; u++) {
-            xv[u] = d0 + u < P.Ea ? x[d0 + u] : 0.0;
-            yv[u] = d0 + u < P.Ea ? y[d0 + u] : 0.0;
-          }
-#pragma unroll
-          for (int u = 0; u < 4; u++)
-            if (d0 + u < P.Ea) {
-              const double dd = xv[u] - yv[u];
-              acc += dd * dd;
-            }
-        }
+        const double acc =
+          e < PAR ? accq[(w * 16 + lane) * PAR + e] : dist_f64<KP>(P.M + (size_t)(c0 + j) * P.Ea, y, P.Ea);
         if (acc == 0.0 || acc == MISSING_D) {
           zeros++;
           continue;
