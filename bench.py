@@ -6,11 +6,12 @@ T = 20000 points, edm(t, y, x, E=3, library = 40 log-spaced sizes 10..19000, num
 from the embedding of y, simplex with k = E+1 = 4, 4000 tasks of 19998 predictions each.
 
 One step = every prediction of the sweep (fedm_plan_run), inputs resident on the GPU.  Reported:
-  value          predictions/s of the batched mixed-precision route (FP32 screening + FP64 re-rank)
+  value          predictions/s of the batched mixed-precision route (long libraries: FP32-screened rank
+                 tables with FP64 re-rank; short libraries: the exact FP64 direct kernel)
   plain          the plain FP64 drop-in route, one edm_predict per task (the reference's per-task seam)
-  plain_batched  FP64 in the batched launch (isolates the precision from the batching)
-  roofline       the batched kernel's algorithmic FP32 rate (3 E_a flops per library-target pair)
-                 from HIP-event kernel times, against the MI355X FP32 peak
+  plain_batched  FP64 in the batched launch, every library through the direct kernel
+  roofline       the stage that dominates the device time, with its algorithmic rate from HIP-event
+                 kernel times against the matching MI355X peak
   cpu_baseline   the CPU oracle (a restatement of the reference CPU path, one thread per host core)
                  timed on a bounded sample of the same sweep
 """
@@ -108,7 +109,7 @@ def roofline(st, libs, reps):
     work = {
         # libraries below the rank threshold are short: the direct kernel runs its FP64 variant
         "direct": ("mfma", 9.0 * reps * float(np.sum(direct_libs)) * N_TARGETS, "TFLOP/s", FP64_PEAK_TFLOPS,
-                   "batch_simplex<4,4,false>"),
+                   "batch_simplex<4,3>"),
         "rank_build": ("mfma", 2 * 9.0 * pool * N_TARGETS, "TFLOP/s", FP32_PEAK_TFLOPS, "rank_build<4>"),
         "rank_scan": ("hbm", reps * float(np.sum(K * pool / rank_libs * 4.0 + K * 8.0 + 16.0)) * N_TARGETS, "GB/s",
                       HBM_PEAK_GBS, "rank_scan_multi<4,1> + rank_scatter"),

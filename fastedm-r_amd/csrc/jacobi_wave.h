@@ -24,10 +24,11 @@ __device__ __forceinline__ Rot make_jacobi(double x, double y, double z)
     return { 1.0, 0.0 };
   const double tau = (x - z) / deno;
   const double w = sqrt(tau * tau + 1.0);
-  const double t = tau > 0.0 ? 1.0 / (tau + w) : 1.0 / (tau - w);
+  const double t = 1.0 / (tau > 0.0 ? tau + w : tau - w);
   const double sign_t = t > 0.0 ? 1.0 : -1.0;
   const double nrm = 1.0 / sqrt(t * t + 1.0);
-  const double s = -sign_t * (y / fabs(y)) * fabs(t) * nrm;
+  // y / |y| is exactly +-1 for finite nonzero y
+  const double s = -sign_t * copysign(1.0, y) * fabs(t) * nrm;
   return { nrm, s };
 }
 

@@ -166,6 +166,31 @@ __device__ __forceinline__ double row_dist(const double* x, const double (&y)[EM
 
 constexpr int SUB = 32; // rows screened together against the threshold they start with
 
+// The rows of `mask` (bit r: LDS row x + r EMAX, library position pos0 + r) in position order, each
+// against the list as it stands.  The update runs on every lane without branches: a row that does not
+// enter passes an infinite key, which changes nothing.
+template <int KCAP, int EMAX, bool FULL>
+__device__ __forceinline__ void walk_rows(unsigned mask, const double* x, const double (&y)[EMAX], int pos0, int k,
+                                          double (&la)[KCAP], int (&li)[KCAP], double& kth, double& thr_g, int& zeros,
+                                          bool& near)
+{
+  while (mask) {
+    const int r = __builtin_ctz(mask);
+    mask &= mask - 1;
+    const double acc = row_dist<EMAX>(x + r * EMAX, y);
+    // keep rule; a sum equal to MISSING cannot occur on this route (coordinates below 1e30)
+    const bool drop = acc == 0.0 || acc == MISSING_D;
+    zeros += drop ? 1 : 0;
+    // kth < acc < kth (1 + 2^-49): its square root may equal the k-th's
+    near = near | (!drop & (acc > kth) & (acc < thr_g));
+    const double out = kth; // leaves the k-th place when the row enters
+    insert_acc<KCAP>(la, li, (!drop & (acc < kth)) ? acc : INF_D, pos0 + r);
+    kth = FULL ? la[KCAP - 1] : kth_of<KCAP>(la, k);
+    near = near | ((out > kth) & (out < kth * SQRT_GUARD));
+    thr_g = kth * SQRT_GUARD;
+  }
+}
+
 template <int KCAP, int EMAX>
 __global__ __launch_bounds__(256) void batch_simplex(BatchParams B)
 {
@@ -262,23 +287,12 @@ __global__ __launch_bounds__(256) void batch_simplex(BatchParams B)
             for (int r = 0; r < SUB; r++)
               mask |= row_dist<EMAX>(lds64 + (s0 + r) * EMAX, y) < thr_g ? 1u << r : 0u;
           }
-          // the passing rows in position order, each against the list as it stands
-          while (mask) {
-            const int r = __builtin_ctz(mask);
-            mask &= mask - 1;
-            const double acc = row_dist<EMAX>(lds64 + (s0 + r) * EMAX, y);
-            // keep rule; a sum equal to MISSING cannot occur on this route (coordinates below 1e30)
-            const bool drop = acc == 0.0 || acc == MISSING_D;
-            zeros += drop ? 1 : 0;
-            // kth < acc < kth (1 + 2^-49): its square root may equal the k-th's
-            near = near | (!drop & (acc > kth) & (acc < thr_g));
-            // the update runs on every lane (an infinite key changes nothing), without branches
-            const double out = kth; // leaves the k-th place when the row enters
-            insert_acc<KCAP>(la, li, (!drop & (acc < kth)) ? acc : INF_D, c0 + s0 + r);
-            kth = k == KCAP ? la[KCAP - 1] : kth_of<KCAP>(la, k);
-            near = near | ((out > kth) & (out < kth * SQRT_GUARD));
-            thr_g = kth * SQRT_GUARD;
-          }
+          // the passing rows in position order, each against the list as it stands (k = KCAP, the
+          // common case, reads the k-th entry directly)
+          if (k == KCAP)
+            walk_rows<KCAP, EMAX, true>(mask, lds64 + s0 * EMAX, y, c0 + s0, k, la, li, kth, thr_g, zeros, near);
+          else
+            walk_rows<KCAP, EMAX, false>(mask, lds64 + s0 * EMAX, y, c0 + s0, k, la, li, kth, thr_g, zeros, near);
         }
       }
       if (!active)
