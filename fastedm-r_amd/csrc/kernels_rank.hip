@@ -35,6 +35,7 @@ __global__ __launch_bounds__(256) void rank_build(RankParams R)
   __shared__ double cs[RANK_SORT];
   __shared__ int cq[RANK_SORT];
   __shared__ int ncand, nzero, overflow, cut_bin, all_rows;
+  __shared__ unsigned wsum[4];
   __shared__ double smax_sh;
   __shared__ int zr[RANK_ZCAP];
 
@@ -96,16 +97,48 @@ __global__ __launch_bounds__(256) void rank_build(RankParams R)
         atomicAdd(&hist[min(RANK_BINS - 1, (int)(__float_as_uint(d[u]) >> 20))], 1u);
   }
   __syncthreads();
-  if (threadIdx.x == 0) {
-    unsigned run = 0;
-    int b = 0;
-    for (; b < RANK_BINS; b++) {
-      run += hist[b];
-      if (run >= (unsigned)RANK_TARGET)
-        break;
+  {
+    // the first bin at which the running count reaches RANK_TARGET (RANK_BINS when it never does):
+    // each thread sums PER consecutive bins, a block scan places the sums, and the thread whose range
+    // crosses RANK_TARGET finds the bin (the block has 256 threads)
+    constexpr int PER = RANK_BINS / 256;
+    unsigned loc[PER], sum = 0;
+#pragma unroll
+    for (int i = 0; i < PER; i++) {
+      loc[i] = hist[PER * threadIdx.x + i];
+      sum += loc[i];
     }
-    all_rows = (R.npool <= RANK_CAP) ? 1 : 0;
-    cut_bin = b;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    unsigned incl = sum;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const unsigned v = __shfl_up(incl, off);
+      if (lane >= off)
+        incl += v;
+    }
+    if (lane == 63)
+      wsum[wv] = incl;
+    if (threadIdx.x == 0) {
+      all_rows = (R.npool <= RANK_CAP) ? 1 : 0;
+      cut_bin = RANK_BINS;
+    }
+    __syncthreads();
+    for (int v = 0; v < wv; v++)
+      incl += wsum[v];
+    const unsigned excl = incl - sum;
+    if (excl < (unsigned)RANK_TARGET && incl >= (unsigned)RANK_TARGET) {
+      unsigned run = excl;
+      int b = PER * threadIdx.x;
+#pragma unroll
+      for (int i = 0; i < PER; i++) {
+        run += loc[i];
+        if (run >= (unsigned)RANK_TARGET) {
+          b = PER * threadIdx.x + i;
+          break;
+        }
+      }
+      cut_bin = b;
+    }
   }
   __syncthreads();
 
@@ -196,12 +229,26 @@ __global__ __launch_bounds__(256) void rank_build(RankParams R)
     cq[c] = INT_MAX;
   }
   __syncthreads();
-  // bitonic sort by (s, row) over the next power of two covering the candidates
-  for (int size = 2; size <= width; size <<= 1) {
-    for (int stride = size >> 1; stride > 0; stride >>= 1) {
-      for (int i = threadIdx.x; i < width; i += blockDim.x) {
-        const int j = i ^ stride;
-        if (j > i) {
+  // Bitonic sort by (s, row) over the next power of two covering the candidates.  Thread t takes the
+  // pairs t, t + 256, ...; for strides up to 64 the pairs of one wavefront stay inside its own
+  // 128-element blocks, so only the stages with a longer stride, and their neighbours, need the
+  // whole block to wait.
+  {
+    bool prev_cross = true;
+    for (int size = 2; size <= width; size <<= 1) {
+      for (int stride = size >> 1; stride > 0; stride >>= 1) {
+        const bool cross = stride > 64;
+        if (cross || prev_cross) {
+          __syncthreads();
+        } else {
+          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+          __builtin_amdgcn_wave_barrier();
+          __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+        }
+        prev_cross = cross;
+        for (int t = threadIdx.x; t < width / 2; t += blockDim.x) {
+          const int i = ((t & ~(stride - 1)) << 1) | (t & (stride - 1));
+          const int j = i + stride;
           const bool asc = (i & size) == 0;
           const double si = cs[i], sj = cs[j];
           const int qi = cq[i], qj = cq[j];
@@ -214,8 +261,8 @@ __global__ __launch_bounds__(256) void rank_build(RankParams R)
           }
         }
       }
-      __syncthreads();
     }
+    __syncthreads();
   }
   int kept = 0;
   if (threadIdx.x == 0) {
