@@ -198,6 +198,7 @@ def main():
     kms = float(st[0])
     print(f"mixed: {sec * 1e3:.1f} ms/step, device {st[0]:.1f} ms (rank build {st[1]:.1f}, rank scan {st[2]:.1f}, "
           f"direct {st[3]:.1f})", file=sys.stderr, flush=True)
+    redos = fe.exact_redos()
     res_mixed = mixed.result()
 
     out = {"metric": METRIC, "value": n_pred / sec, "unit": "predictions/s", "n_gpus": 1, "steps": args.steps,
@@ -212,6 +213,7 @@ def main():
 
     out["stages_ms"] = {"device_total": float(st[0]), "rank_build": float(st[1]), "rank_scan": float(st[2]),
                         "direct": float(st[3])}
+    out["exact_reselections"] = redos  # (task, target) pairs of the last step handed to the exact selection
     out["roofline"] = roofline(st, libs, args.reps)
 
     # plain forms and parity

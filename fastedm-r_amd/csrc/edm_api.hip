@@ -1308,7 +1308,7 @@ int edm_batch_run(edm_batch* b)
     const int2* none = nullptr;
     int zero = 0;
     void* rargs[] = { &B, &none, &zero };
-    HIP_TRY(hipLaunchKernel(batch_redo_kernel(b->kcap), dim3(64), dim3(256), rargs, 0, s));
+    HIP_TRY(hipLaunchKernel(batch_redo_kernel(b->kcap), dim3(256), dim3(256), rargs, 0, s));
     int handed = 0;
     HIP_TRY(hipMemcpyAsync(&handed, B.redo_count, sizeof(int), hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));

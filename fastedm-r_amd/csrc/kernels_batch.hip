@@ -396,7 +396,103 @@ __device__ void exact_target(const BatchParams& B, int tk, int t)
   finish_target<KCAP>(B, tk, t, np, lrows, nl, zeros, k, ls, li);
 }
 
-// items == null: the (task, target) pairs handed over (at most redo_cap of them);
+// The CPU path's selection for one (task, target) on a wavefront: lane l takes library positions l,
+// l + 64, ... and keeps its own k nearest by (sqrt distance, position) as they arrive; the wavefront
+// then merges the 64 lists, smallest (sqrt distance, position) first, into the task's list (it holds
+// the k nearest of the union), and lane 0 finishes the target.
+template <int KCAP>
+__device__ void exact_target_wave(const BatchParams& B, int tk, int t, int lane)
+{
+  const int np = B.pred_count[tk];
+  const int nl = B.lib_count[tk];
+  const int Ea = B.Ea;
+  const double* yp = B.P + (size_t)B.pred_rows[B.pred_begin[tk] + t] * Ea;
+  const int* lrows = B.lib_rows + B.lib_begin[tk];
+  const int k = B.k[tk];
+  double ls[KCAP];
+  int li[KCAP];
+#pragma unroll
+  for (int j = 0; j < KCAP; j++) {
+    ls[j] = INF_D;
+    li[j] = INT_MAX;
+  }
+  double s_kth = INF_D, thr = INF_D;
+  int zeros = 0;
+  // four of the lane's positions at a time, so that their loads are in flight together
+  constexpr int RW = 4;
+  for (int c0 = lane; c0 < nl; c0 += 64 * RW) {
+    const double* xr[RW];
+    double acc[RW];
+#pragma unroll
+    for (int u = 0; u < RW; u++) {
+      const int c = c0 + 64 * u;
+      xr[u] = B.L + (size_t)lrows[c < nl ? c : c0] * Ea;
+      acc[u] = 0.0;
+    }
+    for (int j = 0; j < Ea; j++) {
+      const double yj = yp[j];
+#pragma unroll
+      for (int u = 0; u < RW; u++) {
+        const double d = xr[u][j] - yj;
+        acc[u] += d * d;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < RW; u++) {
+      const int c = c0 + 64 * u;
+      if (c >= nl)
+        break;
+      if (acc[u] == 0.0 || acc[u] == MISSING_D) {
+        zeros++;
+        continue;
+      }
+      if (acc[u] < thr) {
+        const double s = sqrt(acc[u]);
+        if (s < s_kth) {
+          insert_sqrt<KCAP>(ls, li, s, c);
+          s_kth = kth_of<KCAP>(ls, k);
+          if (s_kth != INF_D)
+            thr = s_kth * s_kth * SQRT_GUARD;
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1)
+    zeros += __shfl_xor(zeros, off);
+  double ms[KCAP];
+  int mi[KCAP];
+#pragma unroll
+  for (int r = 0; r < KCAP; r++) {
+    // the smallest (sqrt distance, position) among the lanes' heads; positions are distinct
+    double bs = ls[0];
+    int bi = li[0], bl = lane;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      const double os = __shfl_xor(bs, off);
+      const int oi = __shfl_xor(bi, off), ol = __shfl_xor(bl, off);
+      const bool take = os < bs || (os == bs && (oi < bi || (oi == bi && ol < bl)));
+      bs = take ? os : bs;
+      bi = take ? oi : bi;
+      bl = take ? ol : bl;
+    }
+    ms[r] = bs;
+    mi[r] = bi;
+    if (lane == bl) {
+#pragma unroll
+      for (int j = 0; j + 1 < KCAP; j++) {
+        ls[j] = ls[j + 1];
+        li[j] = li[j + 1];
+      }
+      ls[KCAP - 1] = INF_D;
+      li[KCAP - 1] = INT_MAX;
+    }
+  }
+  if (lane == 0)
+    finish_target<KCAP>(B, tk, t, np, lrows, nl, zeros, k, ms, mi);
+}
+
+// items == null: the (task, target) pairs handed over (at most redo_cap of them), one per wavefront;
 // otherwise every target of the given (task, first target) items (used when more were handed over).
 template <int KCAP>
 __global__ __launch_bounds__(256) void batch_redo(BatchParams B, const int2* items, int num_items)
@@ -410,9 +506,11 @@ __global__ __launch_bounds__(256) void batch_redo(BatchParams B, const int2* ite
     }
   } else {
     const int n = min(*B.redo_count, B.redo_cap);
-    for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += gridDim.x * blockDim.x) {
+    const int lane = threadIdx.x & 63;
+    const int waves = gridDim.x * (blockDim.x >> 6);
+    for (int r = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); r < n; r += waves) {
       const int2 e = B.redo[r];
-      exact_target<KCAP>(B, e.x, e.y);
+      exact_target_wave<KCAP>(B, e.x, e.y, lane);
     }
   }
 }
