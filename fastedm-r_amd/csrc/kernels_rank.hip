@@ -539,7 +539,7 @@ __global__ __launch_bounds__(64 * MS_WAVES) void rank_scan_multi(ScanParams S)
 {
   constexpr int NP = Planes<KCAP>::NP;
   extern __shared__ unsigned ms_lds[];
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int gw = blockIdx.x * MS_WAVES + w;
   const int t = gw / S.parts;
   if (t >= S.np)
@@ -591,13 +591,23 @@ __global__ __launch_bounds__(64 * MS_WAVES) void rank_scan_multi(ScanParams S)
     if (!__ballot(live > 0))
       break;
     const int n = min(64, len - c0);
-    const int row_l = lane < n ? rr[(size_t)(c0 + lane) * S.jstride] : 0;
     unsigned mg[G][WPL];
+    // the prefix rows are contiguous here (jstride 1) and the same for every lane: scalar loads
     auto load_group = [&](int g0) {
+      int rows[G];
+      if (g0 + G <= n) {
+#pragma unroll
+        for (int u = 0; u < G; u++)
+          rows[u] = rr[c0 + g0 + u];
+      } else {
+#pragma unroll
+        for (int u = 0; u < G; u++)
+          rows[u] = g0 + u < n ? rr[c0 + g0 + u] : 0;
+      }
 #pragma unroll
       for (int u = 0; u < G; u++) {
         const int i = g0 + u;
-        const unsigned* mrow = S.member + (size_t)__shfl(row_l, i < n ? i : 0) * S.words + w0;
+        const unsigned* mrow = S.member + (size_t)rows[u] * S.words + w0;
 #pragma unroll
         for (int q = 0; q < WPL; q++)
           mg[u][q] = (i < n && lane + 64 * q < W) ? mrow[lane + 64 * q] : 0u;
