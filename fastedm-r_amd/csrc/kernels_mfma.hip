@@ -188,24 +188,28 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
 #pragma unroll
           for (int u = 0; u < TB; u++)
             acc[u] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[q][kk], b[u][q][kk], acc[u], 0, 0, 0);
-      unsigned cm = 0; // bit 4 u + r: tile u, output row r passes
+      float d2[TB][4];
+      bool any = false; // some value passes the threshold (validity checked on the slow path)
 #pragma unroll
       for (int u = 0; u < TB; u++) {
-        const int j = (jt0 + u) * 16 + l16;
-        const float nx = Bn[j];
+        const float nx = Bn[(jt0 + u) * 16 + l16];
 #pragma unroll
         for (int r = 0; r < 4; r++) {
-          const float d2 = __builtin_fmaf(-2.0f, acc[u][r], ny[r] + nx);
-          cm |= ((d2 <= th[r]) & tok[r] & (j < cn)) ? 1u << (4 * u + r) : 0u;
+          d2[u][r] = __builtin_fmaf(-2.0f, acc[u][r], ny[r] + nx);
+          any = any | (d2[u][r] <= th[r]);
         }
       }
-      if (__ballot(cm != 0)) {
-        for (unsigned m = cm; m; m &= m - 1) {
-          const int bit = __builtin_ctz(m);
-          const int ti = 4 * g + (bit & 3);
-          const int j = (jt0 + (bit >> 2)) * 16 + l16;
-          const int slot = atomicAdd(&qcnt[w * 16 + ti], 1);
-          queue[(w * 16 + ti) * MCH + slot] = (unsigned char)j;
+      if (__ballot(any)) {
+#pragma unroll
+        for (int u = 0; u < TB; u++) {
+          const int j = (jt0 + u) * 16 + l16;
+#pragma unroll
+          for (int r = 0; r < 4; r++)
+            if ((d2[u][r] <= th[r]) & tok[r] & (j < cn)) {
+              const int ti = 4 * g + r;
+              const int slot = atomicAdd(&qcnt[w * 16 + ti], 1);
+              queue[(w * 16 + ti) * MCH + slot] = (unsigned char)j;
+            }
         }
       }
     }
