@@ -18,6 +18,8 @@ namespace fedm {
 
 constexpr int RANK_BINS = 2048;   // FP32 bit-pattern bins: 8 exponent + 3 mantissa bits
 constexpr int RANK_TARGET = 600;  // rows wanted below the histogram cut
+constexpr int RANK_SAMPLE = 8;    // the histogram counts one pool row in RANK_SAMPLE (the hash picks 0..7)
+constexpr int RANK_SAMPLE_TARGET = (RANK_TARGET + RANK_SAMPLE - 1) / RANK_SAMPLE;
 constexpr int RANK_CAP = 1536;    // LDS candidate capacity
 constexpr int RANK_SORT = 2048;   // bitonic sort width (power of two >= RANK_CAP)
 __device__ __forceinline__ float up_float(double x)
@@ -80,27 +82,32 @@ __global__ __launch_bounds__(256) void rank_build(RankParams R)
     return acc;
   };
 
-  // pass 1: histogram of the FP32 squared distances' bit patterns
+  // pass 1: histogram of the FP32 squared distances' bit patterns over one pool row in each run of
+  // RANK_SAMPLE, at a hashed offset (a fixed stride would alias with periodic series).  The cut only
+  // sizes the prefix (any cut gives an exact prefix): the sample puts about RANK_TARGET rows below
+  // it, and pass 2 lowers it while the rows below overflow RANK_CAP.
   for (int b = threadIdx.x; b < RANK_BINS; b += blockDim.x)
     hist[b] = 0;
   __syncthreads();
-  for (int q0 = threadIdx.x; q0 < R.npool; q0 += UNR * blockDim.x) {
+  const int nsample = (R.npool + RANK_SAMPLE - 1) / RANK_SAMPLE;
+  for (int i0 = threadIdx.x; i0 < nsample; i0 += UNR * blockDim.x) {
     float d[UNR];
 #pragma unroll
     for (int u = 0; u < UNR; u++) {
-      const int q = q0 + u * blockDim.x;
-      d[u] = q < R.npool ? d32_of(q) : 0.0f;
+      const int i = i0 + u * blockDim.x;
+      const int q = i * RANK_SAMPLE + (int)(((unsigned)i * 2654435761u) >> 29);
+      d[u] = (i < nsample && q < R.npool) ? d32_of(q) : -1.0f;
     }
 #pragma unroll
     for (int u = 0; u < UNR; u++)
-      if (q0 + u * (int)blockDim.x < R.npool)
+      if (d[u] >= 0.0f)
         atomicAdd(&hist[min(RANK_BINS - 1, (int)(__float_as_uint(d[u]) >> 20))], 1u);
   }
   __syncthreads();
   {
-    // the first bin at which the running count reaches RANK_TARGET (RANK_BINS when it never does):
-    // each thread sums PER consecutive bins, a block scan places the sums, and the thread whose range
-    // crosses RANK_TARGET finds the bin (the block has 256 threads)
+    // the first bin at which the running count reaches RANK_SAMPLE_TARGET (RANK_BINS when it never
+    // does): each thread sums PER consecutive bins, a block scan places the sums, and the thread whose
+    // range crosses the target finds the bin (the block has 256 threads)
     constexpr int PER = RANK_BINS / 256;
     unsigned loc[PER], sum = 0;
 #pragma unroll
@@ -126,13 +133,13 @@ __global__ __launch_bounds__(256) void rank_build(RankParams R)
     for (int v = 0; v < wv; v++)
       incl += wsum[v];
     const unsigned excl = incl - sum;
-    if (excl < (unsigned)RANK_TARGET && incl >= (unsigned)RANK_TARGET) {
+    if (excl < (unsigned)RANK_SAMPLE_TARGET && incl >= (unsigned)RANK_SAMPLE_TARGET) {
       unsigned run = excl;
       int b = PER * threadIdx.x;
 #pragma unroll
       for (int i = 0; i < PER; i++) {
         run += loc[i];
-        if (run >= (unsigned)RANK_TARGET) {
+        if (run >= (unsigned)RANK_SAMPLE_TARGET) {
           b = PER * threadIdx.x + i;
           break;
         }
