@@ -1309,24 +1309,24 @@ int edm_batch_run(edm_batch* b)
     int zero = 0;
     void* rargs[] = { &B, &none, &zero };
     HIP_TRY(hipLaunchKernel(batch_redo_kernel(b->kcap), dim3(256), dim3(256), rargs, 0, s));
-    int handed = 0;
-    HIP_TRY(hipMemcpyAsync(&handed, B.redo_count, sizeof(int), hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipStreamSynchronize(s));
-    if (handed > B.redo_cap) {
-      // more hand-overs than the list holds: repeat every target with the exact selection
-      const int2* items[2] = { B.items, b->rank_route ? b->S.items : nullptr };
-      int counts[2] = { B.num_items, b->rank_route ? b->S.num_items : 0 };
-      for (int q = 0; q < 2; q++)
-        if (counts[q] > 0) {
-          void* aargs[] = { &B, &items[q], &counts[q] };
-          HIP_TRY(hipLaunchKernel(batch_redo_kernel(b->kcap), dim3(b->grid), dim3(256), aargs, 0, s));
-        }
-    }
-    b->redo_last = handed;
-    c->last_redo = handed;
+    // more hand-overs than the list holds: every target again with the exact selection (these
+    // launches return at once otherwise, so the host does not wait for the count mid-step)
+    const int2* items[2] = { B.items, b->rank_route ? b->S.items : nullptr };
+    int counts[2] = { B.num_items, b->rank_route ? b->S.num_items : 0 };
+    for (int q = 0; q < 2; q++)
+      if (counts[q] > 0) {
+        void* aargs[] = { &B, &items[q], &counts[q] };
+        HIP_TRY(hipLaunchKernel(batch_redo_kernel(b->kcap), dim3(b->grid), dim3(256), aargs, 0, s));
+      }
   }
   HIP_TRY(hipEventRecord(c->ev[5], s));
   HIP_TRY(hipEventSynchronize(c->ev[5]));
+  {
+    int handed = 0;
+    HIP_TRY(hipMemcpy(&handed, B.redo_count, sizeof(int), hipMemcpyDeviceToHost));
+    b->redo_last = handed;
+    c->last_redo = handed;
+  }
   b->last_ms = ev_ms(c->ev[4], c->ev[5]);
   // [0] whole batch, [1] rank-table build, [2] rank scan, [3] direct kernel
   c->timings[0] = b->last_ms;

@@ -493,11 +493,14 @@ __device__ void exact_target_wave(const BatchParams& B, int tk, int t, int lane)
 }
 
 // items == null: the (task, target) pairs handed over (at most redo_cap of them), one per wavefront;
-// otherwise every target of the given (task, first target) items (used when more were handed over).
+// otherwise every target of the given (task, first target) items, when more pairs were handed over
+// than the list holds (the launch returns at once when they were not).
 template <int KCAP>
 __global__ __launch_bounds__(256) void batch_redo(BatchParams B, const int2* items, int num_items)
 {
   if (items) {
+    if (*B.redo_count <= B.redo_cap)
+      return;
     for (int item = blockIdx.x; item < num_items; item += gridDim.x) {
       const int2 it = items[item];
       const int t = it.y + threadIdx.x;
