@@ -96,7 +96,9 @@ __global__ __launch_bounds__(256) void rank_build(RankParams R)
     for (int u = 0; u < UNR; u++) {
       const int i = i0 + u * blockDim.x;
       const int q = i * RANK_SAMPLE + (int)(((unsigned)i * 2654435761u) >> 29);
-      d[u] = (i < nsample && q < R.npool) ? d32_of(q) : -1.0f;
+      // the load is unconditional (index clamped) so that the UNR loads are in flight together
+      const float dq = d32_of(min(q, R.npool - 1));
+      d[u] = (i < nsample && q < R.npool) ? dq : -1.0f;
     }
 #pragma unroll
     for (int u = 0; u < UNR; u++)
@@ -167,7 +169,8 @@ __global__ __launch_bounds__(256) void rank_build(RankParams R)
 #pragma unroll
       for (int u = 0; u < UNR; u++) {
         const int q = q0 + u * blockDim.x;
-        d[u] = q < R.npool ? d32_of(q) : __builtin_huge_valf();
+        const float dq = d32_of(min(q, R.npool - 1)); // unconditional, as in pass 1
+        d[u] = q < R.npool ? dq : __builtin_huge_valf();
       }
 #pragma unroll
       for (int u = 0; u < UNR; u++) {
