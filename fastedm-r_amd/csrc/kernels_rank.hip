@@ -755,8 +755,15 @@ __global__ __launch_bounds__(64 * MS_WAVES) void rank_scan_multi(ScanParams S)
 __global__ __launch_bounds__(256) void rank_scatter(ScanParams S)
 {
   __shared__ double tile[64][65];
+  __shared__ int64_t obeg[64]; // output offsets of the tile's 64 tasks
+  __shared__ int kk[64];
   const int tau0 = blockIdx.x * 64, t0 = blockIdx.y * 64, th = blockIdx.z;
   const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  if (ty == 0 && tau0 + tx < S.num_rank) {
+    const int task = S.rank_task[tau0 + tx];
+    obeg[tx] = S.out_begin[task];
+    kk[tx] = S.k[task];
+  }
   for (int r = ty; r < 64; r += 4) {
     const int t = t0 + r, tau = tau0 + tx;
     tile[r][tx] = (t < S.np && tau < S.num_rank) ? S.stage[((size_t)th * S.np + t) * S.num_rank + tau] : 0.0;
@@ -765,12 +772,11 @@ __global__ __launch_bounds__(256) void rank_scatter(ScanParams S)
   for (int r = ty; r < 64; r += 4) {
     const int tau = tau0 + r, t = t0 + tx;
     if (tau < S.num_rank && t < S.np) {
-      const int task = S.rank_task[tau];
-      const int64_t ob = S.out_begin[task];
+      const int64_t ob = obeg[r];
       S.pred_out[ob + (int64_t)th * S.np + t] = tile[tx][r];
       S.rc[ob + (int64_t)th * S.np + t] = RC_SUCCESS;
       if (th == 0)
-        S.k_used[ob / S.T + t] = S.k[task];
+        S.k_used[ob / S.T + t] = kk[r];
     }
   }
 }
