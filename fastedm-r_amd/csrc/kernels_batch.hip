@@ -6,7 +6,7 @@
 // coordinates (a zero coordinate adds exactly 0 to the sum, so the padded sum is the CPU path's sum,
 // src/distances.cpp:186-216).  Both precision forms use this kernel: FP64 arithmetic runs at the FP32
 // rate on CDNA4 vector units, so FP32 screening saves nothing here.  The cost lies in the list
-// updates, which diverge across a wavefront: each thread screens 32 rows against its threshold
+// updates, which diverge across a wavefront: each thread screens 16 rows against its threshold
 // first and then walks only the rows that passed, so a wavefront runs as many updates as its busiest
 // thread needs rather than one per row in which any thread updates.
 //
@@ -164,7 +164,7 @@ __device__ __forceinline__ double row_dist(const double* x, const double (&y)[EM
   return acc;
 }
 
-constexpr int SUB = 32; // rows screened together against the threshold they start with
+constexpr int SUB = 16; // rows screened together against the threshold they start with
 
 // The rows of `mask` (bit r: LDS row x + r EMAX, library position pos0 + r) in position order, each
 // against the list as it stands.  The update runs on every lane without branches: a row that does not
