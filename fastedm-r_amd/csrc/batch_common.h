@@ -8,6 +8,10 @@ namespace fedm {
 struct BatchParams
 {
   const double* L;      // library-side point table [NLrows x Ea] (FP64)
+  const double* Lg;     // the direct kernel's tasks' library rows, gathered per task [.. x Ea] at gbeg[task]
+  const double* Tg;     // their targets, gathered the same way
+  const int64_t* gbeg;  // [tasks] first gathered row of a direct-kernel task
+  const double* Pg;     // prediction rows in prediction-list order [pred_total x Ea]
   const float* L32;     // FP32 copy of L (MIXED)
   const double* Ltgt;   // library-side targets [NLrows]
   const double* P;      // prediction-side point table [NProws x Ea]

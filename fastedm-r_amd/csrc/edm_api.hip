@@ -1081,6 +1081,35 @@ int edm_batch_create(edm_ctx* c, const edm_manifold_view* L, const edm_manifold_
   }
   B.items = b->dev(items.data(), items.size());
   B.num_items = (int)items.size();
+  {
+    // the direct kernel's tasks: library rows and targets gathered contiguously per task, and the
+    // prediction rows in list order, so that its loads need no row-list indirection
+    std::vector<int64_t> gbeg(num_tasks, 0);
+    int64_t gtot = 0;
+    for (int t = 0; t < num_tasks; t++)
+      if (!(b->rank_route && is_rank[t])) {
+        gbeg[t] = gtot;
+        gtot += lib_count[t];
+      }
+    std::vector<double> lg((size_t)gtot * Ea), tg((size_t)gtot);
+    for (int t = 0; t < num_tasks; t++)
+      if (!(b->rank_route && is_rank[t]))
+        for (int i = 0; i < lib_count[t]; i++) {
+          const int row = lib_rows[lib_begin[t] + i];
+          std::copy(b->Ldata.begin() + (size_t)row * Ea, b->Ldata.begin() + (size_t)(row + 1) * Ea,
+                    lg.begin() + (size_t)(gbeg[t] + i) * Ea);
+          tg[(size_t)(gbeg[t] + i)] = b->Ltgt[row];
+        }
+    std::vector<double> pg((size_t)pred_total * Ea);
+    for (size_t i = 0; i < pred_total; i++)
+      std::copy(b->Pdata.begin() + (size_t)pred_rows[i] * Ea, b->Pdata.begin() + (size_t)(pred_rows[i] + 1) * Ea,
+                pg.begin() + i * Ea);
+    B.Lg = b->dev(lg.data(), lg.size());
+    B.Tg = b->dev(tg.data(), tg.size());
+    B.gbeg = b->dev(gbeg.data(), gbeg.size());
+    B.Pg = b->dev(pg.data(), pg.size());
+    hipStreamSynchronize(c->stream);
+  }
   if (b->rank_route) {
     RankParams& R = b->R;
     std::memset(&R, 0, sizeof(R));

@@ -85,9 +85,10 @@ __device__ __forceinline__ float float_up(double x)
 
 // k policy, neighbour order and simplex prediction of target t of task tk.  ls/li: the neighbour list
 // sorted by (sqrt distance, position); zeros: pairs dropped by the keep rule.
+// tgt: the task's library targets gathered by position, or null to read them through the row list.
 template <int KCAP>
 __device__ void finish_target(const BatchParams& B, int tk, int t, int np, const int* lrows, int nl, int zeros,
-                              int k, double (&ls)[KCAP], int (&li)[KCAP])
+                              int k, double (&ls)[KCAP], int (&li)[KCAP], const double* tgt = nullptr)
 {
   const int cnt = nl - zeros;
   const int T = B.T;
@@ -130,7 +131,7 @@ __device__ void finish_target(const BatchParams& B, int tk, int t, int np, const
   double tg[KCAP];
 #pragma unroll
   for (int j = 0; j < KCAP; j++)
-    tg[j] = j < ke ? B.Ltgt[lrows[li[j]]] : 0.0;
+    tg[j] = j < ke ? (tgt ? tgt[li[j]] : B.Ltgt[lrows[li[j]]]) : 0.0;
   for (int th = 0; th < T; th++) {
     const double theta = B.thetas[th];
     double w[KCAP];
@@ -216,14 +217,17 @@ __global__ __launch_bounds__(256) void batch_simplex(BatchParams B)
       const int nl = B.lib_count[tk];
       const int t = it.y + threadIdx.x;
       const bool active = t < np;
-      const int trow = active ? B.pred_rows[B.pred_begin[tk] + t] : 0;
       const int* lrows = B.lib_rows + B.lib_begin[tk];
       const int k = B.k[tk];
+      // this task's library rows and targets, gathered contiguously by position
+      const double* lg = B.Lg + (size_t)B.gbeg[tk] * Ea;
+      const double* tgt = B.Tg + B.gbeg[tk];
 
       double y[EMAX];
+      const double* yp = B.Pg + (size_t)(B.pred_begin[tk] + (active ? t : 0)) * Ea;
 #pragma unroll
       for (int j = 0; j < EMAX; j++)
-        y[j] = (active && j < Ea) ? B.P[(size_t)trow * Ea + j] : 0.0;
+        y[j] = (active && j < Ea) ? yp[j] : 0.0;
       double la[KCAP];
       int li[KCAP];
 #pragma unroll
@@ -241,7 +245,7 @@ __global__ __launch_bounds__(256) void batch_simplex(BatchParams B)
         __syncthreads();
         for (int c = threadIdx.x; c < cpad; c += blockDim.x) {
           if (c < cn) {
-            const double* src = B.L + (size_t)lrows[c0 + c] * Ea;
+            const double* src = lg + (size_t)(c0 + c) * Ea;
 #pragma unroll
             for (int j = 0; j < EMAX; j++)
               lds64[c * EMAX + j] = j < Ea ? src[j] : 0.0;
@@ -327,7 +331,7 @@ __global__ __launch_bounds__(256) void batch_simplex(BatchParams B)
           }
         }
       }
-      finish_target<KCAP>(B, tk, t, np, lrows, nl, zeros, k, ls, li);
+      finish_target<KCAP>(B, tk, t, np, lrows, nl, zeros, k, ls, li, tgt);
     } while (0);
     __syncthreads();
     item = item_sh;
