@@ -26,8 +26,7 @@ __global__ void finish_dense(FinishParams F, const double* D, int t0, int nt, co
 __global__ void wasserstein_dense(WassParams P, int t0, int nt, int slice_len, double* D, int* out_cnt);
 __global__ void select_wide(FinishParams F, const double* D, int t0, int nt, unsigned long long* gkey, int* gidx,
                             int gstride);
-__global__ void smap_multi(SmapParams S, const double* D, int t0, int nt);
-size_t smap_lds_bytes(int n);
+int smap_launch(int n, hipStream_t s, const SmapParams& S, const double* D, int t0, int nt, bool lds_rows);
 
 struct MfmaParams
 {
@@ -264,26 +263,12 @@ static void launch_finish_lists(int kcap, dim3 g, hipStream_t s, const FinishPar
 }
 
 // S-map systems of targets [t0, t0 + nt) at every theta: a persistent grid of one-wavefront blocks.
+// FEDM_SMAP_LDS_ROWS=1 keeps U and V in LDS for every n (the kernel used above 24 unknowns).
 static int launch_smap(edm_ctx* c, int n, hipStream_t s, const SmapParams& S, const double* D, int t0, int nt)
 {
-  if (n > 65)
-    return -22;
-  static int per_cu = [] {
-    hipFuncSetAttribute((const void*)smap_multi, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-    return 0;
-  }();
-  (void)per_cu;
-  const size_t lds = smap_lds_bytes(n);
-  const int G = std::max(1, std::min(8, 64 / n));
-  const long long groups = ((long long)nt * S.T + G - 1) / G;
-  int dev = 0, ncu = 0, occ = 0;
-  hipGetDevice(&dev);
-  hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
-  hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)smap_multi, 64, lds);
-  const int grid = (int)std::max<long long>(1, std::min<long long>(groups, (long long)ncu * std::max(occ, 1)));
   (void)c;
-  hipLaunchKernelGGL(smap_multi, dim3(grid), dim3(64), lds, s, S, D, t0, nt);
-  return 0;
+  const char* env = getenv("FEDM_SMAP_LDS_ROWS");
+  return smap_launch(n, s, S, D, t0, nt, env && env[0] == '1');
 }
 
 extern "C" {

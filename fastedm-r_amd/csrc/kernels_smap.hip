@@ -8,6 +8,7 @@
 // its own rotations.  The solve is the two-sided Jacobi SVD of Eigen 3.4's JacobiSVD, operation for
 // operation as in jacobi_wave.h, with W, U and V of every system in LDS.
 #include <algorithm>
+#include <type_traits>
 
 #include "kernels_common.h"
 #include "fdlibm_exp.h"
@@ -192,13 +193,168 @@ __device__ void jacobi_svd_solve_multi(int n, int G, double* W, double* U, doubl
   __syncthreads();
 }
 
-// Persistent grid over groups of G (target, theta) items of targets [t0, t0 + nt).
+// Forms the systems of the group of G (target, theta) items that starts at item i0: X^T X into the slot
+// W_g (both triangles) and X^T y into b_g, each entry summed neighbour by neighbour in neighbour order.
+// Returns bit g set when item i0 + g has a system.  Weighted neighbour rows are staged sch at a time in
+// stage, or, when stage is null, in the slot W_g of the system being formed (sch (n + 1) <= n^2).
+__device__ unsigned form_group(const SmapParams& S, const double* __restrict__ D, int t0, int nt, int i0, int n,
+                               int G, double* W, double* bvec, double* stage, int sch, double* acc, double* cd,
+                               int* cidx, int lane)
+{
+  const int Ea = n - 1, T = S.T;
+  const int ntri = n * (n + 1) / 2, nacc = ntri + n;
+  const int items = nt * T;
+  unsigned valid = 0; // bit g: item i0 + g has a system
+  for (int g = 0; g < G; g++) {
+    const int item = i0 + g;
+    if (item >= items)
+      break;
+    const int tl = item / T, th = item - (item / T) * T;
+    const int t = t0 + tl;
+    if (t >= S.NP)
+      continue;
+    const int kk = S.nn_k[t];
+    if (kk == 0)
+      continue;
+    valid |= 1u << g;
+    const bool dense = kk < 0;
+    const int k = dense ? S.nn_cnt[t] : kk;
+    const double* drow = dense ? D + (size_t)tl * S.NL : nullptr;
+    const int* li = S.nn_i + (size_t)(t - S.nn_t0) * S.nn_stride;
+    const double* lsd = S.nn_s + (size_t)(t - S.nn_t0) * S.nn_stride;
+    double* Xc = stage ? stage : W + (size_t)g * n * n;
+    double* yc = Xc + (size_t)sch * n;
+    // mean of the selected distances, summed in neighbour order (Eigen mean = sum / size)
+    double sum = 0.0;
+    if (lane == 0) {
+      if (dense) {
+        for (int i = 0; i < S.NL; i++)
+          if (drow[i] >= 0.0)
+            sum += drow[i];
+      } else {
+        for (int j = 0; j < k; j++)
+          sum += lsd[j];
+      }
+    }
+    sum = __shfl(sum, 0);
+    const double mean = sum / (double)k;
+    const double theta = S.thetas[th];
+    for (int e = lane; e < nacc; e += 64)
+      acc[e] = 0.0;
+    int pos = 0, done = 0;
+    while (done < k) {
+      int cnum = 0;
+      if (dense) {
+        // next run of valid entries of the distance row, in library order
+        while (cnum == 0 && pos < S.NL) {
+          const int i = pos + lane;
+          const bool v = lane < sch && i < S.NL && drow[i] >= 0.0;
+          const unsigned long long bb = __ballot(v);
+          const int before = __popcll(bb & ((1ull << lane) - 1ull));
+          if (v) {
+            cidx[before] = i;
+            cd[before] = drow[i];
+          }
+          cnum = __popcll(bb);
+          pos += sch;
+        }
+      } else {
+        cnum = min(sch, k - done);
+        if (lane < cnum) {
+          cidx[lane] = li[done + lane];
+          cd[lane] = lsd[done + lane];
+        }
+      }
+      if (cnum == 0)
+        break;
+      __syncthreads();
+      if (lane < cnum) {
+        const int ci = cidx[lane];
+        const double w = fdlibm_exp(-theta * (cd[lane] / mean));
+        const double* row = S.M + (size_t)ci * Ea;
+        Xc[lane * n] = w;
+        for (int j = 1; j < n; j++)
+          Xc[lane * n + j] = w * row[j - 1];
+        yc[lane] = w * S.targets[ci];
+      }
+      __syncthreads();
+      for (int e = lane; e < nacc; e += 64) {
+        double a = acc[e];
+        if (e < ntri) {
+          int p = 0, rem = e;
+          while (rem >= n - p) {
+            rem -= n - p;
+            p++;
+          }
+          const int q = p + rem;
+          for (int c = 0; c < cnum; c++)
+            a += Xc[c * n + p] * Xc[c * n + q];
+        } else {
+          const int p = e - ntri;
+          for (int c = 0; c < cnum; c++)
+            a += Xc[c * n + p] * yc[c];
+        }
+        acc[e] = a;
+      }
+      __syncthreads();
+      done += cnum;
+    }
+    double* Wg = W + (size_t)g * n * n;
+    for (int e = lane; e < nacc; e += 64) {
+      if (e < ntri) {
+        int p = 0, rem = e;
+        while (rem >= n - p) {
+          rem -= n - p;
+          p++;
+        }
+        const int q = p + rem;
+        Wg[p * n + q] = acc[e];
+        Wg[q * n + p] = acc[e];
+      } else {
+        bvec[g * n + e - ntri] = acc[e];
+      }
+    }
+    __syncthreads();
+  }
+  return valid;
+}
+
+// Prediction (and, at the last theta, the coefficients) of each system of the group.
+__device__ void write_group(const SmapParams& S, int t0, int i0, int n, int G, unsigned valid, const double* xsol,
+                            int lane)
+{
+  const int Ea = n - 1, T = S.T;
+  const int gl = lane / n;
+  const bool mine = gl < G && ((valid >> gl) & 1u);
+  if (!mine)
+    return;
+  const int item = i0 + gl;
+  const int tl = item / T, th = item - (item / T) * T;
+  const int t = t0 + tl;
+  const int j = lane - gl * n;
+  const double* xg = xsol + gl * n;
+  if (j == 0) {
+    double r = xg[0];
+    const double* yp = S.Mp + (size_t)t * Ea;
+    for (int c = 0; c < Ea; c++)
+      if (yp[c] != MISSING_D)
+        r += yp[c] * xg[c + 1];
+    S.pred[(size_t)th * S.NP + t] = r;
+    S.rc[(size_t)th * S.NP + t] = RC_SUCCESS;
+  }
+  if (S.coeffs && th == T - 1) {
+    const double c = xg[j];
+    S.coeffs[(size_t)t * n + j] = fabs(c) < 1.0e-11 ? MISSING_D : c;
+  }
+}
+
+// Persistent grid over groups of G (target, theta) items of targets [t0, t0 + nt); U and V in LDS.
 __global__ __launch_bounds__(64) void smap_multi(SmapParams S, const double* __restrict__ D, int t0, int nt)
 {
   extern __shared__ double smem[];
-  const int Ea = S.Ea, n = Ea + 1, T = S.T;
+  const int n = S.Ea + 1;
   const int G = smap_group(n);
-  const int ntri = n * (n + 1) / 2, nacc = ntri + n;
+  const int nacc = n * (n + 1) / 2 + n;
   const int lane = threadIdx.x;
   double* W = smem;
   double* U = W + (size_t)G * n * n;
@@ -211,155 +367,377 @@ __global__ __launch_bounds__(64) void smap_multi(SmapParams S, const double* __r
   double* acc = xsol + G * n;
   double* cd = acc + nacc;
   int* cidx = (int*)(cd + SM_CH);
-  double* Xc = U;
-  double* yc = Xc + SM_CH * n;
 
-  const int items = nt * T;
+  const int items = nt * S.T;
   for (int i0 = blockIdx.x * G; i0 < items; i0 += gridDim.x * G) {
-    // Gram systems of the group's items, one after the other
-    unsigned valid = 0; // bit g: item i0 + g has a system
-    for (int g = 0; g < G; g++) {
-      const int item = i0 + g;
-      if (item >= items)
-        break;
-      const int tl = item / T, th = item - (item / T) * T;
-      const int t = t0 + tl;
-      if (t >= S.NP)
-        continue;
-      const int kk = S.nn_k[t];
-      if (kk == 0)
-        continue;
-      valid |= 1u << g;
-      const bool dense = kk < 0;
-      const int k = dense ? S.nn_cnt[t] : kk;
-      const double* drow = dense ? D + (size_t)tl * S.NL : nullptr;
-      const int* li = S.nn_i + (size_t)(t - S.nn_t0) * S.nn_stride;
-      const double* lsd = S.nn_s + (size_t)(t - S.nn_t0) * S.nn_stride;
-      // mean of the selected distances, summed in neighbour order (Eigen mean = sum / size)
-      double sum = 0.0;
-      if (lane == 0) {
-        if (dense) {
-          for (int i = 0; i < S.NL; i++)
-            if (drow[i] >= 0.0)
-              sum += drow[i];
-        } else {
-          for (int j = 0; j < k; j++)
-            sum += lsd[j];
-        }
-      }
-      sum = __shfl(sum, 0);
-      const double mean = sum / (double)k;
-      const double theta = S.thetas[th];
-      for (int e = lane; e < nacc; e += 64)
-        acc[e] = 0.0;
-      int pos = 0, done = 0;
-      while (done < k) {
-        int cnum = 0;
-        if (dense) {
-          // next run of valid entries of the distance row, in library order
-          while (cnum == 0 && pos < S.NL) {
-            const int i = pos + lane;
-            const bool v = lane < SM_CH && i < S.NL && drow[i] >= 0.0;
-            const unsigned long long bb = __ballot(v);
-            const int before = __popcll(bb & ((1ull << lane) - 1ull));
-            if (v) {
-              cidx[before] = i;
-              cd[before] = drow[i];
-            }
-            cnum = __popcll(bb);
-            pos += SM_CH;
-          }
-        } else {
-          cnum = min(SM_CH, k - done);
-          if (lane < cnum) {
-            cidx[lane] = li[done + lane];
-            cd[lane] = lsd[done + lane];
-          }
-        }
-        if (cnum == 0)
-          break;
-        __syncthreads();
-        if (lane < cnum) {
-          const int ci = cidx[lane];
-          const double w = fdlibm_exp(-theta * (cd[lane] / mean));
-          const double* row = S.M + (size_t)ci * Ea;
-          Xc[lane * n] = w;
-          for (int j = 1; j < n; j++)
-            Xc[lane * n + j] = w * row[j - 1];
-          yc[lane] = w * S.targets[ci];
-        }
-        __syncthreads();
-        for (int e = lane; e < nacc; e += 64) {
-          double a = acc[e];
-          if (e < ntri) {
-            int p = 0, rem = e;
-            while (rem >= n - p) {
-              rem -= n - p;
-              p++;
-            }
-            const int q = p + rem;
-            for (int c = 0; c < cnum; c++)
-              a += Xc[c * n + p] * Xc[c * n + q];
-          } else {
-            const int p = e - ntri;
-            for (int c = 0; c < cnum; c++)
-              a += Xc[c * n + p] * yc[c];
-          }
-          acc[e] = a;
-        }
-        __syncthreads();
-        done += cnum;
-      }
-      double* Wg = W + (size_t)g * n * n;
-      for (int e = lane; e < nacc; e += 64) {
-        if (e < ntri) {
-          int p = 0, rem = e;
-          while (rem >= n - p) {
-            rem -= n - p;
-            p++;
-          }
-          const int q = p + rem;
-          Wg[p * n + q] = acc[e];
-          Wg[q * n + p] = acc[e];
-        } else {
-          bvec[g * n + e - ntri] = acc[e];
-        }
-      }
-      __syncthreads();
-    }
-
+    // the Gram staging lives in U's slot until the solve starts
+    const unsigned valid = form_group(S, D, t0, nt, i0, n, G, W, bvec, U, SM_CH, acc, cd, cidx, lane);
     const int gl = lane / n;
     const bool mine = gl < G && ((valid >> gl) & 1u);
     jacobi_svd_solve_multi(n, G, W, U, V, bvec, sv, tmp, xsol, !mine, lane);
-
-    if (mine) {
-      const int item = i0 + gl;
-      const int tl = item / T, th = item - (item / T) * T;
-      const int t = t0 + tl;
-      const int j = lane - gl * n;
-      const double* xg = xsol + gl * n;
-      if (j == 0) {
-        double r = xg[0];
-        const double* yp = S.Mp + (size_t)t * Ea;
-        for (int c = 0; c < Ea; c++)
-          if (yp[c] != MISSING_D)
-            r += yp[c] * xg[c + 1];
-        S.pred[(size_t)th * S.NP + t] = r;
-        S.rc[(size_t)th * S.NP + t] = RC_SUCCESS;
-      }
-      if (S.coeffs && th == T - 1) {
-        const double c = xg[j];
-        S.coeffs[(size_t)t * n + j] = fabs(c) < 1.0e-11 ? MISSING_D : c;
-      }
-    }
+    write_group(S, t0, i0, n, G, valid, xsol, lane);
     __syncthreads();
   }
 }
 
-// LDS bytes of smap_multi for n unknowns.
-size_t smap_lds_bytes(int n)
+// Element i (wave-uniform) of a row held in registers as NM + NT doubles.
+template <int NM, int NT, class VM, class VT>
+__device__ __forceinline__ double row_get(const VM& m, const VT& t, int i)
 {
-  return sizeof(double) * smap_lds_doubles(n, smap_group(n));
+  if constexpr (NT == 0) {
+    return m[i & (NM - 1)];
+  } else {
+    const double a = m[i & (NM - 1)], b = t[(i - NM) & (NT - 1)];
+    return i < NM ? a : b;
+  }
+}
+
+// Writes are unconditional (a select per element) so that the indexed register write never sits
+// under a branch.
+template <int NM, int NT, class VM, class VT>
+__device__ __forceinline__ void row_set(VM& m, VT& t, int i, double v)
+{
+  if constexpr (NT == 0) {
+    m[i & (NM - 1)] = v;
+  } else {
+    const int im = i & (NM - 1), it = (i - NM) & (NT - 1);
+    const double om = m[im], ot = t[it];
+    m[im] = i < NM ? v : om;
+    t[it] = i < NM ? ot : v;
+  }
+}
+
+// The solve of jacobi_svd_solve_multi with U and V in registers: lane g n + j keeps row j of U_g and
+// of V_g as NM + NT doubles (n <= NM + NT).  Every rotation touches columns p and q of U and V, and p
+// and q are wave-uniform, so each row element is reached by an indexed register move.  Element p is
+// held in a plain register over the q loop.  The sort of the singular values keeps the column
+// permutation in perm instead of moving columns; at the end the rows of U, then of V, go to W's slot
+// for the two products of the solve.  Every floating-point operation and its order are those of
+// jacobi_svd_solve_multi.
+template <int NM, int NT>
+__device__ void jacobi_svd_solve_rows(int n, int G, double* W, const double* b, double* sv, double* tmp,
+                                      double* x, int* perm, bool skip, int lane)
+{
+  typedef double VM __attribute__((ext_vector_type(NM)));
+  typedef double VT __attribute__((ext_vector_type(NT > 0 ? NT : 1)));
+  constexpr int TM = NT > 0 ? NT - 1 : 0;
+  const int g = lane / n, j = lane - (lane / n) * n;
+  const bool act = g < G;
+  const int gg = act ? g : 0;
+  double* Wg = W + (size_t)gg * n * n;
+  const double* bg = b + gg * n;
+  double* svg = sv + gg * n;
+  double* tmpg = tmp + gg * n;
+  double* xg = x + gg * n;
+  int* pg = perm + gg * n;
+
+  // scale by the largest |w| of the system
+  double cm = 0.0;
+  if (act)
+    for (int i = 0; i < n; i++)
+      cm = fmax(cm, fabs(Wg[i * n + j]));
+  double scale = 0.0;
+  for (int i = 0; i < n; i++)
+    scale = fmax(scale, __shfl(cm, gg * n + i));
+  const bool bad = skip || !act || !isfinite(scale);
+  if (scale == 0.0)
+    scale = 1.0;
+  if (!bad)
+    for (int i = 0; i < n; i++)
+      Wg[i * n + j] = Wg[i * n + j] / scale;
+  // rows of U and V start as the identity; the opaque copy of j keeps the compiler from hoisting
+  // these constants out of the item loop, where they would hold registers through the Gram phase
+  int jo = j;
+  asm volatile("" : "+v"(jo));
+  VM um, vm;
+  VT ut, vt;
+#pragma unroll
+  for (int i = 0; i < NM; i++) {
+    um[i] = i == jo ? 1.0 : 0.0;
+    vm[i] = um[i];
+  }
+#pragma unroll
+  for (int i = 0; i < TM + 1; i++) {
+    ut[i] = NM + i == jo ? 1.0 : 0.0;
+    vt[i] = ut[i];
+  }
+  __syncthreads();
+  const double precision = 2.0 * DBL_EPSILON;
+  double maxDiag = 0.0;
+  if (!bad)
+    for (int i = 0; i < n; i++)
+      maxDiag = fmax(maxDiag, fabs(Wg[i * n + i]));
+
+  // The sweeps and their p loop run as one loop, so that the rows have a single loop-carried home in
+  // registers: sweep s + 1 starts (p back to 1) while some system of the wavefront still rotated in
+  // sweep s, at most 1000 sweeps.
+  bool finished = bad;
+  int sweeps = 0;
+  bool go = __ballot(!finished) != 0ull;
+  bool run = false;
+  if (go) {
+    sweeps = 1;
+    run = !finished;
+    if (run)
+      finished = true;
+  }
+  int p = 1;
+  while (go) {
+    {
+      double up = row_get<NM, NT>(um, ut, p), vp = row_get<NM, NT>(vm, vt, p);
+      auto step = [&](int q, auto tail) {
+        constexpr bool TAIL = decltype(tail)::value;
+        bool rot = false;
+        double wpq = 0.0, wqp = 0.0;
+        if (run) {
+          const double thr = fmax(DBL_MIN, precision * maxDiag);
+          wpq = Wg[p * n + q];
+          wqp = Wg[q * n + p];
+          rot = fabs(wpq) > thr || fabs(wqp) > thr;
+        }
+        if (!__ballot(rot))
+          return;
+        Rot jl{ 1.0, 0.0 }, jr{ 1.0, 0.0 };
+        if (rot) {
+          finished = false;
+          real_2x2(Wg[p * n + p], wpq, wqp, Wg[q * n + q], &jl, &jr);
+        }
+        __syncthreads();
+        // rows p, q of W and columns p, q of U by jl
+        const bool dl = rot && !(jl.c == 1.0 && jl.s == 0.0);
+        if (dl) {
+          const double xv = Wg[p * n + j], yv = Wg[q * n + j];
+          Wg[p * n + j] = jl.c * xv + jl.s * yv;
+          Wg[q * n + j] = -jl.s * xv + jl.c * yv;
+        }
+        {
+          double uq;
+          if constexpr (TAIL)
+            uq = ut[(q - NM) & TM];
+          else
+            uq = um[q & (NM - 1)];
+          const double np = jl.c * up + jl.s * uq, nq = -jl.s * up + jl.c * uq;
+          up = dl ? np : up;
+          if constexpr (TAIL)
+            ut[(q - NM) & TM] = dl ? nq : uq;
+          else
+            um[q & (NM - 1)] = dl ? nq : uq;
+        }
+        __syncthreads();
+        // columns p, q of W and V by jr (applyOnTheRight: rotation (c, -s))
+        const double tc = jr.c, ts = -jr.s;
+        const bool dr = rot && !(tc == 1.0 && ts == 0.0);
+        if (dr) {
+          const double xv = Wg[j * n + p], yv = Wg[j * n + q];
+          Wg[j * n + p] = tc * xv + ts * yv;
+          Wg[j * n + q] = -ts * xv + tc * yv;
+        }
+        {
+          double vq;
+          if constexpr (TAIL)
+            vq = vt[(q - NM) & TM];
+          else
+            vq = vm[q & (NM - 1)];
+          const double np = tc * vp + ts * vq, nq = -ts * vp + tc * vq;
+          vp = dr ? np : vp;
+          if constexpr (TAIL)
+            vt[(q - NM) & TM] = dr ? nq : vq;
+          else
+            vm[q & (NM - 1)] = dr ? nq : vq;
+        }
+        __syncthreads();
+        if (rot)
+          maxDiag = fmax(maxDiag, fmax(fabs(Wg[p * n + p]), fabs(Wg[q * n + q])));
+      };
+      const int qm = p < NM ? p : NM;
+      for (int q = 0; q < qm; q++)
+        step(q, std::false_type{});
+      if constexpr (NT > 0)
+        for (int q = NM; q < p; q++)
+          step(q, std::true_type{});
+      row_set<NM, NT>(um, ut, p, up);
+      row_set<NM, NT>(vm, vt, p, vp);
+    }
+    if (++p == n) {
+      p = 1;
+      go = __ballot(!finished) != 0ull && sweeps < 1000;
+      if (go) {
+        sweeps++;
+        run = !finished;
+        if (run)
+          finished = true;
+      }
+    }
+  }
+  // non-negative singular values: bit i of flip negates column i of U (applied as the rows go out)
+  unsigned flip = 0;
+  for (int i = 0; i < n; i++) {
+    const double a = bad ? 0.0 : Wg[i * n + i];
+    if (!bad && a < 0.0)
+      flip |= 1u << i;
+    if (!bad && j == 0)
+      svg[i] = fabs(a) * scale;
+  }
+  if (act)
+    pg[j] = j;
+  __syncthreads();
+  // decreasing order, first maximum wins; perm[i] is the column of U and V now at position i
+  int nonzero = n;
+  bool stop = bad;
+  for (int i = 0; i < n; i++) {
+    int pos = i;
+    if (!stop) {
+      double mx = svg[i];
+      for (int c = i + 1; c < n; c++)
+        if (svg[c] > mx) {
+          mx = svg[c];
+          pos = c;
+        }
+      if (mx == 0.0) {
+        nonzero = i;
+        stop = true;
+      }
+    }
+    __syncthreads();
+    if (!stop && pos != i && j == 0) {
+      const double a = svg[i];
+      svg[i] = svg[pos];
+      svg[pos] = a;
+      const int c = pg[i];
+      pg[i] = pg[pos];
+      pg[pos] = c;
+    }
+    __syncthreads();
+  }
+  int rank = 0;
+  if (!bad) {
+    const double thr = fmax(svg[0] * ((double)n * DBL_EPSILON), DBL_MIN);
+    int r = nonzero - 1;
+    while (r >= 0 && svg[r] < thr)
+      r--;
+    rank = r + 1;
+  }
+  // rows of U into W's slot
+  if (act) {
+#pragma unroll
+    for (int k = 0; k < NM; k++)
+      if (k < n)
+        Wg[j * n + k] = ((flip >> k) & 1u) ? -um[k] : um[k];
+#pragma unroll
+    for (int k = 0; k < NT; k++)
+      if (NM + k < n)
+        Wg[j * n + NM + k] = ((flip >> (NM + k)) & 1u) ? -ut[k] : ut[k];
+  }
+  __syncthreads();
+  if (!bad && j < rank) {
+    const int cj = pg[j];
+    double s = 0.0;
+    for (int c = 0; c < n; c++)
+      s += Wg[c * n + cj] * bg[c];
+    tmpg[j] = (1.0 / svg[j]) * s;
+  }
+  __syncthreads();
+  // rows of V into W's slot
+  if (act) {
+#pragma unroll
+    for (int k = 0; k < NM; k++)
+      if (k < n)
+        Wg[j * n + k] = vm[k];
+#pragma unroll
+    for (int k = 0; k < NT; k++)
+      if (NM + k < n)
+        Wg[j * n + NM + k] = vt[k];
+  }
+  __syncthreads();
+  if (act) {
+    double s = 0.0;
+    for (int i = 0; i < rank; i++)
+      s += Wg[j * n + pg[i]] * tmpg[i];
+    xg[j] = s;
+  }
+  __syncthreads();
+}
+
+// Gram staging: in the slot of the system being formed when a chunk of at least 8 rows fits there.
+__host__ __device__ inline int smap_rows_stage(int n)
+{
+  const int in_slot = n * n / (n + 1);
+  return in_slot >= 8 ? (in_slot < SM_CH ? in_slot : SM_CH) : 0;
+}
+
+// LDS doubles of smap_rows: W [G n^2] | b, sv, tmp, x [G n each] | acc [n (n+1)/2 + n] | cd [SM_CH] |
+// staging [SM_CH (n + 1)] when it does not fit a slot of W | cidx [SM_CH] and perm [G n] ints.
+__host__ __device__ inline size_t smap_rows_lds_doubles(int n, int G)
+{
+  const size_t nacc = (size_t)n * (n + 1) / 2 + n;
+  const size_t stage = smap_rows_stage(n) > 0 ? 0 : (size_t)SM_CH * (n + 1);
+  return (size_t)G * n * n + 4 * (size_t)G * n + nacc + SM_CH + stage + (SM_CH + (size_t)G * n + 1) / 2;
+}
+
+// Persistent grid over groups of G (target, theta) items of targets [t0, t0 + nt); U and V rows in
+// registers (n <= NM + NT), so a wavefront needs about a third of smap_multi's LDS.
+template <int NM, int NT>
+__global__ __launch_bounds__(64) void smap_rows(SmapParams S, const double* __restrict__ D, int t0, int nt)
+{
+  extern __shared__ double smem[];
+  const int n = S.Ea + 1;
+  const int G = smap_group(n);
+  const int nacc = n * (n + 1) / 2 + n;
+  const int lane = threadIdx.x;
+  const int sin = smap_rows_stage(n);
+  double* W = smem;
+  double* bvec = W + (size_t)G * n * n;
+  double* sv = bvec + G * n;
+  double* tmp = sv + G * n;
+  double* xsol = tmp + G * n;
+  double* acc = xsol + G * n;
+  double* cd = acc + nacc;
+  double* stage = cd + SM_CH;
+  int* cidx = (int*)(stage + (sin > 0 ? 0 : SM_CH * (n + 1)));
+  int* perm = cidx + SM_CH;
+
+  const int items = nt * S.T;
+  for (int i0 = blockIdx.x * G; i0 < items; i0 += gridDim.x * G) {
+    const unsigned valid =
+        form_group(S, D, t0, nt, i0, n, G, W, bvec, sin > 0 ? nullptr : stage, sin > 0 ? sin : SM_CH, acc, cd,
+                   cidx, lane);
+    const int gl = lane / n;
+    const bool mine = gl < G && ((valid >> gl) & 1u);
+    jacobi_svd_solve_rows<NM, NT>(n, G, W, bvec, sv, tmp, xsol, perm, !mine, lane);
+    write_group(S, t0, i0, n, G, valid, xsol, lane);
+    __syncthreads();
+  }
+}
+
+// Launches the S-map kernel for n = E_a + 1 unknowns over targets [t0, t0 + nt) at every theta:
+// register rows for n <= 24, LDS rows above.  A persistent grid of one-wavefront blocks, as many as
+// stay resident.  Returns 0, or -22 when n is beyond 64 (a system needs a lane per unknown).
+int smap_launch(int n, hipStream_t s, const SmapParams& S, const double* D, int t0, int nt, bool lds_rows)
+{
+  if (n > 64 || n < 2)
+    return -22;
+  const int G = smap_group(n);
+  const void* fn = (const void*)smap_multi;
+  size_t lds = sizeof(double) * smap_lds_doubles(n, G);
+  if (!lds_rows && n <= 24) {
+    fn = n <= 8    ? (const void*)smap_rows<8, 0>
+         : n <= 16 ? (const void*)smap_rows<16, 0>
+         : n <= 17 ? (const void*)smap_rows<16, 1>
+         : n <= 20 ? (const void*)smap_rows<16, 4>
+                   : (const void*)smap_rows<16, 8>;
+    lds = sizeof(double) * smap_rows_lds_doubles(n, G);
+  }
+  if (lds > 64 * 1024)
+    hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+  const long long groups = ((long long)nt * S.T + G - 1) / G;
+  int dev = 0, ncu = 0, occ = 0;
+  hipGetDevice(&dev);
+  hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+  hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fn, 64, lds);
+  const int grid = (int)std::max<long long>(1, std::min<long long>(groups, (long long)ncu * std::max(occ, 1)));
+  void* args[] = { (void*)&S, (void*)&D, (void*)&t0, (void*)&nt };
+  const hipError_t e = hipLaunchKernel(fn, dim3(grid), dim3(64), args, lds, s);
+  return e == hipSuccess ? 0 : -5;
 }
 
 } // namespace fedm

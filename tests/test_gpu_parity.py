@@ -32,6 +32,11 @@ CASES = {
     "categorical": dict(NL=300, NP=150, Ea=3, k=4, categorical=True, integer=True),
     "mae_distance": dict(NL=300, NP=150, Ea=3, k=4, mae=True),
     "smap_E16_k18": dict(NL=2000, NP=300, Ea=16, k=18, algorithm=1, thetas=(0, 1, 3)),
+    # the register-row S-map kernels by width: n = 8 (8 + 0), 13 (16 + 0), 20 (16 + 4), 23 (16 + 8)
+    "smap_E7_k9": dict(NL=800, NP=200, Ea=7, k=9, algorithm=1, thetas=(0, 1.5), save_coeffs=True),
+    "smap_E12_k20": dict(NL=800, NP=200, Ea=12, k=20, algorithm=1, thetas=(0.5,), save_coeffs=True),
+    "smap_E19_k21": dict(NL=1000, NP=150, Ea=19, k=21, algorithm=1, thetas=(0, 2.0), save_coeffs=True),
+    "smap_E22_k30": dict(NL=1000, NP=150, Ea=22, k=30, algorithm=1, thetas=(1.0,), save_coeffs=True),
     "smap_E40_k42": dict(NL=1500, NP=120, Ea=40, k=42, algorithm=1, thetas=(2.0,), save_coeffs=True),
     "simplex_E40_k41": dict(NL=1500, NP=200, Ea=40, k=41),
     "invalid_algorithm": dict(NL=100, NP=50, Ea=2, k=3, algorithm=7),
@@ -63,6 +68,27 @@ def test_task_parity(name):
     assert_predictions_close(gp, rp)
     if opts.get("save_smap_coeffs"):
         assert_predictions_close(gc, rc_)
+
+
+@pytest.mark.parametrize("Ea", [1, 3, 7, 8, 12, 15, 16, 17, 19, 22, 23])
+def test_smap_register_rows_match_lds_rows(Ea, monkeypatch):
+    """For n = E_a + 1 <= 24 the S-map solve keeps U and V rows in registers; FEDM_SMAP_LDS_ROWS=1 keeps
+    them in LDS.  Both run the same floating-point operations in the same order, so every prediction
+    and coefficient must agree to the bit (k near n gives rank-deficient and ill-conditioned systems)."""
+    fe = load_product()
+    for k in (Ea + 1, Ea + 3):
+        M, Mp, opts = random_task(100 + 7 * Ea + k, NL=700, NP=150, Ea=Ea, k=k, algorithm=1, thetas=(0, 1, 4),
+                                  save_coeffs=True, integer=(k == Ea + 1))
+        print(f"smap rows: Ea={Ea} k={k}", flush=True)
+        monkeypatch.delenv("FEDM_SMAP_LDS_ROWS", raising=False)
+        gp, gr, gc, gk, gn, status = fe.predict(M, Mp, opts, nn_cap=64)
+        monkeypatch.setenv("FEDM_SMAP_LDS_ROWS", "1")
+        lp, lr, lc, lk, ln, lstatus = fe.predict(M, Mp, opts, nn_cap=64)
+        monkeypatch.delenv("FEDM_SMAP_LDS_ROWS")
+        assert status == 0 and lstatus == 0
+        np.testing.assert_array_equal(gr, lr)
+        np.testing.assert_array_equal(np.ascontiguousarray(gp).view(np.uint64), np.ascontiguousarray(lp).view(np.uint64))
+        np.testing.assert_array_equal(np.ascontiguousarray(gc).view(np.uint64), np.ascontiguousarray(lc).view(np.uint64))
 
 
 WASS_CASES = {
