@@ -81,7 +81,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
   __shared__ float Bn[MCH];
   __shared__ float thr_s[64];
   __shared__ int qcnt[64];
-  __shared__ unsigned char queue[64 * MCH];
+  constexpr int QCAP = 2 * MCH;             // queue slots per target: the candidates of two chunks
+  __shared__ unsigned char queue[64 * QCAP];
+  // Per-target arrays are entry-major (entry e of target t at e * 64 + t): the 16 lanes that work on one
+  // wavefront's targets touch consecutive words, without bank conflicts.
   __shared__ double Ls[64 * KCAP]; // exact lists of the block's targets, by (sqrt distance, position)
   __shared__ int Li[64 * KCAP];
   __shared__ double accq[64 * PAR]; // FP64 distances of each target's first PAR queued candidates
@@ -116,12 +119,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
   // owner lanes 0..15 keep the exact lists of targets tbase + lane (LDS rows)
   const bool owner = lane < 16 && (tbase + lane) < P.NP;
   const int ot = tbase + lane;
-  double* ls = Ls + (w * 16 + (lane & 15)) * KCAP;
-  int* li = Li + (w * 16 + (lane & 15)) * KCAP;
+  double* ls = Ls + (w * 16 + (lane & 15));
+  int* li = Li + (w * 16 + (lane & 15));
   if (lane < 16)
     for (int j = 0; j < KCAP; j++) {
-      ls[j] = INF_D;
-      li[j] = INT_MAX;
+      ls[j * 64] = INF_D;
+      li[j * 64] = INT_MAX;
     }
   double s_kth = INF_D, thr = INF_D;
   int zeros = 0;
@@ -147,6 +150,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
   if (lo < hi)
     prefetch(lo);
 
+  // The queues are drained every second chunk (and after the last): a queued entry is the position
+  // in the pair of chunks that starts at cbase.
+  int half = 0, cbase = lo;
   for (int c0 = lo; c0 < hi; c0 += MCH) {
     const int cn = min(MCH, hi - c0);
     __syncthreads();
@@ -208,20 +214,28 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
             if ((d2[u][r] <= th[r]) & tok[r] & (j < cn)) {
               const int ti = 4 * g + r;
               const int slot = atomicAdd(&qcnt[w * 16 + ti], 1);
-              queue[(w * 16 + ti) * MCH + slot] = (unsigned char)j;
+              queue[slot * 64 + w * 16 + ti] = (unsigned char)(half * MCH + j);
             }
         }
       }
     }
-    __syncthreads();
+    const bool drain = half == 1 || c0 + MCH >= hi;
+    half ^= 1;
+    if (!drain)
+      continue;
+    half = 0;
+    const int cb = cbase;
+    cbase = c0 + MCH;
+    // the queues and lists belong to this wavefront: a wave-level sync suffices
+    lds_wave_sync();
     // exact FP64 recomputation: lane (target lane % 16, slot lane / 16) takes queued candidate `slot`
     // of its target, so the first PAR candidates of all 16 targets cost one round of loads
     {
       const int ti = lane & 15, sl = lane >> 4;
       const int qi = w * 16 + ti;
       if (tbase + ti < P.NP && sl < qcnt[qi]) {
-        const int j = queue[qi * MCH + sl];
-        accq[qi * PAR + sl] = dist_f64<KP>(P.M + (size_t)(c0 + j) * P.Ea, P.Mp + (size_t)(tbase + ti) * P.Ea, P.Ea);
+        const int j = queue[sl * 64 + qi];
+        accq[sl * 64 + qi] = dist_f64<KP>(P.M + (size_t)(cb + j) * P.Ea, P.Mp + (size_t)(tbase + ti) * P.Ea, P.Ea);
       }
     }
     lds_wave_sync();
@@ -229,10 +243,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     if (owner) {
       const int nq = qcnt[w * 16 + lane];
       const double* y = P.Mp + (size_t)ot * P.Ea;
+      bool moved_kth = false;
       for (int e = 0; e < nq; e++) {
-        const int j = queue[(w * 16 + lane) * MCH + e];
+        const int j = queue[e * 64 + w * 16 + lane];
         const double acc =
-          e < PAR ? accq[(w * 16 + lane) * PAR + e] : dist_f64<KP>(P.M + (size_t)(c0 + j) * P.Ea, y, P.Ea);
+          e < PAR ? accq[e * 64 + w * 16 + lane] : dist_f64<KP>(P.M + (size_t)(cb + j) * P.Ea, y, P.Ea);
         if (acc == 0.0 || acc == MISSING_D) {
           zeros++;
           continue;
@@ -240,33 +255,60 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
         if (acc < thr) {
           const double s = sqrt(acc);
           if (s < s_kth) {
-            // sorted insertion; the last entry drops out
-            const int ci = c0 + j;
-            int q = KCAP - 1;
-            while (q > 0 && (s < ls[q - 1] || (s == ls[q - 1] && ci < li[q - 1]))) {
-              ls[q] = ls[q - 1];
-              li[q] = li[q - 1];
-              q--;
+            // sorted insertion among the first k entries (entry k - 1 drops out), walking down four
+            // entries per round of LDS reads; the new k-th entry is known from the first round
+            const int ci = cb + j;
+            int q = k - 1;
+            double kth_next = s;
+            bool first = true;
+            while (q > 0) {
+              double v[4];
+              int iv[4];
+#pragma unroll
+              for (int u = 0; u < 4; u++) {
+                const int r = max(q - 1 - u, 0);
+                v[u] = ls[r * 64];
+                iv[u] = li[r * 64];
+              }
+              if (first && (s < v[0] || (s == v[0] && ci < iv[0])))
+                kth_next = v[0];
+              first = false;
+              int moved = 0;
+              bool more = true;
+#pragma unroll
+              for (int u = 0; u < 4; u++) {
+                more = more && q - 1 - u >= 0 && (s < v[u] || (s == v[u] && ci < iv[u]));
+                if (more) {
+                  ls[(q - u) * 64] = v[u];
+                  li[(q - u) * 64] = iv[u];
+                  moved++;
+                }
+              }
+              q -= moved;
+              if (moved < 4)
+                break;
             }
-            ls[q] = s;
-            li[q] = ci;
-            s_kth = ls[k - 1];
+            ls[q * 64] = s;
+            li[q * 64] = ci;
+            s_kth = kth_next;
+            moved_kth = true;
             if (s_kth != INF_D)
               thr = s_kth * s_kth * SQRT_GUARD;
           }
         }
       }
       qcnt[w * 16 + lane] = 0;
-      thr_s[w * 16 + lane] =
-        thr == INF_D ? __builtin_huge_valf() : up_f32(thr + 2.0 * (c1 * sqrt(thr) + P.c2 * thr + c3));
+      if (moved_kth)
+        thr_s[w * 16 + lane] =
+          thr == INF_D ? __builtin_huge_valf() : up_f32(thr + 2.0 * (c1 * sqrt(thr) + P.c2 * thr + c3));
     }
   }
   if (owner) {
     const size_t base = ((size_t)blockIdx.y * P.NP + ot) * KCAP;
 #pragma unroll
     for (int j = 0; j < KCAP; j++) {
-      out_s[base + j] = ls[j];
-      out_i[base + j] = li[j];
+      out_s[base + j] = ls[j * 64];
+      out_i[base + j] = li[j * 64];
     }
     out_cnt[(size_t)blockIdx.y * P.NP + ot] = (hi > lo ? hi - lo : 0) - zeros;
   }
