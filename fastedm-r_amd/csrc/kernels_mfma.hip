@@ -254,10 +254,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
         }
         if (acc < thr) {
           const double s = sqrt(acc);
-          if (s < s_kth) {
+          const int ci = cb + j;
+          // a tie with the k-th entry enters when its position is smaller, so the list does not depend
+          // on the order in which candidates arrive
+          if (s < s_kth || (s == s_kth && ci < li[(k - 1) * 64])) {
             // sorted insertion among the first k entries (entry k - 1 drops out), walking down four
             // entries per round of LDS reads; the new k-th entry is known from the first round
-            const int ci = cb + j;
             int q = k - 1;
             double kth_next = s;
             bool first = true;

@@ -160,6 +160,52 @@ MIXED_CASES = {
 }
 
 
+def logistic_task(seed, NL, NP, E, k, algorithm=0, thetas=(1.0,), save_coeffs=False, panels=1):
+    """Lagged embeddings of logistic-map series (r = 3.9, a thin attractor) split at random into
+    library and targets."""
+    rng = np.random.default_rng(seed)
+    n = NL + NP + E * panels
+    rows = []
+    for p in range(panels):
+        x = np.empty(n // panels + E)
+        x[0] = rng.uniform(0.1, 0.9)
+        for t in range(1, len(x)):
+            x[t] = 3.9 * x[t - 1] * (1.0 - x[t - 1])
+        rows.append(np.stack([x[i:len(x) - E + 1 + i] for i in range(E)], axis=1))
+    X = np.concatenate(rows)[: NL + NP]
+    perm = rng.permutation(NL + NP)
+    M, Mp = X[perm[:NL]].copy(), X[perm[NL:]].copy()
+    Md, Mpd, opts = random_task(seed, NL=NL, NP=NP, Ea=E, k=k, algorithm=algorithm, thetas=thetas,
+                                save_coeffs=save_coeffs, precision=1)
+    Md["data"], Mpd["data"] = M, Mp
+    return Md, Mpd, opts
+
+
+CLUSTERED_CASES = {
+    "logistic_simplex_E8_k9": dict(NL=20000, NP=3000, E=8, k=9, thetas=(0.5, 2.0)),
+    "logistic_smap_E16_k18": dict(NL=12000, NP=2500, E=16, k=18, algorithm=1, thetas=(0, 1, 3), save_coeffs=True),
+    "logistic_panels_E12_k13": dict(NL=15000, NP=2000, E=12, k=13, panels=7),
+}
+
+
+@pytest.mark.parametrize("name", sorted(CLUSTERED_CASES))
+def test_mixed_clustered_parity(name):
+    """Mixed route on a thin attractor (many near neighbours inside the FP32 screening bound) against
+    the FP64 oracle: indices exact."""
+    fe = load_product()
+    M, Mp, opts = logistic_task(zlib.crc32(name.encode()) % 1000, **CLUSTERED_CASES[name])
+    print(f"case {name}", flush=True)
+    gp, gr, gc, gk, gn, status = fe.predict(M, Mp, opts, nn_cap=64)
+    rp, rr, rc_, rk, rn = oracle.predict(M, Mp, opts, nn_cap=64)
+    assert status == 0
+    np.testing.assert_array_equal(gr, rr)
+    np.testing.assert_array_equal(gk, rk)
+    np.testing.assert_array_equal(gn, rn)
+    assert_predictions_close(gp, rp)
+    if opts.get("save_smap_coeffs"):
+        assert_predictions_close(gc, rc_)
+
+
 @pytest.mark.parametrize("name", sorted(MIXED_CASES))
 def test_mixed_task_parity(name):
     """Mixed route (FP32 matrix-core screening + FP64 re-rank) against the FP64 oracle: indices exact."""
