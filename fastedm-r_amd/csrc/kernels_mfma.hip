@@ -36,12 +36,13 @@ __device__ __forceinline__ float up_f32(double x)
   return f;
 }
 
-// FP64 squared distance in the reference's order (src/distances.cpp:186-216); the loads of up to 16
-// coordinates of both rows (8 for wider points, to bound the registers) are issued together.
+// FP64 squared distance in the reference's order (src/distances.cpp:186-216); the loads of 8 coordinates
+// of both rows are issued together (16 VGPR pairs in flight, which leaves room for four wavefronts per
+// SIMD).
 template <int KP>
 __device__ __forceinline__ double dist_f64(const double* x, const double* y, int Ea)
 {
-  constexpr int LB = KP <= 16 ? KP : 8;
+  constexpr int LB = 8;
   double acc = 0.0;
 #pragma unroll 1
   for (int d0 = 0; d0 < KP; d0 += LB) {
@@ -72,10 +73,10 @@ __device__ __forceinline__ void lds_wave_sync()
 constexpr int PAR = 4; // queued candidates per target recomputed by the whole wavefront at once
 
 template <int KCAP, int KP>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void knn_mfma(MfmaParams P, int k, int slice_len, double* __restrict__ out_s,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(KP == 64 ? 3 : 4))) void knn_mfma(MfmaParams P, int k, int slice_len, double* __restrict__ out_s,
                                                int* __restrict__ out_i, int* __restrict__ out_cnt)
 {
-  constexpr int MCH = KP == 64 ? 64 : 128;  // library points per LDS chunk (16-point tiles)
+  constexpr int MCH = 64; // library points per LDS chunk (16-point tiles); LDS for four blocks per CU
   constexpr int F4 = MCH * KP / 4 / 256;    // float4 of the next chunk each thread prefetches
   __shared__ float Bs[MCH * KP];
   __shared__ float Bn[MCH];
