@@ -36,8 +36,9 @@ struct MfmaParams
   const float* Pn;
   const double* M;
   const double* Mp;
-  const double* c1t;
-  const double* c3t;
+  const double* ynorm;
+  double R;
+  double k1, k3;
   double c2;
   int NL, NP, Ea;
 };
@@ -128,7 +129,6 @@ enum
   B_MIX2,
   B_MIX3,
   B_MIX4,
-  B_MIX5,
   B_NUM
 };
 } // namespace fedm
@@ -523,7 +523,7 @@ int edm_predict(edm_ctx* c, const edm_manifold_view* M, const edm_manifold_view*
     for (int d = 0; d < Ea; d++)
       ctr[d] /= std::max(NL, 1);
     std::vector<float> m32((size_t)NL * kp, 0.f), mn(NL), p32((size_t)NP * kp, 0.f), pn(NP);
-    std::vector<double> c1t(NP), c3t(NP);
+    std::vector<double> ynorm(NP);
     double R = 0.0;
     for (int j = 0; j < NL; j++) {
       double r2 = 0.0;
@@ -550,17 +550,18 @@ int edm_predict(edm_ctx* c, const edm_manifold_view* M, const edm_manifold_view*
         r2 += v * v;
       }
       pn[i] = n2;
-      const double rr = (R + std::sqrt(r2)) * (1.0 + 1e-6);
-      c1t[i] = 2.02 * u * rr;
-      c3t[i] = ((kp + 3) * 1.02 * u + u * u) * rr * rr;
+      ynorm[i] = std::sqrt(r2);
     }
     MfmaParams MP;
     MP.M32 = up(c, B_MIX0, m32.data(), m32.size(), &err);
     MP.Mn = up(c, B_MIX1, mn.data(), mn.size(), &err);
     MP.P32 = up(c, B_MIX2, p32.data(), p32.size(), &err);
     MP.Pn = up(c, B_MIX3, pn.data(), pn.size(), &err);
-    MP.c1t = up(c, B_MIX4, c1t.data(), c1t.size(), &err);
-    MP.c3t = up(c, B_MIX5, c3t.data(), c3t.size(), &err);
+    // the screening bound per target (kernel): 2 (k1 rr sqrt(D) + c2 D + k3 rr^2), rr >= |x| + |y|
+    MP.ynorm = up(c, B_MIX4, ynorm.data(), ynorm.size(), &err);
+    MP.R = R;
+    MP.k1 = 2.02 * u;
+    MP.k3 = (kp + 3) * 1.02 * u + u * u;
     MP.c2 = 4.0 * u;
     MP.M = P.M;
     MP.Mp = P.Mp;

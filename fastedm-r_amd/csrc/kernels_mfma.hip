@@ -22,9 +22,10 @@ struct MfmaParams
   const float* Pn;    // [NP]
   const double* M;    // FP64 library [NL x Ea]
   const double* Mp;   // FP64 targets [NP x Ea]
-  const double* c1t;  // [NP] bound: coefficient of sqrt(D)
-  const double* c3t;  // [NP] bound: constant term
-  double c2;          // bound: coefficient of D
+  const double* ynorm; // [NP] |centred target| (FP64)
+  double R;            // largest |centred library row|
+  double k1, k3;       // bound: coefficient of sqrt(D) per unit rr, constant term per unit rr^2
+  double c2;           // bound: coefficient of D
   int NL, NP, Ea;
 };
 
@@ -129,7 +130,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(KP == 64 ? 
     }
   double s_kth = INF_D, thr = INF_D;
   int zeros = 0;
-  const double c1 = owner ? P.c1t[ot] : 0.0, c3 = owner ? P.c3t[ot] : 0.0;
+  const double yn = owner ? P.ynorm[ot] : 0.0;
   if (lane < 16) {
     thr_s[w * 16 + lane] = __builtin_huge_valf();
     qcnt[w * 16 + lane] = 0;
@@ -301,9 +302,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(KP == 64 ? 
         }
       }
       qcnt[w * 16 + lane] = 0;
-      if (moved_kth)
-        thr_s[w * 16 + lane] =
-          thr == INF_D ? __builtin_huge_valf() : up_f32(thr + 2.0 * (c1 * sqrt(thr) + P.c2 * thr + c3));
+      if (moved_kth && thr != INF_D) {
+        // FP32 screening threshold: thr plus the bound B(thr) = 2 (c1 sqrt(thr) + c2 thr + c3) on
+        // |FP32 d2 - D| for pairs with D <= thr, where c1 = k1 rr and c3 = k3 rr^2 for any rr >=
+        // |x| + |y| (centred rows).  Such a pair has |x| <= |y| + sqrt(D), so rr = 2 |y| + sqrt(thr) bounds
+        // it as well as R + |y| does, and more tightly once thr is small.
+        const double sq = sqrt(thr);
+        const double rr = fmin(P.R + yn, 2.0 * yn + sq) * (1.0 + 1e-6);
+        thr_s[w * 16 + lane] = up_f32(thr + 2.0 * (P.k1 * rr * sq + P.c2 * thr + P.k3 * rr * rr));
+      }
     }
   }
   if (owner) {
