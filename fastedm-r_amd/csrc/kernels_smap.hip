@@ -22,14 +22,18 @@ __host__ __device__ inline int smap_group(int n)
   return n > 64 ? 0 : (64 / n > 8 ? 8 : 64 / n);
 }
 
-// LDS doubles: W [G n^2] | U, V [G n^2 each] (the Gram staging Xc [SM_CH n], yc [SM_CH] lives there
-// before the solve starts) | b, sv, tmp, x [G n each] | acc [n (n+1)/2 + n] | cd [SM_CH] | cidx [SM_CH]
-// ints.
-__host__ __device__ inline size_t smap_lds_doubles(int n, int G)
+// LDS doubles of smap_multi: W [G n^2] | U, V [G n^2 each] | b [G n] | (n < 3 only) sv, tmp, x [G n each].
+// Before the solve starts, U's and V's slots hold the Gram staging Xc [SM_CH n], yc [SM_CH], then acc
+// [n (n+1)/2 + n], cd [SM_CH] and cidx [SM_CH] ints; after the sweeps, sv, tmp and x live in W's slot
+// (n >= 3).  For n = 41 that is 40.7 KB, four wavefronts per CU.
+__host__ __device__ inline size_t smap_uv_doubles(int n, int G)
 {
   const size_t nacc = (size_t)n * (n + 1) / 2 + n;
-  const size_t uv = std::max((size_t)2 * G * n * n, (size_t)SM_CH * n + SM_CH);
-  return (size_t)G * n * n + uv + 4 * (size_t)G * n + nacc + SM_CH + SM_CH / 2;
+  return std::max((size_t)2 * G * n * n, (size_t)SM_CH * n + SM_CH + nacc + SM_CH + SM_CH / 2);
+}
+__host__ __device__ inline size_t smap_lds_doubles(int n, int G)
+{
+  return (size_t)G * n * n + smap_uv_doubles(n, G) + (size_t)G * n + (n >= 3 ? 0 : 3 * (size_t)G * n);
 }
 
 // Segmented Jacobi SVD solve.  Lane g n + j (g < G) owns row/column j of system g.
@@ -127,14 +131,16 @@ __device__ void jacobi_svd_solve_multi(int n, int G, double* W, double* U, doubl
       }
     }
   }
-  // non-negative singular values
+  // non-negative singular values; each lane reads its diagonal entry first, since sv may share W's slot
+  const double dj = bad ? 0.0 : Wg[j * n + j];
+  __syncthreads();
   for (int i = 0; i < n; i++) {
-    const double a = bad ? 0.0 : Wg[i * n + i];
+    const double a = __shfl(dj, gg * n + i);
     if (!bad && a < 0.0)
       Ug[j * n + i] = -Ug[j * n + i];
-    if (!bad && j == 0)
-      svg[i] = fabs(a) * scale;
   }
+  if (!bad)
+    svg[j] = fabs(dj) * scale;
   __syncthreads();
   // decreasing order, first maximum wins
   int nonzero = n;
@@ -359,12 +365,11 @@ __global__ __launch_bounds__(64) void smap_multi(SmapParams S, const double* __r
   double* W = smem;
   double* U = W + (size_t)G * n * n;
   double* V = U + (size_t)G * n * n;
-  const size_t uv = max((size_t)2 * G * n * n, (size_t)SM_CH * n + SM_CH);
-  double* bvec = U + uv;
-  double* sv = bvec + G * n;
+  double* bvec = U + smap_uv_doubles(n, G);
+  double* sv = n >= 3 ? W : bvec + G * n;
   double* tmp = sv + G * n;
   double* xsol = tmp + G * n;
-  double* acc = xsol + G * n;
+  double* acc = U + SM_CH * n + SM_CH; // after the staging
   double* cd = acc + nacc;
   int* cidx = (int*)(cd + SM_CH);
 
