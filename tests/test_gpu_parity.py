@@ -37,6 +37,9 @@ CASES = {
     "smap_E12_k20": dict(NL=800, NP=200, Ea=12, k=20, algorithm=1, thetas=(0.5,), save_coeffs=True),
     "smap_E19_k21": dict(NL=1000, NP=150, Ea=19, k=21, algorithm=1, thetas=(0, 2.0), save_coeffs=True),
     "smap_E22_k30": dict(NL=1000, NP=150, Ea=22, k=30, algorithm=1, thetas=(1.0,), save_coeffs=True),
+    # the LDS-row S-map kernel: two systems per wavefront (n = 29), one (n = 34)
+    "smap_E28_k31": dict(NL=1200, NP=130, Ea=28, k=31, algorithm=1, thetas=(0, 2.0), save_coeffs=True),
+    "smap_E33_k36": dict(NL=1200, NP=100, Ea=33, k=36, algorithm=1, thetas=(0.5,), save_coeffs=True),
     "smap_E40_k42": dict(NL=1500, NP=120, Ea=40, k=42, algorithm=1, thetas=(2.0,), save_coeffs=True),
     "simplex_E40_k41": dict(NL=1500, NP=200, Ea=40, k=41),
     "invalid_algorithm": dict(NL=100, NP=50, Ea=2, k=3, algorithm=7),
