@@ -28,20 +28,6 @@ __global__ void select_wide(FinishParams F, const double* D, int t0, int nt, uns
                             int gstride);
 int smap_launch(int n, hipStream_t s, const SmapParams& S, const double* D, int t0, int nt, bool lds_rows);
 
-struct MfmaParams
-{
-  const float* M32;
-  const float* Mn;
-  const float* P32;
-  const float* Pn;
-  const double* M;
-  const double* Mp;
-  const double* ynorm;
-  double R;
-  double k1, k3;
-  double c2;
-  int NL, NP, Ea;
-};
 template <int KCAP, int KP>
 __global__ void knn_mfma(MfmaParams P, int k, int slice_len, double* out_s, int* out_i, int* out_cnt);
 __global__ void mfma_layout_probe(float* out);

@@ -109,6 +109,23 @@ struct SmapParams
   double* coeffs; // [NP x (Ea+1)] or null
 };
 
+// Matrix-core k-NN (knn_mfma): FP32 screening operands, the FP64 rows of the exact re-rank and the
+// screening bound's constants.
+struct MfmaParams
+{
+  const float* M32;   // [NL x KP] centred FP32 library rows, zero padded
+  const float* Mn;    // [NL] FP32 squared norms of those rows
+  const float* P32;   // [NP x KP] centred FP32 targets
+  const float* Pn;    // [NP]
+  const double* M;    // FP64 library [NL x Ea]
+  const double* Mp;   // FP64 targets [NP x Ea]
+  const double* ynorm; // [NP] |centred target| (FP64)
+  double R;            // largest |centred library row|
+  double k1, k3;       // bound: coefficient of sqrt(D) per unit rr, constant term per unit rr^2
+  double c2;           // bound: coefficient of D
+  int NL, NP, Ea;
+};
+
 } // namespace fedm
 
 #include "jacobi_wave.h"

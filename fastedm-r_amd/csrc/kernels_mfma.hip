@@ -14,21 +14,6 @@ namespace fedm {
 
 typedef float v4f __attribute__((ext_vector_type(4)));
 
-struct MfmaParams
-{
-  const float* M32;   // [NL x KP] centred FP32 library rows, zero padded
-  const float* Mn;    // [NL] FP32 squared norms of those rows
-  const float* P32;   // [NP x KP] centred FP32 targets
-  const float* Pn;    // [NP]
-  const double* M;    // FP64 library [NL x Ea]
-  const double* Mp;   // FP64 targets [NP x Ea]
-  const double* ynorm; // [NP] |centred target| (FP64)
-  double R;            // largest |centred library row|
-  double k1, k3;       // bound: coefficient of sqrt(D) per unit rr, constant term per unit rr^2
-  double c2;           // bound: coefficient of D
-  int NL, NP, Ea;
-};
-
 __device__ __forceinline__ float up_f32(double x)
 {
   float f = (float)x;
