@@ -714,8 +714,256 @@ __global__ __launch_bounds__(64) void smap_rows(SmapParams S, const double* __re
   }
 }
 
+// One system per wavefront (33 <= n <= 48): lane j keeps row j of U in registers as three 16-wide
+// segments and row j of V in LDS with W.  The operations and their order are those of
+// jacobi_svd_solve_multi, as in jacobi_svd_solve_rows.
+__device__ void jacobi_svd_solve_hyb(int n, double* W, double* V, const double* b, double* sv, double* tmp,
+                                     double* x, int* perm, bool skip, int lane)
+{
+  typedef double V16 __attribute__((ext_vector_type(16)));
+  const int j = lane;
+  const bool act = j < n;
+  // scale by the largest |w| of the system
+  double cm = 0.0;
+  if (act)
+    for (int i = 0; i < n; i++)
+      cm = fmax(cm, fabs(W[i * n + j]));
+  double scale = 0.0;
+  for (int i = 0; i < n; i++)
+    scale = fmax(scale, __shfl(cm, i));
+  const bool bad = skip || !act || !isfinite(scale);
+  if (scale == 0.0)
+    scale = 1.0;
+  if (!bad)
+    for (int i = 0; i < n; i++)
+      W[i * n + j] = W[i * n + j] / scale;
+  // U rows in registers (identity); the opaque copy of j keeps the constants from being hoisted
+  int jo = j;
+  asm volatile("" : "+v"(jo));
+  V16 u0, u1, u2;
+#pragma unroll
+  for (int i = 0; i < 16; i++) {
+    u0[i] = i == jo ? 1.0 : 0.0;
+    u1[i] = 16 + i == jo ? 1.0 : 0.0;
+    u2[i] = 32 + i == jo ? 1.0 : 0.0;
+  }
+  if (act)
+    for (int i = 0; i < n; i++)
+      V[j * n + i] = i == j ? 1.0 : 0.0;
+  __syncthreads();
+  auto uget = [&](int i) {
+    const double a0 = u0[i & 15], a1 = u1[i & 15], a2 = u2[i & 15];
+    return i < 16 ? a0 : (i < 32 ? a1 : a2);
+  };
+  auto uset = [&](int i, double v) {
+    const int ii = i & 15;
+    const double o0 = u0[ii], o1 = u1[ii], o2 = u2[ii];
+    u0[ii] = i < 16 ? v : o0;
+    u1[ii] = (i >= 16 && i < 32) ? v : o1;
+    u2[ii] = i >= 32 ? v : o2;
+  };
+  const double precision = 2.0 * DBL_EPSILON;
+  double maxDiag = 0.0;
+  if (!bad)
+    for (int i = 0; i < n; i++)
+      maxDiag = fmax(maxDiag, fabs(W[i * n + i]));
+
+  bool finished = bad;
+  int sweeps = 0;
+  bool go = __ballot(!finished) != 0ull;
+  bool run = false;
+  if (go) {
+    sweeps = 1;
+    run = !finished;
+    if (run)
+      finished = true;
+  }
+  int p = 1;
+  while (go) {
+    {
+      double up = uget(p);
+      auto step = [&](int q, auto seg) {
+        constexpr int S = decltype(seg)::value;
+        bool rot = false;
+        double wpq = 0.0, wqp = 0.0;
+        if (run) {
+          const double thr = fmax(DBL_MIN, precision * maxDiag);
+          wpq = W[p * n + q];
+          wqp = W[q * n + p];
+          rot = fabs(wpq) > thr || fabs(wqp) > thr;
+        }
+        if (!__ballot(rot))
+          return;
+        Rot jl{ 1.0, 0.0 }, jr{ 1.0, 0.0 };
+        double vp = 0.0, vq = 0.0;
+        if (rot) {
+          finished = false;
+          vp = V[j * n + p];
+          vq = V[j * n + q];
+          real_2x2(W[p * n + p], wpq, wqp, W[q * n + q], &jl, &jr);
+        }
+        __syncthreads();
+        // rows p, q of W and columns p, q of U by jl
+        const bool dl = rot && !(jl.c == 1.0 && jl.s == 0.0);
+        if (dl) {
+          const double xv = W[p * n + j], yv = W[q * n + j];
+          W[p * n + j] = jl.c * xv + jl.s * yv;
+          W[q * n + j] = -jl.s * xv + jl.c * yv;
+        }
+        {
+          V16& us = S == 0 ? u0 : (S == 1 ? u1 : u2);
+          const double uq = us[q & 15];
+          const double np = jl.c * up + jl.s * uq, nq = -jl.s * up + jl.c * uq;
+          up = dl ? np : up;
+          us[q & 15] = dl ? nq : uq;
+        }
+        __syncthreads();
+        // columns p, q of W and V by jr (applyOnTheRight: rotation (c, -s))
+        const double tc = jr.c, ts = -jr.s;
+        if (rot && !(tc == 1.0 && ts == 0.0)) {
+          const double xv = W[j * n + p], yv = W[j * n + q];
+          W[j * n + p] = tc * xv + ts * yv;
+          W[j * n + q] = -ts * xv + tc * yv;
+          V[j * n + p] = tc * vp + ts * vq;
+          V[j * n + q] = -ts * vp + tc * vq;
+        }
+        __syncthreads();
+        if (rot)
+          maxDiag = fmax(maxDiag, fmax(fabs(W[p * n + p]), fabs(W[q * n + q])));
+      };
+      const int q1 = p < 16 ? p : 16, q2 = p < 32 ? p : 32;
+      for (int q = 0; q < q1; q++)
+        step(q, std::integral_constant<int, 0>{});
+      for (int q = 16; q < q2; q++)
+        step(q, std::integral_constant<int, 1>{});
+      for (int q = 32; q < p; q++)
+        step(q, std::integral_constant<int, 2>{});
+      uset(p, up);
+    }
+    if (++p == n) {
+      p = 1;
+      go = __ballot(!finished) != 0ull && sweeps < 1000;
+      if (go) {
+        sweeps++;
+        run = !finished;
+        if (run)
+          finished = true;
+      }
+    }
+  }
+  // non-negative singular values: bit i of flip negates column i of U (applied as the rows go out)
+  unsigned long long flip = 0;
+  for (int i = 0; i < n; i++) {
+    const double a = bad ? 0.0 : W[i * n + i];
+    if (!bad && a < 0.0)
+      flip |= 1ull << i;
+    if (!bad && j == 0)
+      sv[i] = fabs(a) * scale;
+  }
+  if (act)
+    perm[j] = j;
+  __syncthreads();
+  // decreasing order, first maximum wins; perm[i] is the column of U and V now at position i
+  int nonzero = n;
+  bool stop = bad;
+  for (int i = 0; i < n; i++) {
+    int pos = i;
+    if (!stop) {
+      double mx = sv[i];
+      for (int c = i + 1; c < n; c++)
+        if (sv[c] > mx) {
+          mx = sv[c];
+          pos = c;
+        }
+      if (mx == 0.0) {
+        nonzero = i;
+        stop = true;
+      }
+    }
+    __syncthreads();
+    if (!stop && pos != i && j == 0) {
+      const double a = sv[i];
+      sv[i] = sv[pos];
+      sv[pos] = a;
+      const int c = perm[i];
+      perm[i] = perm[pos];
+      perm[pos] = c;
+    }
+    __syncthreads();
+  }
+  int rank = 0;
+  if (!bad) {
+    const double thr = fmax(sv[0] * ((double)n * DBL_EPSILON), DBL_MIN);
+    int r = nonzero - 1;
+    while (r >= 0 && sv[r] < thr)
+      r--;
+    rank = r + 1;
+  }
+  // rows of U into W's slot
+  if (act) {
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+      W[j * n + i] = ((flip >> i) & 1ull) ? -u0[i] : u0[i];
+      W[j * n + 16 + i] = ((flip >> (16 + i)) & 1ull) ? -u1[i] : u1[i];
+      if (32 + i < n)
+        W[j * n + 32 + i] = ((flip >> (32 + i)) & 1ull) ? -u2[i] : u2[i];
+    }
+  }
+  __syncthreads();
+  if (!bad && j < rank) {
+    const int cj = perm[j];
+    double t = 0.0;
+    for (int c = 0; c < n; c++)
+      t += W[c * n + cj] * b[c];
+    tmp[j] = (1.0 / sv[j]) * t;
+  }
+  __syncthreads();
+  if (act) {
+    double t = 0.0;
+    for (int i = 0; i < rank; i++)
+      t += V[j * n + perm[i]] * tmp[i];
+    x[j] = t;
+  }
+  __syncthreads();
+}
+
+// LDS doubles of smap_hyb (one system): W [n^2] (the Gram staging before the solve, U's rows after it)
+// | V [n^2] (acc, cd, cidx before the solve) | b, sv, tmp, x [n each] | perm [n] ints.
+__host__ __device__ inline size_t smap_hyb_lds_doubles(int n)
+{
+  return 2 * (size_t)n * n + 4 * (size_t)n + ((size_t)n + 1) / 2;
+}
+
+// Persistent grid over (target, theta) items of targets [t0, t0 + nt), one system per wavefront, for
+// 33 <= n <= 48: about 28 KB of LDS per wavefront for n = 41, against 40.7 KB in smap_multi.
+__global__ __launch_bounds__(64) void smap_hyb(SmapParams S, const double* __restrict__ D, int t0, int nt)
+{
+  extern __shared__ double smem[];
+  const int n = S.Ea + 1;
+  const int nacc = n * (n + 1) / 2 + n;
+  const int lane = threadIdx.x;
+  double* W = smem;
+  double* V = W + (size_t)n * n;
+  double* bvec = V + (size_t)n * n;
+  double* sv = bvec + n;
+  double* tmp = sv + n;
+  double* xsol = tmp + n;
+  int* perm = (int*)(xsol + n);
+  double* acc = V;
+  double* cd = acc + nacc;
+  int* cidx = (int*)(cd + SM_CH);
+  const int items = nt * S.T;
+  for (int i0 = blockIdx.x; i0 < items; i0 += gridDim.x) {
+    // the Gram staging lives in W's slot (SM_CH (n + 1) <= n^2), the accumulators in V's
+    const unsigned valid = form_group(S, D, t0, nt, i0, n, 1, W, bvec, nullptr, SM_CH, acc, cd, cidx, lane);
+    jacobi_svd_solve_hyb(n, W, V, bvec, sv, tmp, xsol, perm, !(valid & 1u), lane);
+    write_group(S, t0, i0, n, 1, valid, xsol, lane);
+    __syncthreads();
+  }
+}
+
 // Launches the S-map kernel for n = E_a + 1 unknowns over targets [t0, t0 + nt) at every theta:
-// register rows for n <= 24, LDS rows above.  A persistent grid of one-wavefront blocks, as many as
+// register rows for n <= 24, U rows in registers for 33 <= n <= 48, LDS rows otherwise.  A persistent grid of one-wavefront blocks, as many as
 // stay resident.  Returns 0, or -22 when n is beyond 64 (a system needs a lane per unknown).
 int smap_launch(int n, hipStream_t s, const SmapParams& S, const double* D, int t0, int nt, bool lds_rows)
 {
@@ -731,6 +979,9 @@ int smap_launch(int n, hipStream_t s, const SmapParams& S, const double* D, int 
          : n <= 20 ? (const void*)smap_rows<16, 4>
                    : (const void*)smap_rows<16, 8>;
     lds = sizeof(double) * smap_rows_lds_doubles(n, G);
+  } else if (!lds_rows && n >= 33 && n <= 48) {
+    fn = (const void*)smap_hyb;
+    lds = sizeof(double) * smap_hyb_lds_doubles(n);
   }
   if (lds > 64 * 1024)
     hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);

@@ -73,11 +73,12 @@ def test_task_parity(name):
         assert_predictions_close(gc, rc_)
 
 
-@pytest.mark.parametrize("Ea", [1, 3, 7, 8, 12, 15, 16, 17, 19, 22, 23])
+@pytest.mark.parametrize("Ea", [1, 3, 7, 8, 12, 15, 16, 17, 19, 22, 23, 32, 40, 47])
 def test_smap_register_rows_match_lds_rows(Ea, monkeypatch):
-    """For n = E_a + 1 <= 24 the S-map solve keeps U and V rows in registers; FEDM_SMAP_LDS_ROWS=1 keeps
-    them in LDS.  Both run the same floating-point operations in the same order, so every prediction
-    and coefficient must agree to the bit (k near n gives rank-deficient and ill-conditioned systems)."""
+    """For n = E_a + 1 <= 24 the S-map solve keeps U and V rows in registers, for 33 <= n <= 48 U's rows;
+    FEDM_SMAP_LDS_ROWS=1 keeps them in LDS.  Both run the same floating-point operations in the same
+    order, so every prediction and coefficient must agree to the bit (k near n gives rank-deficient and
+    ill-conditioned systems)."""
     fe = load_product()
     for k in (Ea + 1, Ea + 3):
         M, Mp, opts = random_task(100 + 7 * Ea + k, NL=700, NP=150, Ea=Ea, k=k, algorithm=1, thetas=(0, 1, 4),
