@@ -180,7 +180,7 @@ def main():
     ap.add_argument("--no-panel", action="store_true", help="skip the panel S-map line")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported"}))
+        print(json.dumps({"error": "the library runs on one GPU; --gpus above 1 is not supported"}))
         sys.exit(2)
 
     fe = load_product()
