@@ -58,8 +58,17 @@ __device__ __forceinline__ void lds_wave_sync()
 
 constexpr int PAR = 4; // queued candidates per target recomputed by the whole wavefront at once
 
+// Wavefronts per SIMD the register budget aims at: the blocks of 4 wavefronts the static LDS of
+// knn_mfma<KCAP, KP> (about 256 KP + 768 KCAP + 11 008 bytes) leaves room for on a CU, at most 4 (3 for
+// KP = 64, whose operands need more registers).
+constexpr int mfma_waves(int kcap, int kp)
+{
+  return (163840 / (256 * kp + 768 * kcap + 11008)) >= 4 ? (kp == 64 ? 3 : 4)
+                                                         : 163840 / (256 * kp + 768 * kcap + 11008);
+}
+
 template <int KCAP, int KP>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(KP == 64 ? 3 : 4))) void knn_mfma(MfmaParams P, int k, int slice_len, double* __restrict__ out_s,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(mfma_waves(KCAP, KP)))) void knn_mfma(MfmaParams P, int k, int slice_len, double* __restrict__ out_s,
                                                int* __restrict__ out_i, int* __restrict__ out_cnt)
 {
   constexpr int MCH = 64; // library points per LDS chunk (16-point tiles); LDS for four blocks per CU

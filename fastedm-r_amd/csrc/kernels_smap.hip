@@ -983,17 +983,24 @@ int smap_launch(int n, hipStream_t s, const SmapParams& S, const double* D, int 
     fn = (const void*)smap_hyb;
     lds = sizeof(double) * smap_hyb_lds_doubles(n);
   }
+  // HIP errors come back as -(error) - 1000, as elsewhere in the library
+  hipError_t e = hipSuccess;
   if (lds > 64 * 1024)
-    hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
   const long long groups = ((long long)nt * S.T + G - 1) / G;
   int dev = 0, ncu = 0, occ = 0;
-  hipGetDevice(&dev);
-  hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
-  hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fn, 64, lds);
+  if (e == hipSuccess)
+    e = hipGetDevice(&dev);
+  if (e == hipSuccess)
+    e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+  if (e == hipSuccess)
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fn, 64, lds);
+  if (e != hipSuccess)
+    return -(int)e - 1000;
   const int grid = (int)std::max<long long>(1, std::min<long long>(groups, (long long)ncu * std::max(occ, 1)));
   void* args[] = { (void*)&S, (void*)&D, (void*)&t0, (void*)&nt };
-  const hipError_t e = hipLaunchKernel(fn, dim3(grid), dim3(64), args, lds, s);
-  return e == hipSuccess ? 0 : -5;
+  e = hipLaunchKernel(fn, dim3(grid), dim3(64), args, lds, s);
+  return e == hipSuccess ? 0 : -(int)e - 1000;
 }
 
 } // namespace fedm
