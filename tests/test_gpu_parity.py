@@ -95,6 +95,20 @@ def test_smap_register_rows_match_lds_rows(Ea, monkeypatch):
         np.testing.assert_array_equal(np.ascontiguousarray(gc).view(np.uint64), np.ascontiguousarray(lc).view(np.uint64))
 
 
+def test_smap_width_limit():
+    """An S-map system needs a lane per unknown: E_a = 64 (65 unknowns) is refused with -22, while the
+    same task with simplex runs."""
+    fe = load_product()
+    M, Mp, opts = random_task(7, NL=300, NP=20, Ea=64, k=66, algorithm=1)
+    print("smap width limit: Ea=64", flush=True)
+    with pytest.raises(fe.FastEDMError, match=r"\(-22\)"):
+        fe.predict(M, Mp, opts, nn_cap=0)
+    opts["algorithm"] = 0
+    opts["k"] = 5
+    gp, gr, gc, gk, gn, status = fe.predict(M, Mp, opts, nn_cap=0)
+    assert status == 0
+
+
 WASS_CASES = {
     "wass_E5_k4": dict(NL=400, NP=100, E=5, k=4),
     "wass_dt_aspect": dict(NL=300, NP=80, E=6, k=5, dt=True, aspect_ratio=0.7, thetas=(0.5, 2.0)),
