@@ -29,6 +29,7 @@ struct BatchParams
   const double* thetas;
   int T, Ea, force_compute;
   double c1, c2, c3;     // mixed-precision error bound B(D) = c1 sqrt(D) + c2 D + c3
+  double c4;             // bound on the rounding of the direct kernel's expanded FP64 screen
   double* pred;
   int* rc;
   int* k_used;

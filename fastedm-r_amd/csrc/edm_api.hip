@@ -1040,6 +1040,15 @@ int edm_batch_create(edm_ctx* c, const edm_manifold_view* L, const edm_manifold_
   B.c1 = (4 * u * Ap + 4 * u * u * Ap) * std::sqrt((double)Ea);
   B.c2 = 2.01 * u + 1.01 * Ea * u / (1 - Ea * u);
   B.c3 = 4.04 * Ea * u * u * Ap * Ap;
+  {
+    // |(|x|^2 - 2 x.y) + |y|^2 - sum (x_j - y_j)^2| <= (3 Ea + 2) 2^-53 (|x| + |y|)^2 for the FP64 forms
+    // (|x|^2 and |y|^2 summed, the cross terms by fused multiply-adds, the direct sum in coordinate
+    // order), and (|x| + |y|)^2 <= 4 Ea A^2; the threshold's own rounding adds at most a few 2^-53
+    // (|x| + |y|)^2 more
+    const double u64 = 1.1102230246251565e-16; // 2^-53
+    const double A2 = A * A * (1 + 1e-12);
+    B.c4 = 4.0 * (Ea + 4) * u64 * 4.0 * Ea * A2;
+  }
   B.pred = b->dev<double>(nullptr, (size_t)b->total_out);
   B.rc = b->dev<int>(nullptr, (size_t)b->total_out);
   B.k_used = b->dev<int>(nullptr, (size_t)b->total_k);

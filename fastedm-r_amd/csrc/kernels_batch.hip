@@ -25,9 +25,10 @@
 namespace fedm {
 
 // Sorted by (squared distance, position); the new entry has the largest position so far, so it goes
-// after equal squared distances, and every entry behind it moves one place down.
+// after equal squared distances, and every entry behind it moves one place down.  Returns the squared
+// distance that leaves the list: the last entry when the new one enters, else the new one.
 template <int KCAP>
-__device__ __forceinline__ void insert_acc(double (&la)[KCAP], int (&li)[KCAP], double a, int i)
+__device__ __forceinline__ double insert_acc(double (&la)[KCAP], int (&li)[KCAP], double a, int i)
 {
   double cs = a;
   int ci = i;
@@ -43,6 +44,7 @@ __device__ __forceinline__ void insert_acc(double (&la)[KCAP], int (&li)[KCAP], 
     la[j] = ts;
     li[j] = ti;
   }
+  return cs;
 }
 
 // Sorted by (sqrt distance, position).
@@ -165,9 +167,25 @@ __device__ __forceinline__ double row_dist(const double* x, const double (&y)[EM
   return acc;
 }
 
-constexpr int SUB = 16; // rows screened together against the threshold they start with
+constexpr int SUB0 = 16; // rows of a library's first block, screened against their own k-th distance
+constexpr int SUB = 32;  // rows screened together against the threshold they start with (one mask bit each)
 
-// The rows of `mask` (bit r: LDS row x + r EMAX, library position pos0 + r) in position order, each
+// LDS row of the direct kernel: EMAX coordinates (zero padded), the squared norm, padded to an even
+// number of doubles so that rows stay 16-byte aligned
+template <int EMAX>
+constexpr int row_stride() { return (EMAX + 2) & ~1; }
+
+// Appends a row's screen result to a block mask: after a block's rows, row r of the block is bit
+// 31 - r when the block has 32 rows.  The empty asm keeps the compiler from holding every row's bit in a
+// register of its own until the block ends.
+__device__ __forceinline__ unsigned shift_in(unsigned mask, bool pass)
+{
+  mask = (mask << 1) | (pass ? 1u : 0u);
+  asm volatile("" : "+v"(mask));
+  return mask;
+}
+
+// The rows of `mask` (bit 31 - r: LDS row x + r ST, library position pos0 + r) in position order, each
 // against the list as it stands.  The update runs on every lane without branches: a row that does not
 // enter passes an infinite key, which changes nothing.
 template <int KCAP, int EMAX, bool FULL>
@@ -175,19 +193,29 @@ __device__ __forceinline__ void walk_rows(unsigned mask, const double* x, const 
                                           double (&la)[KCAP], int (&li)[KCAP], double& kth, double& thr_g, int& zeros,
                                           bool& near)
 {
+  constexpr int ST = row_stride<EMAX>();
   while (mask) {
-    const int r = __builtin_ctz(mask);
-    mask &= mask - 1;
-    const double acc = row_dist<EMAX>(x + r * EMAX, y);
+    const int r = __builtin_clz(mask);
+    mask ^= 0x80000000u >> r;
+    const double acc = row_dist<EMAX>(x + r * ST, y);
     // keep rule; a sum equal to MISSING cannot occur on this route (coordinates below 1e30)
     const bool drop = acc == 0.0 || acc == MISSING_D;
     zeros += drop ? 1 : 0;
-    // kth < acc < kth (1 + 2^-49): its square root may equal the k-th's
-    near = near | (!drop & (acc > kth) & (acc < thr_g));
-    const double out = kth; // leaves the k-th place when the row enters
-    insert_acc<KCAP>(la, li, (!drop & (acc < kth)) ? acc : INF_D, pos0 + r);
-    kth = FULL ? la[KCAP - 1] : kth_of<KCAP>(la, k);
-    near = near | ((out > kth) & (out < kth * SQRT_GUARD));
+    if (FULL) {
+      // k = KCAP: the squared distance that leaves the list is the row's own when it does not enter
+      // and the old k-th when it does; either lies strictly between the new k-th and the k-th times
+      // (1 + 2^-49) when its square root may equal the k-th's
+      const double out = insert_acc<KCAP>(la, li, drop ? INF_D : acc, pos0 + r);
+      kth = la[KCAP - 1];
+      near = near | ((out > kth) & (out < kth * SQRT_GUARD));
+    } else {
+      // kth < acc < kth (1 + 2^-49): its square root may equal the k-th's
+      near = near | (!drop & (acc > kth) & (acc < thr_g));
+      const double out = kth; // leaves the k-th place when the row enters
+      insert_acc<KCAP>(la, li, (!drop & (acc < kth)) ? acc : INF_D, pos0 + r);
+      kth = kth_of<KCAP>(la, k);
+      near = near | ((out > kth) & (out < kth * SQRT_GUARD));
+    }
     thr_g = kth * SQRT_GUARD;
   }
 }
@@ -196,8 +224,10 @@ template <int KCAP, int EMAX>
 __global__ __launch_bounds__(256) void batch_simplex(BatchParams B)
 {
   // library rows per LDS chunk (a multiple of SUB); rows past the chunk's end hold +inf and never pass
-  constexpr int BCH = EMAX <= 4 ? 512 : 4096 / EMAX;
-  __shared__ double lds64[BCH * EMAX];
+  constexpr int ST = row_stride<EMAX>();
+  constexpr int BCH = EMAX <= 4 ? 512 : (4096 / ST) / SUB * SUB;
+  // the first chunk's blocks start at SUB0, SUB0 + SUB, ...: its last block may reach SUB0 rows past BCH
+  __shared__ double lds64[(BCH + SUB0) * ST];
   __shared__ int item_sh;
 
   const int Ea = B.Ea;
@@ -224,10 +254,13 @@ __global__ __launch_bounds__(256) void batch_simplex(BatchParams B)
       const double* tgt = B.Tg + B.gbeg[tk];
 
       double y[EMAX];
+      double yy = 0.0;
       const double* yp = B.Pg + (size_t)(B.pred_begin[tk] + (active ? t : 0)) * Ea;
 #pragma unroll
-      for (int j = 0; j < EMAX; j++)
+      for (int j = 0; j < EMAX; j++) {
         y[j] = (active && j < Ea) ? yp[j] : 0.0;
+        yy += y[j] * y[j];
+      }
       double la[KCAP];
       int li[KCAP];
 #pragma unroll
@@ -241,62 +274,95 @@ __global__ __launch_bounds__(256) void batch_simplex(BatchParams B)
 
       for (int c0 = 0; c0 < nl; c0 += BCH) {
         const int cn = min(BCH, nl - c0);
-        const int cpad = (cn + SUB - 1) / SUB * SUB;
+        const int cpad = c0 == 0 ? (cn <= SUB0 ? SUB0 : SUB0 + (cn - SUB0 + SUB - 1) / SUB * SUB)
+                                 : (cn + SUB - 1) / SUB * SUB;
         __syncthreads();
         for (int c = threadIdx.x; c < cpad; c += blockDim.x) {
+          double* dst = lds64 + c * ST;
           if (c < cn) {
             const double* src = lg + (size_t)(c0 + c) * Ea;
+            double xx = 0.0;
 #pragma unroll
-            for (int j = 0; j < EMAX; j++)
-              lds64[c * EMAX + j] = j < Ea ? src[j] : 0.0;
+            for (int j = 0; j < EMAX; j++) {
+              const double v = j < Ea ? src[j] : 0.0;
+              dst[j] = v;
+              xx += v * v;
+            }
+            dst[EMAX] = -0.5 * xx;
           } else {
+            // infinite distance, and an expanded screen value of -inf or NaN, which never passes
 #pragma unroll
             for (int j = 0; j < EMAX; j++)
-              lds64[c * EMAX + j] = j == 0 ? INF_D : 0.0;
+              dst[j] = j == 0 ? INF_D : 0.0;
+            dst[EMAX] = -INF_D;
           }
         }
         __syncthreads();
         if (!active)
           continue;
-        for (int s0 = 0; s0 < cpad; s0 += SUB) {
-          // Screen SUB rows against a threshold, then walk the rows that pass in position order with
-          // the list as it stands.  Any threshold at or above (final k-th) (1 + 2^-49) gives the same
-          // list, zero count and hand-over decision as walking every row: the rows it drops can
+        for (int s0 = 0; s0 < cpad;) {
+          // Screen a block of rows against a threshold, then walk the rows that pass in position order
+          // with the list as it stands.  Any threshold at or above (final k-th) (1 + 2^-49) gives the
+          // same list, zero count and hand-over decision as walking every row: the rows it drops can
           // neither enter the final list nor lie just above its k-th entry.  The current threshold is
-          // one (it only falls); for the first rows, before it is finite, the KCAP-th smallest nonzero
-          // distance among them is another.  Rows at distance 0 always pass (every bound is > 0).
+          // one (it only falls); for a library's first SUB0 rows, before it is finite, the KCAP-th
+          // smallest nonzero distance among them is another.  Rows at distance 0 always pass (every
+          // bound is > 0).
           unsigned mask = 0;
-          if (KCAP * 4 <= SUB && c0 == 0 && s0 == 0) {
-            double dr[SUB], m[KCAP];
+          int nb;
+          if (c0 == 0 && s0 == 0) {
+            nb = SUB0;
+            if (KCAP * 4 <= SUB0) {
+              double dr[SUB0], m[KCAP];
 #pragma unroll
-            for (int j = 0; j < KCAP; j++)
-              m[j] = INF_D;
+              for (int j = 0; j < KCAP; j++)
+                m[j] = INF_D;
+#pragma unroll
+              for (int r = 0; r < SUB0; r++) {
+                dr[r] = row_dist<EMAX>(lds64 + r * ST, y);
+                double v = dr[r] == 0.0 ? INF_D : dr[r];
+#pragma unroll
+                for (int j = 0; j < KCAP; j++) {
+                  const double lo = fmin(m[j], v);
+                  v = fmax(m[j], v);
+                  m[j] = lo;
+                }
+              }
+              const double t0 = m[KCAP - 1] == INF_D ? INF_D : m[KCAP - 1] * SQRT_GUARD;
+#pragma unroll
+              for (int r = 0; r < SUB0; r++)
+                mask = shift_in(mask, dr[r] < t0);
+            } else {
+#pragma unroll
+              for (int r = 0; r < SUB0; r++)
+                mask = shift_in(mask, row_dist<EMAX>(lds64 + r * ST, y) < thr_g);
+            }
+            mask <<= 32 - SUB0;
+          } else {
+            // Expanded screen: e = x.y - |x|^2 / 2 by fused multiply-adds, so that the distance is
+            // |y|^2 - 2 e; the row passes when |y|^2 - 2 e < thr_g + c4, c4 bounding the rounding of
+            // both sides (the direct sum's included).  Every row whose distance is below thr_g
+            // passes; a row that passes needlessly lies at or above the k-th distance times
+            // (1 + 2^-49) and changes nothing in the walk.
+            nb = SUB;
+            const double lo = ((yy - thr_g) - B.c4) * 0.5;
 #pragma unroll
             for (int r = 0; r < SUB; r++) {
-              dr[r] = row_dist<EMAX>(lds64 + r * EMAX, y);
-              double v = dr[r] == 0.0 ? INF_D : dr[r];
+              const double* x = lds64 + (s0 + r) * ST;
+              double e = x[EMAX];
 #pragma unroll
-              for (int j = 0; j < KCAP; j++) {
-                const double lo = fmin(m[j], v);
-                v = fmax(m[j], v);
-                m[j] = lo;
-              }
+              for (int j = 0; j < EMAX; j++)
+                e = __builtin_fma(x[j], y[j], e);
+              mask = shift_in(mask, e > lo);
             }
-            const double t0 = m[KCAP - 1] == INF_D ? INF_D : m[KCAP - 1] * SQRT_GUARD;
-#pragma unroll
-            for (int r = 0; r < SUB; r++)
-              mask |= dr[r] < t0 ? 1u << r : 0u;
-          } else {
-#pragma unroll
-            for (int r = 0; r < SUB; r++)
-              mask |= row_dist<EMAX>(lds64 + (s0 + r) * EMAX, y) < thr_g ? 1u << r : 0u;
           }
           // the passing rows in position order, each against the list as it stands (k = KCAP, the
           // common case, reads the k-th entry directly)
           if (k == KCAP)
-            walk_rows<KCAP, EMAX, true>(mask, lds64 + s0 * EMAX, y, c0 + s0, k, la, li, kth, thr_g, zeros, near);
+            walk_rows<KCAP, EMAX, true>(mask, lds64 + s0 * ST, y, c0 + s0, k, la, li, kth, thr_g, zeros, near);
           else
-            walk_rows<KCAP, EMAX, false>(mask, lds64 + s0 * EMAX, y, c0 + s0, k, la, li, kth, thr_g, zeros, near);
+            walk_rows<KCAP, EMAX, false>(mask, lds64 + s0 * ST, y, c0 + s0, k, la, li, kth, thr_g, zeros, near);
+          s0 += nb;
         }
       }
       if (!active)
