@@ -20,8 +20,10 @@ constexpr int RANK_BINS = 2048;   // FP32 bit-pattern bins: 8 exponent + 3 manti
 constexpr int RANK_TARGET = 600;  // rows wanted below the histogram cut
 constexpr int RANK_SAMPLE = 8;    // the histogram counts one pool row in RANK_SAMPLE (the hash picks 0..7)
 constexpr int RANK_SAMPLE_TARGET = (RANK_TARGET + RANK_SAMPLE - 1) / RANK_SAMPLE;
-constexpr int RANK_CAP = 1536;    // LDS candidate capacity
-constexpr int RANK_SORT = 2048;   // bitonic sort width (power of two >= RANK_CAP)
+constexpr int RANK_CAP = 1024;    // LDS candidate capacity (the stored prefix is at most RANK_STORE)
+constexpr int RANK_SORT = 1024;   // bitonic sort width (power of two >= RANK_CAP)
+static_assert(RANK_STORE <= RANK_CAP, "the stored prefix is cut from the sorted candidates");
+static_assert(RANK_BINS * sizeof(unsigned) <= RANK_SORT * sizeof(double), "the histogram shares the sort keys' LDS");
 __device__ __forceinline__ float up_float(double x)
 {
   float f = (float)x;
@@ -33,8 +35,10 @@ __device__ __forceinline__ float up_float(double x)
 template <int EMAX>
 __global__ __launch_bounds__(256) void rank_build(RankParams R)
 {
-  __shared__ unsigned hist[RANK_BINS];
+  // The histogram is done with before pass 2 writes the first candidate, so it shares the sort keys'
+  // storage: about 12 KiB of LDS per block, eight blocks per CU.
   __shared__ double cs[RANK_SORT];
+  unsigned* hist = reinterpret_cast<unsigned*>(cs);
   __shared__ int cq[RANK_SORT];
   __shared__ int ncand, nzero, overflow, cut_bin, all_rows;
   __shared__ unsigned wsum[4];
