@@ -680,7 +680,7 @@ __global__ __launch_bounds__(64 * MS_WAVES) void rank_scan_multi(ScanParams S)
     if (lane + 64 * q < W)
       act_sh[lane + 64 * q] = act[q];
   ms_wave_sync();
-  constexpr int U = KCAP <= 4 ? 4 : KCAP <= 8 ? 2 : 1;
+  constexpr int U = KCAP <= 8 ? 2 : 1; // more side by side costs registers, and so occupancy
   const double* rs = S.rank_s + (size_t)t * S.tstride;
   for (int tau0 = lane * U; tau0 < ntask; tau0 += 64 * U) {
     int task[U], kq[U];
