@@ -220,8 +220,11 @@ __device__ __forceinline__ void walk_rows(unsigned mask, const double* x, const 
   }
 }
 
+// waves per SIMD the short-list instances are held to (five fit their registers without spills)
+constexpr int batch_waves(int kcap, int emax) { return kcap <= 4 && emax <= 4 ? 5 : 1; }
+
 template <int KCAP, int EMAX>
-__global__ __launch_bounds__(256) void batch_simplex(BatchParams B)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(batch_waves(KCAP, EMAX)))) void batch_simplex(BatchParams B)
 {
   // library rows per LDS chunk (a multiple of SUB); rows past the chunk's end hold +inf and never pass
   constexpr int ST = row_stride<EMAX>();
