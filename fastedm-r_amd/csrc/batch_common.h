@@ -40,6 +40,7 @@ struct BatchParams
 
 constexpr int RANK_STORE = 1024; // longest stored prefix
 constexpr int RANK_ZCAP = 8;     // rows at distance 0 kept per target
+constexpr int MS_WAVES = 1;      // wavefronts per block of rank_scan_multi (its LDS, 8 KiB a wavefront, sets occupancy)
 
 struct RankParams
 {

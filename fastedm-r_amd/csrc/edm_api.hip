@@ -1195,7 +1195,7 @@ int edm_batch_create(edm_ctx* c, const edm_manifold_view* L, const edm_manifold_
       }
       S.member = b->dev(member.data(), member.size());
       S.rank_task = b->dev(rank_task.data(), rank_task.size());
-      b->multi_lds = wave_lds * 4; // 4 wavefronts per block
+      b->multi_lds = wave_lds * MS_WAVES;
       if (b->multi_lds > 48 * 1024)
         hipFuncSetAttribute(rank_scan_multi_kernel(kcap, S.words), hipFuncAttributeMaxDynamicSharedMemorySize,
                             (int)b->multi_lds);
@@ -1297,8 +1297,9 @@ int edm_batch_run(edm_batch* b)
     HIP_TRY(hipEventRecord(c->ev[6], s));
     void* sargs[] = { &b->S };
     if (b->multi_scan) {
-      HIP_TRY(hipLaunchKernel(rank_scan_multi_kernel(b->kcap, b->S.words), dim3(((size_t)b->S.np * b->S.parts + 3) / 4),
-                              dim3(256), sargs, b->multi_lds, s));
+      HIP_TRY(hipLaunchKernel(rank_scan_multi_kernel(b->kcap, b->S.words),
+                              dim3(((size_t)b->S.np * b->S.parts + MS_WAVES - 1) / MS_WAVES), dim3(64 * MS_WAVES), sargs,
+                              b->multi_lds, s));
       HIP_TRY(hipLaunchKernel((const void*)rank_scatter, dim3((b->S.num_rank + 63) / 64, (b->S.np + 63) / 64, b->S.T),
                               dim3(256), sargs, 0, s));
     }

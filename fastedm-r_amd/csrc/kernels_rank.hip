@@ -533,7 +533,6 @@ __device__ __forceinline__ void ms_wave_sync()
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
 }
 
-constexpr int MS_WAVES = 4;
 
 // S.parts wavefronts share a target, each taking a contiguous range of task words.  Per lane and task
 // word the hit counts of its 32 tasks are kept as NP bit planes in registers (bit b of plane p = bit p of
