@@ -47,6 +47,31 @@ __device__ __forceinline__ double insert_acc(double (&la)[KCAP], int (&li)[KCAP]
   return cs;
 }
 
+// The same insertion with every compare made first: the list is sorted, so the compares are monotone,
+// entry j takes entry j - 1 when the new one goes before j - 1, the new one when it goes at j, and
+// keeps its own otherwise.  Updated from the last entry down, each entry is overwritten in place, which
+// spares the register copies the chained form costs in a loop.  Returns the squared distance that
+// leaves the list.
+template <int KCAP>
+__device__ __forceinline__ double insert_acc_par(double (&la)[KCAP], int (&li)[KCAP], double a, int i)
+{
+  bool c[KCAP];
+#pragma unroll
+  for (int j = 0; j < KCAP; j++)
+    c[j] = a < la[j];
+  const double out = c[KCAP - 1] ? la[KCAP - 1] : a;
+#pragma unroll
+  for (int j = KCAP - 1; j >= 1; j--) {
+    const double v = c[j] ? a : la[j];
+    const int vi = c[j] ? i : li[j];
+    la[j] = c[j - 1] ? la[j - 1] : v;
+    li[j] = c[j - 1] ? li[j - 1] : vi;
+  }
+  la[0] = c[0] ? a : la[0];
+  li[0] = c[0] ? i : li[0];
+  return out;
+}
+
 // Sorted by (sqrt distance, position).
 template <int KCAP>
 __device__ __forceinline__ void insert_sqrt(double (&ls)[KCAP], int (&li)[KCAP], double s, int i)
@@ -205,7 +230,7 @@ __device__ __forceinline__ void walk_rows(unsigned mask, const double* x, const 
       // k = KCAP: the squared distance that leaves the list is the row's own when it does not enter
       // and the old k-th when it does; either lies strictly between the new k-th and the k-th times
       // (1 + 2^-49) when its square root may equal the k-th's
-      const double out = insert_acc<KCAP>(la, li, drop ? INF_D : acc, pos0 + r);
+      const double out = insert_acc_par<KCAP>(la, li, drop ? INF_D : acc, pos0 + r);
       kth = la[KCAP - 1];
       near = near | ((out > kth) & (out < kth * SQRT_GUARD));
     } else {
