@@ -13,9 +13,10 @@ from tests.tasks import assert_predictions_close
 pytestmark = pytest.mark.gpu
 
 
-def ccm_like(seed, n, Ea, sizes, reps, integer=False):
+def ccm_like(seed, n, Ea, sizes, reps, integer=False, offset=0.0):
     rng = np.random.default_rng(seed)
     table = rng.integers(0, 5, size=(n, Ea)).astype(float) if integer else rng.random((n, Ea))
+    table = table + offset
     targets = rng.random(n)
     L = {"data": table, "targets": targets}
     libs, preds = [], []
@@ -27,17 +28,22 @@ def ccm_like(seed, n, Ea, sizes, reps, integer=False):
 
 
 @pytest.mark.parametrize("precision", [0, 1], ids=["plain", "mixed"])
-@pytest.mark.parametrize("case", ["E3_k4", "E5_k6_integer", "E8_k9_thetas", "tiny_libraries", "E3_k4_long"])
+@pytest.mark.parametrize("case", ["E3_k4", "E5_k6_integer", "E8_k9_thetas", "tiny_libraries", "E3_k4_long",
+                                  "E3_k4_offset"])
 def test_batch_matches_oracle(precision, case):
     fe = load_product()
     spec = {"E3_k4": (3, 4, [10, 37, 150, 900], 3, False, (1.0,)),
             "E5_k6_integer": (5, 6, [12, 80, 400], 2, True, (1.0,)),
             "E8_k9_thetas": (8, 9, [20, 300, 1200], 2, False, (0.0, 1.0, 4.0)),
             "tiny_libraries": (2, 3, [1, 2, 3, 4, 6], 2, False, (1.0,)),
-            "E3_k4_long": (3, 4, [2100, 2900], 2, False, (1.0,))}[case]  # FP32-screened direct kernel
+            "E3_k4_long": (3, 4, [2100, 2900], 2, False, (1.0,)),  # FP32-screened direct kernel
+            # coordinates near 3e7 with unit spread: |x|^2 - 2 x.y + |y|^2 loses about 1 to cancellation, far
+            # more than the k-th distances, so the direct kernel's expanded screen holds only with its bound
+            "E3_k4_offset": (3, 4, [10, 37, 150, 900], 2, False, (1.0,))}[case]
     Ea, k, sizes, reps, integer, thetas = spec
     n = 3000 if case == "E3_k4_long" else 1500
-    L, libs, preds = ccm_like(zlib.crc32(case.encode()) % 997, n, Ea, sizes, reps, integer)
+    offset = 3.0e7 if case == "E3_k4_offset" else 0.0
+    L, libs, preds = ccm_like(zlib.crc32(case.encode()) % 997, n, Ea, sizes, reps, integer, offset)
     opts = {"algorithm": 0, "k": k, "thetas": np.asarray(thetas), "precision": precision}
     print(f"case {case} precision {precision}: {len(libs)} tasks", flush=True)
     got = fe.predict_batch(L, L, opts, libs, preds, [k] * len(libs))
