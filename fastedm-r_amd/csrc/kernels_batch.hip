@@ -6,9 +6,12 @@
 // coordinates (a zero coordinate adds exactly 0 to the sum, so the padded sum is the CPU path's sum,
 // src/distances.cpp:186-216).  Both precision forms use this kernel: FP64 arithmetic runs at the FP32
 // rate on CDNA4 vector units, so FP32 screening saves nothing here.  The cost lies in the list
-// updates, which diverge across a wavefront: each thread screens 16 rows against its threshold
-// first and then walks only the rows that passed, so a wavefront runs as many updates as its busiest
-// thread needs rather than one per row in which any thread updates.
+// updates, which diverge across a wavefront: each thread screens a block of rows (a library's first 16,
+// then 32 at a time) against its threshold first and then walks only the rows that passed, so a
+// wavefront runs as many updates as its busiest thread needs rather than one per row in which any
+// thread updates.  The screen after the first block is an expanded FP64 form with a rounding bound
+// (three fused multiply-adds a row); the walk recomputes each passing row's distance in coordinate
+// order.
 //
 // The reference orders neighbours by (sqrt distance, library position) (src/edm.cpp:652-681).  The
 // scan keeps the k smallest by (squared distance, position) instead — no square root per candidate,
