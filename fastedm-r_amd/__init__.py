@@ -472,9 +472,16 @@ def predict_batch(L, P, opts, lib_lists, pred_lists, ks):
     lb = np.cumsum([0] + [len(a) for a in lib_lists[:-1]]).astype(np.int32)
     lc = np.array([len(a) for a in lib_lists], np.int32)
     lr = np.concatenate([np.asarray(a, np.int32) for a in lib_lists]) if lib_lists else np.zeros(1, np.int32)
-    pb = np.cumsum([0] + [len(a) for a in pred_lists[:-1]]).astype(np.int32)
-    pc = np.array([len(a) for a in pred_lists], np.int32)
-    pr = np.concatenate([np.asarray(a, np.int32) for a in pred_lists]) if pred_lists else np.zeros(1, np.int32)
+    shared = len(pred_lists) > 1 and all(np.array_equal(pred_lists[0], a) for a in pred_lists[1:])
+    if shared:
+        # one prediction list for every task, as a CCM sweep passes it (the rank-table route needs it)
+        pb = np.zeros(len(pred_lists), np.int32)
+        pc = np.full(len(pred_lists), len(pred_lists[0]), np.int32)
+        pr = np.asarray(pred_lists[0], np.int32) if len(pred_lists[0]) else np.zeros(1, np.int32)
+    else:
+        pb = np.cumsum([0] + [len(a) for a in pred_lists[:-1]]).astype(np.int32)
+        pc = np.array([len(a) for a in pred_lists], np.int32)
+        pr = np.concatenate([np.asarray(a, np.int32) for a in pred_lists]) if pred_lists else np.zeros(1, np.int32)
     ob = np.cumsum([0] + [T * len(a) for a in pred_lists[:-1]]).astype(np.int64)
     total = int(T * pc.sum())
     pred = np.zeros(max(total, 1))

@@ -725,6 +725,7 @@ struct edm_batch
   size_t multi_lds = 0;
   float last_ms = 0.f;
   int redo_last = 0; // targets handed to the exact selection in the last run
+  bool outputs_filled = false; // the outputs have held their fill values once (first run)
   // host copies for the task-by-task route
   edm_manifold_view Lv, Pv;
   edm_options o;
@@ -1285,9 +1286,15 @@ int edm_batch_run(edm_batch* b)
   BatchParams& B = b->B;
   hipStream_t s = c->stream;
   dfill(c, B.counter, 1, 0);
-  dfill(c, B.pred, (size_t)b->total_out, FEDM_MISSING_D);
-  dfill(c, B.rc, (size_t)b->total_out, (int)FEDM_UNKNOWN_ERROR);
-  dfill(c, B.k_used, (size_t)b->total_k, (int)FEDM_MISSING_I);
+  if (!b->outputs_filled) {
+    // Every run writes every output of every target (a prediction, or the fill values themselves for a
+    // target left unpredicted), so the outputs are filled on the first run only, as a safety net: a
+    // refill would cost each run about 16 bytes of HBM writes per prediction.
+    dfill(c, B.pred, (size_t)b->total_out, FEDM_MISSING_D);
+    dfill(c, B.rc, (size_t)b->total_out, (int)FEDM_UNKNOWN_ERROR);
+    dfill(c, B.k_used, (size_t)b->total_k, (int)FEDM_MISSING_I);
+    b->outputs_filled = true;
+  }
   dfill(c, B.redo_count, 1, 0);
   HIP_TRY(hipEventRecord(c->ev[4], s));
   if (b->rank_route) {

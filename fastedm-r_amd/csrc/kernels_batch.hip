@@ -128,12 +128,12 @@ __device__ void finish_target(const BatchParams& B, int tk, int t, int np, const
     if (B.force_compute) {
       ke = cnt;
     } else {
-      B.rc[ob + t] = RC_INSUFFICIENT_UNIQUE;
+      write_unpredicted(B.pred, B.rc, B.k_used, ob, T, np, t, RC_INSUFFICIENT_UNIQUE);
       return;
     }
   }
   if (ke == 0 || cnt == 0) {
-    B.rc[ob + t] = B.force_compute ? RC_SUCCESS : RC_INSUFFICIENT_UNIQUE;
+    write_unpredicted(B.pred, B.rc, B.k_used, ob, T, np, t, B.force_compute ? RC_SUCCESS : RC_INSUFFICIENT_UNIQUE);
     return;
   }
   if (ke == cnt) {

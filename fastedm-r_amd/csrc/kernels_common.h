@@ -26,6 +26,20 @@ enum
   RC_INSUFFICIENT_UNIQUE = 503,
   RC_UNKNOWN_ERROR = 8000
 };
+
+// A target left without a prediction (too few valid neighbours, k = 0): every theta's prediction
+// missing, the code at theta 0 and unknown-error at the others, k_used missing — the values the
+// reference's outputs are filled with before its per-target calls (src/edm.cpp:338-432), so that the
+// kernels leave no output of the target to an earlier fill.
+__device__ __forceinline__ void write_unpredicted(double* pred, int* rc, int* k_used, int64_t ob, int T, int np,
+                                                  int t, int code)
+{
+  for (int th = 0; th < T; th++) {
+    pred[ob + (int64_t)th * np + t] = 1.0e+100;
+    rc[ob + (int64_t)th * np + t] = th == 0 ? code : (int)RC_UNKNOWN_ERROR;
+  }
+  k_used[ob / T + t] = (int)(-2147483647 - 1);
+}
 enum
 {
   ALG_SIMPLEX = 0,

@@ -450,12 +450,13 @@ __global__ __launch_bounds__(256) void rank_scan(ScanParams S)
         if (S.force_compute)
           ke = vcnt;
         else {
-          S.rc[ob + t] = RC_INSUFFICIENT_UNIQUE;
+          write_unpredicted(S.pred_out, S.rc, S.k_used, ob, S.T, S.np, t, RC_INSUFFICIENT_UNIQUE);
           continue;
         }
       }
       if (ke == 0 || vcnt == 0) {
-        S.rc[ob + t] = S.force_compute ? RC_SUCCESS : RC_INSUFFICIENT_UNIQUE;
+        write_unpredicted(S.pred_out, S.rc, S.k_used, ob, S.T, S.np, t,
+                          S.force_compute ? RC_SUCCESS : RC_INSUFFICIENT_UNIQUE);
         continue;
       }
       if (ke == vcnt) {

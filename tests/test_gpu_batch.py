@@ -126,7 +126,7 @@ def test_rank_route_matches_oracle(case):
     opts = {"algorithm": 0, "k": 4, "thetas": np.asarray([1.0, 3.0]), "precision": 1}
     print(f"case {case}: {len(libs)} tasks over {n} rows", flush=True)
     got = fe.predict_batch(L, L, opts, libs, preds, [4] * len(libs))
-    assert fe.timings()[1] > 0.0  # the rank table was built on this call
+    assert fe.timings()[1] > 0.02 and fe.timings()[2] > 0.02  # the rank route ran (build and scan, ms)
     for t, (lib, pr) in enumerate(zip(libs, preds)):
         M = {"data": L["data"][lib], "targets": L["targets"][lib]}
         Mp = {"data": L["data"][pr]}
@@ -135,6 +135,36 @@ def test_rank_route_matches_oracle(case):
         np.testing.assert_array_equal(gr, rr)
         np.testing.assert_array_equal(gk, rk)
         assert_predictions_close(gp, rp)
+
+
+def test_rank_route_unpredicted_targets():
+    """Rank-route targets handed to the exact selection that end without a prediction: a table that is
+    constant but for one row in 500, so a constant target has more than eight library rows at distance 0
+    (no ranked prefix) and often fewer than k valid neighbours (force_compute off).  Every theta's output must hold the
+    unpredicted values (missing prediction, the code at theta 0, unknown-error at theta 1, k_used missing),
+    whatever the rank scan had written for the target before the hand-over."""
+    fe = load_product()
+    n, Ea = 5000, 2
+    L, libs, preds = ccm_like(4099, n, Ea, [40, 700, 2600, n - 100], 4)
+    data = np.full((n, Ea), 3.0)
+    data[::500] = L["data"][::500]
+    L["data"] = data
+    opts = {"algorithm": 0, "k": 4, "thetas": np.asarray([1.0, 3.0]), "precision": 1, "force_compute": 0}
+    print(f"mostly constant table: {len(libs)} tasks over {n} rows", flush=True)
+    got = fe.predict_batch(L, L, opts, libs, preds, [4] * len(libs))
+    assert fe.timings()[1] > 0.02 and fe.timings()[2] > 0.02  # the rank route ran (build and scan, ms)
+    unpredicted = 0
+    for t, (lib, pr) in enumerate(zip(libs, preds)):
+        M = {"data": L["data"][lib], "targets": L["targets"][lib]}
+        Mp = {"data": L["data"][pr]}
+        rp, rr, _, rk, _ = oracle.predict(M, Mp, dict(opts, k=4, nthreads=16))
+        gp, gr, gk = got[t]
+        np.testing.assert_array_equal(gr, rr)
+        np.testing.assert_array_equal(gk, rk)
+        assert_predictions_close(gp, rp)
+        if len(lib) >= 700:
+            unpredicted += int(np.sum(np.asarray(rr).reshape(2, -1)[0] == 503))
+    assert unpredicted > 0  # the case is exercised on rank-route tasks
 
 
 def test_ccm_plan_rank_route_matches_oracle():
