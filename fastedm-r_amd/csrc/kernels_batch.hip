@@ -248,8 +248,9 @@ __device__ __forceinline__ void walk_rows(unsigned mask, const double* x, const 
   }
 }
 
-// waves per SIMD the short-list instances are held to (five fit their registers without spills)
-constexpr int batch_waves(int kcap, int emax) { return kcap <= 4 && emax <= 4 ? 5 : 1; }
+// waves per SIMD the short-list instances are held to: at six (80 VGPRs) a few registers spill, and the
+// kernel still runs faster than at five (94 VGPRs, no spills), being bound by issue more than by latency
+constexpr int batch_waves(int kcap, int emax) { return kcap <= 4 && emax <= 4 ? 6 : 1; }
 
 template <int KCAP, int EMAX>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(batch_waves(KCAP, EMAX)))) void batch_simplex(BatchParams B)
