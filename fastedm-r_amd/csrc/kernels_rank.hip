@@ -75,14 +75,19 @@ __global__ __launch_bounds__(256) void rank_build(RankParams R)
   };
   constexpr int UNR = 4; // pool entries per thread whose loads are in flight together
   auto d64_of = [&](int row) {
+    // every coordinate load is issued at a clamped index before the sum, so that they are in flight
+    // together; a padding coordinate adds +0.0, which leaves the sum (and its order) unchanged
     const double* x = R.L + (size_t)row * Ea;
-    double acc = 0.0;
+    double v[EMAX];
 #pragma unroll
     for (int j = 0; j < EMAX; j++)
-      if (j < Ea) {
-        const double u = x[j] - y[j];
-        acc += u * u;
-      }
+      v[j] = x[j < Ea ? j : Ea - 1];
+    double acc = 0.0;
+#pragma unroll
+    for (int j = 0; j < EMAX; j++) {
+      const double u = j < Ea ? v[j] - y[j] : 0.0;
+      acc += u * u;
+    }
     return acc;
   };
 
