@@ -28,35 +28,13 @@
 namespace fedm {
 
 // Sorted by (squared distance, position); the new entry has the largest position so far, so it goes
-// after equal squared distances, and every entry behind it moves one place down.  Returns the squared
-// distance that leaves the list: the last entry when the new one enters, else the new one.
+// after equal squared distances, and every entry behind it moves one place down.  Every compare is made
+// first: the list is sorted, so the compares are monotone, entry j takes entry j - 1 when the new one goes
+// before j - 1, the new one when it goes at j, and keeps its own otherwise.  Updated from the last entry
+// down, each entry is overwritten in place, which spares the register copies a chained insertion costs
+// in a loop.  Returns the squared distance that leaves the list.
 template <int KCAP>
 __device__ __forceinline__ double insert_acc(double (&la)[KCAP], int (&li)[KCAP], double a, int i)
-{
-  double cs = a;
-  int ci = i;
-  bool placed = false;
-#pragma unroll
-  for (int j = 0; j < KCAP; j++) {
-    const bool take = placed || cs < la[j];
-    placed = take;
-    const double ts = take ? cs : la[j];
-    const int ti = take ? ci : li[j];
-    cs = take ? la[j] : cs;
-    ci = take ? li[j] : ci;
-    la[j] = ts;
-    li[j] = ti;
-  }
-  return cs;
-}
-
-// The same insertion with every compare made first: the list is sorted, so the compares are monotone,
-// entry j takes entry j - 1 when the new one goes before j - 1, the new one when it goes at j, and
-// keeps its own otherwise.  Updated from the last entry down, each entry is overwritten in place, which
-// spares the register copies the chained form costs in a loop.  Returns the squared distance that
-// leaves the list.
-template <int KCAP>
-__device__ __forceinline__ double insert_acc_par(double (&la)[KCAP], int (&li)[KCAP], double a, int i)
 {
   bool c[KCAP];
 #pragma unroll
@@ -233,7 +211,7 @@ __device__ __forceinline__ void walk_rows(unsigned mask, const double* x, const 
       // k = KCAP: the squared distance that leaves the list is the row's own when it does not enter
       // and the old k-th when it does; either lies strictly between the new k-th and the k-th times
       // (1 + 2^-49) when its square root may equal the k-th's
-      const double out = insert_acc_par<KCAP>(la, li, drop ? INF_D : acc, pos0 + r);
+      const double out = insert_acc<KCAP>(la, li, drop ? INF_D : acc, pos0 + r);
       kth = la[KCAP - 1];
       near = near | ((out > kth) & (out < kth * SQRT_GUARD));
     } else {
