@@ -114,13 +114,18 @@ def roofline(st, libs, reps):
         "rank_scan": ("hbm", reps * float(np.sum(K * pool / rank_libs * 4.0 + K * 8.0 + 16.0)) * N_TARGETS, "GB/s",
                       HBM_PEAK_GBS, "rank_scan_multi<4,1> + rank_scatter"),
     }
+    # HBM bytes per launch from rocprofv3 FETCH_SIZE (x2.0) and WRITE_SIZE (x1.0) passes over a steady-state
+    # step, scaled by the same counters on a 1 GiB copy (profiles/ccm_sweep_E3_k4.md); the direct kernel's
+    # writes include its register spills (0.56 GB) beside the 0.64 GB of outputs
+    traffic = {"direct": 1.40e9, "rank_build": 0.16e9, "rank_scan": 1.03e9 + 0.98e9}
     times = {"rank_build": st[1], "rank_scan": st[2], "direct": st[3]}
     name = max(times, key=lambda n: times[n])
     bound, amount, unit, peak, kernel = work[name]
     ms = float(times[name])
     achieved = amount / (ms * 1e-3) / (1e12 if unit == "TFLOP/s" else 1e9)
     return {"bound": bound, "achieved": achieved, "peak": peak, "unit": unit, "frac": achieved / peak,
-            "traffic": None, "kernel": kernel, "kernel_ms": ms,
+            "traffic": traffic[name], "traffic_source": "rocprofv3 FETCH_SIZE/WRITE_SIZE, profiles/ccm_sweep_E3_k4.md",
+            "kernel": kernel, "kernel_ms": ms,
             "algorithmic_work_per_launch": amount,
             "engine": {"direct": "fp64 VALU", "rank_build": "fp32 VALU", "rank_scan": "HBM3E"}[name]}
 
