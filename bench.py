@@ -94,6 +94,22 @@ def rank_threshold(k, npool):
     return f(k, npool)
 
 
+def rank_route_rates(st, libs, reps):
+    """The rank-table route in its own units: the build ranks every (pool row, target) pair once per
+    step (two FP32 passes over the pool per target), the scan walks ranked-prefix entries per (task,
+    target): about k * pool / L of them for a library of L rows drawn from the pool."""
+    pool = N_TARGETS
+    lthr = rank_threshold(K, pool)
+    rank_libs = libs[libs >= lthr]
+    pairs = float(pool) * N_TARGETS
+    entries = reps * float(np.sum(K * pool / rank_libs)) * N_TARGETS
+    return {"rank_tasks": int(reps * rank_libs.size), "library_threshold": int(lthr),
+            "build_pool_target_pairs": pairs, "build_pairs_per_s": pairs / (st[1] * 1e-3),
+            "scan_prefix_entries": entries, "scan_entries_per_s": entries / (st[2] * 1e-3),
+            "note": "one ranked prefix per target serves every long-library task, so the brute-force "
+                    "pair count of the step (215.8e9) overstates the work this route does"}
+
+
 def roofline(st, libs, reps):
     """Roofline of the stage that dominates the device time, with that stage's algorithmic work.
 
@@ -107,10 +123,11 @@ def roofline(st, libs, reps):
     direct_libs = libs[libs < lthr]
     rank_libs = libs[libs >= lthr]
     work = {
-        # libraries below the rank threshold are short: the direct kernel runs its FP64 variant
-        "direct": ("mfma", 9.0 * reps * float(np.sum(direct_libs)) * N_TARGETS, "TFLOP/s", FP64_PEAK_TFLOPS,
+        # libraries below the rank threshold are short: the direct kernel runs its FP64 variant; both
+        # kernels run on the vector units (no matrix-core instruction), hence "valu"
+        "direct": ("valu", 9.0 * reps * float(np.sum(direct_libs)) * N_TARGETS, "TFLOP/s", FP64_PEAK_TFLOPS,
                    "batch_simplex<4,3>"),
-        "rank_build": ("mfma", 2 * 9.0 * pool * N_TARGETS, "TFLOP/s", FP32_PEAK_TFLOPS, "rank_build<4>"),
+        "rank_build": ("valu", 2 * 9.0 * pool * N_TARGETS, "TFLOP/s", FP32_PEAK_TFLOPS, "rank_build<4>"),
         "rank_scan": ("hbm", reps * float(np.sum(K * pool / rank_libs * 4.0 + K * 8.0 + 16.0)) * N_TARGETS, "GB/s",
                       HBM_PEAK_GBS, "rank_scan_multi<4,1> + rank_scatter"),
     }
@@ -124,7 +141,9 @@ def roofline(st, libs, reps):
     ms = float(times[name])
     achieved = amount / (ms * 1e-3) / (1e12 if unit == "TFLOP/s" else 1e9)
     return {"bound": bound, "achieved": achieved, "peak": peak, "unit": unit, "frac": achieved / peak,
-            "traffic": traffic[name], "traffic_source": "rocprofv3 FETCH_SIZE/WRITE_SIZE, profiles/ccm_sweep_E3_k4.md",
+            "traffic": traffic[name],
+            "traffic_source": "not measured by this run: rocprofv3 FETCH_SIZE/WRITE_SIZE passes over the same "
+                              "command, scaled by a 1 GiB copy (profiles/ccm_sweep_E3_k4.md)",
             "kernel": kernel, "kernel_ms": ms,
             "algorithmic_work_per_launch": amount,
             "engine": {"direct": "fp64 VALU", "rank_build": "fp32 VALU", "rank_scan": "HBM3E"}[name]}
@@ -154,23 +173,52 @@ def panel_smap_leg(fe):
     return out
 
 
-def cpu_baseline(t, x, y, kw, seconds):
+def baseline_oracle():
+    """The oracle built -O3 -march=native -ffp-contract=off on this host (4 s); the prebuilt -O2 one if
+    the compiler is missing.  Returns (module, build flags)."""
+    import subprocess
+    import tempfile
     from oracle import oracle
+    out = tempfile.mkdtemp(prefix="fedm_baseline_")
+    try:
+        subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "native", f"NATIVE_OUT={out}"],
+                       check=True, capture_output=True, timeout=120)
+        oracle.use_library(os.path.join(out, "liboracle_native.so"))
+        return oracle, "-O3 -march=native -ffp-contract=off"
+    except Exception:
+        return oracle, "-O2 -ffp-contract=off (prebuilt; the native build failed)"
+
+
+def cpu_baseline(t, x, y, kw, seconds):
+    """The oracle's edm() (tasks serial, targets over a thread per host core, as the reference CPU path
+    runs them) on the first replicates of the sweep: a sample sized to about `seconds`, timed three
+    times, median reported.  The timed call includes the oracle's manifold builds and rho/MAE."""
+    oracle, flags = baseline_oracle()
     cores = os.cpu_count() or 1
     reps = 1
     libs = kw["library"]
     while True:
         sample = dict(kw, numReps=reps, numThreads=cores)
         t0 = time.perf_counter()
-        res = oracle.edm(t, x, y, **sample)
+        oracle.edm(t, x, y, **sample)
         dt = time.perf_counter() - t0
-        if dt >= seconds or reps >= 100:
+        if dt >= seconds / 2 or reps >= 100:
             break
         reps = min(100, max(reps + 1, int(reps * seconds / max(dt, 1e-3)) + 1))
-    preds = reps * len(libs) * 19998
+    times = []
+    res = None
+    for _ in range(3):
+        sample = dict(kw, numReps=reps, numThreads=cores, savePredictions=True)
+        t0 = time.perf_counter()
+        res = oracle.edm(t, x, y, **sample)
+        times.append(time.perf_counter() - t0)
+    dt = float(np.median(times))
+    preds = reps * len(libs) * N_TARGETS
     return {"value": preds / dt, "unit": "predictions/s", "cores": cores, "kind": "port",
-            "sample": f"the first {reps} of the 100 replicates x 40 libraries ({preds} predictions, {dt:.1f} s, "
-                      f"{cores} threads), rc={res['rc']}"}, res, reps
+            "build": flags, "runs_s": [round(v, 3) for v in times],
+            "sample": f"the first {reps} of the 100 replicates x 40 libraries ({preds} predictions), median of 3 "
+                      f"runs ({dt:.1f} s, {cores} threads), manifold builds and rho/MAE included; rc={res['rc']}"}, \
+        res, reps
 
 
 def main():
@@ -205,10 +253,14 @@ def main():
           f"direct {st[3]:.1f})", file=sys.stderr, flush=True)
     redos = fe.exact_redos()
     res_mixed = mixed.result()
+    out_mixed = mixed.fetch()  # every prediction of the last timed step
 
     out = {"metric": METRIC, "value": n_pred / sec, "unit": "predictions/s", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": sec * 1e3, "higher_is_better": True, "scaling": "weak",
            "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic",
+           "value_excludes": "fedm_plan_create (host normalisation, splits, point tables and their upload, once "
+                             "per command) and fedm_plan_result (copy-back and rho/MAE); the step is fedm_plan_run "
+                             "with every input resident in HBM",
            "config": {"workload": "ccm_sweep_coupled_logistic_T20000_E3_40lib_10to19000",
                       "replicates": args.reps, "tasks": int(args.reps * len(libs)), "predictions_per_step": n_pred,
                       "library_target_pairs_per_step": pairs, "k": 4, "algorithm": "simplex",
@@ -220,18 +272,28 @@ def main():
                         "direct": float(st[3])}
     out["exact_reselections"] = redos  # (task, target) pairs of the last step handed to the exact selection
     out["roofline"] = roofline(st, libs, args.reps)
+    out["rank_route"] = rank_route_rates(st, libs, args.reps)
 
     # plain forms and parity
     print("building the plain plans", file=sys.stderr, flush=True)
     pb = fe.Plan(t, x, y, precision=fe.PLAIN, **kw)
-    sec_pb, st_pb = timed_steps(pb, fe, 1, 0)
+    sec_pb, st_pb = timed_steps(pb, fe, 3, 1)
     kms_pb = float(st_pb[0])
     res_pb = pb.result()
-    out["plain_batched"] = {"value": n_pred / sec_pb, "ms_per_step": sec_pb * 1e3, "kernel_ms": kms_pb}
+    out_pb = pb.fetch()
+    out["plain_batched"] = {"value": n_pred / sec_pb, "ms_per_step": sec_pb * 1e3, "kernel_ms": kms_pb, "steps": 3}
     d_rho = np.nanmax(np.abs(res_mixed["stats"]["rho"] - res_pb["stats"]["rho"]))
     d_mae = np.nanmax(np.abs(res_mixed["stats"]["mae"] - res_pb["stats"]["mae"]))
+    pm, pp = out_mixed[0], out_pb[0]
     out["parity"] = {"max_abs_d_rho_mixed_vs_plain": float(d_rho), "max_abs_d_mae_mixed_vs_plain": float(d_mae),
-                     "k_range": [res_mixed["kMin"], res_mixed["kMax"]]}
+                     "k_range": [res_mixed["kMin"], res_mixed["kMax"]],
+                     "predictions_compared": int(pm.size),
+                     "max_abs_d_yhat_mixed_vs_plain_batched": float(np.max(np.abs(pm - pp))) if pm.size else 0.0,
+                     "predictions_differing_bits_mixed_vs_plain_batched": int(np.sum(pm.view(np.int64) !=
+                                                                                     pp.view(np.int64))),
+                     "rc_and_k_used_equal_mixed_vs_plain_batched": bool(np.array_equal(out_mixed[1], out_pb[1]) and
+                                                                       np.array_equal(out_mixed[2], out_pb[2]))}
+    del out_pb
     if not args.no_plain:
         pt = fe.Plan(t, x, y, precision=2, **kw)
         t0 = time.perf_counter()
@@ -260,6 +322,19 @@ def main():
             np.nanmax(np.abs(res_mixed["stats"]["rho"][:n] - res_cpu["stats"]["rho"])))
         out["parity"]["max_abs_d_mae_mixed_vs_oracle"] = float(
             np.nanmax(np.abs(res_mixed["stats"]["mae"][:n] - res_cpu["stats"]["mae"])))
+        # the oracle saves the last task's predictions (replicate `reps`, library 19000): all of them
+        # against the same task of the mixed step
+        # (R's layout: one row per table row, missing where a row is not a target; here every usable row
+        # is one, in row order, as in the task's outputs)
+        ref = res_cpu["predictions"][:, 0]
+        ref = ref[~np.isnan(ref)]
+        task = reps * len(libs) - 1
+        got = pm[task * N_TARGETS:(task + 1) * N_TARGETS]
+        if ref.size == got.size:
+            out["parity"]["oracle_saved_task"] = int(task)
+            out["parity"]["max_abs_d_yhat_mixed_vs_oracle_saved_task"] = float(np.max(np.abs(got - ref)))
+            out["parity"]["max_rel_d_yhat_mixed_vs_oracle_saved_task"] = float(
+                np.max(np.abs(got - ref) / np.maximum(np.abs(ref), 1e-300)))
     print(json.dumps(out))
 
 

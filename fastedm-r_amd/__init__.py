@@ -51,13 +51,15 @@ class Command(C.Structure):
                 ("dt_weight", C.c_double), ("allow_missing", C.c_int), ("missing_distance", C.c_double),
                 ("panel_weight", C.c_double), ("panel_weights", _dp), ("panel_weights_dim", C.c_int),
                 ("num_threads", C.c_int), ("low_memory", C.c_int), ("predict_with_past", C.c_int),
-                ("distance", C.c_int), ("aspect_ratio", C.c_double), ("precision", C.c_int)]
+                ("distance", C.c_int), ("aspect_ratio", C.c_double), ("precision", C.c_int),
+                ("keep_going", C.c_void_p)]
 
 
 EXPORTS = ["edm_ctx_create", "edm_ctx_destroy", "edm_predict", "edm_predict_batch", "edm_ctx_timings",
            "edm_ctx_exact_redos",
            "edm_batch_create", "edm_batch_run", "edm_batch_fetch", "edm_batch_destroy", "fedm_plan_create",
-           "fedm_plan_run", "fedm_plan_result", "fedm_plan_num_predictions", "fedm_plan_device_batches", "fedm_plan_destroy", "edm_batch_on_device", "edm_selftest_mfma", "fedm_rank_threshold", "fedm_run_command", "fedm_run_task_group", "fedm_result_rc", "fedm_result_num_stats", "fedm_result_stats", "fedm_result_k",
+           "fedm_plan_run", "fedm_plan_result", "fedm_plan_num_predictions", "fedm_plan_device_batches", "fedm_plan_destroy",
+           "fedm_plan_num_tasks", "fedm_plan_task", "fedm_plan_fetch", "edm_batch_set_keep_going", "edm_batch_on_device", "edm_selftest_mfma", "fedm_rank_threshold", "fedm_run_command", "fedm_run_task_group", "fedm_result_rc", "fedm_result_num_stats", "fedm_result_stats", "fedm_result_k",
            "fedm_result_scalar", "fedm_result_matrix_shape", "fedm_result_matrix", "fedm_result_num_manifolds",
            "fedm_result_manifold_shape", "fedm_result_manifold", "fedm_result_free"]
 
@@ -96,6 +98,9 @@ def load_library():
         L.fedm_plan_num_predictions.restype = C.c_int64
         L.fedm_plan_destroy.argtypes = [C.c_void_p]
         L.fedm_plan_device_batches.argtypes = [C.c_void_p]
+        L.fedm_plan_num_tasks.argtypes = [C.c_void_p]
+        L.fedm_plan_task.argtypes = [C.c_void_p, C.c_int, _ip, _ip, _ip]
+        L.fedm_plan_fetch.argtypes = [C.c_void_p, _dp, _ip, _ip]
         for name in ["fedm_result_rc", "fedm_result_num_stats", "fedm_result_stats", "fedm_result_k",
                      "fedm_result_scalar", "fedm_result_matrix_shape", "fedm_result_matrix",
                      "fedm_result_num_manifolds", "fedm_result_manifold_shape", "fedm_result_manifold",
@@ -149,7 +154,7 @@ def _isnothing(v):
 def _command(t, x, y, panel, E, tau, theta, library, k, algorithm, p, crossfold, full, shuffle, seed, copredict,
              savePredictions, saveCoPredictions, saveManifolds, saveSMAPCoeffs, extras, allowMissing,
              missingDistance, dt, reldt, dtWeight, numReps, panelWeight, panelWeights, numThreads, lowMemory,
-             predictWithPast, distance, aspectRatio, precision, keep):
+             predictWithPast, distance, aspectRatio, precision, keep, keep_going=None):
     n = len(t)
     if len(x) != n:
         raise ValueError("The time and x variables should be the same length")
@@ -225,6 +230,10 @@ def _command(t, x, y, panel, E, tau, theta, library, k, algorithm, p, crossfold,
     c.distance = {"mae": 0, "euclidean": 1, "wasserstein": 2}[distance]
     c.aspect_ratio = float(aspectRatio)
     c.precision = int(precision)
+    if keep_going is not None:
+        cb = KEEP_GOING(keep_going)
+        keep.append(cb)
+        c.keep_going = C.cast(cb, C.c_void_p)
     return c
 
 
@@ -252,19 +261,21 @@ def edm(t, x, y=None, panel=None, E=2, tau=1, theta=1, library=None, k=0, algori
         saveManifolds=False, saveSMAPCoeffs=False, extras=None, allowMissing=False, missingDistance=0.0,
         dt=False, reldt=False, dtWeight=0.0, numReps=1, panelWeight=0, panelWeights=None, verbosity=0,
         showProgressBar=None, numThreads=1, lowMemory=False, predictWithPast=False, distance="euclidean",
-        aspectRatio=1.0, precision=PLAIN):
+        aspectRatio=1.0, precision=PLAIN, keep_going=None):
     """The reference's edm() (R/edm.R:232), predictions made on the GPU.
 
     Returns a dict shaped like R's result list: rc, stats (E, library, theta, rho, mae; NaN = NA),
     summary, kMin, kMax and, when requested, copredStats, predictions, copredictions, coeffs, Ms, Mps,
-    missingdistance, dtWeight.  ``precision`` selects the plain FP64 or the mixed form.
+    missingdistance, dtWeight.  ``precision`` selects the plain FP64 or the mixed form.  ``keep_going``
+    (a function returning a truth value, or None) plays the part of R's interrupt check: once it says
+    stop, the call returns with rc 1 (BREAK_HIT) and no statistics.
     """
     L = load_library()
     keep = []
     c = _command(t, x, y, panel, E, tau, theta, library, k, algorithm, p, crossfold, full, shuffle, seed,
                  copredict, savePredictions, saveCoPredictions, saveManifolds, saveSMAPCoeffs, extras, allowMissing,
                  missingDistance, dt, reldt, dtWeight, numReps, panelWeight, panelWeights, numThreads, lowMemory,
-                 predictWithPast, distance, aspectRatio, precision, keep)
+                 predictWithPast, distance, aspectRatio, precision, keep, keep_going)
     h = C.c_void_p()
     status = L.fedm_run_command(context(), C.byref(c), C.byref(h))
     if status < 0:
@@ -415,7 +426,7 @@ class Plan:
                     saveCoPredictions=False, saveManifolds=False, saveSMAPCoeffs=False, extras=None,
                     allowMissing=False, missingDistance=0.0, dt=False, reldt=False, dtWeight=0.0, numReps=1,
                     panelWeight=0, panelWeights=None, numThreads=1, lowMemory=False, predictWithPast=False,
-                    distance="euclidean", aspectRatio=1.0, precision=PLAIN)
+                    distance="euclidean", aspectRatio=1.0, precision=PLAIN, keep_going=None)
         args.update(kw)
         self._flags = (args["copredict"] is not None and len(args["copredict"]) > 0, args["allowMissing"],
                        args["dt"] or args["reldt"])
@@ -435,9 +446,46 @@ class Plan:
         return int(load_library().fedm_plan_device_batches(self._h))
 
     def run(self):
+        """One device run of every task; returns 0, or 1 (BREAK_HIT) when keep_going stopped it."""
         status = load_library().fedm_plan_run(self._h)
-        if status != 0:
+        if status < 0 or status > 1:
             raise FastEDMError(f"fedm_plan_run failed ({status})")
+        return status
+
+    def tasks(self, rows=False, indices=None):
+        """The plan's tasks in the reference's order (or those at ``indices``): dicts with E_actual,
+        library, k, copredict, E and the numbers of library and prediction rows (and, with rows=True,
+        the table rows themselves)."""
+        L = load_library()
+        out = []
+        info = np.zeros(7, np.int32)
+        if indices is None:
+            indices = range(int(L.fedm_plan_num_tasks(self._h)))
+        for i in indices:
+            L.fedm_plan_task(self._h, i, info.ctypes.data_as(_ip), None, None)
+            d = {"E_actual": int(info[0]), "library": int(info[1]), "k": int(info[2]), "num_lib": int(info[3]),
+                 "num_pred": int(info[4]), "copredict": bool(info[5]), "E": int(info[6])}
+            if rows:
+                lr = np.zeros(max(d["num_lib"], 1), np.int32)
+                pr = np.zeros(max(d["num_pred"], 1), np.int32)
+                L.fedm_plan_task(self._h, i, info.ctypes.data_as(_ip), lr.ctypes.data_as(_ip), pr.ctypes.data_as(_ip))
+                d["lib_rows"], d["pred_rows"] = lr[:d["num_lib"]], pr[:d["num_pred"]]
+            out.append(d)
+        return out
+
+    def fetch(self):
+        """Every task's outputs of the last run, concatenated in task order: pred and rc ([T x Np] per
+        task, flattened) and k_used ([Np] per task)."""
+        L = load_library()
+        T = max(int(self._c.num_thetas), 1)
+        n = self.num_predictions
+        pred = np.zeros(max(n, 1))
+        rc = np.zeros(max(n, 1), np.int32)
+        ku = np.zeros(max(n // T, 1), np.int32)
+        status = L.fedm_plan_fetch(self._h, pred.ctypes.data_as(_dp), rc.ctypes.data_as(_ip), ku.ctypes.data_as(_ip))
+        if status != 0:
+            raise FastEDMError(f"fedm_plan_fetch failed ({status})")
+        return pred[:n], rc[:n], ku[:n // T]
 
     def result(self):
         L = load_library()

@@ -724,6 +724,7 @@ struct edm_batch
   bool multi_scan = false; // one pass per target over every rank task (rank_scan_multi)
   size_t multi_lds = 0;
   float last_ms = 0.f;
+  int (*keep_going)(void) = nullptr; // polled before each task or stage launch
   int redo_last = 0; // targets handed to the exact selection in the last run
   bool outputs_filled = false; // the outputs have held their fill values once (first run)
   // host copies for the task-by-task route
@@ -1227,10 +1228,35 @@ int edm_batch_create(edm_ctx* c, const edm_manifold_view* L, const edm_manifold_
   return 0;
 }
 
+void edm_batch_set_keep_going(edm_batch* b, int (*keep_going)(void))
+{
+  b->keep_going = keep_going;
+}
+
+// A stop requested through keep_going: every output takes the interrupted values (prediction missing,
+// BREAK_HIT, k_used missing) and the next run fills the outputs afresh before its stages.
+static int batch_break(edm_batch* b)
+{
+  edm_ctx* c = b->ctx;
+  if (!b->device_route) {
+    std::fill(b->hpred.begin(), b->hpred.end(), FEDM_MISSING_D);
+    std::fill(b->hrc.begin(), b->hrc.end(), (int)FEDM_BREAK_HIT);
+    std::fill(b->hku.begin(), b->hku.end(), (int)FEDM_MISSING_I);
+    return FEDM_BREAK_HIT;
+  }
+  dfill(c, b->B.pred, (size_t)b->total_out, FEDM_MISSING_D);
+  dfill(c, b->B.rc, (size_t)b->total_out, (int)FEDM_BREAK_HIT);
+  dfill(c, b->B.k_used, (size_t)b->total_k, (int)FEDM_MISSING_I);
+  b->outputs_filled = false;
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return FEDM_BREAK_HIT;
+}
+
 int edm_batch_run(edm_batch* b)
 {
   edm_ctx* c = b->ctx;
   hipSetDevice(c->device);
+  auto stop = [b] { return b->keep_going && !b->keep_going(); };
   if (!b->device_route) {
     // task-by-task: gather each task's manifolds and predict them with edm_predict
     const int Ea = b->Ea;
@@ -1238,6 +1264,8 @@ int edm_batch_run(edm_batch* b)
     b->hrc.assign((size_t)b->total_out, FEDM_UNKNOWN_ERROR);
     b->hku.assign((size_t)b->total_k, FEDM_MISSING_I);
     for (int tk = 0; tk < b->tasks; tk++) {
+      if (stop())
+        return batch_break(b);
       const int nl = b->lc[tk], np = b->pc[tk];
       std::vector<double> md((size_t)nl * Ea), mt(nl), mtt(nl), mpd((size_t)np * Ea), mppt(np);
       std::vector<int> mpan, mppan;
@@ -1277,12 +1305,16 @@ int edm_batch_run(edm_batch* b)
       to.metrics = b->metrics.empty() ? nullptr : b->metrics.data();
       const int64_t ob = b->ob[tk];
       const int r = edm_predict(c, &M, &Mp, &to, b->hpred.data() + ob, b->hrc.data() + ob, nullptr,
-                                b->hku.data() + ob / b->T, nullptr, 0, nullptr);
+                                b->hku.data() + ob / b->T, nullptr, 0, b->keep_going);
+      if (r == FEDM_BREAK_HIT)
+        return batch_break(b);
       if (r != 0)
         return r;
     }
     return 0;
   }
+  if (stop())
+    return batch_break(b);
   BatchParams& B = b->B;
   hipStream_t s = c->stream;
   dfill(c, B.counter, 1, 0);
@@ -1302,6 +1334,8 @@ int edm_batch_run(edm_batch* b)
     void* rargs[] = { &b->R };
     HIP_TRY(hipLaunchKernel(rank_build_kernel(b->emax), dim3(b->R.np), dim3(256), rargs, 0, s));
     HIP_TRY(hipEventRecord(c->ev[6], s));
+    if (stop())
+      return batch_break(b);
     void* sargs[] = { &b->S };
     if (b->multi_scan) {
       HIP_TRY(hipLaunchKernel(rank_scan_multi_kernel(b->kcap, b->S.words),
@@ -1317,6 +1351,8 @@ int edm_batch_run(edm_batch* b)
     HIP_TRY(hipEventRecord(c->ev[6], s));
   }
   HIP_TRY(hipEventRecord(c->ev[7], s));
+  if (stop())
+    return batch_break(b);
   const void* fn = batch_kernel(b->kcap, b->Ea);
   void* args[] = { &B };
   if (B.num_items > 0)
@@ -1393,9 +1429,12 @@ int edm_predict_batch(edm_ctx* c, const edm_manifold_view* L, const edm_manifold
                            k, out_begin, &b);
   if (r != 0)
     return r;
+  edm_batch_set_keep_going(b, keep_going);
   r = edm_batch_run(b);
-  if (r == 0)
-    r = edm_batch_fetch(b, pred, rc, k_used);
+  if (r == 0 || r == FEDM_BREAK_HIT) {
+    const int q = edm_batch_fetch(b, pred, rc, k_used);
+    r = q != 0 ? q : r;
+  }
   edm_batch_destroy(b);
   return r;
 }

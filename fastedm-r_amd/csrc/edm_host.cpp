@@ -524,6 +524,9 @@ struct fedm_plan
   bool copred_mode = false;
   int save_manifolds = 0, save_smap_coeffs = 0;
   std::vector<int> extra_metrics; // per extra: 0 Diff, 1 CheckSame (empty: all Diff)
+  int (*keep_going)(void) = nullptr; // polled between tasks and stages (src/edm.cpp:134, 377, 419)
+  bool broken = false;               // the last run stopped because keep_going said so
+  bool fetched = false;              // group outputs copied back since the last run
   struct Task
   {
     int E, k, library, mask; // mask: index into masks (prediction rows)
@@ -572,6 +575,7 @@ static int plan_tasks(edm_ctx* ctx, fedm_plan* P, const TaskGrid& grid);
 static int plan_command(edm_ctx* ctx, const fedm_command* c, fedm_plan* P)
 {
   P->ctx = ctx;
+  P->keep_going = c->keep_going;
   const int n = c->n;
   Series& s = P->s;
   s.t = normalised(c->t, n);
@@ -889,30 +893,51 @@ static int run_group_taskwise(fedm_plan* P, fedm_plan::Group& g)
     to.k = tk.k;
     to.metrics = metrics.data();
     const int64_t ob = g.out_begin[m];
+    if (P->keep_going && !P->keep_going())
+      return FEDM_BREAK_HIT;
     if (Mp.n > 0) {
       const edm_manifold_view mv = M.view(), mpv = Mp.view();
       const int r = edm_predict(P->ctx, &mv, &mpv, &to, g.pred.data() + ob, g.rc.data() + ob,
-                                g.coeffs.data() + (size_t)(ob / T) * C, g.ku.data() + ob / T, nullptr, 0, nullptr);
-      if (r < 0)
+                                g.coeffs.data() + (size_t)(ob / T) * C, g.ku.data() + ob / T, nullptr, 0,
+                                P->keep_going);
+      if (r != 0)
         return r;
     }
   }
   return 0;
 }
 
+// Every group in turn; keep_going is polled before each group and passed on to the batches and the
+// task-by-task route.  A stop returns FEDM_BREAK_HIT and leaves the plan marked as broken.
 static int run_plan(fedm_plan* P)
 {
+  P->broken = false;
+  P->fetched = false;
   for (auto& g : P->groups) {
-    const int r = g.taskwise ? run_group_taskwise(P, g) : g.batch ? edm_batch_run(g.batch) : 0;
+    if (P->keep_going && !P->keep_going()) {
+      P->broken = true;
+      return FEDM_BREAK_HIT;
+    }
+    int r = 0;
+    if (g.taskwise)
+      r = run_group_taskwise(P, g);
+    else if (g.batch) {
+      edm_batch_set_keep_going(g.batch, P->keep_going);
+      r = edm_batch_run(g.batch);
+    }
+    if (r == FEDM_BREAK_HIT)
+      P->broken = true;
     if (r != 0)
       return r;
   }
   return 0;
 }
 
-static int finish_plan(fedm_plan* P, fedm_result* res)
+// Copies the batched groups' outputs back to the host (the task-by-task groups hold theirs already).
+static int fetch_groups(fedm_plan* P)
 {
-  const Series& s = P->s;
+  if (P->fetched)
+    return 0;
   const int T = P->o.num_thetas;
   for (auto& g : P->groups) {
     if (g.taskwise)
@@ -925,6 +950,19 @@ static int finish_plan(fedm_plan* P, fedm_result* res)
       if (r != 0)
         return r;
     }
+  }
+  P->fetched = true;
+  return 0;
+}
+
+static int finish_plan(fedm_plan* P, fedm_result* res)
+{
+  const Series& s = P->s;
+  const int T = P->o.num_thetas;
+  {
+    const int r = fetch_groups(P);
+    if (r != 0)
+      return r;
   }
   bool first = true;
   for (auto& tk : P->tasks) {
@@ -1024,8 +1062,9 @@ int fedm_plan_result(fedm_plan* P, fedm_result** out)
 {
   *out = nullptr;
   auto res = std::make_unique<fedm_result>();
-  if (P->rc0) {
-    res->rc = P->rc0;
+  if (P->rc0 || P->broken) {
+    // no usable rows, or an interrupted run: the reference stores no statistics (src/edm.cpp:438-439)
+    res->rc = P->rc0 ? P->rc0 : FEDM_BREAK_HIT;
     *out = res.release();
     return 0;
   }
@@ -1042,6 +1081,53 @@ int64_t fedm_plan_num_predictions(const fedm_plan* P)
   for (const auto& g : P->groups)
     n += g.total_out;
   return n;
+}
+
+int fedm_plan_num_tasks(const fedm_plan* P)
+{
+  return (int)P->tasks.size();
+}
+
+int fedm_plan_task(const fedm_plan* P, int task, int* info, int* lib_rows, int* pred_rows)
+{
+  if (task < 0 || task >= (int)P->tasks.size())
+    return -22;
+  const auto& tk = P->tasks[task];
+  const auto& rows = P->pred_rows[tk.mask];
+  info[0] = P->s.E_actual(tk.E);
+  info[1] = tk.library;
+  info[2] = tk.k;
+  info[3] = (int)tk.lib.size();
+  info[4] = (int)rows.size();
+  info[5] = tk.co ? 1 : 0;
+  info[6] = tk.E;
+  if (lib_rows)
+    std::copy(tk.lib.begin(), tk.lib.end(), lib_rows);
+  if (pred_rows)
+    std::copy(rows.begin(), rows.end(), pred_rows);
+  return 0;
+}
+
+int fedm_plan_fetch(fedm_plan* P, double* pred, int* rc, int* k_used)
+{
+  if (P->rc0)
+    return 0;
+  const int r = fetch_groups(P);
+  if (r != 0)
+    return r;
+  const int T = P->o.num_thetas;
+  size_t o = 0, ko = 0;
+  for (const auto& tk : P->tasks) {
+    const auto& g = P->groups[tk.group];
+    const int64_t ob = g.out_begin[tk.slot];
+    const size_t Np = P->pred_rows[tk.mask].size();
+    std::copy(g.pred.begin() + ob, g.pred.begin() + ob + (int64_t)(T * Np), pred + o);
+    std::copy(g.rc.begin() + ob, g.rc.begin() + ob + (int64_t)(T * Np), rc + o);
+    std::copy(g.ku.begin() + ob / T, g.ku.begin() + ob / T + (int64_t)Np, k_used + ko);
+    o += (size_t)T * Np;
+    ko += Np;
+  }
+  return 0;
 }
 
 int fedm_plan_device_batches(const fedm_plan* P)
@@ -1064,8 +1150,10 @@ int fedm_run_command(edm_ctx* ctx, const fedm_command* c, fedm_result** out)
   int r = fedm_plan_create(ctx, c, &P);
   if (r == 0)
     r = fedm_plan_run(P);
-  if (r == 0)
-    r = fedm_plan_result(P, out);
+  if (r == 0 || r == FEDM_BREAK_HIT) {
+    const int q = fedm_plan_result(P, out);
+    r = q != 0 ? q : r;
+  }
   fedm_plan_destroy(P);
   return r;
 }

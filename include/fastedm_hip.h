@@ -46,7 +46,8 @@ enum
 enum
 {
   FEDM_PLAIN = 0, /* FP64 throughout: predictions to 1e-10 of the CPU path, neighbour indices bit-exact */
-  FEDM_MIXED = 1, /* FP32 distance screening + FP64 re-rank; FP32 S-map Gram, FP64 solve */
+  FEDM_MIXED = 1, /* FP32/matrix-core distance screening with an exact FP64 re-rank (neighbours and
+                     predictions identical to plain); the S-map Gram and solve are FP64 in both forms */
   FEDM_PLAIN_TASKWISE = 2 /* command level only: plain, one edm_predict per task (the drop-in seam) */
 };
 
@@ -136,6 +137,11 @@ int edm_batch_create(edm_ctx* ctx, const edm_manifold_view* L, const edm_manifol
 int edm_batch_run(edm_batch* b);
 int edm_batch_fetch(edm_batch* b, double* pred, int* rc, int* k_used);
 void edm_batch_destroy(edm_batch* b);
+/* keep_going for the next edm_batch_run (may be NULL): polled before each task on the task-by-task
+ * route and before each stage launch on the device route, as edm_task polls it between targets
+ * (src/edm.cpp:377, 419).  A stop returns FEDM_BREAK_HIT; the outputs then hold missing predictions,
+ * rc BREAK_HIT and missing k_used until the next complete run. */
+void edm_batch_set_keep_going(edm_batch* b, int (*keep_going)(void));
 /* 1 when the batch runs as one persistent device launch (simplex, Euclidean, no missing/panel
  * options, E_actual <= 32, 1 <= k <= 32), 0 when it runs task by task through edm_predict. */
 int edm_batch_on_device(const edm_batch* b);
@@ -191,6 +197,9 @@ typedef struct
   int distance;        /* the R front end always passes FEDM_DIST_EUCLIDEAN (src/r.cpp:156) */
   double aspect_ratio;
   int precision;       /* FEDM_PLAIN / FEDM_MIXED */
+  int (*keep_going)(void); /* or NULL: polled between groups, tasks and stage launches, as launch_tasks and
+                              edm_task poll theirs (src/edm.cpp:134, 377, 419); a stop makes the run
+                              return FEDM_BREAK_HIT and the result hold rc BREAK_HIT and no statistics */
 } fedm_command;
 
 typedef struct fedm_result fedm_result;
@@ -206,6 +215,14 @@ int fedm_plan_run(fedm_plan* p);
 int fedm_plan_result(fedm_plan* p, fedm_result** out);
 int64_t fedm_plan_num_predictions(const fedm_plan* p);
 int fedm_plan_device_batches(const fedm_plan* p); /* groups that run as one persistent launch */
+/* The plan's tasks in the reference's task order (src/edm.cpp:128-213), for parity checks (no reference
+ * counterpart).  fedm_plan_task: info[7] = {E_actual, library, k, library rows, prediction rows,
+ * copredict, E}; lib_rows / pred_rows (or NULL) receive the table rows the task uses.
+ * fedm_plan_fetch copies every task's outputs of the last run back in task order: pred and rc
+ * [num_thetas x prediction rows] per task, k_used [prediction rows] per task, concatenated. */
+int fedm_plan_num_tasks(const fedm_plan* p);
+int fedm_plan_task(const fedm_plan* p, int task, int* info, int* lib_rows, int* pred_rows);
+int fedm_plan_fetch(fedm_plan* p, double* pred, int* rc, int* k_used);
 void fedm_plan_destroy(fedm_plan* p);
 
 /* fedm_plan_create + fedm_plan_run + fedm_plan_result. */

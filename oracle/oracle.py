@@ -56,6 +56,13 @@ def build():
     subprocess.run(["make", "-s", "-C", HERE], check=True)
 
 
+def use_library(path):
+    """Binds a different build of the same sources (the CPU baseline's -O3 -march=native one)."""
+    global _lib, LIB_PATH
+    LIB_PATH = path
+    _lib = None
+
+
 def lib():
     global _lib
     if _lib is None:

@@ -28,6 +28,11 @@ def coupled_maps(T):
     return x, y
 
 
+def ccm_libraries():
+    """40 log-spaced integer library sizes from 10 to 19000 (they sum to 107 910)."""
+    return np.round(np.exp(np.linspace(np.log(10), np.log(19000), 40))).astype(int)
+
+
 def cfg4_panels(num_panels=512, seed=4):
     rng = np.random.default_rng(seed)
     ts, xs, ps = [], [], []

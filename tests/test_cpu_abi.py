@@ -31,3 +31,27 @@ def test_package_exports_match_header():
     fe = load_product()
     assert sorted(fe.EXPORTS) == sorted(n for n in declared_functions() if n in fe.EXPORTS)
     assert set(fe.EXPORTS) <= set(declared_functions())
+
+
+def _c_layout(tmp_path, struct, names):
+    import subprocess
+    src = tmp_path / f"layout_{struct}.c"
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "fastedm_hip.h"', "int main(void) {",
+             f'  printf("%zu\\n", sizeof({struct}));']
+    lines += [f'  printf("%zu\\n", offsetof({struct}, {n}));' for n in names]
+    lines += ["  return 0;", "}"]
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / f"layout_{struct}"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    return [int(v) for v in subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()]
+
+
+def test_ctypes_mirrors_match_header(tmp_path):
+    """Field offsets and sizes of the package's ctypes structures equal the C compiler's."""
+    fe = load_product()
+    for cls, struct in [(fe.Command, "fedm_command"), (fe.Options, "edm_options"),
+                        (fe.ManifoldView, "edm_manifold_view")]:
+        names = [f[0] for f in cls._fields_]
+        out = _c_layout(tmp_path, struct, names)
+        assert out[0] == ctypes.sizeof(cls), struct
+        assert out[1:] == [getattr(cls, n).offset for n in names], struct
