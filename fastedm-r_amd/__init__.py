@@ -59,7 +59,9 @@ EXPORTS = ["edm_ctx_create", "edm_ctx_destroy", "edm_predict", "edm_predict_batc
            "edm_ctx_exact_redos",
            "edm_batch_create", "edm_batch_run", "edm_batch_fetch", "edm_batch_destroy", "fedm_plan_create",
            "fedm_plan_run", "fedm_plan_result", "fedm_plan_num_predictions", "fedm_plan_device_batches", "fedm_plan_destroy",
-           "fedm_plan_num_tasks", "fedm_plan_task", "fedm_plan_fetch", "edm_batch_set_keep_going", "edm_batch_on_device", "edm_selftest_mfma", "fedm_rank_threshold", "fedm_run_command", "fedm_run_task_group", "fedm_result_rc", "fedm_result_num_stats", "fedm_result_stats", "fedm_result_k",
+           "fedm_plan_num_tasks", "fedm_plan_task", "fedm_plan_fetch", "edm_batch_set_keep_going",
+           "edm_batch_route", "edm_batch_set_graph", "edm_batch_fetch_coeffs", "fedm_plan_set_graph",
+           "fedm_plan_resident_batches", "edm_batch_on_device", "edm_selftest_mfma", "fedm_rank_threshold", "fedm_run_command", "fedm_run_task_group", "fedm_result_rc", "fedm_result_num_stats", "fedm_result_stats", "fedm_result_k",
            "fedm_result_scalar", "fedm_result_matrix_shape", "fedm_result_matrix", "fedm_result_num_manifolds",
            "fedm_result_manifold_shape", "fedm_result_manifold", "fedm_result_free"]
 
@@ -101,6 +103,8 @@ def load_library():
         L.fedm_plan_num_tasks.argtypes = [C.c_void_p]
         L.fedm_plan_task.argtypes = [C.c_void_p, C.c_int, _ip, _ip, _ip]
         L.fedm_plan_fetch.argtypes = [C.c_void_p, _dp, _ip, _ip]
+        L.fedm_plan_set_graph.argtypes = [C.c_void_p, C.c_int]
+        L.fedm_plan_resident_batches.argtypes = [C.c_void_p]
         for name in ["fedm_result_rc", "fedm_result_num_stats", "fedm_result_stats", "fedm_result_k",
                      "fedm_result_scalar", "fedm_result_matrix_shape", "fedm_result_matrix",
                      "fedm_result_num_manifolds", "fedm_result_manifold_shape", "fedm_result_manifold",
@@ -439,6 +443,16 @@ class Plan:
     @property
     def num_predictions(self):
         return int(load_library().fedm_plan_num_predictions(self._h))
+
+    def set_graph(self, on=True):
+        """Replays each group's launches as one captured hipGraph from the second run on; returns the
+        number of groups that do."""
+        return int(load_library().fedm_plan_set_graph(self._h, int(bool(on))))
+
+    @property
+    def resident_batches(self):
+        """Number of groups on the resident task route (kernels per task, inputs uploaded once)."""
+        return int(load_library().fedm_plan_resident_batches(self._h))
 
     @property
     def device_batches(self):

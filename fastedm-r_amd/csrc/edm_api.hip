@@ -7,7 +7,9 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <array>
 #include <map>
+#include <type_traits>
 #include <cstring>
 #include <vector>
 
@@ -248,13 +250,12 @@ static void launch_finish_lists(int kcap, dim3 g, hipStream_t s, const FinishPar
   }
 }
 
-// S-map systems of targets [t0, t0 + nt) at every theta: a persistent grid of one-wavefront blocks.
-// FEDM_SMAP_LDS_ROWS=1 keeps U and V in LDS for every n (the kernel used above 24 unknowns).
-static int launch_smap(edm_ctx* c, int n, hipStream_t s, const SmapParams& S, const double* D, int t0, int nt)
+// FEDM_SMAP_LDS_ROWS=1 keeps U and V of the S-map solve in LDS for every width (the kernel used above
+// 24 unknowns); the default keeps rows in registers where they fit.  Both give the same bits.
+static bool smap_lds_rows()
 {
-  (void)c;
   const char* env = getenv("FEDM_SMAP_LDS_ROWS");
-  return smap_launch(n, s, S, D, t0, nt, env && env[0] == '1');
+  return env && env[0] == '1';
 }
 
 extern "C" {
@@ -326,62 +327,118 @@ static void host_fill_outputs(const edm_manifold_view* Mp, const edm_options* o,
       nn_idx[i] = -1;
 }
 
-int edm_predict(edm_ctx* c, const edm_manifold_view* M, const edm_manifold_view* Mp, const edm_options* o,
-                double* pred, int* rc, double* coeffs, int* k_used, int* nn_idx, int nn_cap, int (*keep_going)(void))
+} // extern "C"
+
+// ------------------------------------------------------------------------------------------------
+// One task on the device in four steps, shared by edm_predict, the batches' task route and
+// edm_predict_noalloc:
+//   plan    route, grid shapes and buffer sizes, from the host views (and the FP32 operands of the
+//           matrix-core route, which need the host rows);
+//   bind    a device pointer per buffer, from the context pool, a batch's arena or the static scratch;
+//   upload  the host inputs into their buffers;
+//   launch  the output fills and the kernels, on one stream.
+
+namespace fedm {
+
+enum TaskRoute
 {
-  const int NL = M->num_points, NP = Mp->num_points, Ea = M->E_actual, T = o->num_thetas, n = Ea + 1;
-  if (!c || Ea != Mp->E_actual || T <= 0 || Ea <= 0)
-    return -22;
-  if (NP == 0)
-    return 0;
-  if (keep_going && !keep_going()) {
-    host_fill_outputs(Mp, o, pred, rc, coeffs, k_used, nn_idx, nn_cap, Ea, FEDM_UNKNOWN_ERROR);
-    for (int i = 0; i < NP; i++)
-      rc[i] = FEDM_BREAK_HIT;
-    return FEDM_BREAK_HIT;
+  R_MFMA,
+  R_LIST,
+  R_DENSE
+};
+
+// inputs come from the host views, outputs receive the results, scratch lives for one launch
+enum BufRole
+{
+  ROLE_IN,
+  ROLE_SCRATCH,
+  ROLE_OUT
+};
+
+static BufRole buf_role(int slot)
+{
+  switch (slot) {
+    case B_PS:
+    case B_PI:
+    case B_PCNT:
+    case B_NNI:
+    case B_NNS:
+    case B_NNK:
+    case B_NNCNT:
+    case B_D:
+    case B_WKEY:
+    case B_WIDX:
+      return ROLE_SCRATCH;
+    case B_PRED:
+    case B_RC:
+    case B_KUSED:
+    case B_COEFFS:
+    case B_NNOUT:
+      return ROLE_OUT;
+    default:
+      return ROLE_IN;
   }
-  const int emax = emax_for(Ea);
+}
+
+struct TaskPlan
+{
+  int NL = 0, NP = 0, Ea = 0, T = 0, n = 0, k = 0, emax = 0, nmet = 0, nn_cap = 0;
+  int route = R_LIST;
+  bool wass = false, smap = false, generic = false, wide = false;
+  int kcap = 0, kp = 0, S = 1, sl = 0, bx = 0, nt = 0, gstride = 0;
+  size_t bytes[B_NUM] = {};
+  void* ptr[B_NUM] = {};
+  // host copies the upload needs beyond the views
+  std::vector<int> metrics;
+  std::vector<float> m32, mn, p32, pn;
+  std::vector<double> ynorm;
+  LpParams P;
+  WassParams W;
+  FinishParams F;
+  SmapParams Sm;
+  MfmaParams MP;
+};
+
+// Chooses the route and sizes every buffer.  Returns 0, -22 (invalid arguments) or -95 (a width or
+// option this build does not take; the caller fills the outputs).  `mfma_ok` allows the matrix-core
+// route (it needs the host rows); `max_slices` caps the library slices of the list routes.
+static int plan_task(const edm_manifold_view* M, const edm_manifold_view* Mp, const edm_options* o, bool want_coeffs,
+                     int nn_cap, bool mfma_ok, int max_slices, TaskPlan& tp)
+{
+  const int NL = M->num_points, NP = Mp->num_points, Ea = M->E_actual, T = o->num_thetas;
+  if (Ea != Mp->E_actual || T <= 0 || Ea <= 0 || NL < 0 || NP < 0)
+    return -22;
+  tp.NL = NL;
+  tp.NP = NP;
+  tp.Ea = Ea;
+  tp.T = T;
+  tp.n = Ea + 1;
+  tp.k = o->k;
+  tp.nn_cap = nn_cap;
+  tp.emax = emax_for(Ea);
   const int k = o->k;
-  const bool wass = o->distance == FEDM_DIST_WASSERSTEIN;
-  const bool list_route = !wass && k > 0 && k <= 64;
+  tp.wass = o->distance == FEDM_DIST_WASSERSTEIN;
+  tp.smap = o->algorithm == FEDM_SMAP;
   // Wasserstein point layout [x lags | dt lags | lagged extras | unlagged extras] (src/distances.cpp:318-326)
   const int wE = M->E_x;
   const int wrows = wE > 0 ? 1 + (M->E_dt > 0) + M->E_lagged_extras / wE : 0;
   const int wunl = M->E_extras - M->E_lagged_extras;
   bool wass_ok = wE >= 1 && wE <= 32 && (M->E_dt == 0 || M->E_dt == wE) && M->E_lagged_extras % wE == 0 &&
                  wunl >= 0 && wrows * wE + wunl == Ea && o->missing_distance >= 0.0;
-  if (wass && wass_ok && o->panel_mode && o->idw <= 0 && o->id_weights)
+  if (tp.wass && wass_ok && o->panel_mode && o->idw <= 0 && o->id_weights)
     for (int a = 0; a < o->num_panels * o->num_panels; a++)
       wass_ok = wass_ok && o->id_weights[a] >= 0.0f;
-  if (emax < 0 || (wass && !wass_ok)) {
-    host_fill_outputs(Mp, o, pred, rc, coeffs, k_used, nn_idx, nn_cap, Ea, FEDM_UNKNOWN_ERROR);
-    return -95; // not supported on this route
-  }
-  hipSetDevice(c->device);
-  hipStream_t s = c->stream;
-  int err = 0;
+  if (tp.emax < 0 || (tp.wass && !wass_ok))
+    return -95;
 
   // metrics: one per coordinate for L^p, one per variable for Wasserstein (src/distances.cpp:29-67)
-  const int nmet = wass ? wrows + wunl : Ea;
-  std::vector<int> metrics(nmet, 0);
+  tp.nmet = tp.wass ? wrows + wunl : Ea;
+  tp.metrics.assign(tp.nmet, 0);
   if (o->metrics)
-    std::copy(o->metrics, o->metrics + nmet, metrics.begin());
+    std::copy(o->metrics, o->metrics + tp.nmet, tp.metrics.begin());
 
-  LpParams P;
+  LpParams& P = tp.P;
   std::memset(&P, 0, sizeof(P));
-  P.M = up(c, B_M, M->data, (size_t)NL * Ea, &err);
-  P.Mp = up(c, B_MP, Mp->data, (size_t)NP * Ea, &err);
-  const double* dtargets = up(c, B_MT, M->targets, (size_t)NL, &err);
-  P.Mtt = up(c, B_MTT, M->target_times, (size_t)NL, &err);
-  P.Mppt = up(c, B_MPPT, Mp->point_times, (size_t)NP, &err);
-  if (o->panel_mode) {
-    P.Mpanel = up(c, B_MPAN, M->panel, (size_t)NL, &err);
-    P.Mppanel = up(c, B_MPPAN, Mp->panel, (size_t)NP, &err);
-  }
-  P.metrics = up(c, B_METRICS, metrics.data(), (size_t)nmet, &err);
-  if (o->id_weights && o->num_panels > 0)
-    P.id_weights = up(c, B_IDW, o->id_weights, (size_t)o->num_panels * o->num_panels, &err);
-  const double* dthetas = up(c, B_THETAS, o->thetas, (size_t)T, &err);
   P.NL = NL;
   P.NP = NP;
   P.Ea = Ea;
@@ -402,14 +459,181 @@ int edm_predict(edm_ctx* c, const edm_manifold_view* M, const edm_manifold_view*
     f |= F_MAE;
   if (o->panel_mode && o->idw > 0)
     f |= F_IDW;
-  else if (o->panel_mode && P.id_weights)
+  else if (o->panel_mode && o->id_weights && o->num_panels > 0)
     f |= F_IDWEIGHTS;
   P.flags = f;
-  const bool generic = f != 0;
+  tp.generic = f != 0;
 
-  WassParams W;
+  size_t* by = tp.bytes;
+  by[B_M] = sizeof(double) * (size_t)NL * Ea;
+  by[B_MP] = sizeof(double) * (size_t)NP * Ea;
+  by[B_MT] = M->targets ? sizeof(double) * (size_t)NL : 0;
+  by[B_MTT] = M->target_times ? sizeof(double) * (size_t)NL : 0;
+  by[B_MPPT] = Mp->point_times ? sizeof(double) * (size_t)NP : 0;
+  if (o->panel_mode) {
+    by[B_MPAN] = M->panel ? sizeof(int) * (size_t)NL : 0;
+    by[B_MPPAN] = Mp->panel ? sizeof(int) * (size_t)NP : 0;
+  }
+  by[B_METRICS] = sizeof(int) * (size_t)tp.nmet;
+  by[B_IDW] = (o->id_weights && o->num_panels > 0) ? sizeof(float) * (size_t)o->num_panels * o->num_panels : 0;
+  by[B_THETAS] = sizeof(double) * (size_t)T;
+  by[B_PRED] = sizeof(double) * (size_t)T * NP;
+  by[B_RC] = sizeof(int) * (size_t)T * NP;
+  by[B_KUSED] = sizeof(int) * (size_t)NP;
+  by[B_COEFFS] = want_coeffs ? sizeof(double) * (size_t)NP * tp.n : 0;
+  by[B_NNOUT] = nn_cap > 0 ? sizeof(int) * (size_t)NP * nn_cap : 0;
+  by[B_NNK] = sizeof(int) * (size_t)NP;
+  by[B_NNCNT] = sizeof(int) * (size_t)NP;
+
+  const bool list_route = !tp.wass && k > 0 && k <= 64;
+  double amax = 0.0;
+  const bool want_mfma = mfma_ok && list_route && o->precision == FEDM_MIXED && !tp.generic && Ea >= 8;
+  if (want_mfma) {
+    for (size_t i = 0; i < (size_t)NL * Ea; i++)
+      amax = std::max(amax, std::fabs(M->data[i]));
+    for (size_t i = 0; i < (size_t)NP * Ea; i++)
+      amax = std::max(amax, std::fabs(Mp->data[i]));
+  }
+  if (want_mfma && amax < 1e30) {
+    tp.route = R_MFMA;
+    tp.kcap = kcap_for(k);
+    tp.kp = Ea <= 16 ? 16 : Ea <= 32 ? 32 : 64;
+    tp.bx = (NP + 63) / 64;
+    int S = std::max(1, std::min(MAX_SLICES, (2048 + tp.bx - 1) / tp.bx));
+    S = std::max(1, std::min(S, NL / 256));
+    tp.S = std::max(1, std::min(S, max_slices));
+    tp.sl = (NL + tp.S - 1) / tp.S;
+    const int kp = tp.kp;
+    // centred FP32 operands, their FP32 norms and the per-target screening bound
+    std::vector<double> ctr(Ea, 0.0);
+    for (int j = 0; j < NL; j++)
+      for (int d = 0; d < Ea; d++)
+        ctr[d] += M->data[(size_t)j * Ea + d];
+    for (int d = 0; d < Ea; d++)
+      ctr[d] /= std::max(NL, 1);
+    tp.m32.assign((size_t)NL * kp, 0.f);
+    tp.mn.assign(NL, 0.f);
+    tp.p32.assign((size_t)NP * kp, 0.f);
+    tp.pn.assign(NP, 0.f);
+    tp.ynorm.assign(NP, 0.0);
+    double R = 0.0;
+    for (int j = 0; j < NL; j++) {
+      double r2 = 0.0;
+      float n2 = 0.f;
+      for (int d = 0; d < Ea; d++) {
+        const double v = M->data[(size_t)j * Ea + d] - ctr[d];
+        const float fv = (float)v;
+        tp.m32[(size_t)j * kp + d] = fv;
+        n2 += fv * fv;
+        r2 += v * v;
+      }
+      tp.mn[j] = n2;
+      R = std::max(R, std::sqrt(r2));
+    }
+    for (int i = 0; i < NP; i++) {
+      double r2 = 0.0;
+      float n2 = 0.f;
+      for (int d = 0; d < Ea; d++) {
+        const double v = Mp->data[(size_t)i * Ea + d] - ctr[d];
+        const float fv = (float)v;
+        tp.p32[(size_t)i * kp + d] = fv;
+        n2 += fv * fv;
+        r2 += v * v;
+      }
+      tp.pn[i] = n2;
+      tp.ynorm[i] = std::sqrt(r2);
+    }
+    const double u = 5.9604644775390625e-08;
+    MfmaParams& MP = tp.MP;
+    std::memset(&MP, 0, sizeof(MP));
+    // the screening bound per target (kernel): 2 (k1 rr sqrt(D) + c2 D + k3 rr^2), rr >= |x| + |y|
+    MP.R = R;
+    MP.k1 = 2.02 * u;
+    MP.k3 = (kp + 3) * 1.02 * u + u * u;
+    MP.c2 = 4.0 * u;
+    MP.NL = NL;
+    MP.NP = NP;
+    MP.Ea = Ea;
+    by[B_MIX0] = sizeof(float) * tp.m32.size();
+    by[B_MIX1] = sizeof(float) * tp.mn.size();
+    by[B_MIX2] = sizeof(float) * tp.p32.size();
+    by[B_MIX3] = sizeof(float) * tp.pn.size();
+    by[B_MIX4] = sizeof(double) * tp.ynorm.size();
+    by[B_PS] = sizeof(double) * (size_t)tp.S * NP * tp.kcap;
+    by[B_PI] = sizeof(int) * (size_t)tp.S * NP * tp.kcap;
+    by[B_PCNT] = sizeof(int) * (size_t)tp.S * NP;
+    by[B_NNI] = sizeof(int) * (size_t)NP * tp.kcap;
+    by[B_NNS] = sizeof(double) * (size_t)NP * tp.kcap;
+  } else if (list_route) {
+    tp.route = R_LIST;
+    tp.kcap = kcap_for(k);
+    tp.bx = (NP + 255) / 256;
+    int S = std::max(1, std::min(MAX_SLICES, (1024 + tp.bx - 1) / tp.bx));
+    S = std::max(1, std::min(S, NL / 256));
+    tp.S = std::max(1, std::min(S, max_slices));
+    tp.sl = (NL + tp.S - 1) / tp.S;
+    by[B_PS] = sizeof(double) * (size_t)tp.S * NP * tp.kcap;
+    by[B_PI] = sizeof(int) * (size_t)tp.S * NP * tp.kcap;
+    by[B_PCNT] = sizeof(int) * (size_t)tp.S * NP;
+    by[B_NNI] = sizeof(int) * (size_t)NP * tp.kcap;
+    by[B_NNS] = sizeof(double) * (size_t)NP * tp.kcap;
+  } else {
+    // dense distance rows: every valid neighbour (k < 0), or an exact selection wider than the
+    // register lists (k > 64) made from the rows by select_wide
+    tp.route = R_DENSE;
+    tp.wide = tp.wass ? k > 0 : k > 64;
+    const int n2 =
+      tp.wide ? (int)std::min<long long>(1ll << 30, 1ll << (64 - __builtin_clzll((unsigned long long)k - 1))) : 0;
+    tp.gstride = tp.wide && n2 > 4096 ? n2 : 0;
+    const size_t per_target = sizeof(double) * (size_t)std::max(NL, 1) +
+                              (tp.wide ? (sizeof(int) + sizeof(double)) * (size_t)k : 0) +
+                              (sizeof(unsigned long long) + sizeof(int)) * (size_t)tp.gstride;
+    const size_t budget = (size_t)1 << 30;
+    tp.nt = (int)std::max<size_t>(1, std::min<size_t>((size_t)std::max(NP, 1), budget / per_target));
+    const int bxmax = tp.wass ? tp.nt : (tp.nt + 255) / 256;
+    int S = std::max(1, std::min(MAX_SLICES, (1024 + bxmax - 1) / bxmax));
+    S = std::max(1, std::min(S, NL / (tp.wass ? 64 : 256)));
+    tp.S = std::max(1, std::min(S, max_slices));
+    tp.sl = (NL + tp.S - 1) / tp.S;
+    by[B_D] = sizeof(double) * (size_t)tp.nt * std::max(NL, 1);
+    by[B_PCNT] = sizeof(int) * (size_t)tp.S * NP;
+    by[B_NNI] = tp.wide ? sizeof(int) * (size_t)tp.nt * k : 0;
+    by[B_NNS] = tp.wide ? sizeof(double) * (size_t)tp.nt * k : 0;
+    by[B_WKEY] = tp.gstride ? sizeof(unsigned long long) * (size_t)tp.nt * tp.gstride : 0;
+    by[B_WIDX] = tp.gstride ? sizeof(int) * (size_t)tp.nt * tp.gstride : 0;
+  }
+  return 0;
+}
+
+// Gives every buffer of the plan a device pointer: get(slot, role, bytes) returns one (nullptr on
+// failure); then the kernels' parameter blocks are filled in.
+template <class Get>
+static int bind_task(TaskPlan& tp, const edm_options* o, Get&& get)
+{
+  for (int s = 0; s < B_NUM; s++) {
+    tp.ptr[s] = nullptr;
+    if (tp.bytes[s] == 0)
+      continue;
+    tp.ptr[s] = get(s, buf_role(s), tp.bytes[s]);
+    if (!tp.ptr[s])
+      return -2;
+  }
+  const int NL = tp.NL, NP = tp.NP, Ea = tp.Ea;
+  LpParams& P = tp.P;
+  P.M = (const double*)tp.ptr[B_M];
+  P.Mp = (const double*)tp.ptr[B_MP];
+  P.Mtt = (const double*)tp.ptr[B_MTT];
+  P.Mppt = (const double*)tp.ptr[B_MPPT];
+  P.Mpanel = (const int*)tp.ptr[B_MPAN];
+  P.Mppanel = (const int*)tp.ptr[B_MPPAN];
+  P.metrics = (const int*)tp.ptr[B_METRICS];
+  P.id_weights = (const float*)tp.ptr[B_IDW];
+  const double* dthetas = (const double*)tp.ptr[B_THETAS];
+  const double* dtargets = (const double*)tp.ptr[B_MT];
+
+  WassParams& W = tp.W;
   std::memset(&W, 0, sizeof(W));
-  if (wass) {
+  if (tp.wass) {
     W.M = P.M;
     W.Mp = P.Mp;
     W.Mpanel = P.Mpanel;
@@ -419,10 +643,6 @@ int edm_predict(edm_ctx* c, const edm_manifold_view* M, const edm_manifold_view*
     W.NL = NL;
     W.NP = NP;
     W.Ea = Ea;
-    W.E = wE;
-    W.E_dt = M->E_dt;
-    W.rows = wrows;
-    W.n_unlagged = wunl;
     W.num_panels = o->num_panels;
     W.missing_distance = o->missing_distance;
     W.idw = o->idw;
@@ -431,246 +651,221 @@ int edm_predict(edm_ctx* c, const edm_manifold_view* M, const edm_manifold_view*
     W.skip_other = o->panel_mode && o->idw < 0;
   }
 
-  double* dpred = (double*)c->get(B_PRED, sizeof(double) * (size_t)T * NP);
-  int* drc = (int*)c->get(B_RC, sizeof(int) * (size_t)T * NP);
-  int* dku = (int*)c->get(B_KUSED, sizeof(int) * (size_t)NP);
-  double* dco = coeffs ? (double*)c->get(B_COEFFS, sizeof(double) * (size_t)NP * n) : nullptr;
-  int* dnno = (nn_idx && nn_cap > 0) ? (int*)c->get(B_NNOUT, sizeof(int) * (size_t)NP * nn_cap) : nullptr;
-  int* nnk = (int*)c->get(B_NNK, sizeof(int) * (size_t)NP);
-  int* nncnt = (int*)c->get(B_NNCNT, sizeof(int) * (size_t)NP);
-  if (err || !dpred || !drc || !dku || !nnk || !nncnt || (coeffs && !dco) || (nn_idx && nn_cap > 0 && !dnno))
-    return err ? err : -2;
-  dfill(c, dpred, (size_t)T * NP, FEDM_MISSING_D);
-  dfill(c, drc, (size_t)T * NP, (int)FEDM_UNKNOWN_ERROR);
-  dfill(c, dku, (size_t)NP, (int)FEDM_MISSING_I);
-  dfill(c, dco, (size_t)NP * n, FEDM_MISSING_D);
-  dfill(c, dnno, (size_t)NP * nn_cap, -1);
-  dfill(c, nnk, (size_t)NP, 0);
-
-  FinishParams F;
+  FinishParams& F = tp.F;
   std::memset(&F, 0, sizeof(F));
   F.NP = NP;
   F.NL = NL;
-  F.k = k;
-  F.T = T;
+  F.k = tp.k;
+  F.T = tp.T;
   F.force_compute = o->force_compute;
   F.algorithm = o->algorithm;
   F.thetas = dthetas;
   F.targets = dtargets;
-  F.pred = dpred;
-  F.rc = drc;
-  F.k_used = dku;
-  F.nn_k = nnk;
-  F.nn_cnt = nncnt;
-  F.nn_out = dnno;
-  F.nn_cap = dnno ? nn_cap : 0;
+  F.pred = (double*)tp.ptr[B_PRED];
+  F.rc = (int*)tp.ptr[B_RC];
+  F.k_used = (int*)tp.ptr[B_KUSED];
+  F.nn_k = (int*)tp.ptr[B_NNK];
+  F.nn_cnt = (int*)tp.ptr[B_NNCNT];
+  F.nn_out = (int*)tp.ptr[B_NNOUT];
+  F.nn_cap = F.nn_out ? tp.nn_cap : 0;
+  F.nn_i = (int*)tp.ptr[B_NNI];
+  F.nn_s = (double*)tp.ptr[B_NNS];
+  F.nn_stride = tp.route == R_DENSE ? (tp.wide ? tp.k : 0) : tp.kcap;
 
-  SmapParams Sm;
+  SmapParams& Sm = tp.Sm;
   std::memset(&Sm, 0, sizeof(Sm));
   Sm.NP = NP;
   Sm.NL = NL;
   Sm.Ea = Ea;
-  Sm.T = T;
+  Sm.T = tp.T;
   Sm.M = P.M;
   Sm.Mp = P.Mp;
   Sm.targets = dtargets;
   Sm.thetas = dthetas;
-  Sm.nn_k = nnk;
-  Sm.nn_cnt = nncnt;
-  Sm.pred = dpred;
-  Sm.rc = drc;
-  Sm.coeffs = (o->save_smap_coeffs && dco) ? dco : nullptr;
+  Sm.nn_k = F.nn_k;
+  Sm.nn_cnt = F.nn_cnt;
+  Sm.pred = F.pred;
+  Sm.rc = F.rc;
+  Sm.coeffs = o->save_smap_coeffs ? (double*)tp.ptr[B_COEFFS] : nullptr;
+  Sm.nn_i = F.nn_i;
+  Sm.nn_s = F.nn_s;
+  Sm.nn_stride = F.nn_stride;
 
-  const bool smap = o->algorithm == FEDM_SMAP;
-  HIP_TRY(hipEventRecord(c->ev[0], s));
-
-  // matrix-core screening pays once a point has a few dozen coordinates
-  double amax = 0.0;
-  const bool want_mfma = list_route && o->precision == FEDM_MIXED && !generic && Ea >= 8;
-  if (want_mfma) {
-    for (size_t i = 0; i < (size_t)NL * Ea; i++)
-      amax = std::max(amax, std::fabs(M->data[i]));
-    for (size_t i = 0; i < (size_t)NP * Ea; i++)
-      amax = std::max(amax, std::fabs(Mp->data[i]));
-  }
-  const bool mfma = want_mfma && amax < 1e30;
-  if (list_route && mfma) {
-    const int kcap = kcap_for(k);
-    const int kp = Ea <= 16 ? 16 : Ea <= 32 ? 32 : 64;
-    const int bx = (NP + 63) / 64;
-    int S = std::max(1, std::min(MAX_SLICES, (2048 + bx - 1) / bx));
-    S = std::max(1, std::min(S, NL / 256));
-    const int sl = (NL + S - 1) / S;
-    // centred FP32 operands, their FP32 norms and the per-target screening bound
-    std::vector<double> ctr(Ea, 0.0);
-    for (int j = 0; j < NL; j++)
-      for (int d = 0; d < Ea; d++)
-        ctr[d] += M->data[(size_t)j * Ea + d];
-    for (int d = 0; d < Ea; d++)
-      ctr[d] /= std::max(NL, 1);
-    std::vector<float> m32((size_t)NL * kp, 0.f), mn(NL), p32((size_t)NP * kp, 0.f), pn(NP);
-    std::vector<double> ynorm(NP);
-    double R = 0.0;
-    for (int j = 0; j < NL; j++) {
-      double r2 = 0.0;
-      float n2 = 0.f;
-      for (int d = 0; d < Ea; d++) {
-        const double v = M->data[(size_t)j * Ea + d] - ctr[d];
-        const float f = (float)v;
-        m32[(size_t)j * kp + d] = f;
-        n2 += f * f;
-        r2 += v * v;
-      }
-      mn[j] = n2;
-      R = std::max(R, std::sqrt(r2));
-    }
-    const double u = 5.9604644775390625e-08;
-    for (int i = 0; i < NP; i++) {
-      double r2 = 0.0;
-      float n2 = 0.f;
-      for (int d = 0; d < Ea; d++) {
-        const double v = Mp->data[(size_t)i * Ea + d] - ctr[d];
-        const float f = (float)v;
-        p32[(size_t)i * kp + d] = f;
-        n2 += f * f;
-        r2 += v * v;
-      }
-      pn[i] = n2;
-      ynorm[i] = std::sqrt(r2);
-    }
-    MfmaParams MP;
-    MP.M32 = up(c, B_MIX0, m32.data(), m32.size(), &err);
-    MP.Mn = up(c, B_MIX1, mn.data(), mn.size(), &err);
-    MP.P32 = up(c, B_MIX2, p32.data(), p32.size(), &err);
-    MP.Pn = up(c, B_MIX3, pn.data(), pn.size(), &err);
-    // the screening bound per target (kernel): 2 (k1 rr sqrt(D) + c2 D + k3 rr^2), rr >= |x| + |y|
-    MP.ynorm = up(c, B_MIX4, ynorm.data(), ynorm.size(), &err);
-    MP.R = R;
-    MP.k1 = 2.02 * u;
-    MP.k3 = (kp + 3) * 1.02 * u + u * u;
-    MP.c2 = 4.0 * u;
+  if (tp.route == R_MFMA) {
+    MfmaParams& MP = tp.MP;
+    MP.M32 = (const float*)tp.ptr[B_MIX0];
+    MP.Mn = (const float*)tp.ptr[B_MIX1];
+    MP.P32 = (const float*)tp.ptr[B_MIX2];
+    MP.Pn = (const float*)tp.ptr[B_MIX3];
+    MP.ynorm = (const double*)tp.ptr[B_MIX4];
     MP.M = P.M;
     MP.Mp = P.Mp;
-    MP.NL = NL;
-    MP.NP = NP;
-    MP.Ea = Ea;
-    double* ps = (double*)c->get(B_PS, sizeof(double) * (size_t)S * NP * kcap);
-    int* pi = (int*)c->get(B_PI, sizeof(int) * (size_t)S * NP * kcap);
-    int* pc = (int*)c->get(B_PCNT, sizeof(int) * (size_t)S * NP);
-    F.nn_i = (int*)c->get(B_NNI, sizeof(int) * (size_t)NP * kcap);
-    F.nn_s = (double*)c->get(B_NNS, sizeof(double) * (size_t)NP * kcap);
-    F.nn_stride = kcap;
-    if (err || !ps || !pi || !pc || !F.nn_i || !F.nn_s)
-      return err ? err : -2;
-    HIP_TRY(hipEventRecord(c->ev[0], s));
-    launch_mfma(kcap, kp, dim3(bx, S), s, MP, k, sl, ps, pi, pc);
-    HIP_TRY(hipGetLastError());
-    HIP_TRY(hipEventRecord(c->ev[1], s));
-    launch_finish_lists(kcap, dim3((NP + 255) / 256), s, F, ps, pi, pc, S);
-    HIP_TRY(hipGetLastError());
-    HIP_TRY(hipEventRecord(c->ev[2], s));
-    if (smap) {
-      Sm.nn_i = F.nn_i;
-      Sm.nn_s = F.nn_s;
-      Sm.nn_stride = kcap;
-      int r = launch_smap(c, n, s, Sm, nullptr, 0, NP);
-      if (r)
-        return r;
-      HIP_TRY(hipGetLastError());
-    }
-    HIP_TRY(hipEventRecord(c->ev[3], s));
-  } else if (list_route) {
-    const int kcap = kcap_for(k);
-    const int bx = (NP + 255) / 256;
-    int S = std::max(1, std::min(MAX_SLICES, (1024 + bx - 1) / bx));
-    S = std::max(1, std::min(S, NL / 256));
-    const int sl = (NL + S - 1) / S;
-    double* ps = (double*)c->get(B_PS, sizeof(double) * (size_t)S * NP * kcap);
-    int* pi = (int*)c->get(B_PI, sizeof(int) * (size_t)S * NP * kcap);
-    int* pc = (int*)c->get(B_PCNT, sizeof(int) * (size_t)S * NP);
-    F.nn_i = (int*)c->get(B_NNI, sizeof(int) * (size_t)NP * kcap);
-    F.nn_s = (double*)c->get(B_NNS, sizeof(double) * (size_t)NP * kcap);
-    F.nn_stride = kcap;
-    if (!ps || !pi || !pc || !F.nn_i || !F.nn_s)
-      return -2;
-    launch_knn(kcap, emax, generic, dim3(bx, S), s, P, k, sl, ps, pi, pc);
-    HIP_TRY(hipGetLastError());
-    HIP_TRY(hipEventRecord(c->ev[1], s));
-    launch_finish_lists(kcap, dim3(bx), s, F, ps, pi, pc, S);
-    HIP_TRY(hipGetLastError());
-    HIP_TRY(hipEventRecord(c->ev[2], s));
-    if (smap) {
-      Sm.nn_i = F.nn_i;
-      Sm.nn_s = F.nn_s;
-      Sm.nn_stride = kcap;
-      int r = launch_smap(c, n, s, Sm, nullptr, 0, NP);
-      if (r)
-        return r;
-      HIP_TRY(hipGetLastError());
-    }
-    HIP_TRY(hipEventRecord(c->ev[3], s));
-  } else {
-    // dense distance rows: every valid neighbour (k < 0), or an exact selection wider than the
-    // register lists (k > 64) made from the rows by select_wide
-    const bool wide = wass ? k > 0 : k > 64;
-    const int n2 = wide ? (int)std::min<long long>(1ll << 30, 1ll << (64 - __builtin_clzll((unsigned long long)k - 1))) : 0;
-    const int gstride = wide && n2 > 4096 ? n2 : 0;
-    const size_t per_target = sizeof(double) * (size_t)std::max(NL, 1) +
-                              (wide ? (sizeof(int) + sizeof(double)) * (size_t)k : 0) +
-                              (sizeof(unsigned long long) + sizeof(int)) * (size_t)gstride;
-    const size_t budget = (size_t)1 << 30;
-    int nt = (int)std::max<size_t>(1, std::min<size_t>((size_t)NP, budget / per_target));
-    const int bxmax = wass ? nt : (nt + 255) / 256;
-    int S = std::max(1, std::min(MAX_SLICES, (1024 + bxmax - 1) / bxmax));
-    S = std::max(1, std::min(S, NL / (wass ? 64 : 256)));
-    const int sl = (NL + S - 1) / S;
-    double* D = (double*)c->get(B_D, sizeof(double) * (size_t)nt * std::max(NL, 1));
-    int* pc = (int*)c->get(B_PCNT, sizeof(int) * (size_t)S * NP);
-    F.nn_i = wide ? (int*)c->get(B_NNI, sizeof(int) * (size_t)nt * k) : nullptr;
-    F.nn_s = wide ? (double*)c->get(B_NNS, sizeof(double) * (size_t)nt * k) : nullptr;
-    F.nn_stride = wide ? k : 0;
-    unsigned long long* gkey =
-      gstride ? (unsigned long long*)c->get(B_WKEY, sizeof(unsigned long long) * (size_t)nt * gstride) : nullptr;
-    int* gidx = gstride ? (int*)c->get(B_WIDX, sizeof(int) * (size_t)nt * gstride) : nullptr;
-    if (!D || !pc || (wide && (!F.nn_i || !F.nn_s)) || (gstride && (!gkey || !gidx)))
-      return -2;
-    Sm.nn_i = F.nn_i;
-    Sm.nn_s = F.nn_s;
-    Sm.nn_stride = F.nn_stride;
-    HIP_TRY(hipEventRecord(c->ev[1], s));
-    for (int t0 = 0; t0 < NP; t0 += nt) {
-      const int cnt = std::min(nt, NP - t0);
-      if (wass)
-        hipLaunchKernelGGL(wasserstein_dense, dim3(cnt, S), dim3(256), 0, s, W, t0, cnt, sl, D, pc);
-      else
-        launch_dense(emax, generic, dim3((cnt + 255) / 256, S), s, P, t0, cnt, sl, D, pc);
-      HIP_TRY(hipGetLastError());
-      hipLaunchKernelGGL(finish_dense, dim3((cnt + 255) / 256), dim3(256), 0, s, F, D, t0, cnt, pc, S);
-      HIP_TRY(hipGetLastError());
-      if (wide) {
-        hipLaunchKernelGGL(select_wide, dim3(cnt), dim3(256), 0, s, F, D, t0, cnt, gkey, gidx, gstride);
-        HIP_TRY(hipGetLastError());
-      }
-      Sm.nn_t0 = t0;
-      if (smap) {
-        int r = launch_smap(c, n, s, Sm, D, t0, cnt);
-        if (r)
-          return r;
-        HIP_TRY(hipGetLastError());
-      }
-      if (keep_going && !keep_going())
-        break;
-    }
-    HIP_TRY(hipEventRecord(c->ev[2], s));
-    HIP_TRY(hipEventRecord(c->ev[3], s));
   }
-  HIP_TRY(hipMemcpyAsync(pred, dpred, sizeof(double) * (size_t)T * NP, hipMemcpyDeviceToHost, s));
-  HIP_TRY(hipMemcpyAsync(rc, drc, sizeof(int) * (size_t)T * NP, hipMemcpyDeviceToHost, s));
-  HIP_TRY(hipMemcpyAsync(k_used, dku, sizeof(int) * (size_t)NP, hipMemcpyDeviceToHost, s));
+  return 0;
+}
+
+// Copies the host inputs of the task into its bound buffers (slots of role ROLE_IN whose pointer is
+// set; `skip_views` leaves out the manifold arrays, which a device-side caller passes itself).
+static int upload_task(const TaskPlan& tp, const edm_manifold_view* M, const edm_manifold_view* Mp,
+                       const edm_options* o, hipStream_t s, bool skip_views, const bool* skip = nullptr)
+{
+  auto cp = [&](int slot, const void* host) {
+    if (!tp.ptr[slot] || !host || !tp.bytes[slot] || (skip && skip[slot]))
+      return hipSuccess;
+    return hipMemcpyAsync(tp.ptr[slot], host, tp.bytes[slot], hipMemcpyHostToDevice, s);
+  };
+  hipError_t e = hipSuccess;
+  if (!skip_views) {
+    const void* src[] = { M->data, Mp->data, M->targets, M->target_times, Mp->point_times, M->panel, Mp->panel };
+    const int slots[] = { B_M, B_MP, B_MT, B_MTT, B_MPPT, B_MPAN, B_MPPAN };
+    for (int i = 0; i < 7 && e == hipSuccess; i++)
+      e = cp(slots[i], src[i]);
+  }
+  if (e == hipSuccess)
+    e = cp(B_METRICS, tp.metrics.data());
+  if (e == hipSuccess)
+    e = cp(B_IDW, o->id_weights);
+  if (e == hipSuccess)
+    e = cp(B_THETAS, o->thetas);
+  if (tp.route == R_MFMA) {
+    if (e == hipSuccess)
+      e = cp(B_MIX0, tp.m32.data());
+    if (e == hipSuccess)
+      e = cp(B_MIX1, tp.mn.data());
+    if (e == hipSuccess)
+      e = cp(B_MIX2, tp.p32.data());
+    if (e == hipSuccess)
+      e = cp(B_MIX3, tp.pn.data());
+    if (e == hipSuccess)
+      e = cp(B_MIX4, tp.ynorm.data());
+  }
+  return e == hipSuccess ? 0 : -3;
+}
+
+// Enqueues the task's fills and kernels on stream s.  ev (or nullptr) receives four stage events
+// [start, after select, after merge, after S-map].  keep_going is polled between target blocks of the
+// dense route.
+static int launch_task(TaskPlan& tp, hipStream_t s, hipEvent_t* ev, int (*keep_going)(void))
+{
+  const int NP = tp.NP, T = tp.T;
+  FinishParams& F = tp.F;
+  SmapParams& Sm = tp.Sm;
+  auto fill = [s](auto* p, size_t n, auto v) {
+    using V = std::remove_pointer_t<decltype(p)>;
+    if (!p || n == 0)
+      return;
+    int blocks = (int)std::min<size_t>(4096, (n + 255) / 256);
+    hipLaunchKernelGGL(fill_kernel<V>, dim3(blocks), dim3(256), 0, s, p, n, (V)v);
+  };
+  fill(F.pred, (size_t)T * NP, FEDM_MISSING_D);
+  fill(F.rc, (size_t)T * NP, (int)FEDM_UNKNOWN_ERROR);
+  fill(F.k_used, (size_t)NP, (int)FEDM_MISSING_I);
+  fill((double*)tp.ptr[B_COEFFS], (size_t)NP * tp.n, FEDM_MISSING_D);
+  fill(F.nn_out, (size_t)NP * tp.nn_cap, -1);
+  fill(F.nn_k, (size_t)NP, 0);
+  if (ev)
+    HIP_TRY(hipEventRecord(ev[0], s));
+  double* ps = (double*)tp.ptr[B_PS];
+  int* pi = (int*)tp.ptr[B_PI];
+  int* pc = (int*)tp.ptr[B_PCNT];
+  if (tp.route != R_DENSE) {
+    if (tp.route == R_MFMA)
+      launch_mfma(tp.kcap, tp.kp, dim3(tp.bx, tp.S), s, tp.MP, tp.k, tp.sl, ps, pi, pc);
+    else
+      launch_knn(tp.kcap, tp.emax, tp.generic, dim3(tp.bx, tp.S), s, tp.P, tp.k, tp.sl, ps, pi, pc);
+    HIP_TRY(hipGetLastError());
+    if (ev)
+      HIP_TRY(hipEventRecord(ev[1], s));
+    launch_finish_lists(tp.kcap, dim3((NP + 255) / 256), s, F, ps, pi, pc, tp.S);
+    HIP_TRY(hipGetLastError());
+    if (ev)
+      HIP_TRY(hipEventRecord(ev[2], s));
+    if (tp.smap) {
+      const int r = smap_launch(tp.n, s, Sm, nullptr, 0, NP, smap_lds_rows());
+      if (r)
+        return r;
+      HIP_TRY(hipGetLastError());
+    }
+    if (ev)
+      HIP_TRY(hipEventRecord(ev[3], s));
+    return 0;
+  }
+  double* D = (double*)tp.ptr[B_D];
+  unsigned long long* gkey = (unsigned long long*)tp.ptr[B_WKEY];
+  int* gidx = (int*)tp.ptr[B_WIDX];
+  if (ev)
+    HIP_TRY(hipEventRecord(ev[1], s));
+  for (int t0 = 0; t0 < NP; t0 += tp.nt) {
+    const int cnt = std::min(tp.nt, NP - t0);
+    if (tp.wass)
+      hipLaunchKernelGGL(wasserstein_dense, dim3(cnt, tp.S), dim3(256), 0, s, tp.W, t0, cnt, tp.sl, D, pc);
+    else
+      launch_dense(tp.emax, tp.generic, dim3((cnt + 255) / 256, tp.S), s, tp.P, t0, cnt, tp.sl, D, pc);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(finish_dense, dim3((cnt + 255) / 256), dim3(256), 0, s, F, D, t0, cnt, pc, tp.S);
+    HIP_TRY(hipGetLastError());
+    if (tp.wide) {
+      hipLaunchKernelGGL(select_wide, dim3(cnt), dim3(256), 0, s, F, D, t0, cnt, gkey, gidx, tp.gstride);
+      HIP_TRY(hipGetLastError());
+    }
+    Sm.nn_t0 = t0;
+    if (tp.smap) {
+      const int r = smap_launch(tp.n, s, Sm, D, t0, cnt, smap_lds_rows());
+      if (r)
+        return r;
+      HIP_TRY(hipGetLastError());
+    }
+    if (keep_going && !keep_going())
+      break;
+  }
+  if (ev) {
+    HIP_TRY(hipEventRecord(ev[2], s));
+    HIP_TRY(hipEventRecord(ev[3], s));
+  }
+  return 0;
+}
+
+} // namespace fedm
+
+extern "C" {
+
+int edm_predict(edm_ctx* c, const edm_manifold_view* M, const edm_manifold_view* Mp, const edm_options* o,
+                double* pred, int* rc, double* coeffs, int* k_used, int* nn_idx, int nn_cap, int (*keep_going)(void))
+{
+  const int NP = Mp->num_points, Ea = M->E_actual;
+  if (!c || Ea != Mp->E_actual || o->num_thetas <= 0 || Ea <= 0)
+    return -22;
+  if (NP == 0)
+    return 0;
+  if (keep_going && !keep_going()) {
+    host_fill_outputs(Mp, o, pred, rc, coeffs, k_used, nn_idx, nn_cap, Ea, FEDM_UNKNOWN_ERROR);
+    for (int i = 0; i < NP; i++)
+      rc[i] = FEDM_BREAK_HIT;
+    return FEDM_BREAK_HIT;
+  }
+  TaskPlan tp;
+  const int pr = plan_task(M, Mp, o, coeffs != nullptr, nn_idx ? nn_cap : 0, true, MAX_SLICES, tp);
+  if (pr == -95)
+    host_fill_outputs(Mp, o, pred, rc, coeffs, k_used, nn_idx, nn_cap, Ea, FEDM_UNKNOWN_ERROR);
+  if (pr != 0)
+    return pr;
+  hipSetDevice(c->device);
+  hipStream_t s = c->stream;
+  int r = bind_task(tp, o, [c](int slot, BufRole, size_t bytes) { return c->get(slot, bytes); });
+  if (r == 0)
+    r = upload_task(tp, M, Mp, o, s, false);
+  if (r == 0)
+    r = launch_task(tp, s, c->ev, keep_going);
+  if (r != 0)
+    return r;
+  const size_t T = o->num_thetas;
+  HIP_TRY(hipMemcpyAsync(pred, tp.F.pred, sizeof(double) * T * NP, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipMemcpyAsync(rc, tp.F.rc, sizeof(int) * T * NP, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipMemcpyAsync(k_used, tp.F.k_used, sizeof(int) * (size_t)NP, hipMemcpyDeviceToHost, s));
   if (coeffs)
-    HIP_TRY(hipMemcpyAsync(coeffs, dco, sizeof(double) * (size_t)NP * n, hipMemcpyDeviceToHost, s));
-  if (dnno)
-    HIP_TRY(hipMemcpyAsync(nn_idx, dnno, sizeof(int) * (size_t)NP * nn_cap, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(coeffs, tp.ptr[B_COEFFS], tp.bytes[B_COEFFS], hipMemcpyDeviceToHost, s));
+  if (tp.F.nn_out)
+    HIP_TRY(hipMemcpyAsync(nn_idx, tp.F.nn_out, tp.bytes[B_NNOUT], hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
   c->timings[0] = ev_ms(c->ev[0], c->ev[1]);
   c->timings[1] = ev_ms(c->ev[1], c->ev[2]);
@@ -684,7 +879,6 @@ int edm_predict(edm_ctx* c, const edm_manifold_view* M, const edm_manifold_view*
   }
   return 0;
 }
-
 } // extern "C"
 
 // ------------------------------------------------------------------------------------------------
@@ -735,6 +929,16 @@ struct edm_batch
   std::vector<int64_t> ob;
   std::vector<double> hpred;
   std::vector<int> hrc, hku;
+  std::vector<double> hcoeffs;
+  // resident task route: every task planned, gathered and uploaded once at create, its kernels
+  // launched in turn on each run (or replayed as one captured graph)
+  bool resident = false;
+  std::vector<TaskPlan> plans;
+  double* coeffs = nullptr;  // [total_k x (Ea+1)] when S-map coefficients are saved
+  bool use_graph = false;    // capture the steady-state run into a hipGraph and replay it
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  int runs = 0;              // completed runs (the graph captures the second one)
 
   template <typename T>
   T* dev(const T* host, size_t n)
@@ -751,6 +955,10 @@ struct edm_batch
   {
     if (ctx)
       hipStreamSynchronize(ctx->stream);
+    if (graph_exec)
+      hipGraphExecDestroy(graph_exec);
+    if (graph)
+      hipGraphDestroy(graph);
     for (void* p : owned)
       hipFree(p);
   }
@@ -862,6 +1070,236 @@ static const void* rank_scan_kernel(int kcap, int emax)
   }
 }
 
+// A task's library and prediction manifolds, gathered from the batch's host tables by row lists.
+struct GatheredTask
+{
+  std::vector<double> md, mt, mtt, mpd, mppt;
+  std::vector<int> mpan, mppan;
+  edm_manifold_view M, Mp;
+};
+
+static void gather_task(const edm_batch* b, int tk, GatheredTask& g)
+{
+  const int Ea = b->Ea, nl = b->lc[tk], np = b->pc[tk];
+  g.md.resize((size_t)nl * Ea);
+  g.mt.resize(nl);
+  g.mtt.resize(nl);
+  g.mpd.resize((size_t)np * Ea);
+  g.mppt.resize(np);
+  g.mpan.clear();
+  g.mppan.clear();
+  for (int r = 0; r < nl; r++) {
+    const int row = b->lr[b->lb[tk] + r];
+    std::copy(b->Ldata.begin() + (size_t)row * Ea, b->Ldata.begin() + (size_t)(row + 1) * Ea,
+              g.md.begin() + (size_t)r * Ea);
+    g.mt[r] = b->Ltgt[row];
+    g.mtt[r] = b->Ltt.empty() ? 0.0 : b->Ltt[row];
+    if (!b->Lpan.empty())
+      g.mpan.push_back(b->Lpan[row]);
+  }
+  for (int r = 0; r < np; r++) {
+    const int row = b->pr[b->pb[tk] + r];
+    std::copy(b->Pdata.begin() + (size_t)row * Ea, b->Pdata.begin() + (size_t)(row + 1) * Ea,
+              g.mpd.begin() + (size_t)r * Ea);
+    g.mppt[r] = b->Ppt.empty() ? 0.0 : b->Ppt[row];
+    if (!b->Ppan.empty())
+      g.mppan.push_back(b->Ppan[row]);
+  }
+  g.M = b->Lv;
+  g.Mp = b->Pv;
+  g.M.data = g.md.data();
+  g.M.targets = g.mt.data();
+  g.M.target_times = g.mtt.data();
+  g.M.point_times = nullptr;
+  g.M.panel = g.mpan.empty() ? nullptr : g.mpan.data();
+  g.M.num_points = nl;
+  g.Mp.data = g.mpd.data();
+  g.Mp.targets = nullptr;
+  g.Mp.point_times = g.mppt.data();
+  g.Mp.target_times = nullptr;
+  g.Mp.panel = g.mppan.empty() ? nullptr : g.mppan.data();
+  g.Mp.num_points = np;
+}
+
+static edm_options task_options(const edm_batch* b, int tk)
+{
+  edm_options to = b->o;
+  to.k = b->kk[tk];
+  to.thetas = b->thetas.data();
+  to.metrics = b->metrics.empty() ? nullptr : b->metrics.data();
+  return to;
+}
+
+// The resident task route for batches the persistent kernels do not take (S-map, missing values,
+// panels, other distances, wide k): every task is planned, gathered and uploaded once, here; the
+// prediction side of tasks that share a prediction list is uploaded once; the scratch of the
+// kernels is one set of buffers sized for the largest task, since tasks run one after another on
+// the stream.  Results go straight to the batch's output arrays.  Returns 0, 1 when the route does
+// not apply (a task this build cannot run, or inputs beyond the budget: the batch then gathers and
+// uploads task by task on each run, as before), or a negative error.
+static int build_resident(edm_batch* b)
+{
+  edm_ctx* c = b->ctx;
+  const int Ea = b->Ea, T = b->T, C = Ea + 1;
+  const bool coeffs = b->o.save_smap_coeffs && b->o.algorithm == FEDM_SMAP;
+  {
+    size_t in_bytes = 0;
+    std::map<std::pair<int, int>, int> lists;
+    for (int tk = 0; tk < b->tasks; tk++) {
+      in_bytes += (size_t)b->lc[tk] * (sizeof(double) * (Ea + 2) + sizeof(int));
+      if (lists.emplace(std::make_pair(b->pb[tk], b->pc[tk]), tk).second)
+        in_bytes += (size_t)b->pc[tk] * (sizeof(double) * (Ea + 1) + sizeof(int));
+    }
+    if (in_bytes > ((size_t)16 << 30))
+      return 1;
+  }
+  double* pred = b->dev<double>(nullptr, (size_t)b->total_out);
+  int* rc = b->dev<int>(nullptr, (size_t)b->total_out);
+  int* ku = b->dev<int>(nullptr, (size_t)b->total_k);
+  b->coeffs = coeffs ? b->dev<double>(nullptr, (size_t)b->total_k * C) : nullptr;
+  if (!pred || !rc || !ku || (coeffs && !b->coeffs))
+    return -2;
+  // tasks without targets launch nothing: their outputs keep these values
+  dfill(c, pred, (size_t)b->total_out, FEDM_MISSING_D);
+  dfill(c, rc, (size_t)b->total_out, (int)FEDM_UNKNOWN_ERROR);
+  dfill(c, ku, (size_t)b->total_k, (int)FEDM_MISSING_I);
+  if (b->coeffs)
+    dfill(c, b->coeffs, (size_t)b->total_k * C, FEDM_MISSING_D);
+  b->plans.assign(b->tasks, TaskPlan());
+  size_t smax[B_NUM] = {};
+  std::map<std::pair<int, int>, std::array<void*, 3>> shared;
+  const int pslots[3] = { B_MP, B_MPPT, B_MPPAN };
+  GatheredTask g;
+  for (int tk = 0; tk < b->tasks; tk++) {
+    TaskPlan& tp = b->plans[tk];
+    if (b->pc[tk] == 0)
+      continue;
+    gather_task(b, tk, g);
+    const edm_options to = task_options(b, tk);
+    int r = plan_task(&g.M, &g.Mp, &to, coeffs, 0, true, MAX_SLICES, tp);
+    if (r != 0)
+      return r == -95 ? 1 : r;
+    const int64_t ob = b->ob[tk];
+    const auto key = std::make_pair(b->pb[tk], b->pc[tk]);
+    const auto it = shared.find(key);
+    bool skip[B_NUM] = {};
+    r = bind_task(tp, &to, [&](int slot, BufRole role, size_t bytes) -> void* {
+      if (role == ROLE_SCRATCH) {
+        smax[slot] = std::max(smax[slot], bytes);
+        return (void*)&smax[slot]; // a placeholder until the shared scratch exists (never launched)
+      }
+      if (role == ROLE_OUT) {
+        switch (slot) {
+          case B_PRED: return pred + ob;
+          case B_RC: return rc + ob;
+          case B_KUSED: return ku + ob / T;
+          case B_COEFFS: return b->coeffs ? b->coeffs + (size_t)(ob / T) * C : nullptr;
+          default: return nullptr;
+        }
+      }
+      for (int q = 0; q < 3; q++)
+        if (slot == pslots[q] && it != shared.end() && it->second[q]) {
+          skip[slot] = true;
+          return it->second[q];
+        }
+      return b->dev<char>(nullptr, bytes);
+    });
+    if (r != 0)
+      return r;
+    if (it == shared.end())
+      shared[key] = { tp.ptr[B_MP], tp.ptr[B_MPPT], tp.ptr[B_MPPAN] };
+    r = upload_task(tp, &g.M, &g.Mp, &to, c->stream, false, skip);
+    if (r != 0)
+      return r;
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    // the host copies of the matrix-core operands are uploaded: drop them
+    std::vector<float>().swap(tp.m32);
+    std::vector<float>().swap(tp.mn);
+    std::vector<float>().swap(tp.p32);
+    std::vector<float>().swap(tp.pn);
+    std::vector<double>().swap(tp.ynorm);
+  }
+  void* sp[B_NUM] = {};
+  for (int q = 0; q < B_NUM; q++)
+    if (smax[q]) {
+      sp[q] = b->dev<char>(nullptr, smax[q]);
+      if (!sp[q])
+        return -2;
+    }
+  for (int tk = 0; tk < b->tasks; tk++) {
+    TaskPlan& tp = b->plans[tk];
+    if (tp.NP == 0)
+      continue;
+    void* old[B_NUM];
+    std::copy(tp.ptr, tp.ptr + B_NUM, old);
+    const edm_options to = task_options(b, tk);
+    const int r = bind_task(tp, &to, [&](int slot, BufRole role, size_t) { return role == ROLE_SCRATCH ? sp[slot] : old[slot]; });
+    if (r != 0)
+      return r;
+  }
+  b->B.pred = pred;
+  b->B.rc = rc;
+  b->B.k_used = ku;
+  b->resident = true;
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+static int batch_break(edm_batch* b);
+
+// One run of the resident route: each task's fills and kernels in task order, or, with graphs on
+// and after one eager run, the captured sequence replayed as a single graph launch.
+static int run_resident(edm_batch* b)
+{
+  edm_ctx* c = b->ctx;
+  hipStream_t s = c->stream;
+  if (b->keep_going && !b->keep_going())
+    return batch_break(b);
+  HIP_TRY(hipEventRecord(c->ev[4], s));
+  if (b->use_graph && b->runs >= 1) {
+    if (!b->graph_exec) {
+      HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
+      int r = 0;
+      for (int tk = 0; tk < b->tasks && r == 0; tk++)
+        if (b->plans[tk].NP > 0)
+          r = launch_task(b->plans[tk], s, nullptr, nullptr);
+      hipGraph_t gr = nullptr;
+      const hipError_t e = hipStreamEndCapture(s, &gr);
+      if (r != 0 || e != hipSuccess) {
+        if (gr)
+          hipGraphDestroy(gr);
+        return r != 0 ? r : -(int)e - 1000;
+      }
+      b->graph = gr;
+      HIP_TRY(hipGraphInstantiate(&b->graph_exec, gr, nullptr, nullptr, 0));
+    }
+    HIP_TRY(hipGraphLaunch(b->graph_exec, s));
+  } else {
+    for (int tk = 0; tk < b->tasks; tk++) {
+      if (b->plans[tk].NP == 0)
+        continue;
+      const int r = launch_task(b->plans[tk], s, nullptr, nullptr);
+      if (r != 0)
+        return r;
+      if (b->keep_going) {
+        // with a callback, the host waits for each task so that a stop takes effect between tasks
+        HIP_TRY(hipStreamSynchronize(s));
+        if (!b->keep_going())
+          return batch_break(b);
+      }
+    }
+  }
+  HIP_TRY(hipEventRecord(c->ev[5], s));
+  HIP_TRY(hipEventSynchronize(c->ev[5]));
+  b->runs++;
+  b->last_ms = ev_ms(c->ev[4], c->ev[5]);
+  c->timings[0] = b->last_ms;
+  c->timings[1] = c->timings[2] = 0.0;
+  c->timings[3] = b->last_ms;
+  c->last_redo = 0;
+  return 0;
+}
+
 extern "C" {
 
 int edm_batch_create(edm_ctx* c, const edm_manifold_view* L, const edm_manifold_view* P, const edm_options* o,
@@ -939,6 +1377,12 @@ int edm_batch_create(edm_ctx* c, const edm_manifold_view* L, const edm_manifold_
   }
   b->device_route = o->algorithm == FEDM_SIMPLEX && !generic && emax > 0 && kcap > 0 && kmin >= 1 && A < 1e30;
   if (!b->device_route) {
+    const char* env = getenv("FEDM_RESIDENT");
+    const int r = (env && env[0] == '0') ? 1 : build_resident(b);
+    if (r < 0) {
+      delete b;
+      return r;
+    }
     *out = b;
     return 0;
   }
@@ -1238,7 +1682,7 @@ void edm_batch_set_keep_going(edm_batch* b, int (*keep_going)(void))
 static int batch_break(edm_batch* b)
 {
   edm_ctx* c = b->ctx;
-  if (!b->device_route) {
+  if (!b->device_route && !b->resident) {
     std::fill(b->hpred.begin(), b->hpred.end(), FEDM_MISSING_D);
     std::fill(b->hrc.begin(), b->hrc.end(), (int)FEDM_BREAK_HIT);
     std::fill(b->hku.begin(), b->hku.end(), (int)FEDM_MISSING_I);
@@ -1252,71 +1696,15 @@ static int batch_break(edm_batch* b)
   return FEDM_BREAK_HIT;
 }
 
-int edm_batch_run(edm_batch* b)
+// The persistent route's stages on the context stream: work-queue counters, the outputs' fill values
+// (first run only), the rank-table build and scan, the direct kernel and the exact re-selection.
+// `timed` records the stage events and polls keep_going between stages (not under graph capture).
+static int enqueue_device(edm_batch* b, bool timed)
 {
   edm_ctx* c = b->ctx;
-  hipSetDevice(c->device);
-  auto stop = [b] { return b->keep_going && !b->keep_going(); };
-  if (!b->device_route) {
-    // task-by-task: gather each task's manifolds and predict them with edm_predict
-    const int Ea = b->Ea;
-    b->hpred.assign((size_t)b->total_out, FEDM_MISSING_D);
-    b->hrc.assign((size_t)b->total_out, FEDM_UNKNOWN_ERROR);
-    b->hku.assign((size_t)b->total_k, FEDM_MISSING_I);
-    for (int tk = 0; tk < b->tasks; tk++) {
-      if (stop())
-        return batch_break(b);
-      const int nl = b->lc[tk], np = b->pc[tk];
-      std::vector<double> md((size_t)nl * Ea), mt(nl), mtt(nl), mpd((size_t)np * Ea), mppt(np);
-      std::vector<int> mpan, mppan;
-      for (int r = 0; r < nl; r++) {
-        const int row = b->lr[b->lb[tk] + r];
-        std::copy(b->Ldata.begin() + (size_t)row * Ea, b->Ldata.begin() + (size_t)(row + 1) * Ea,
-                  md.begin() + (size_t)r * Ea);
-        mt[r] = b->Ltgt[row];
-        mtt[r] = b->Ltt.empty() ? 0.0 : b->Ltt[row];
-        if (!b->Lpan.empty())
-          mpan.push_back(b->Lpan[row]);
-      }
-      for (int r = 0; r < np; r++) {
-        const int row = b->pr[b->pb[tk] + r];
-        std::copy(b->Pdata.begin() + (size_t)row * Ea, b->Pdata.begin() + (size_t)(row + 1) * Ea,
-                  mpd.begin() + (size_t)r * Ea);
-        mppt[r] = b->Ppt.empty() ? 0.0 : b->Ppt[row];
-        if (!b->Ppan.empty())
-          mppan.push_back(b->Ppan[row]);
-      }
-      edm_manifold_view M = b->Lv, Mp = b->Pv;
-      M.data = md.data();
-      M.targets = mt.data();
-      M.target_times = mtt.data();
-      M.point_times = nullptr;
-      M.panel = mpan.empty() ? nullptr : mpan.data();
-      M.num_points = nl;
-      Mp.data = mpd.data();
-      Mp.targets = nullptr;
-      Mp.point_times = mppt.data();
-      Mp.target_times = nullptr;
-      Mp.panel = mppan.empty() ? nullptr : mppan.data();
-      Mp.num_points = np;
-      edm_options to = b->o;
-      to.k = b->kk[tk];
-      to.thetas = b->thetas.data();
-      to.metrics = b->metrics.empty() ? nullptr : b->metrics.data();
-      const int64_t ob = b->ob[tk];
-      const int r = edm_predict(c, &M, &Mp, &to, b->hpred.data() + ob, b->hrc.data() + ob, nullptr,
-                                b->hku.data() + ob / b->T, nullptr, 0, b->keep_going);
-      if (r == FEDM_BREAK_HIT)
-        return batch_break(b);
-      if (r != 0)
-        return r;
-    }
-    return 0;
-  }
-  if (stop())
-    return batch_break(b);
   BatchParams& B = b->B;
   hipStream_t s = c->stream;
+  auto stop = [b, timed] { return timed && b->keep_going && !b->keep_going(); };
   dfill(c, B.counter, 1, 0);
   if (!b->outputs_filled) {
     // Every run writes every output of every target (a prediction, or the fill values themselves for a
@@ -1328,12 +1716,14 @@ int edm_batch_run(edm_batch* b)
     b->outputs_filled = true;
   }
   dfill(c, B.redo_count, 1, 0);
-  HIP_TRY(hipEventRecord(c->ev[4], s));
+  if (timed)
+    HIP_TRY(hipEventRecord(c->ev[4], s));
   if (b->rank_route) {
     dfill(c, b->S.counter, 1, 0);
     void* rargs[] = { &b->R };
     HIP_TRY(hipLaunchKernel(rank_build_kernel(b->emax), dim3(b->R.np), dim3(256), rargs, 0, s));
-    HIP_TRY(hipEventRecord(c->ev[6], s));
+    if (timed)
+      HIP_TRY(hipEventRecord(c->ev[6], s));
     if (stop())
       return batch_break(b);
     void* sargs[] = { &b->S };
@@ -1343,14 +1733,14 @@ int edm_batch_run(edm_batch* b)
                               b->multi_lds, s));
       HIP_TRY(hipLaunchKernel((const void*)rank_scatter, dim3((b->S.num_rank + 63) / 64, (b->S.np + 63) / 64, b->S.T),
                               dim3(256), sargs, 0, s));
-    }
-    else if (b->S.num_items > 0)
+    } else if (b->S.num_items > 0)
       HIP_TRY(hipLaunchKernel(rank_scan_kernel(b->kcap, b->emax), dim3(b->scan_grid), dim3(256), sargs,
                               b->scan_lds, s));
-  } else {
+  } else if (timed) {
     HIP_TRY(hipEventRecord(c->ev[6], s));
   }
-  HIP_TRY(hipEventRecord(c->ev[7], s));
+  if (timed)
+    HIP_TRY(hipEventRecord(c->ev[7], s));
   if (stop())
     return batch_break(b);
   const void* fn = batch_kernel(b->kcap, b->Ea);
@@ -1373,20 +1763,82 @@ int edm_batch_run(edm_batch* b)
         HIP_TRY(hipLaunchKernel(batch_redo_kernel(b->kcap), dim3(b->grid), dim3(256), aargs, 0, s));
       }
   }
+  return 0;
+}
+
+int edm_batch_run(edm_batch* b)
+{
+  edm_ctx* c = b->ctx;
+  hipSetDevice(c->device);
+  auto stop = [b] { return b->keep_going && !b->keep_going(); };
+  if (b->resident)
+    return run_resident(b);
+  if (!b->device_route) {
+    // task-by-task: gather each task's manifolds and predict them with edm_predict (uploads per task)
+    b->hpred.assign((size_t)b->total_out, FEDM_MISSING_D);
+    b->hrc.assign((size_t)b->total_out, FEDM_UNKNOWN_ERROR);
+    b->hku.assign((size_t)b->total_k, FEDM_MISSING_I);
+    const bool want_coeffs = b->o.save_smap_coeffs && b->o.algorithm == FEDM_SMAP;
+    const size_t C = (size_t)b->Ea + 1;
+    b->hcoeffs.assign(want_coeffs ? (size_t)b->total_k * C : 0, FEDM_MISSING_D);
+    GatheredTask g;
+    for (int tk = 0; tk < b->tasks; tk++) {
+      if (stop())
+        return batch_break(b);
+      gather_task(b, tk, g);
+      const edm_options to = task_options(b, tk);
+      const int64_t ob = b->ob[tk];
+      const int r = edm_predict(c, &g.M, &g.Mp, &to, b->hpred.data() + ob, b->hrc.data() + ob,
+                                want_coeffs ? b->hcoeffs.data() + (size_t)(ob / b->T) * C : nullptr,
+                                b->hku.data() + ob / b->T, nullptr, 0, b->keep_going);
+      if (r == FEDM_BREAK_HIT)
+        return batch_break(b);
+      if (r != 0)
+        return r;
+    }
+    return 0;
+  }
+  if (stop())
+    return batch_break(b);
+  hipStream_t s = c->stream;
+  const bool graphed = b->use_graph && b->outputs_filled;
+  if (graphed) {
+    // the steady-state run (outputs filled once already) captured once and replayed
+    if (!b->graph_exec) {
+      HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
+      const int r = enqueue_device(b, false);
+      hipGraph_t gr = nullptr;
+      const hipError_t e = hipStreamEndCapture(s, &gr);
+      if (r != 0 || e != hipSuccess) {
+        if (gr)
+          hipGraphDestroy(gr);
+        return r != 0 ? r : -(int)e - 1000;
+      }
+      b->graph = gr;
+      HIP_TRY(hipGraphInstantiate(&b->graph_exec, gr, nullptr, nullptr, 0));
+    }
+    HIP_TRY(hipEventRecord(c->ev[4], s));
+    HIP_TRY(hipGraphLaunch(b->graph_exec, s));
+  } else {
+    const int r = enqueue_device(b, true);
+    if (r != 0)
+      return r;
+  }
   HIP_TRY(hipEventRecord(c->ev[5], s));
   HIP_TRY(hipEventSynchronize(c->ev[5]));
   {
     int handed = 0;
-    HIP_TRY(hipMemcpy(&handed, B.redo_count, sizeof(int), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(&handed, b->B.redo_count, sizeof(int), hipMemcpyDeviceToHost));
     b->redo_last = handed;
     c->last_redo = handed;
   }
+  b->runs++;
   b->last_ms = ev_ms(c->ev[4], c->ev[5]);
-  // [0] whole batch, [1] rank-table build, [2] rank scan, [3] direct kernel
+  // [0] whole batch, [1] rank-table build, [2] rank scan, [3] direct kernel (stages: eager runs only)
   c->timings[0] = b->last_ms;
-  c->timings[1] = ev_ms(c->ev[4], c->ev[6]);
-  c->timings[2] = ev_ms(c->ev[6], c->ev[7]);
-  c->timings[3] = ev_ms(c->ev[7], c->ev[5]);
+  c->timings[1] = graphed ? 0.0 : ev_ms(c->ev[4], c->ev[6]);
+  c->timings[2] = graphed ? 0.0 : ev_ms(c->ev[6], c->ev[7]);
+  c->timings[3] = graphed ? 0.0 : ev_ms(c->ev[7], c->ev[5]);
   return 0;
 }
 
@@ -1394,7 +1846,7 @@ int edm_batch_fetch(edm_batch* b, double* pred, int* rc, int* k_used)
 {
   edm_ctx* c = b->ctx;
   hipSetDevice(c->device);
-  if (!b->device_route) {
+  if (!b->device_route && !b->resident) {
     std::copy(b->hpred.begin(), b->hpred.end(), pred);
     std::copy(b->hrc.begin(), b->hrc.end(), rc);
     std::copy(b->hku.begin(), b->hku.end(), k_used);
@@ -1410,6 +1862,40 @@ int edm_batch_fetch(edm_batch* b, double* pred, int* rc, int* k_used)
 int edm_batch_on_device(const edm_batch* b)
 {
   return b->device_route ? 1 : 0;
+}
+
+int edm_batch_route(const edm_batch* b)
+{
+  return b->device_route ? 2 : b->resident ? 1 : 0;
+}
+
+int edm_batch_set_graph(edm_batch* b, int on)
+{
+  b->use_graph = on != 0 && (b->device_route || b->resident);
+  if (!b->use_graph && b->graph_exec) {
+    hipGraphExecDestroy(b->graph_exec);
+    hipGraphDestroy(b->graph);
+    b->graph_exec = nullptr;
+    b->graph = nullptr;
+  }
+  return b->use_graph ? 1 : 0;
+}
+
+int edm_batch_fetch_coeffs(edm_batch* b, double* coeffs)
+{
+  edm_ctx* c = b->ctx;
+  const size_t n = (size_t)b->total_k * (b->Ea + 1);
+  if (!b->coeffs) {
+    if (b->hcoeffs.size() == n)
+      std::copy(b->hcoeffs.begin(), b->hcoeffs.end(), coeffs);
+    else
+      std::fill(coeffs, coeffs + n, FEDM_MISSING_D);
+    return 0;
+  }
+  hipSetDevice(c->device);
+  HIP_TRY(hipMemcpyAsync(coeffs, b->coeffs, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return 0;
 }
 
 void edm_batch_destroy(edm_batch* b)

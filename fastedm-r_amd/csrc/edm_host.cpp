@@ -547,7 +547,8 @@ struct fedm_plan
     std::vector<int64_t> out_begin;
     int64_t total_out = 0;
     edm_batch* batch = nullptr;
-    bool taskwise = false; // one edm_predict per task (S-map coefficients, or the task-by-task plain route)
+    bool taskwise = false; // one edm_predict per task (the task-by-task plain route)
+    bool want_coeffs = false;
     std::vector<double> pred;
     std::vector<int> rc, ku;
     std::vector<double> coeffs; // per task [Np x C] (S-map coefficient route)
@@ -858,7 +859,9 @@ static int plan_tasks(edm_ctx* ctx, fedm_plan* P, const TaskGrid& grid)
     edm_manifold_view Pv{ g.ptable.data(), g.targets.data(), g.ptimes.data(), g.ttimes.data(),
                           g.panel.empty() ? nullptr : g.panel.data(), n, g.E, s.E_dt(g.E), s.E_extras(), 0, Ea };
     const bool want_coeffs = !g.co && P->save_smap_coeffs && o.algorithm == FEDM_SMAP;
-    g.taskwise = want_coeffs || o.precision == FEDM_PLAIN_TASKWISE;
+    go.save_smap_coeffs = want_coeffs;
+    g.want_coeffs = want_coeffs;
+    g.taskwise = o.precision == FEDM_PLAIN_TASKWISE;
     if (g.taskwise)
       continue;
     const int r = edm_batch_create(ctx, &Lv, &Pv, &go, (int)g.members.size(), lb.data(), lc.data(), lr.data(),
@@ -946,7 +949,11 @@ static int fetch_groups(fedm_plan* P)
     g.rc.assign((size_t)g.total_out, FEDM_UNKNOWN_ERROR);
     g.ku.assign((size_t)(g.total_out / T), MISS_I);
     if (g.batch) {
-      const int r = edm_batch_fetch(g.batch, g.pred.data(), g.rc.data(), g.ku.data());
+      int r = edm_batch_fetch(g.batch, g.pred.data(), g.rc.data(), g.ku.data());
+      if (r == 0 && g.want_coeffs) {
+        g.coeffs.assign((size_t)(g.total_out / T) * (P->s.E_actual(g.E) + 1), MISS);
+        r = edm_batch_fetch_coeffs(g.batch, g.coeffs.data());
+      }
       if (r != 0)
         return r;
     }
@@ -1088,6 +1095,15 @@ int fedm_plan_num_tasks(const fedm_plan* P)
   return (int)P->tasks.size();
 }
 
+int fedm_plan_set_graph(fedm_plan* P, int on)
+{
+  int n = 0;
+  for (auto& g : P->groups)
+    if (g.batch)
+      n += edm_batch_set_graph(g.batch, on);
+  return n;
+}
+
 int fedm_plan_task(const fedm_plan* P, int task, int* info, int* lib_rows, int* pred_rows)
 {
   if (task < 0 || task >= (int)P->tasks.size())
@@ -1128,6 +1144,14 @@ int fedm_plan_fetch(fedm_plan* P, double* pred, int* rc, int* k_used)
     ko += Np;
   }
   return 0;
+}
+
+int fedm_plan_resident_batches(const fedm_plan* P)
+{
+  int n = 0;
+  for (const auto& g : P->groups)
+    n += g.batch != nullptr && edm_batch_route(g.batch) == 1;
+  return n;
 }
 
 int fedm_plan_device_batches(const fedm_plan* P)

@@ -143,8 +143,22 @@ void edm_batch_destroy(edm_batch* b);
  * rc BREAK_HIT and missing k_used until the next complete run. */
 void edm_batch_set_keep_going(edm_batch* b, int (*keep_going)(void));
 /* 1 when the batch runs as one persistent device launch (simplex, Euclidean, no missing/panel
- * options, E_actual <= 32, 1 <= k <= 32), 0 when it runs task by task through edm_predict. */
+ * options, E_actual <= 32, 1 <= k <= 32), 0 when it runs task by task. */
 int edm_batch_on_device(const edm_batch* b);
+/* How the batch runs: 2 one persistent launch per stage; 1 the resident task route (every other
+ * option set: each task's manifolds gathered and uploaded once at create, its kernels launched in
+ * turn on each run, nothing copied from the host); 0 task by task through edm_predict, with a host
+ * gather and upload per task on each run (only when the inputs would exceed 16 GiB or a task needs a
+ * width this build does not take). */
+int edm_batch_route(const edm_batch* b);
+/* With on != 0, the second and later runs of the batch replay its launches as one captured hipGraph
+ * (the first run executes them directly; routes 1 and 2 only).  Stage timings are then not
+ * recorded (edm_ctx_timings gives the whole run).  Returns 1 when graphs are on. */
+int edm_batch_set_graph(edm_batch* b, int on);
+/* S-map coefficients of the last theta (routes 1 and 0 save them when the options ask for
+ * them): [prediction rows x (E_actual + 1)] per task at out_begin[t] / num_thetas * (E_actual + 1),
+ * missing elsewhere. */
+int edm_batch_fetch_coeffs(edm_batch* b, double* coeffs);
 
 /* Smallest library size that a batch sends to the rank-table route for k neighbours over a pool of
  * npool rows (scale from FEDM_RANK_LTHR_SCALE, default 1.5 * sqrt(k * npool)). */
@@ -214,13 +228,16 @@ int fedm_plan_create(edm_ctx* ctx, const fedm_command* c, fedm_plan** out);
 int fedm_plan_run(fedm_plan* p);
 int fedm_plan_result(fedm_plan* p, fedm_result** out);
 int64_t fedm_plan_num_predictions(const fedm_plan* p);
-int fedm_plan_device_batches(const fedm_plan* p); /* groups that run as one persistent launch */
+int fedm_plan_device_batches(const fedm_plan* p);   /* groups that run as one persistent launch */
+int fedm_plan_resident_batches(const fedm_plan* p); /* groups on the resident task route */
 /* The plan's tasks in the reference's task order (src/edm.cpp:128-213), for parity checks (no reference
  * counterpart).  fedm_plan_task: info[7] = {E_actual, library, k, library rows, prediction rows,
  * copredict, E}; lib_rows / pred_rows (or NULL) receive the table rows the task uses.
  * fedm_plan_fetch copies every task's outputs of the last run back in task order: pred and rc
  * [num_thetas x prediction rows] per task, k_used [prediction rows] per task, concatenated. */
 int fedm_plan_num_tasks(const fedm_plan* p);
+/* Graph replay (edm_batch_set_graph) for every group of the plan; returns the groups that have it. */
+int fedm_plan_set_graph(fedm_plan* p, int on);
 int fedm_plan_task(const fedm_plan* p, int task, int* info, int* lib_rows, int* pred_rows);
 int fedm_plan_fetch(fedm_plan* p, double* pred, int* rc, int* k_used);
 void fedm_plan_destroy(fedm_plan* p);
