@@ -386,6 +386,7 @@ struct TaskPlan
   int route = R_LIST;
   bool wass = false, smap = false, generic = false, wide = false;
   int kcap = 0, kp = 0, S = 1, sl = 0, bx = 0, nt = 0, gstride = 0;
+  int wE = 0, wE_dt = 0, wrows = 0, wunl = 0; // Wasserstein layout
   size_t bytes[B_NUM] = {};
   void* ptr[B_NUM] = {};
   // host copies the upload needs beyond the views
@@ -430,6 +431,10 @@ static int plan_task(const edm_manifold_view* M, const edm_manifold_view* Mp, co
       wass_ok = wass_ok && o->id_weights[a] >= 0.0f;
   if (tp.emax < 0 || (tp.wass && !wass_ok))
     return -95;
+  tp.wE = wE;
+  tp.wE_dt = M->E_dt;
+  tp.wrows = wrows;
+  tp.wunl = wunl;
 
   // metrics: one per coordinate for L^p, one per variable for Wasserstein (src/distances.cpp:29-67)
   tp.nmet = tp.wass ? wrows + wunl : Ea;
@@ -643,6 +648,10 @@ static int bind_task(TaskPlan& tp, const edm_options* o, Get&& get)
     W.NL = NL;
     W.NP = NP;
     W.Ea = Ea;
+    W.E = tp.wE;
+    W.E_dt = tp.wE_dt;
+    W.rows = tp.wrows;
+    W.n_unlagged = tp.wunl;
     W.num_panels = o->num_panels;
     W.missing_distance = o->missing_distance;
     W.idw = o->idw;
