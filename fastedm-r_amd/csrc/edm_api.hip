@@ -9,6 +9,7 @@
 #include <cstdlib>
 #include <array>
 #include <map>
+#include <mutex>
 #include <type_traits>
 #include <cstring>
 #include <vector>
@@ -61,6 +62,8 @@ struct edm_ctx
   double timings[4] = { 0, 0, 0, 0 };
   int last_error = 0;
   int last_redo = 0;
+  unsigned char* scratch = nullptr; // this device's copy of the no-allocation scratch array
+  int scratch_err = 0;
 
   void* get(int slot, size_t bytes)
   {
@@ -258,6 +261,36 @@ static bool smap_lds_rows()
   return env && env[0] == '1';
 }
 
+// Scratch of edm_predict_noalloc: the kernels' slice and neighbour lists, counts and dense distance
+// rows, and the small host-side option arrays, for callers that cannot allocate device memory.  Each
+// device holds its own copy of the array for the life of the process; its address on a device comes
+// from hipGetSymbolAddress, looked up once per device when a context is created there.
+namespace fedm {
+constexpr size_t NOALLOC_SCRATCH_BYTES = (size_t)32 << 20;
+}
+static __device__ __align__(256) unsigned char scratch[fedm::NOALLOC_SCRATCH_BYTES];
+
+static std::mutex g_scratch_mu;
+static std::map<int, unsigned char*> g_scratch_addr; // device -> address of `scratch` there
+
+// With `device` current: the address of its copy of the scratch array (looked up on first use).
+static int scratch_address(int device, unsigned char** out)
+{
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  auto it = g_scratch_addr.find(device);
+  if (it == g_scratch_addr.end()) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess)
+      return -(int)e - 1000;
+    if (!p)
+      return -2;
+    it = g_scratch_addr.emplace(device, (unsigned char*)p).first;
+  }
+  *out = it->second;
+  return 0;
+}
+
 extern "C" {
 
 int edm_ctx_create(edm_ctx** out, int device)
@@ -272,6 +305,7 @@ int edm_ctx_create(edm_ctx** out, int device)
   for (auto& e : c->ev)
     hipEventCreate(&e);
   c->bufs.resize(B_NUM);
+  c->scratch_err = scratch_address(device, &c->scratch);
   *out = c;
   return 0;
 }
@@ -404,7 +438,7 @@ struct TaskPlan
 // option this build does not take; the caller fills the outputs).  `mfma_ok` allows the matrix-core
 // route (it needs the host rows); `max_slices` caps the library slices of the list routes.
 static int plan_task(const edm_manifold_view* M, const edm_manifold_view* Mp, const edm_options* o, bool want_coeffs,
-                     int nn_cap, bool mfma_ok, int max_slices, TaskPlan& tp)
+                     int nn_cap, bool mfma_ok, int max_slices, TaskPlan& tp, size_t dense_budget = (size_t)1 << 30)
 {
   const int NL = M->num_points, NP = Mp->num_points, Ea = M->E_actual, T = o->num_thetas;
   if (Ea != Mp->E_actual || T <= 0 || Ea <= 0 || NL < 0 || NP < 0)
@@ -593,8 +627,7 @@ static int plan_task(const edm_manifold_view* M, const edm_manifold_view* Mp, co
     const size_t per_target = sizeof(double) * (size_t)std::max(NL, 1) +
                               (tp.wide ? (sizeof(int) + sizeof(double)) * (size_t)k : 0) +
                               (sizeof(unsigned long long) + sizeof(int)) * (size_t)tp.gstride;
-    const size_t budget = (size_t)1 << 30;
-    tp.nt = (int)std::max<size_t>(1, std::min<size_t>((size_t)std::max(NP, 1), budget / per_target));
+    tp.nt = (int)std::max<size_t>(1, std::min<size_t>((size_t)std::max(NP, 1), dense_budget / per_target));
     const int bxmax = tp.wass ? tp.nt : (tp.nt + 255) / 256;
     int S = std::max(1, std::min(MAX_SLICES, (1024 + bxmax - 1) / bxmax));
     S = std::max(1, std::min(S, NL / (tp.wass ? 64 : 256)));
@@ -888,6 +921,80 @@ int edm_predict(edm_ctx* c, const edm_manifold_view* M, const edm_manifold_view*
   }
   return 0;
 }
+int edm_predict_noalloc(edm_ctx* c, const edm_manifold_view* M, const edm_manifold_view* Mp, const edm_options* o,
+                        double* pred, int* rc, double* coeffs, int* k_used)
+{
+  if (!c || !M || !Mp || !o || !pred || !rc || !k_used || M->E_actual != Mp->E_actual || o->num_thetas <= 0 ||
+      M->E_actual <= 0)
+    return -22;
+  if (Mp->num_points == 0)
+    return 0;
+  if (!c->scratch)
+    return c->scratch_err ? c->scratch_err : -2;
+  const size_t cap = NOALLOC_SCRATCH_BYTES;
+  auto rounded = [](size_t b) { return (b + 255) / 256 * 256; };
+  auto in_scratch = [](int slot) {
+    return buf_role(slot) == ROLE_SCRATCH || slot == B_METRICS || slot == B_IDW || slot == B_THETAS;
+  };
+  // the most library slices (and, on the dense route, targets per block) whose buffers fit
+  TaskPlan tp;
+  size_t need = 0;
+  for (int slices = MAX_SLICES;; slices /= 2) {
+    tp = TaskPlan();
+    const int r = plan_task(M, Mp, o, coeffs != nullptr, 0, false, slices, tp, cap / 2);
+    if (r != 0)
+      return r;
+    need = 0;
+    for (int q = 0; q < B_NUM; q++)
+      if (in_scratch(q))
+        need += rounded(tp.bytes[q]);
+    if (need <= cap)
+      break;
+    if (slices == 1) {
+      fprintf(stderr,
+              "fastedm_hip: edm_predict_noalloc: this task needs %zu bytes of scratch, more than the %zu bytes "
+              "the library holds for it (fewer targets or a smaller k per call)\n",
+              need, cap);
+      return -12;
+    }
+  }
+  unsigned char* base = c->scratch;
+  size_t off = 0;
+  const void* view[B_NUM] = {};
+  view[B_M] = M->data;
+  view[B_MP] = Mp->data;
+  view[B_MT] = M->targets;
+  view[B_MTT] = M->target_times;
+  view[B_MPPT] = Mp->point_times;
+  view[B_MPAN] = M->panel;
+  view[B_MPPAN] = Mp->panel;
+  view[B_PRED] = pred;
+  view[B_RC] = rc;
+  view[B_KUSED] = k_used;
+  view[B_COEFFS] = coeffs;
+  hipSetDevice(c->device);
+  int r = bind_task(tp, o, [&](int slot, BufRole, size_t bytes) -> void* {
+    if (in_scratch(slot)) {
+      void* p = base + off;
+      off += rounded(bytes);
+      return p;
+    }
+    return const_cast<void*>(view[slot]);
+  });
+  if (r == 0)
+    r = upload_task(tp, M, Mp, o, c->stream, true);
+  if (r == 0)
+    r = launch_task(tp, c->stream, c->ev, nullptr);
+  if (r != 0)
+    return r;
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  c->timings[0] = ev_ms(c->ev[0], c->ev[1]);
+  c->timings[1] = ev_ms(c->ev[1], c->ev[2]);
+  c->timings[2] = ev_ms(c->ev[2], c->ev[3]);
+  c->timings[3] = ev_ms(c->ev[0], c->ev[3]);
+  return 0;
+}
+
 } // extern "C"
 
 // ------------------------------------------------------------------------------------------------

@@ -110,6 +110,23 @@ int edm_predict(edm_ctx* ctx, const edm_manifold_view* M, const edm_manifold_vie
                 double* pred, int* rc, double* coeffs, int* k_used, int* nn_idx, int nn_cap, int (*keep_going)(void));
 
 /*
+ * edm_predict_noalloc — edm_predict for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  No counterpart in the reference.
+ *   M, Mp: every array in the views (data, targets, times, panel) is a DEVICE pointer;
+ *   pred [num_thetas x Np], rc [num_thetas x Np], k_used [Np], coeffs [Np x (E_actual+1)] or NULL: DEVICE
+ *   pointers; o->thetas, o->metrics and o->id_weights stay host arrays.
+ * Results equal edm_predict's.  The kernels' scratch lives in a static 32 MiB device array (one copy per
+ * device, held until the process ends), so the call allocates nothing; a task whose scratch would not
+ * fit (about Np x k x 24 bytes on the list routes, at least Nl x 8 bytes on the dense ones) returns -12
+ * with a message on stderr.  The matrix-core screening is not used (FP64 routes only).  Calls that
+ * overlap on one device race on the shared array: make them from one thread or one context at a time.
+ * Synchronous on ctx's stream.  Returns 0, -22 (invalid arguments), -95 (a width this build does not
+ * take; outputs untouched), -12, or a HIP error.
+ */
+int edm_predict_noalloc(edm_ctx* ctx, const edm_manifold_view* M, const edm_manifold_view* Mp, const edm_options* o,
+                        double* pred, int* rc, double* coeffs, int* k_used);
+
+/*
  * edm_predict_batch — many tasks that index one shared point table, in one persistent launch per
  * stage (CCM library-size x replicate sweeps, E/theta grids).  Task t uses library rows
  * lib_rows[lib_begin[t] .. lib_begin[t]+lib_count[t]) (ascending, rows of L) and prediction rows
