@@ -61,7 +61,7 @@ EXPORTS = ["edm_ctx_create", "edm_ctx_destroy", "edm_predict", "edm_predict_batc
            "fedm_plan_run", "fedm_plan_result", "fedm_plan_num_predictions", "fedm_plan_device_batches", "fedm_plan_destroy",
            "fedm_plan_num_tasks", "fedm_plan_task", "fedm_plan_fetch", "edm_batch_set_keep_going",
            "edm_batch_route", "edm_batch_set_graph", "edm_batch_fetch_coeffs", "fedm_plan_set_graph",
-           "fedm_plan_resident_batches", "edm_predict_noalloc", "edm_batch_on_device", "edm_selftest_mfma", "fedm_rank_threshold", "fedm_run_command", "fedm_run_task_group", "fedm_result_rc", "fedm_result_num_stats", "fedm_result_stats", "fedm_result_k",
+           "fedm_plan_resident_batches", "edm_predict_noalloc", "edm_selftest_exp", "edm_batch_on_device", "edm_selftest_mfma", "fedm_rank_threshold", "fedm_run_command", "fedm_run_task_group", "fedm_result_rc", "fedm_result_num_stats", "fedm_result_stats", "fedm_result_k",
            "fedm_result_scalar", "fedm_result_matrix_shape", "fedm_result_matrix", "fedm_result_num_manifolds",
            "fedm_result_manifold_shape", "fedm_result_manifold", "fedm_result_free"]
 

@@ -16,6 +16,7 @@
 
 #include "../../include/fastedm_hip.h"
 #include "batch_common.h"
+#include "fdlibm_exp.h"
 
 namespace fedm {
 
@@ -1149,6 +1150,36 @@ extern "C" int edm_selftest_mfma(float* out256)
   hipLaunchKernelGGL(mfma_layout_probe, dim3(1), dim3(64), 0, 0, d);
   hipError_t e = hipMemcpy(out256, d, 256 * sizeof(float), hipMemcpyDeviceToHost);
   hipFree(d);
+  return e == hipSuccess ? 0 : -3;
+}
+
+namespace fedm {
+__global__ void exp_probe(const double* x, double* out, int n)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n)
+    out[i] = fdlibm_exp(x[i]);
+}
+} // namespace fedm
+
+extern "C" int edm_selftest_exp(const double* x, double* out, int n)
+{
+  if (n <= 0)
+    return 0;
+  double *dx = nullptr, *dy = nullptr;
+  if (hipMalloc(&dx, sizeof(double) * n) != hipSuccess)
+    return -2;
+  if (hipMalloc(&dy, sizeof(double) * n) != hipSuccess) {
+    hipFree(dx);
+    return -2;
+  }
+  hipError_t e = hipMemcpy(dx, x, sizeof(double) * n, hipMemcpyHostToDevice);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(exp_probe, dim3((n + 255) / 256), dim3(256), 0, 0, dx, dy, n);
+    e = hipMemcpy(out, dy, sizeof(double) * n, hipMemcpyDeviceToHost);
+  }
+  hipFree(dx);
+  hipFree(dy);
   return e == hipSuccess ? 0 : -3;
 }
 

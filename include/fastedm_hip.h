@@ -186,6 +186,10 @@ int fedm_rank_threshold(int kmax, int npool);
  * on a conforming device.  Returns 0 when the probe ran. */
 int edm_selftest_mfma(float* out256);
 
+/* Diagnostic: out[i] = the device exp the simplex and S-map weights use, at x[i] (FDLIBM's method; the
+ * CPU oracle evaluates the same expression).  Returns 0 when the probe ran. */
+int edm_selftest_exp(const double* x, double* out, int n);
+
 /* Last stage timings of the most recent call on ctx (milliseconds, HIP events on the ctx stream):
  * [0] distance/selection kernel, [1] merge/predict kernel, [2] S-map kernel, [3] whole device section. */
 void edm_ctx_timings(edm_ctx* ctx, double* ms4);

@@ -243,3 +243,23 @@ def test_mixed_task_parity(name):
     print(f"max |d yhat| {d:.3e}")
     if opts.get("save_smap_coeffs"):
         assert_predictions_close(gc, rc_)
+
+
+def test_device_exp_matches_oracle_bitwise():
+    """The device exp of the simplex and S-map weights (FDLIBM's method, fdlibm_exp.h) against the
+    oracle's over the dense grid of the weights' domain that pins the oracle's to within 1 ulp of libm:
+    identical bits everywhere, so the two copies of the method cannot drift apart."""
+    import ctypes as C
+    from tests.test_oracle_math import _exp_grid
+    fe = load_product()
+    L = fe.load_library()
+    L.edm_selftest_exp.argtypes = [C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int]
+    x = np.ascontiguousarray(_exp_grid())
+    out = np.zeros_like(x)
+    assert L.edm_selftest_exp(x.ctypes.data_as(C.POINTER(C.c_double)), out.ctypes.data_as(C.POINTER(C.c_double)),
+                              x.size) == 0
+    O = oracle.lib()
+    O.orc_weight_exp.restype = C.c_double
+    O.orc_weight_exp.argtypes = [C.c_double]
+    ref = np.array([O.orc_weight_exp(float(v)) for v in x])
+    assert np.array_equal(out.view(np.int64), ref.view(np.int64))

@@ -344,3 +344,92 @@ def test_crossfold_groups_partition_usable_rows():
     lib_out = lib_out.reshape(5, n)
     assert (pred_out.sum(0) == 1).all()  # every usable row is predicted exactly once
     assert ((lib_out + pred_out) == 1).all()
+
+
+def _exp_grid():
+    """Dense arguments over the weights' domain: exp(-theta d / d_ref) takes x <= 0 down to where it
+    underflows (-745.13), plus the reduction's case boundaries and the special values."""
+    rng = np.random.default_rng(31)
+    x = np.concatenate([
+        -np.logspace(-12, np.log10(745.2), 120000),
+        rng.uniform(-745.2, 0.0, 60000),
+        rng.uniform(-1.2, 0.0, 40000),
+        -np.arange(1, 1076) * np.log(2.0),          # multiples of ln 2 (k changes)
+        np.array([0.0, -0.0, -3.725290298461914e-09, -3.7252902e-09, -0.5 * np.log(2.0), -1.5 * np.log(2.0),
+                  -708.3964185322641, -709.0, -744.44007192138126, -745.0, -745.13321910194110842,
+                  -745.1332191019412, -746.0, -1e308, 1e-300, 0.5, 2.0, 700.0]),
+    ])
+    return x
+
+
+def test_weight_exp_within_one_ulp_of_libm():
+    """The oracle's FDLIBM exp (the weights' exp on the CPU and the GPU) against math.exp over a dense grid
+    of the weights' domain: within 1 ulp everywhere, exact at the special values."""
+    import math
+    L = oracle.lib()
+    L.orc_weight_exp.restype = oracle.C.c_double
+    L.orc_weight_exp.argtypes = [oracle.C.c_double]
+    x = _exp_grid()
+    got = np.array([L.orc_weight_exp(float(v)) for v in x])
+    ref = np.array([math.exp(float(v)) for v in x])
+    ulp = np.spacing(np.abs(ref))
+    ulp[ref == 0] = np.spacing(0.0)
+    assert np.all(np.abs(got - ref) <= ulp), float(np.max(np.abs(got - ref) / ulp))
+    assert L.orc_weight_exp(0.0) == 1.0 and L.orc_weight_exp(-0.0) == 1.0
+    assert L.orc_weight_exp(-746.0) == 0.0 and L.orc_weight_exp(-1e308) == 0.0
+    assert np.isnan(L.orc_weight_exp(float("nan")))
+    assert L.orc_weight_exp(float("-inf")) == 0.0
+    # FDLIBM's method is within 1 ulp but not correctly rounded: about 6 % of these arguments are one ulp
+    # away from glibc's value (a weight's last bit, which the reference leaves to its libm/Eigen exp)
+    assert np.mean(got != ref) < 0.1
+
+
+def test_panel_weight_matrix_brute_force():
+    """The panel-weight matrix branch (src/distances.cpp:182-183, keys built at src/r.cpp:204-220): the
+    penalty W[(library panel, prediction panel)] joins the squared distance inside the sqrt, as a float,
+    keyed by position in the run-length-distinct panel ids, a later run of a panel overwriting an earlier
+    one.  Panel 5 appears in two runs and W is asymmetric, so a transposed key or the first run's
+    values fail the check."""
+    rng = np.random.default_rng(21)
+    blocks = [(5, 30), (2, 25), (5, 20), (9, 28)]
+    t, x, panel = [], [], []
+    for pid, n in blocks:
+        t += list(range(1, n + 1))
+        x += list(rng.uniform(0.1, 0.9, n))
+        panel += [pid] * n
+    t, x, panel = np.array(t, float), np.array(x), np.array(panel)
+    uniq = [5, 2, 5, 9]
+    W = rng.uniform(0.0, 0.4, (4, 4))
+    k, theta = 4, 1.3
+    res = oracle.edm(t, x, panel=panel, E=2, k=k, theta=[theta], panelWeights=W, savePredictions=True)
+    assert res["rc"] == 0
+    key = {}
+    for i, a in enumerate(uniq):
+        for j, b in enumerate(uniq):
+            key[(a, b)] = float(np.float32(W[i, j]))
+    n = len(t)
+    same = lambda r, q: 0 <= q < n and panel[q] == panel[r] and t[q] == t[r] + (q - r)
+    usable = [r for r in range(n) if same(r, r - 1)]
+    nu = len(usable)
+    lib = [r for r in usable[:nu // 2] if same(r, r + 1)]
+    pred_rows = usable[nu // 2:]
+    expect = np.full(n, np.nan)
+    for i in pred_rows:
+        cand = []
+        for pos, j in enumerate(lib):
+            d2 = key[(panel[j], panel[i])]
+            d2 += (x[j] - x[i]) ** 2
+            d2 += (x[j - 1] - x[i - 1]) ** 2
+            if d2 != 0:
+                cand.append((np.sqrt(d2), pos, j))
+        nn = sorted(cand)[:k]
+        dmin = nn[0][0]
+        w = np.array([np.exp(-theta * d / dmin) for d, _, _ in nn])
+        expect[i] = sum(x[j + 1] * (wi / w.sum()) for (_, _, j), wi in zip(nn, w))
+    got = res["predictions"][:, 0]
+    ok = ~np.isnan(expect)
+    assert np.array_equal(np.isnan(got), ~ok)
+    np.testing.assert_allclose(got[ok], expect[ok], rtol=1e-13, atol=0)
+    # the transposed key gives a different answer (the check can fail)
+    res_t = oracle.edm(t, x, panel=panel, E=2, k=k, theta=[theta], panelWeights=W.T.copy(), savePredictions=True)
+    assert np.max(np.abs(res_t["predictions"][ok, 0] - expect[ok])) > 1e-6
