@@ -12,8 +12,11 @@ One step = every prediction of the sweep (fedm_plan_run), inputs resident on the
   plain_batched  FP64 in the batched launch, every library through the direct kernel
   roofline       the stage that dominates the device time, with its algorithmic rate from HIP-event
                  kernel times against the matching MI355X peak
-  cpu_baseline   the CPU oracle (a restatement of the reference CPU path, one thread per host core)
-                 timed on a bounded sample of the same sweep
+  cpu_baseline   the CPU oracle (a restatement of the reference CPU path, one thread per host core, built
+                 -O3 -march=native on the host it runs on) timed on a bounded sample of the same sweep
+  graph_replay   the same step replayed as one captured hipGraph
+  configs        the other workloads (S-map theta sweep, panel S-map, Lorenz-96 S-map): mixed and plain
+                 timed runs with inputs resident, their rooflines, CPU baselines and parity figures
 """
 import argparse
 import json
@@ -149,28 +152,228 @@ def roofline(st, libs, reps):
             "engine": {"direct": "fp64 VALU", "rank_build": "fp32 VALU", "rank_scan": "HBM3E"}[name]}
 
 
-def panel_smap_leg(fe):
-    """Secondary line: S-map on 512 logistic-map panels (E = 8 with dt, E_a = 16, k = 18, theta 0, 1, 3;
-    265 665 library rows x 265 665 targets), mixed against plain, device stage times of one call each."""
+SMAP_UNIT = "2k(E_a+1)^2 + 2k(E_a+1) + k flops per S-map prediction (Gram X^T X, X^T y and weights; the SVD is not counted)"
+
+
+def smap_flops(k, Ea):
+    return 2.0 * k * (Ea + 1) ** 2 + 2.0 * k * (Ea + 1) + k
+
+
+def stage_roofline(stages, work):
+    """Roofline of the stage with the largest device time of a single-task run: stages = [select, merge,
+    S-map] ms (HIP events around the kernels on the context stream); work[name] = (kernel, algorithmic
+    flops per launch, engine, peak TFLOP/s, bound)."""
+    times = {"select": stages[0], "smap": stages[2]}
+    name = max(times, key=lambda n: times[n])
+    kernel, flops, engine, peak, bound = work[name]
+    ms = float(times[name])
+    achieved = flops / (ms * 1e-3) / 1e12
+    return {"bound": bound, "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
+            "traffic": None, "kernel": kernel, "kernel_ms": ms, "algorithmic_work_per_launch": flops,
+            "engine": engine, "stage": name}
+
+
+def timed_runs(obj, fe, steps, warmup):
+    for _ in range(warmup):
+        obj.run()
+    st = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        obj.run()
+        st.append(fe.timings())
+    return (time.perf_counter() - t0) / steps, np.mean(np.asarray(st), axis=0)
+
+
+def median_time(fn, runs=3):
+    times = []
+    out = None
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        out = fn()
+        times.append(time.perf_counter() - t0)
+    return float(np.median(times)), times, out
+
+
+def form_line(n_pred, sec, st):
+    return {"predictions_per_s": n_pred / sec, "ms_per_step": sec * 1e3, "device_ms": float(st[3]),
+            "select_ms": float(st[0]), "merge_ms": float(st[1]), "smap_ms": float(st[2])}
+
+
+def bits_equal(a, b):
+    a, b = np.asarray(a, np.float64), np.asarray(b, np.float64)
+    return int(np.sum(a.view(np.int64) != b.view(np.int64)))
+
+
+def cfg2_leg(fe, oracle, cores, steps):
+    """S-map theta sweep on the coupled maps, T = 5000, E = 4, k = all library points (2498), six thetas:
+    one task of 2499 targets (14 994 predictions) through the plan's resident task route."""
+    from tests import configs
+    x, _ = configs.coupled_maps(5000)
+    t = np.arange(1, 5001, dtype=float)
+    kw = dict(E=4, k=np.inf, theta=[0, 0.5, 1, 2, 4, 8], algorithm="smap", savePredictions=True)
+    out = {"workload": "smap_theta_sweep_coupled_logistic_T5000_E4_kall_6thetas"}
+    raw, res = {}, {}
+    for name, prec in (("mixed", fe.MIXED), ("plain", fe.PLAIN)):
+        plan = fe.Plan(t, x, precision=prec, **kw)
+        sec, st = timed_runs(plan, fe, steps, 2)
+        n_pred = plan.num_predictions
+        out[name] = form_line(n_pred, sec, st)
+        raw[name] = plan.fetch()
+        res[name] = plan.result()
+        plan.close()
+    out["predictions"] = n_pred
+    out["ratio_mixed_vs_plain"] = out["mixed"]["predictions_per_s"] / out["plain"]["predictions_per_s"]
+    out["note"] = ("k = all takes the dense distance route in both forms: there is no candidate list to "
+                   "screen, so the forms run the same FP64 kernels")
+    out["predictions_differing_bits_mixed_vs_plain"] = bits_equal(raw["mixed"][0], raw["plain"][0])
+    k, Ea = 2498, 4
+    st = [out["mixed"]["select_ms"], out["mixed"]["merge_ms"], out["mixed"]["smap_ms"]]
+    out["roofline"] = stage_roofline(st, {
+        "select": ("dense_dist_f64<8>", 3.0 * Ea * 2498 * 2499, "fp64 VALU", FP64_PEAK_TFLOPS, "valu"),
+        "smap": ("smap_rows<8,0> (Gram + solve)", smap_flops(k, Ea) * n_pred, "fp64 VALU", FP64_PEAK_TFLOPS, "valu")})
+    out["roofline"]["unit_of_work"] = SMAP_UNIT
+    if oracle is not None:
+        dt, times, b = median_time(lambda: oracle.edm(t, x, numThreads=cores, **kw))
+        out["cpu_baseline"] = {"value": n_pred / dt, "unit": "predictions/s", "cores": cores, "kind": "port",
+                               "runs_s": [round(v, 3) for v in times],
+                               "sample": f"the whole command ({n_pred} predictions), median of 3"}
+        a_, b_ = res["mixed"]["predictions"], b["predictions"]
+        ok = ~np.isnan(b_)
+        out["max_abs_d_yhat_mixed_vs_oracle"] = float(np.max(np.abs(a_[ok] - b_[ok])))
+        out["max_rel_d_yhat_mixed_vs_oracle"] = float(np.max(np.abs(a_[ok] - b_[ok]) / np.abs(b_[ok])))
+        out["max_abs_d_rho_mixed_vs_oracle"] = float(np.nanmax(np.abs(res["mixed"]["stats"]["rho"] -
+                                                                      b["stats"]["rho"])))
+    return out
+
+
+def cfg4_leg(fe, oracle, cores, steps):
+    """S-map on 512 logistic-map panels of lengths 200..2000 with 5 % missing values, dt on, E = 8 (E_a = 16),
+    k = 18, thetas 0, 1, 3: one task of 265 665 library rows and 265 665 targets (796 995 predictions),
+    through the plan's resident task route."""
     from tests import configs
     t, x, panel = configs.cfg4_panels()
-    kw = dict(panel=panel, E=8, dt=True, theta=[0, 1, 3], algorithm="smap", savePredictions=True)
-    out = {"workload": "panel_smap_512_logistic_panels_E8_dt_k18_3thetas", "predictions": None}
-    res = {}
-    for name, prec in (("mixed", fe.MIXED), ("plain", fe.PLAIN)):
-        t0 = time.perf_counter()
-        r = fe.edm(t, x, precision=prec, **kw)
-        wall = time.perf_counter() - t0
-        st = fe.timings()
-        res[name] = r
-        out[name] = {"device_ms": float(st[3]), "select_ms": float(st[0]), "merge_ms": float(st[1]),
-                     "smap_ms": float(st[2]), "wall_s": wall}
-    out["predictions"] = int(np.sum(~np.isnan(res["plain"]["predictions"]))) * 3
+    kw = dict(panel=panel, E=8, dt=True, theta=[0, 1, 3], algorithm="smap")
+    out = {"workload": "panel_smap_512_logistic_panels_E8_dt_k18_3thetas"}
+    raw = {}
+    for name, prec, st_ in (("mixed", fe.MIXED, steps), ("plain", fe.PLAIN, max(1, steps // 2))):
+        plan = fe.Plan(t, x, precision=prec, **kw)
+        sec, st = timed_runs(plan, fe, st_, 1)
+        n_pred = plan.num_predictions
+        out[name] = form_line(n_pred, sec, st)
+        out[name]["steps"] = st_
+        raw[name] = plan.fetch()
+        plan.close()
+    out["predictions"] = n_pred
+    out["ratio_mixed_vs_plain"] = out["mixed"]["predictions_per_s"] / out["plain"]["predictions_per_s"]
     out["device_ratio_mixed_vs_plain"] = out["plain"]["device_ms"] / out["mixed"]["device_ms"]
-    a, b = res["mixed"]["predictions"], res["plain"]["predictions"]
-    ok = ~np.isnan(b)
-    out["max_abs_d_yhat_mixed_vs_plain"] = float(np.max(np.abs(a[ok] - b[ok]))) if ok.any() else 0.0
+    out["predictions_differing_bits_mixed_vs_plain"] = bits_equal(raw["mixed"][0], raw["plain"][0])
+    out["rc_and_k_used_equal_mixed_vs_plain"] = bool(np.array_equal(raw["mixed"][1], raw["plain"][1]) and
+                                                     np.array_equal(raw["mixed"][2], raw["plain"][2]))
+    Ea, k, NL = 16, 18, n_pred // 3
+    st = [out["mixed"]["select_ms"], out["mixed"]["merge_ms"], out["mixed"]["smap_ms"]]
+    out["roofline"] = stage_roofline(st, {
+        "select": ("knn_mfma<32,16> (FP32 matrix cores, FP64 re-rank)", 3.0 * Ea * float(NL) * NL,
+                   "fp32 MFMA", FP32_PEAK_TFLOPS, "mfma"),
+        "smap": ("smap_rows<16,1> (Gram + Eigen-order Jacobi SVD)", smap_flops(k, Ea) * n_pred, "fp64 VALU",
+                 FP64_PEAK_TFLOPS, "valu")})
+    out["roofline"]["unit_of_work"] = SMAP_UNIT
+    if oracle is not None:
+        # the oracle's own manifold of the command; a prefix of the targets against the full library
+        c, keep, _ = oracle.make_command(t, x, **kw)
+        n = len(t)
+        usable = np.zeros(n, np.int32)
+        nu = oracle.lib().orc_usable(oracle.C.byref(c), 8, 0, oracle.iptr(usable))
+        rows = np.nonzero(usable)[0]
+
+        def manifold(sel, ps):
+            f = np.zeros(n, np.int32)
+            f[sel] = 1
+            o_ = np.zeros(n * Ea)
+            tg = np.zeros(n)
+            m = oracle.lib().orc_manifold(oracle.C.byref(c), 8, oracle.iptr(f), int(ps), 0, oracle.dptr(o_),
+                                          oracle.dptr(tg), n)
+            return o_[:m * Ea].reshape(m, Ea), tg[:m]
+
+        M, Mt = manifold(rows[:nu // 2], False)
+        Mp, _ = manifold(rows[nu // 2:], True)
+        m = 60
+        pi = panel.astype(np.int32)
+        Md = {"data": M, "targets": Mt, "panel": pi[rows[:nu // 2]]}
+        Mpd = {"data": Mp[:m], "panel": pi[rows[nu // 2:]][:m]}
+        opts = {"algorithm": 1, "k": k, "thetas": [0.0, 1.0, 3.0], "panel_mode": 1, "nthreads": cores}
+        dt, times, r = median_time(lambda: oracle.predict(Md, Mpd, opts))
+        out["cpu_baseline"] = {"value": 3 * m / dt, "unit": "predictions/s", "cores": cores, "kind": "port",
+                               "runs_s": [round(v, 3) for v in times],
+                               "sample": f"the first {m} targets x 3 thetas against the full library (manifold "
+                                         f"build not timed), median of 3"}
+        got = raw["mixed"][0].reshape(3, -1)[:, :m]
+        ref = r[0]
+        ok = ref != 1e100
+        out["max_abs_d_yhat_mixed_vs_oracle_sample"] = float(np.max(np.abs(got[ok] - ref[ok])))
+        out["max_rel_d_yhat_mixed_vs_oracle_sample"] = float(np.max(np.abs(got[ok] - ref[ok]) / np.abs(ref[ok])))
     return out
+
+
+def cfg5_leg(fe, oracle, cores, steps):
+    """S-map on Lorenz-96 (N = 40, F = 8, RK4 dt = 0.01, 100 000 steps): E = 10 lags of x_0 plus 30
+    unlagged coordinates (E_a = 40), theta 2, k = 42, 80/20 rows: one task of 79 992 library rows and
+    19 998 targets through a resident batch."""
+    from tests import configs
+    series = configs.lorenz96()
+    lib, pred = configs.cfg5_manifold(series)
+    nl, npd = lib["data"].shape[0], pred["data"].shape[0]
+    table = {key: np.concatenate([lib[key], pred[key]]) for key in ("data", "targets")}
+    table.update(E_x=10, E_extras=30)
+    opts = {"algorithm": 1, "k": 42, "thetas": [2.0]}
+    out = {"workload": "lorenz96_N40_smap_E10_plus30extras_k42_theta2"}
+    raw = {}
+    for name, prec in (("mixed", fe.MIXED), ("plain", fe.PLAIN)):
+        b = fe.Batch(table, table, dict(opts, precision=prec), [np.arange(nl)], [np.arange(nl, nl + npd)], [42])
+        assert b.route == 1
+        sec, st = timed_runs(b, fe, steps, 1)
+        out[name] = form_line(npd, sec, st)
+        raw[name] = b.fetch()[0]
+        b.close()
+    out["predictions"] = npd
+    out["ratio_mixed_vs_plain"] = out["mixed"]["predictions_per_s"] / out["plain"]["predictions_per_s"]
+    out["predictions_differing_bits_mixed_vs_plain"] = bits_equal(raw["mixed"][0], raw["plain"][0])
+    Ea, k = 40, 42
+    st = [out["mixed"]["select_ms"], out["mixed"]["merge_ms"], out["mixed"]["smap_ms"]]
+    out["roofline"] = stage_roofline(st, {
+        "select": ("knn_mfma<64,64> (FP32 matrix cores, FP64 re-rank)", 3.0 * Ea * float(nl) * npd, "fp32 MFMA",
+                   FP32_PEAK_TFLOPS, "mfma"),
+        "smap": ("smap_hyb (Gram + Eigen-order Jacobi SVD, n = 41)", smap_flops(k, Ea) * npd, "fp64 VALU",
+                 FP64_PEAK_TFLOPS, "valu")})
+    out["roofline"]["unit_of_work"] = SMAP_UNIT
+    if oracle is not None:
+        m = 300
+        dt, times, r = median_time(lambda: oracle.predict(lib, {"data": pred["data"][:m]}, dict(opts, nthreads=cores)))
+        out["cpu_baseline"] = {"value": m / dt, "unit": "predictions/s", "cores": cores, "kind": "port",
+                               "runs_s": [round(v, 3) for v in times],
+                               "sample": f"the first {m} targets against the full library, median of 3"}
+        got = raw["mixed"][0][0, :m]
+        ref = r[0][0]
+        out["max_abs_d_yhat_mixed_vs_oracle_sample"] = float(np.max(np.abs(got - ref)))
+        out["max_rel_d_yhat_mixed_vs_oracle_sample"] = float(np.max(np.abs(got - ref) / np.abs(ref)))
+    return out
+
+
+def graph_leg(fe, t, x, y, kw, steps):
+    """The headline sweep with graph replay on: the second and later runs replay the captured launches."""
+    plan = fe.Plan(t, x, y, precision=fe.MIXED, **kw)
+    plan.run()
+    ref = plan.fetch()[0].copy()
+    plan.set_graph(True)
+    for _ in range(2):
+        plan.run()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        plan.run()
+    sec = (time.perf_counter() - t0) / steps
+    got = plan.fetch()[0]
+    plan.close()
+    return {"ms_per_step": sec * 1e3, "predictions_per_s": got.size / sec,
+            "predictions_differing_bits_vs_eager": bits_equal(got, ref)}
 
 
 def baseline_oracle():
@@ -189,12 +392,16 @@ def baseline_oracle():
         return oracle, "-O2 -ffp-contract=off (prebuilt; the native build failed)"
 
 
-def cpu_baseline(t, x, y, kw, seconds):
+def host_cores():
+    """Threads for the CPU baseline: the host's cores, capped by the box's thread allowance."""
+    return max(1, min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", "1024"))))
+
+
+def cpu_baseline(oracle, flags, t, x, y, kw, seconds):
     """The oracle's edm() (tasks serial, targets over a thread per host core, as the reference CPU path
     runs them) on the first replicates of the sweep: a sample sized to about `seconds`, timed three
     times, median reported.  The timed call includes the oracle's manifold builds and rho/MAE."""
-    oracle, flags = baseline_oracle()
-    cores = os.cpu_count() or 1
+    cores = host_cores()
     reps = 1
     libs = kw["library"]
     while True:
@@ -227,10 +434,11 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--reps", type=int, default=100, help="replicates of the CCM sweep (100 = the full workload)")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-seconds", type=float, default=8.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
-    ap.add_argument("--no-panel", action="store_true", help="skip the panel S-map line")
+    ap.add_argument("--configs", default="2,4,5",
+                    help="other workloads to measure beside the CCM sweep (comma list of 2, 4, 5; empty: none)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "the library runs on one GPU; --gpus above 1 is not supported"}))
@@ -309,13 +517,24 @@ def main():
     pb.close()
     mixed.close()
 
-    if not args.no_panel:
-        print("panel S-map line (mixed and plain)", file=sys.stderr, flush=True)
-        out["panel_smap"] = panel_smap_leg(fe)
+    print("graph replay of the sweep", file=sys.stderr, flush=True)
+    out["graph_replay"] = graph_leg(fe, t, x, y, kw, args.steps)
+
+    oracle_mod, flags = (None, None) if args.no_cpu else baseline_oracle()
+    cores = host_cores()
+    legs = {"2": ("cfg2_smap_theta_sweep", cfg2_leg), "4": ("cfg4_panel_smap", cfg4_leg),
+            "5": ("cfg5_lorenz96_smap", cfg5_leg)}
+    out["configs"] = {}
+    for key in [v for v in args.configs.split(",") if v]:
+        name, fn = legs[key]
+        print(f"{name}: mixed and plain", file=sys.stderr, flush=True)
+        out["configs"][name] = fn(fe, oracle_mod, cores, 5)
+        if oracle_mod is not None:
+            out["configs"][name]["cpu_baseline"]["build"] = flags
 
     if not args.no_cpu:
         print("timing the CPU oracle sample", file=sys.stderr, flush=True)
-        out["cpu_baseline"], res_cpu, reps = cpu_baseline(t, x, y, kw, args.cpu_seconds)
+        out["cpu_baseline"], res_cpu, reps = cpu_baseline(oracle_mod, flags, t, x, y, kw, args.cpu_seconds)
         n = len(res_cpu["stats"]["rho"])
         out["parity"]["oracle_sample_tasks"] = n
         out["parity"]["max_abs_d_rho_mixed_vs_oracle"] = float(

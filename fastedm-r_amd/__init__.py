@@ -526,6 +526,85 @@ class Plan:
             pass
 
 
+class Batch:
+    """A batch prepared for repeated device runs (edm_batch_*): tasks over shared tables L and P, task t
+    using library rows lib_lists[t] and prediction rows pred_lists[t].  run() makes every prediction
+    with the inputs resident on the GPU; fetch() returns per task (pred [T x Np], rc [T x Np],
+    k_used [Np]); route is 2 (persistent launch), 1 (resident task route) or 0 (task by task)."""
+
+    def __init__(self, L, P, opts, lib_lists, pred_lists, ks):
+        lib = load_library()
+        lib.edm_batch_create.argtypes = [C.c_void_p, C.POINTER(ManifoldView), C.POINTER(ManifoldView),
+                                         C.POINTER(Options), C.c_int, _ip, _ip, _ip, _ip, _ip, _ip, _ip, _i64p,
+                                         C.POINTER(C.c_void_p)]
+        for name in ("edm_batch_run", "edm_batch_destroy", "edm_batch_route"):
+            getattr(lib, name).argtypes = [C.c_void_p]
+        lib.edm_batch_fetch.argtypes = [C.c_void_p, _dp, _ip, _ip]
+        lib.edm_batch_fetch_coeffs.argtypes = [C.c_void_p, _dp]
+        lib.edm_batch_set_graph.argtypes = [C.c_void_p, C.c_int]
+        self._keep = []
+        keep = self._keep
+        Lv, Pv, o = manifold_view(L, keep), manifold_view(P, keep), options(opts, keep)
+        self.T = int(np.atleast_1d(opts["thetas"]).size)
+        self.Ea = int(np.shape(L["data"])[1])
+        self.counts = [len(a) for a in pred_lists]
+        lb = np.cumsum([0] + [len(a) for a in lib_lists[:-1]]).astype(np.int32)
+        lc = np.array([len(a) for a in lib_lists], np.int32)
+        lr = np.concatenate([np.asarray(a, np.int32) for a in lib_lists])
+        pb = np.cumsum([0] + self.counts[:-1]).astype(np.int32)
+        pc = np.array(self.counts, np.int32)
+        pr = np.concatenate([np.asarray(a, np.int32) for a in pred_lists])
+        self.ob = np.cumsum([0] + [self.T * n for n in self.counts[:-1]]).astype(np.int64)
+        self.total = int(self.T * pc.sum())
+        k = np.asarray(ks, np.int32)
+        keep += [lb, lc, lr, pb, pc, pr, k, self.ob]
+        self._h = C.c_void_p()
+        status = lib.edm_batch_create(context(), C.byref(Lv), C.byref(Pv), C.byref(o), len(lib_lists),
+                                      lb.ctypes.data_as(_ip), lc.ctypes.data_as(_ip), lr.ctypes.data_as(_ip),
+                                      pb.ctypes.data_as(_ip), pc.ctypes.data_as(_ip), pr.ctypes.data_as(_ip),
+                                      k.ctypes.data_as(_ip), self.ob.ctypes.data_as(_i64p), C.byref(self._h))
+        if status != 0:
+            raise FastEDMError(f"edm_batch_create failed ({status})")
+
+    @property
+    def route(self):
+        return int(load_library().edm_batch_route(self._h))
+
+    def set_graph(self, on=True):
+        return int(load_library().edm_batch_set_graph(self._h, int(bool(on))))
+
+    def run(self):
+        status = load_library().edm_batch_run(self._h)
+        if status != 0:
+            raise FastEDMError(f"edm_batch_run failed ({status})")
+
+    def fetch(self):
+        pred = np.zeros(max(self.total, 1))
+        rc = np.zeros(max(self.total, 1), np.int32)
+        ku = np.zeros(max(self.total // self.T, 1), np.int32)
+        status = load_library().edm_batch_fetch(self._h, pred.ctypes.data_as(_dp), rc.ctypes.data_as(_ip),
+                                                ku.ctypes.data_as(_ip))
+        if status != 0:
+            raise FastEDMError(f"edm_batch_fetch failed ({status})")
+        out = []
+        for t, n in enumerate(self.counts):
+            o0 = int(self.ob[t])
+            out.append((pred[o0:o0 + self.T * n].reshape(self.T, n), rc[o0:o0 + self.T * n].reshape(self.T, n),
+                        ku[o0 // self.T:o0 // self.T + n]))
+        return out
+
+    def close(self):
+        if self._h:
+            load_library().edm_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def predict_batch(L, P, opts, lib_lists, pred_lists, ks):
     """edm_predict_batch over shared tables L and P: task t uses library rows lib_lists[t] and prediction
     rows pred_lists[t].  Returns per task (pred [T x Np], rc [T x Np], k_used [Np])."""

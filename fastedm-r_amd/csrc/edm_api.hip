@@ -1422,10 +1422,15 @@ static int run_resident(edm_batch* b)
     }
     HIP_TRY(hipGraphLaunch(b->graph_exec, s));
   } else {
+    int active = 0;
+    for (int tk = 0; tk < b->tasks; tk++)
+      active += b->plans[tk].NP > 0;
+    // a single task records its stage events, as edm_predict does
+    hipEvent_t* ev = active == 1 ? c->ev : nullptr;
     for (int tk = 0; tk < b->tasks; tk++) {
       if (b->plans[tk].NP == 0)
         continue;
-      const int r = launch_task(b->plans[tk], s, nullptr, nullptr);
+      const int r = launch_task(b->plans[tk], s, ev, nullptr);
       if (r != 0)
         return r;
       if (b->keep_going) {
@@ -1440,8 +1445,18 @@ static int run_resident(edm_batch* b)
   HIP_TRY(hipEventSynchronize(c->ev[5]));
   b->runs++;
   b->last_ms = ev_ms(c->ev[4], c->ev[5]);
-  c->timings[0] = b->last_ms;
-  c->timings[1] = c->timings[2] = 0.0;
+  int active = 0;
+  for (int tk = 0; tk < b->tasks; tk++)
+    active += b->plans[tk].NP > 0;
+  if (active == 1 && !(b->use_graph && b->graph_exec)) {
+    // [0] distance/selection, [1] merge, [2] S-map, [3] whole run (edm_predict's layout)
+    c->timings[0] = ev_ms(c->ev[0], c->ev[1]);
+    c->timings[1] = ev_ms(c->ev[1], c->ev[2]);
+    c->timings[2] = ev_ms(c->ev[2], c->ev[3]);
+  } else {
+    c->timings[0] = b->last_ms;
+    c->timings[1] = c->timings[2] = 0.0;
+  }
   c->timings[3] = b->last_ms;
   c->last_redo = 0;
   return 0;
