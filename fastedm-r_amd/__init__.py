@@ -29,6 +29,15 @@ class ManifoldView(C.Structure):
                 ("E_lagged_extras", C.c_int), ("E_actual", C.c_int)]
 
 
+class Knobs(C.Structure):
+    """fedm_knobs: launch and route choices that leave every result unchanged."""
+    _fields_ = [("rank_scale", C.c_double), ("direct_blocks_per_cu", C.c_int), ("direct_alt_waves", C.c_int),
+                ("list_blocks", C.c_int), ("mfma_blocks", C.c_int), ("use_mfma", C.c_int), ("smap_lds", C.c_int)]
+
+    def as_dict(self):
+        return {f[0]: getattr(self, f[0]) for f in self._fields_}
+
+
 class Options(C.Structure):
     """edm_options"""
     _fields_ = [("algorithm", C.c_int), ("distance", C.c_int), ("k", C.c_int), ("num_thetas", C.c_int),
@@ -61,7 +70,8 @@ EXPORTS = ["edm_ctx_create", "edm_ctx_destroy", "edm_predict", "edm_predict_batc
            "fedm_plan_run", "fedm_plan_result", "fedm_plan_num_predictions", "fedm_plan_device_batches", "fedm_plan_destroy",
            "fedm_plan_num_tasks", "fedm_plan_task", "fedm_plan_fetch", "edm_batch_set_keep_going",
            "edm_batch_route", "edm_batch_set_graph", "edm_batch_fetch_coeffs", "fedm_plan_set_graph",
-           "fedm_plan_resident_batches", "edm_predict_noalloc", "edm_selftest_exp", "edm_batch_on_device", "edm_selftest_mfma", "fedm_rank_threshold", "fedm_run_command", "fedm_run_task_group", "fedm_result_rc", "fedm_result_num_stats", "fedm_result_stats", "fedm_result_k",
+           "fedm_plan_resident_batches", "edm_predict_noalloc", "edm_selftest_exp", "fedm_default_knobs",
+           "edm_batch_create_knobs", "edm_batch_knobs", "fedm_plan_tune", "fedm_plan_group_knobs", "edm_batch_on_device", "edm_selftest_mfma", "fedm_rank_threshold", "fedm_run_command", "fedm_run_task_group", "fedm_result_rc", "fedm_result_num_stats", "fedm_result_stats", "fedm_result_k",
            "fedm_result_scalar", "fedm_result_matrix_shape", "fedm_result_matrix", "fedm_result_num_manifolds",
            "fedm_result_manifold_shape", "fedm_result_manifold", "fedm_result_free"]
 
@@ -445,6 +455,24 @@ class Plan:
     @property
     def num_predictions(self):
         return int(load_library().fedm_plan_num_predictions(self._h))
+
+    def tune(self, trials=3, report_len=1 << 16):
+        """The autotuner over every group's knobs (fedm_plan_tune): returns (groups tuned, report)."""
+        L = load_library()
+        L.fedm_plan_tune.argtypes = [C.c_void_p, C.c_int, C.c_char_p, C.c_int]
+        buf = C.create_string_buffer(report_len)
+        n = L.fedm_plan_tune(self._h, int(trials), buf, report_len)
+        if n < 0:
+            raise FastEDMError(f"fedm_plan_tune failed ({n})")
+        return n, buf.value.decode()
+
+    def knobs(self, group=0):
+        L = load_library()
+        L.fedm_plan_group_knobs.argtypes = [C.c_void_p, C.c_int, C.POINTER(Knobs)]
+        k = Knobs()
+        if L.fedm_plan_group_knobs(self._h, int(group), C.byref(k)) != 0:
+            return None
+        return k.as_dict()
 
     def set_graph(self, on=True):
         """Replays each group's launches as one captured hipGraph from the second run on; returns the

@@ -38,6 +38,11 @@ struct BatchParams
   int redo_cap;
 };
 
+constexpr int batch_waves(int kcap, int emax) { return kcap <= 4 && emax <= 4 ? 6 : 1; }
+constexpr int batch_waves_alt(int kcap, int emax) { return kcap <= 4 && emax <= 4 ? 5 : 2; }
+template <int KCAP, int EMAX, int WPE>
+__global__ void batch_simplex(BatchParams B);
+
 constexpr int RANK_STORE = 1024; // longest stored prefix
 constexpr int RANK_ZCAP = 8;     // rows at distance 0 kept per target
 constexpr int MS_WAVES = 1;      // wavefronts per block of rank_scan_multi (its LDS, 8 KiB a wavefront, sets occupancy)

@@ -254,12 +254,27 @@ static void launch_finish_lists(int kcap, dim3 g, hipStream_t s, const FinishPar
   }
 }
 
-// FEDM_SMAP_LDS_ROWS=1 keeps U and V of the S-map solve in LDS for every width (the kernel used above
-// 24 unknowns); the default keeps rows in registers where they fit.  Both give the same bits.
-static bool smap_lds_rows()
+extern "C" void fedm_default_knobs(fedm_knobs* k)
 {
-  const char* env = getenv("FEDM_SMAP_LDS_ROWS");
-  return env && env[0] == '1';
+  k->rank_scale = 1.5;
+  if (const char* e = getenv("FEDM_RANK_LTHR_SCALE"))
+    k->rank_scale = atof(e);
+  k->direct_blocks_per_cu = 0;
+  k->direct_alt_waves = 0;
+  k->list_blocks = 1024;
+  k->mfma_blocks = 2048;
+  k->use_mfma = 1;
+  // FEDM_SMAP_LDS_ROWS=1 keeps U and V of the S-map solve in LDS for every width (the kernel used above
+  // 24 unknowns); the default keeps rows in registers where they fit.  Both give the same bits.
+  const char* e = getenv("FEDM_SMAP_LDS_ROWS");
+  k->smap_lds = e && e[0] == '1';
+}
+
+static fedm_knobs default_knobs()
+{
+  fedm_knobs k;
+  fedm_default_knobs(&k);
+  return k;
 }
 
 // Scratch of edm_predict_noalloc: the kernels' slice and neighbour lists, counts and dense distance
@@ -422,6 +437,7 @@ struct TaskPlan
   bool wass = false, smap = false, generic = false, wide = false;
   int kcap = 0, kp = 0, S = 1, sl = 0, bx = 0, nt = 0, gstride = 0;
   int wE = 0, wE_dt = 0, wrows = 0, wunl = 0; // Wasserstein layout
+  bool smap_lds = false;                      // S-map solve with U and V in LDS
   size_t bytes[B_NUM] = {};
   void* ptr[B_NUM] = {};
   // host copies the upload needs beyond the views
@@ -439,8 +455,12 @@ struct TaskPlan
 // option this build does not take; the caller fills the outputs).  `mfma_ok` allows the matrix-core
 // route (it needs the host rows); `max_slices` caps the library slices of the list routes.
 static int plan_task(const edm_manifold_view* M, const edm_manifold_view* Mp, const edm_options* o, bool want_coeffs,
-                     int nn_cap, bool mfma_ok, int max_slices, TaskPlan& tp, size_t dense_budget = (size_t)1 << 30)
+                     int nn_cap, bool mfma_ok, int max_slices, TaskPlan& tp, size_t dense_budget = (size_t)1 << 30,
+                     const fedm_knobs* kn = nullptr)
 {
+  const fedm_knobs knobs = kn ? *kn : default_knobs();
+  mfma_ok = mfma_ok && knobs.use_mfma;
+  tp.smap_lds = knobs.smap_lds != 0;
   const int NL = M->num_points, NP = Mp->num_points, Ea = M->E_actual, T = o->num_thetas;
   if (Ea != Mp->E_actual || T <= 0 || Ea <= 0 || NL < 0 || NP < 0)
     return -22;
@@ -539,7 +559,7 @@ static int plan_task(const edm_manifold_view* M, const edm_manifold_view* Mp, co
     tp.kcap = kcap_for(k);
     tp.kp = Ea <= 16 ? 16 : Ea <= 32 ? 32 : 64;
     tp.bx = (NP + 63) / 64;
-    int S = std::max(1, std::min(MAX_SLICES, (2048 + tp.bx - 1) / tp.bx));
+    int S = std::max(1, std::min(MAX_SLICES, (std::max(knobs.mfma_blocks, 1) + tp.bx - 1) / tp.bx));
     S = std::max(1, std::min(S, NL / 256));
     tp.S = std::max(1, std::min(S, max_slices));
     tp.sl = (NL + tp.S - 1) / tp.S;
@@ -608,7 +628,7 @@ static int plan_task(const edm_manifold_view* M, const edm_manifold_view* Mp, co
     tp.route = R_LIST;
     tp.kcap = kcap_for(k);
     tp.bx = (NP + 255) / 256;
-    int S = std::max(1, std::min(MAX_SLICES, (1024 + tp.bx - 1) / tp.bx));
+    int S = std::max(1, std::min(MAX_SLICES, (std::max(knobs.list_blocks, 1) + tp.bx - 1) / tp.bx));
     S = std::max(1, std::min(S, NL / 256));
     tp.S = std::max(1, std::min(S, max_slices));
     tp.sl = (NL + tp.S - 1) / tp.S;
@@ -824,7 +844,7 @@ static int launch_task(TaskPlan& tp, hipStream_t s, hipEvent_t* ev, int (*keep_g
     if (ev)
       HIP_TRY(hipEventRecord(ev[2], s));
     if (tp.smap) {
-      const int r = smap_launch(tp.n, s, Sm, nullptr, 0, NP, smap_lds_rows());
+      const int r = smap_launch(tp.n, s, Sm, nullptr, 0, NP, tp.smap_lds);
       if (r)
         return r;
       HIP_TRY(hipGetLastError());
@@ -853,7 +873,7 @@ static int launch_task(TaskPlan& tp, hipStream_t s, hipEvent_t* ev, int (*keep_g
     }
     Sm.nn_t0 = t0;
     if (tp.smap) {
-      const int r = smap_launch(tp.n, s, Sm, D, t0, cnt, smap_lds_rows());
+      const int r = smap_launch(tp.n, s, Sm, D, t0, cnt, tp.smap_lds);
       if (r)
         return r;
       HIP_TRY(hipGetLastError());
@@ -1002,8 +1022,6 @@ int edm_predict_noalloc(edm_ctx* c, const edm_manifold_view* M, const edm_manifo
 // Batches of tasks over shared point tables.
 
 namespace fedm {
-template <int KCAP, int EMAX>
-__global__ void batch_simplex(BatchParams B);
 template <int KCAP>
 __global__ void batch_redo(BatchParams B, const int2* items, int num_items);
 
@@ -1050,6 +1068,7 @@ struct edm_batch
   // resident task route: every task planned, gathered and uploaded once at create, its kernels
   // launched in turn on each run (or replayed as one captured graph)
   bool resident = false;
+  fedm_knobs kn;             // the launch and route choices this batch was built with
   std::vector<TaskPlan> plans;
   double* coeffs = nullptr;  // [total_k x (Ea+1)] when S-map coefficients are saved
   bool use_graph = false;    // capture the steady-state run into a hipGraph and replay it
@@ -1081,16 +1100,22 @@ struct edm_batch
   }
 };
 
+template <int KCAP, int EMAX>
+static const void* batch_kernel_ke(bool alt)
+{
+  return alt ? (const void*)batch_simplex<KCAP, EMAX, batch_waves_alt(KCAP, EMAX)>
+             : (const void*)batch_simplex<KCAP, EMAX, batch_waves(KCAP, EMAX)>;
+}
 template <int KCAP>
-static const void* batch_kernel_k(int demax)
+static const void* batch_kernel_k(int demax, bool alt)
 {
   switch (demax) {
-    case 2: return (const void*)batch_simplex<KCAP, 2>;
-    case 3: return (const void*)batch_simplex<KCAP, 3>;
-    case 4: return (const void*)batch_simplex<KCAP, 4>;
-    case 8: return (const void*)batch_simplex<KCAP, 8>;
-    case 16: return (const void*)batch_simplex<KCAP, 16>;
-    default: return (const void*)batch_simplex<KCAP, 32>;
+    case 2: return batch_kernel_ke<KCAP, 2>(alt);
+    case 3: return batch_kernel_ke<KCAP, 3>(alt);
+    case 4: return batch_kernel_ke<KCAP, 4>(alt);
+    case 8: return batch_kernel_ke<KCAP, 8>(alt);
+    case 16: return batch_kernel_ke<KCAP, 16>(alt);
+    default: return batch_kernel_ke<KCAP, 32>(alt);
   }
 }
 template <int KCAP>
@@ -1122,23 +1147,20 @@ static int direct_emax(int Ea)
 {
   return Ea <= 2 ? 2 : Ea == 3 ? 3 : Ea <= 4 ? 4 : Ea <= 8 ? 8 : Ea <= 16 ? 16 : 32;
 }
-static const void* batch_kernel(int kcap, int Ea)
+static const void* batch_kernel(int kcap, int Ea, bool alt)
 {
   switch (kcap) {
-    case 4: return batch_kernel_k<4>(direct_emax(Ea));
-    case 8: return batch_kernel_k<8>(direct_emax(Ea));
-    case 16: return batch_kernel_k<16>(direct_emax(Ea));
-    default: return batch_kernel_k<32>(direct_emax(Ea));
+    case 4: return batch_kernel_k<4>(direct_emax(Ea), alt);
+    case 8: return batch_kernel_k<8>(direct_emax(Ea), alt);
+    case 16: return batch_kernel_k<16>(direct_emax(Ea), alt);
+    default: return batch_kernel_k<32>(direct_emax(Ea), alt);
   }
 }
 
 // Smallest library that takes the rank route: scanning a ranked prefix costs about k * pool / L
-// entries per target, the direct scan L pairs; FEDM_RANK_LTHR_SCALE overrides the scale (tuning).
-static int rank_threshold(int kmax, int npool)
+// entries per target, the direct scan L pairs; the scale is a knob (default 1.5, FEDM_RANK_LTHR_SCALE).
+static int rank_threshold(int kmax, int npool, double scale)
 {
-  double scale = 1.5;
-  if (const char* e = getenv("FEDM_RANK_LTHR_SCALE"))
-    scale = atof(e);
   return (int)std::max(scale * std::sqrt((double)std::max(kmax, 1) * npool), 2.0 * (kmax + RANK_ZCAP));
 }
 
@@ -1185,7 +1207,7 @@ extern "C" int edm_selftest_exp(const double* x, double* out, int n)
 
 extern "C" int fedm_rank_threshold(int kmax, int npool)
 {
-  return rank_threshold(kmax, npool);
+  return rank_threshold(kmax, npool, default_knobs().rank_scale);
 }
 
 static const void* rank_build_kernel(int emax)
@@ -1323,7 +1345,7 @@ static int build_resident(edm_batch* b)
       continue;
     gather_task(b, tk, g);
     const edm_options to = task_options(b, tk);
-    int r = plan_task(&g.M, &g.Mp, &to, coeffs, 0, true, MAX_SLICES, tp);
+    int r = plan_task(&g.M, &g.Mp, &to, coeffs, 0, true, MAX_SLICES, tp, (size_t)1 << 30, &b->kn);
     if (r != 0)
       return r == -95 ? 1 : r;
     const int64_t ob = b->ob[tk];
@@ -1469,12 +1491,27 @@ int edm_batch_create(edm_ctx* c, const edm_manifold_view* L, const edm_manifold_
                      const int* pred_begin, const int* pred_count, const int* pred_rows, const int* k,
                      const int64_t* out_begin, edm_batch** out)
 {
+  return edm_batch_create_knobs(c, L, P, o, num_tasks, lib_begin, lib_count, lib_rows, pred_begin, pred_count,
+                                pred_rows, k, out_begin, nullptr, out);
+}
+
+void edm_batch_knobs(const edm_batch* b, fedm_knobs* out)
+{
+  *out = b->kn;
+}
+
+int edm_batch_create_knobs(edm_ctx* c, const edm_manifold_view* L, const edm_manifold_view* P, const edm_options* o,
+                           int num_tasks, const int* lib_begin, const int* lib_count, const int* lib_rows,
+                           const int* pred_begin, const int* pred_count, const int* pred_rows, const int* k,
+                           const int64_t* out_begin, const fedm_knobs* knobs, edm_batch** out)
+{
   *out = nullptr;
   if (!c || num_tasks < 0 || L->E_actual != P->E_actual || o->num_thetas <= 0)
     return -22;
   hipSetDevice(c->device);
   auto b = new edm_batch();
   b->ctx = c;
+  b->kn = knobs ? *knobs : default_knobs();
   b->tasks = num_tasks;
   b->T = o->num_thetas;
   b->Ea = L->E_actual;
@@ -1596,7 +1633,7 @@ int edm_batch_create(edm_ctx* c, const edm_manifold_view* L, const edm_manifold_
       if (inpool[r])
         pool.push_back(r);
     const double npool = (double)pool.size();
-    const int lthr = rank_threshold(kmax, (int)pool.size());
+    const int lthr = rank_threshold(kmax, (int)pool.size(), b->kn.rank_scale);
     double work = 0;
     int nrank = 0;
     // library positions must follow table rows (ties between equal distances go by position)
@@ -1819,7 +1856,9 @@ int edm_batch_create(edm_ctx* c, const edm_manifold_view* L, const edm_manifold_
   int dev = 0, ncu = 0, per = 0;
   hipGetDevice(&dev);
   hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
-  hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, batch_kernel(kcap, Ea), 256, 0);
+  hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, batch_kernel(kcap, Ea, b->kn.direct_alt_waves != 0), 256, 0);
+  if (b->kn.direct_blocks_per_cu > 0)
+    per = std::min(std::max(per, 1), b->kn.direct_blocks_per_cu);
   b->grid = std::max(1, std::min(B.num_items, ncu * std::max(per, 1)));
   if (b->rank_route) {
     int sper = 0;
@@ -1905,7 +1944,7 @@ static int enqueue_device(edm_batch* b, bool timed)
     HIP_TRY(hipEventRecord(c->ev[7], s));
   if (stop())
     return batch_break(b);
-  const void* fn = batch_kernel(b->kcap, b->Ea);
+  const void* fn = batch_kernel(b->kcap, b->Ea, b->kn.direct_alt_waves != 0);
   void* args[] = { &B };
   if (B.num_items > 0)
     HIP_TRY(hipLaunchKernel(fn, dim3(b->grid), dim3(256), args, 0, s));

@@ -8,6 +8,10 @@
 //  * tasks that share E and prediction rows are dispatched with one device context and reused
 //    device buffers.
 #include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <string>
+#include <sys/stat.h>
 #include <climits>
 #include <cmath>
 #include <cstdint>
@@ -552,7 +556,12 @@ struct fedm_plan
     std::vector<double> pred;
     std::vector<int> rc, ku;
     std::vector<double> coeffs; // per task [Np x C] (S-map coefficient route)
+    // the batch's arguments, kept for the autotuner's rebuilds
+    std::vector<int> lb, lc, lr, pb, pc, pr, kk, metrics;
+    edm_options go;
+    std::string tune_key;       // workload class in the tuning cache
   };
+  bool graph = false;
   std::vector<Group> groups;
   ~fedm_plan()
   {
@@ -572,6 +581,9 @@ struct TaskGrid
   int seed;
 };
 static int plan_tasks(edm_ctx* ctx, fedm_plan* P, const TaskGrid& grid);
+static std::string tune_key(const fedm_plan* P, const fedm_plan::Group& g);
+static bool tune_cache_lookup(const std::string& key, fedm_knobs* k);
+static int create_group_batch(fedm_plan* P, fedm_plan::Group& g, const fedm_knobs* kn, edm_batch** out);
 
 static int plan_command(edm_ctx* ctx, const fedm_command* c, fedm_plan* P)
 {
@@ -823,7 +835,7 @@ static int plan_tasks(edm_ctx* ctx, fedm_plan* P, const TaskGrid& grid)
   const int T = o.num_thetas;
   for (auto& g : P->groups) {
     const int Ea = s.E_actual(g.E);
-    std::vector<int> lb, lc, lr, pb, pc, pr, kk;
+    std::vector<int>&lb = g.lb, &lc = g.lc, &lr = g.lr, &pb = g.pb, &pc = g.pc, &pr = g.pr, &kk = g.kk;
     std::vector<int> mask_begin(P->pred_rows.size(), -1);
     for (int ti : g.members) {
       const auto& tk = P->tasks[ti];
@@ -841,7 +853,8 @@ static int plan_tasks(edm_ctx* ctx, fedm_plan* P, const TaskGrid& grid)
       g.total_out += (int64_t)T * P->pred_rows[tk.mask].size();
     }
     // expanded metrics (src/distances.cpp:29-67): x, dt and lagged coordinates Diff, each extra its own
-    std::vector<int> metrics(Ea, 0);
+    std::vector<int>& metrics = g.metrics;
+    metrics.assign(Ea, 0);
     if (!P->extra_metrics.empty()) {
       if (o.distance == FEDM_DIST_WASSERSTEIN) {
         const int rows = 1 + (s.E_dt(g.E) > 0);
@@ -852,24 +865,313 @@ static int plan_tasks(edm_ctx* ctx, fedm_plan* P, const TaskGrid& grid)
           metrics[g.E + s.E_dt(g.E) + e] = P->extra_metrics[e];
       }
     }
-    edm_options go = o;
+    edm_options& go = g.go;
+    go = o;
     go.metrics = metrics.data();
-    edm_manifold_view Lv{ g.table.data(), g.targets.data(), g.ptimes.data(), g.ttimes.data(),
-                          g.panel.empty() ? nullptr : g.panel.data(), n, g.E, s.E_dt(g.E), s.E_extras(), 0, Ea };
-    edm_manifold_view Pv{ g.ptable.data(), g.targets.data(), g.ptimes.data(), g.ttimes.data(),
-                          g.panel.empty() ? nullptr : g.panel.data(), n, g.E, s.E_dt(g.E), s.E_extras(), 0, Ea };
     const bool want_coeffs = !g.co && P->save_smap_coeffs && o.algorithm == FEDM_SMAP;
     go.save_smap_coeffs = want_coeffs;
     g.want_coeffs = want_coeffs;
     g.taskwise = o.precision == FEDM_PLAIN_TASKWISE;
     if (g.taskwise)
       continue;
-    const int r = edm_batch_create(ctx, &Lv, &Pv, &go, (int)g.members.size(), lb.data(), lc.data(), lr.data(),
-                                   pb.data(), pc.data(), pr.data(), kk.data(), g.out_begin.data(), &g.batch);
+    g.tune_key = tune_key(P, g);
+    fedm_knobs kn;
+    const bool cached = tune_cache_lookup(g.tune_key, &kn);
+    const int r = create_group_batch(P, g, cached ? &kn : nullptr, &g.batch);
     if (r != 0)
       return r;
   }
   return 0;
+}
+
+static int create_group_batch(fedm_plan* P, fedm_plan::Group& g, const fedm_knobs* kn, edm_batch** out)
+{
+  const Series& s = P->s;
+  const int n = s.n(), Ea = s.E_actual(g.E);
+  edm_manifold_view Lv{ g.table.data(), g.targets.data(), g.ptimes.data(), g.ttimes.data(),
+                        g.panel.empty() ? nullptr : g.panel.data(), n, g.E, s.E_dt(g.E), s.E_extras(), 0, Ea };
+  edm_manifold_view Pv{ g.ptable.data(), g.targets.data(), g.ptimes.data(), g.ttimes.data(),
+                        g.panel.empty() ? nullptr : g.panel.data(), n, g.E, s.E_dt(g.E), s.E_extras(), 0, Ea };
+  return edm_batch_create_knobs(P->ctx, &Lv, &Pv, &g.go, (int)g.members.size(), g.lb.data(), g.lc.data(),
+                                g.lr.data(), g.pb.data(), g.pc.data(), g.pr.data(), g.kk.data(), g.out_begin.data(),
+                                kn, out);
+}
+
+// ------------------------------------------------------------------ autotuner and its cache
+
+static int log2_bucket(long long v)
+{
+  int b = 0;
+  while (v > 1) {
+    v >>= 1;
+    b++;
+  }
+  return b;
+}
+
+// The workload class of a group: what the best launch choices depend on, independent of the route the
+// choices themselves select.
+static std::string tune_key(const fedm_plan* P, const fedm_plan::Group& g)
+{
+  const edm_options& o = g.go;
+  int kmax = 0, lmax = 0;
+  for (size_t i = 0; i < g.kk.size(); i++) {
+    kmax = std::max(kmax, g.kk[i]);
+    lmax = std::max(lmax, g.lc[i]);
+  }
+  const bool generic = o.has_missing || o.has_categorical || o.panel_mode || o.use_only_past || o.low_memory ||
+                       o.distance != FEDM_DIST_EUCLIDEAN;
+  char buf[256];
+  snprintf(buf, sizeof buf, "alg%d|prec%d|gen%d|dist%d|Ea%d|k%d|T%d|tasks%d|lib%d|pred%d|coef%d", o.algorithm,
+           o.precision, generic ? 1 : 0, o.distance, P->s.E_actual(g.E), kmax, o.num_thetas,
+           log2_bucket((long long)g.kk.size()), log2_bucket(lmax), log2_bucket((long long)g.pr.size()),
+           g.want_coeffs ? 1 : 0);
+  return buf;
+}
+
+static std::string tune_cache_path()
+{
+  if (const char* e = getenv("FEDM_TUNE_CACHE"))
+    return e;
+  const char* h = getenv("HOME");
+  return std::string(h ? h : ".") + "/.cache/fastedm_hip/tune.txt";
+}
+
+static std::string knobs_text(const fedm_knobs& k)
+{
+  char buf[256];
+  snprintf(buf, sizeof buf,
+           "rank_scale=%.3f direct_blocks_per_cu=%d direct_alt_waves=%d list_blocks=%d mfma_blocks=%d use_mfma=%d "
+           "smap_lds=%d",
+           k.rank_scale, k.direct_blocks_per_cu, k.direct_alt_waves, k.list_blocks, k.mfma_blocks, k.use_mfma,
+           k.smap_lds);
+  return buf;
+}
+
+static bool parse_knobs(const std::string& line, fedm_knobs* k)
+{
+  fedm_default_knobs(k);
+  int got = 0;
+  size_t pos = 0;
+  while (pos < line.size()) {
+    const size_t end = line.find(' ', pos);
+    const std::string tok = line.substr(pos, end == std::string::npos ? std::string::npos : end - pos);
+    pos = end == std::string::npos ? line.size() : end + 1;
+    const size_t eq = tok.find('=');
+    if (eq == std::string::npos)
+      continue;
+    const std::string name = tok.substr(0, eq);
+    const double v = atof(tok.c_str() + eq + 1);
+    got++;
+    if (name == "rank_scale")
+      k->rank_scale = v;
+    else if (name == "direct_blocks_per_cu")
+      k->direct_blocks_per_cu = (int)v;
+    else if (name == "direct_alt_waves")
+      k->direct_alt_waves = (int)v;
+    else if (name == "list_blocks")
+      k->list_blocks = (int)v;
+    else if (name == "mfma_blocks")
+      k->mfma_blocks = (int)v;
+    else if (name == "use_mfma")
+      k->use_mfma = (int)v;
+    else if (name == "smap_lds")
+      k->smap_lds = (int)v;
+    else
+      got--;
+  }
+  return got > 0;
+}
+
+// The cache: one line per workload class, "<class>\t<knobs>\t<ms>"; a later line for a class wins.
+static std::map<std::string, std::pair<std::string, std::string>> tune_cache_read()
+{
+  std::map<std::string, std::pair<std::string, std::string>> m;
+  FILE* f = fopen(tune_cache_path().c_str(), "r");
+  if (!f)
+    return m;
+  char buf[1024];
+  while (fgets(buf, sizeof buf, f)) {
+    std::string line(buf);
+    while (!line.empty() && (line.back() == '\n' || line.back() == '\r'))
+      line.pop_back();
+    const size_t a = line.find('\t');
+    if (a == std::string::npos)
+      continue;
+    const size_t b = line.find('\t', a + 1);
+    m[line.substr(0, a)] = { line.substr(a + 1, b == std::string::npos ? std::string::npos : b - a - 1),
+                             b == std::string::npos ? "" : line.substr(b + 1) };
+  }
+  fclose(f);
+  return m;
+}
+
+static bool tune_cache_lookup(const std::string& key, fedm_knobs* k)
+{
+  const auto m = tune_cache_read();
+  const auto it = m.find(key);
+  return it != m.end() && parse_knobs(it->second.first, k);
+}
+
+static void tune_cache_write(const std::string& key, const fedm_knobs& k, double ms)
+{
+  auto m = tune_cache_read();
+  char t[64];
+  snprintf(t, sizeof t, "%.4f", ms);
+  m[key] = { knobs_text(k), t };
+  const std::string path = tune_cache_path();
+  const size_t slash = path.rfind('/');
+  if (slash != std::string::npos) {
+    // create the directory chain (best effort)
+    std::string dir;
+    for (size_t i = 0; i <= slash; i++) {
+      if (path[i] == '/' && i > 0)
+        mkdir(dir.c_str(), 0755);
+      dir += path[i];
+    }
+  }
+  FILE* f = fopen(path.c_str(), "w");
+  if (!f)
+    return;
+  for (const auto& kv : m)
+    fprintf(f, "%s\t%s\t%s\n", kv.first.c_str(), kv.second.first.c_str(), kv.second.second.c_str());
+  fclose(f);
+}
+
+// Median wall time of `trials` runs after one warm-up run, in ms (negative on an error).
+static double time_batch(edm_batch* b, int trials)
+{
+  if (edm_batch_run(b) != 0)
+    return -1.0;
+  std::vector<double> t;
+  for (int i = 0; i < trials; i++) {
+    const auto t0 = std::chrono::steady_clock::now();
+    if (edm_batch_run(b) != 0)
+      return -1.0;
+    t.push_back(std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
+  }
+  std::sort(t.begin(), t.end());
+  return t[t.size() / 2];
+}
+
+struct BatchOutputs
+{
+  std::vector<double> pred;
+  std::vector<int> rc, ku;
+  bool operator==(const BatchOutputs& o) const
+  {
+    return pred.size() == o.pred.size() && std::memcmp(pred.data(), o.pred.data(), pred.size() * sizeof(double)) == 0 &&
+           rc == o.rc && ku == o.ku;
+  }
+};
+
+static int fetch_outputs(edm_batch* b, const fedm_plan::Group& g, int T, BatchOutputs& out)
+{
+  out.pred.assign((size_t)g.total_out, 0.0);
+  out.rc.assign((size_t)g.total_out, 0);
+  out.ku.assign((size_t)(g.total_out / T), 0);
+  return edm_batch_fetch(b, out.pred.data(), out.rc.data(), out.ku.data());
+}
+
+static int tune_group(fedm_plan* P, fedm_plan::Group& g, int trials, std::string& report)
+{
+  const int T = P->o.num_thetas;
+  const int route = edm_batch_route(g.batch);
+  if (route == 0)
+    return 0;
+  fedm_knobs best;
+  edm_batch_knobs(g.batch, &best);
+  double best_ms = time_batch(g.batch, trials);
+  if (best_ms < 0)
+    return -3;
+  BatchOutputs ref;
+  int r = fetch_outputs(g.batch, g, T, ref);
+  if (r != 0)
+    return r;
+  char line[512];
+  snprintf(line, sizeof line, "group E=%d%s route=%s class=%s\n  start: %s  %.3f ms\n", g.E, g.co ? " copredict" : "",
+           route == 2 ? "persistent" : "resident", g.tune_key.c_str(), knobs_text(best).c_str(), best_ms);
+  report += line;
+  // coordinate descent over the knobs of the route
+  struct Axis
+  {
+    const char* name;
+    std::vector<double> values;
+  };
+  std::vector<Axis> axes;
+  const bool mixed = g.go.precision == FEDM_MIXED;
+  const int Ea = P->s.E_actual(g.E);
+  if (route == 2) {
+    axes.push_back({ "rank_scale", { 1.0, 1.25, 1.5, 2.0, 3.0 } });
+    axes.push_back({ "direct_alt_waves", { 0, 1 } });
+    axes.push_back({ "direct_blocks_per_cu", { 0, 4, 2 } });
+  } else {
+    axes.push_back({ "list_blocks", { 512, 1024, 2048, 4096 } });
+    if (mixed && Ea >= 8) {
+      axes.push_back({ "use_mfma", { 1, 0 } });
+      axes.push_back({ "mfma_blocks", { 1024, 2048, 4096 } });
+    }
+    if (g.go.algorithm == FEDM_SMAP)
+      axes.push_back({ "smap_lds", { 0, 1 } });
+  }
+  auto with = [](fedm_knobs k, const char* name, double v) {
+    const std::string n(name);
+    if (n == "rank_scale")
+      k.rank_scale = v;
+    else if (n == "direct_alt_waves")
+      k.direct_alt_waves = (int)v;
+    else if (n == "direct_blocks_per_cu")
+      k.direct_blocks_per_cu = (int)v;
+    else if (n == "list_blocks")
+      k.list_blocks = (int)v;
+    else if (n == "use_mfma")
+      k.use_mfma = (int)v;
+    else if (n == "mfma_blocks")
+      k.mfma_blocks = (int)v;
+    else if (n == "smap_lds")
+      k.smap_lds = (int)v;
+    return k;
+  };
+  for (const auto& ax : axes) {
+    report += std::string("  ") + ax.name + ":";
+    for (double v : ax.values) {
+      const fedm_knobs cand = with(best, ax.name, v);
+      if (std::memcmp(&cand, &best, sizeof cand) == 0) {
+        snprintf(line, sizeof line, " %g %.3f ms (current) |", v, best_ms);
+        report += line;
+        continue;
+      }
+      edm_batch* nb = nullptr;
+      r = create_group_batch(P, g, &cand, &nb);
+      if (r != 0)
+        return r;
+      const double ms = time_batch(nb, trials);
+      BatchOutputs got;
+      const int fr = ms < 0 ? -3 : fetch_outputs(nb, g, T, got);
+      if (fr != 0) {
+        edm_batch_destroy(nb);
+        return fr;
+      }
+      const bool same = got == ref;
+      snprintf(line, sizeof line, " %g %.3f ms%s |", v, ms, same ? "" : " (outputs differ: rejected)");
+      report += line;
+      if (same && ms < best_ms) {
+        edm_batch_destroy(g.batch);
+        g.batch = nb;
+        best = cand;
+        best_ms = ms;
+      } else {
+        edm_batch_destroy(nb);
+      }
+    }
+    report += "\n";
+  }
+  snprintf(line, sizeof line, "  best: %s  %.3f ms\n", knobs_text(best).c_str(), best_ms);
+  report += line;
+  if (P->graph)
+    edm_batch_set_graph(g.batch, 1);
+  tune_cache_write(g.tune_key, best, best_ms);
+  return 1;
 }
 
 // One edm_predict per task of a group: the drop-in route of the reference's per-task seam.
@@ -1095,8 +1397,39 @@ int fedm_plan_num_tasks(const fedm_plan* P)
   return (int)P->tasks.size();
 }
 
+int fedm_plan_tune(fedm_plan* P, int trials, char* report, int report_len)
+{
+  std::string rep;
+  int tuned = 0;
+  trials = std::max(trials, 1);
+  for (auto& g : P->groups) {
+    if (!g.batch)
+      continue;
+    const int r = tune_group(P, g, trials, rep);
+    if (r < 0)
+      return r;
+    tuned += r;
+  }
+  P->fetched = false;
+  if (report && report_len > 0) {
+    const size_t n = std::min(rep.size(), (size_t)report_len - 1);
+    std::memcpy(report, rep.data(), n);
+    report[n] = 0;
+  }
+  return tuned;
+}
+
+int fedm_plan_group_knobs(const fedm_plan* P, int group, fedm_knobs* out)
+{
+  if (group < 0 || group >= (int)P->groups.size() || !P->groups[group].batch)
+    return -22;
+  edm_batch_knobs(P->groups[group].batch, out);
+  return 0;
+}
+
 int fedm_plan_set_graph(fedm_plan* P, int on)
 {
+  P->graph = on != 0;
   int n = 0;
   for (auto& g : P->groups)
     if (g.batch)

@@ -226,12 +226,12 @@ __device__ __forceinline__ void walk_rows(unsigned mask, const double* x, const 
   }
 }
 
-// waves per SIMD the short-list instances are held to: at six (80 VGPRs) a few registers spill, and the
-// kernel still runs faster than at five (94 VGPRs, no spills), being bound by issue more than by latency
-constexpr int batch_waves(int kcap, int emax) { return kcap <= 4 && emax <= 4 ? 6 : 1; }
-
-template <int KCAP, int EMAX>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(batch_waves(KCAP, EMAX)))) void batch_simplex(BatchParams B)
+// Waves per SIMD the short-list instances are held to (batch_waves, batch_common.h): at six (80 VGPRs) a
+// few registers spill, and the kernel still runs faster than at five (94 VGPRs, no spills), being bound
+// by issue more than by latency.  WPE is the waves per SIMD of an instance; the autotuner can pick the
+// batch_waves_alt instance instead.
+template <int KCAP, int EMAX, int WPE>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void batch_simplex(BatchParams B)
 {
   // library rows per LDS chunk (a multiple of SUB); rows past the chunk's end hold +inf and never pass
   constexpr int ST = row_stride<EMAX>();
@@ -598,7 +598,9 @@ __global__ __launch_bounds__(256) void batch_redo(BatchParams B, const int2* ite
   }
 }
 
-#define FEDM_BATCH(KC, EM) template __global__ void batch_simplex<KC, EM>(BatchParams);
+#define FEDM_BATCH(KC, EM)                                                                                       \
+  template __global__ void batch_simplex<KC, EM, batch_waves(KC, EM)>(BatchParams);                              \
+  template __global__ void batch_simplex<KC, EM, batch_waves_alt(KC, EM)>(BatchParams);
 #define FEDM_BATCH_E(KC)                                                                                         \
   FEDM_BATCH(KC, 2) FEDM_BATCH(KC, 3) FEDM_BATCH(KC, 4) FEDM_BATCH(KC, 8) FEDM_BATCH(KC, 16) FEDM_BATCH(KC, 32)  \
   template __global__ void batch_redo<KC>(BatchParams, const int2*, int);

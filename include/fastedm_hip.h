@@ -142,6 +142,21 @@ int edm_predict_batch(edm_ctx* ctx, const edm_manifold_view* L, const edm_manifo
                       const int* pred_begin, const int* pred_count, const int* pred_rows, const int* k,
                       const int64_t* out_begin, double* pred, int* rc, int* k_used, int (*keep_going)(void));
 
+/* Launch and route choices that leave every result unchanged (the autotuner's search space; no
+ * reference counterpart).  fedm_default_knobs fills the defaults (FEDM_RANK_LTHR_SCALE and
+ * FEDM_SMAP_LDS_ROWS in the environment override the first and the last). */
+typedef struct
+{
+  double rank_scale;        /* the rank-table route takes libraries of >= rank_scale sqrt(k npool) rows (1.5) */
+  int direct_blocks_per_cu; /* persistent direct kernel: blocks per CU, 0 = as many as fit */
+  int direct_alt_waves;     /* persistent direct kernel: 1 = the instance compiled for fewer waves per SIMD */
+  int list_blocks;          /* FP64 list route: library slices chosen for about this many blocks (1024) */
+  int mfma_blocks;          /* matrix-core list route: the same (2048) */
+  int use_mfma;             /* mixed form may screen on the matrix cores (E_actual >= 8) (1) */
+  int smap_lds;             /* S-map solve keeps U and V in LDS for every width (0) */
+} fedm_knobs;
+void fedm_default_knobs(fedm_knobs* k);
+
 /* The same batch in three steps, so that the device-resident part can be timed or replayed:
  * edm_batch_create uploads the tables, row lists and work queue once; edm_batch_run makes every
  * prediction on the device (inputs resident in HBM, results left there); edm_batch_fetch copies
@@ -151,6 +166,12 @@ int edm_batch_create(edm_ctx* ctx, const edm_manifold_view* L, const edm_manifol
                      int num_tasks, const int* lib_begin, const int* lib_count, const int* lib_rows,
                      const int* pred_begin, const int* pred_count, const int* pred_rows, const int* k,
                      const int64_t* out_begin, edm_batch** out);
+/* edm_batch_create with explicit knobs (NULL = the defaults); edm_batch_knobs reports a batch's. */
+int edm_batch_create_knobs(edm_ctx* ctx, const edm_manifold_view* L, const edm_manifold_view* P,
+                           const edm_options* o, int num_tasks, const int* lib_begin, const int* lib_count,
+                           const int* lib_rows, const int* pred_begin, const int* pred_count, const int* pred_rows,
+                           const int* k, const int64_t* out_begin, const fedm_knobs* knobs, edm_batch** out);
+void edm_batch_knobs(const edm_batch* b, fedm_knobs* out);
 int edm_batch_run(edm_batch* b);
 int edm_batch_fetch(edm_batch* b, double* pred, int* rc, int* k_used);
 void edm_batch_destroy(edm_batch* b);
@@ -259,6 +280,19 @@ int fedm_plan_resident_batches(const fedm_plan* p); /* groups on the resident ta
 int fedm_plan_num_tasks(const fedm_plan* p);
 /* Graph replay (edm_batch_set_graph) for every group of the plan; returns the groups that have it. */
 int fedm_plan_set_graph(fedm_plan* p, int on);
+/* The autotuner (no reference counterpart).  For every group of the plan it searches the knobs of the
+ * group's route one at a time (coordinate descent): the persistent route's rank-table scale and direct
+ * kernel instance and occupancy; the resident route's slice targets, matrix-core screening and S-map
+ * layout.  Each candidate batch is built, run `trials` times after a warm-up (median device time), and
+ * kept only if its outputs equal the default's bit for bit.  The fastest replaces the group's batch;
+ * the choice is written to the tuning cache (FEDM_TUNE_CACHE, else $HOME/.cache/fastedm_hip/tune.txt)
+ * under the group's workload class (route-independent: algorithm, options, E_actual, largest k,
+ * thetas, and the log2 of the task count, longest library and prediction rows), which later plans
+ * read when they create their batches.  A text report of every measured alternative goes to `report`.
+ * Returns the number of groups tuned, or a negative error. */
+int fedm_plan_tune(fedm_plan* p, int trials, char* report, int report_len);
+/* The knobs group `group` (in (E, copredict) order of first use) of the plan runs with; -22 if it has no batch. */
+int fedm_plan_group_knobs(const fedm_plan* p, int group, fedm_knobs* out);
 int fedm_plan_task(const fedm_plan* p, int task, int* info, int* lib_rows, int* pred_rows);
 int fedm_plan_fetch(fedm_plan* p, double* pred, int* rc, int* k_used);
 void fedm_plan_destroy(fedm_plan* p);

@@ -50,8 +50,19 @@ def test_ctypes_mirrors_match_header(tmp_path):
     """Field offsets and sizes of the package's ctypes structures equal the C compiler's."""
     fe = load_product()
     for cls, struct in [(fe.Command, "fedm_command"), (fe.Options, "edm_options"),
-                        (fe.ManifoldView, "edm_manifold_view")]:
+                        (fe.ManifoldView, "edm_manifold_view"), (fe.Knobs, "fedm_knobs")]:
         names = [f[0] for f in cls._fields_]
         out = _c_layout(tmp_path, struct, names)
         assert out[0] == ctypes.sizeof(cls), struct
         assert out[1:] == [getattr(cls, n).offset for n in names], struct
+
+
+def test_default_knobs():
+    """fedm_default_knobs (no device needed): the documented defaults."""
+    fe = load_product()
+    L = fe.load_library()
+    L.fedm_default_knobs.argtypes = [ctypes.POINTER(fe.Knobs)]
+    k = fe.Knobs()
+    L.fedm_default_knobs(ctypes.byref(k))
+    assert k.as_dict() == {"rank_scale": 1.5, "direct_blocks_per_cu": 0, "direct_alt_waves": 0, "list_blocks": 1024,
+                           "mfma_blocks": 2048, "use_mfma": 1, "smap_lds": 0}
