@@ -24,6 +24,8 @@ template <int KCAP, int EMAX, bool GENERIC>
 __global__ void knn_slices_f64(LpParams P, int k, int slice_len, double* out_s, int* out_i, int* out_cnt);
 template <int EMAX, bool GENERIC>
 __global__ void dense_dist_f64(LpParams P, int t0, int nt, int slice_len, double* D, int* out_cnt);
+template <bool GENERIC>
+__global__ void dense_dist_rows(LpParams P, int t0, int nt, int slice_len, double* D, int* out_cnt);
 template <int KCAP>
 __global__ void finish_lists(FinishParams F, const double* ps, const int* pi, const int* pcnt, int S);
 __global__ void finish_dense(FinishParams F, const double* D, int t0, int nt, const int* pcnt, int S);
@@ -121,6 +123,7 @@ enum
   B_MIX2,
   B_MIX3,
   B_MIX4,
+  B_SMAPBIG,
   B_NUM
 };
 } // namespace fedm
@@ -234,6 +237,13 @@ static void launch_dense(int emax, bool generic, dim3 g, hipStream_t s, const Lp
   else                                                                                                           \
     hipLaunchKernelGGL((dense_dist_f64<EM, false>), g, dim3(256), 0, s, P, t0, nt, sl, D, pc);
   switch (emax) {
+    case 0:
+      // points wider than the register rows
+      if (generic)
+        hipLaunchKernelGGL((dense_dist_rows<true>), g, dim3(256), 0, s, P, t0, nt, sl, D, pc);
+      else
+        hipLaunchKernelGGL((dense_dist_rows<false>), g, dim3(256), 0, s, P, t0, nt, sl, D, pc);
+      break;
     case 8: FEDM_DENSE(8) break;
     case 16: FEDM_DENSE(16) break;
     case 32: FEDM_DENSE(32) break;
@@ -418,6 +428,7 @@ static BufRole buf_role(int slot)
     case B_D:
     case B_WKEY:
     case B_WIDX:
+    case B_SMAPBIG:
       return ROLE_SCRATCH;
     case B_PRED:
     case B_RC:
@@ -484,7 +495,11 @@ static int plan_task(const edm_manifold_view* M, const edm_manifold_view* Mp, co
   if (tp.wass && wass_ok && o->panel_mode && o->idw <= 0 && o->id_weights)
     for (int a = 0; a < o->num_panels * o->num_panels; a++)
       wass_ok = wass_ok && o->id_weights[a] >= 0.0f;
-  if (tp.emax < 0 || (tp.wass && !wass_ok))
+  // points wider than 64 coordinates take the dense route with the row-wise distance kernel (emax 0);
+  // the S-map solve takes at most SMAP_BIG_MAX unknowns
+  if (tp.emax < 0 && Ea <= LDS_CHUNK_DOUBLES)
+    tp.emax = 0;
+  if (tp.emax < 0 || (tp.wass && !wass_ok) || (tp.smap && tp.n > SMAP_BIG_MAX))
     return -95;
   tp.wE = wE;
   tp.wE_dt = M->E_dt;
@@ -545,7 +560,9 @@ static int plan_task(const edm_manifold_view* M, const edm_manifold_view* Mp, co
   by[B_NNK] = sizeof(int) * (size_t)NP;
   by[B_NNCNT] = sizeof(int) * (size_t)NP;
 
-  const bool list_route = !tp.wass && k > 0 && k <= 64;
+  const bool list_route = !tp.wass && k > 0 && k <= 64 && tp.emax > 0;
+  if (tp.smap && tp.n > 64)
+    by[B_SMAPBIG] = sizeof(double) * 2 * (size_t)tp.n * tp.n * SMAP_BIG_BLOCKS;
   double amax = 0.0;
   const bool want_mfma = mfma_ok && list_route && o->precision == FEDM_MIXED && !tp.generic && Ea >= 8;
   if (want_mfma) {
@@ -641,7 +658,7 @@ static int plan_task(const edm_manifold_view* M, const edm_manifold_view* Mp, co
     // dense distance rows: every valid neighbour (k < 0), or an exact selection wider than the
     // register lists (k > 64) made from the rows by select_wide
     tp.route = R_DENSE;
-    tp.wide = tp.wass ? k > 0 : k > 64;
+    tp.wide = (tp.wass || tp.emax == 0) ? k > 0 : k > 64;
     const int n2 =
       tp.wide ? (int)std::min<long long>(1ll << 30, 1ll << (64 - __builtin_clzll((unsigned long long)k - 1))) : 0;
     tp.gstride = tp.wide && n2 > 4096 ? n2 : 0;
@@ -753,6 +770,8 @@ static int bind_task(TaskPlan& tp, const edm_options* o, Get&& get)
   Sm.nn_i = F.nn_i;
   Sm.nn_s = F.nn_s;
   Sm.nn_stride = F.nn_stride;
+  Sm.big = (double*)tp.ptr[B_SMAPBIG];
+  Sm.big_blocks = Sm.big ? SMAP_BIG_BLOCKS : 0;
 
   if (tp.route == R_MFMA) {
     MfmaParams& MP = tp.MP;

@@ -236,6 +236,96 @@ __global__ __launch_bounds__(256) void dense_dist_f64(LpParams P, int t0, int nt
 }
 
 // ----------------------------------------------------------------------------------------------
+// Dense distances for points wider than the register rows (E_actual > 64): pair_acc's arithmetic with
+// the target's coordinates read from memory one at a time (they are shared by the block's threads only
+// through the cache) and the library rows staged in LDS in chunks.  Same output as dense_dist_f64.
+template <bool GENERIC>
+__device__ __forceinline__ double pair_acc_rows(const LpParams& P, const double* __restrict__ xs,
+                                                const double* __restrict__ y, int lib_panel, double lib_tt,
+                                                int tgt_panel, double tgt_pt)
+{
+  double acc = 0.0;
+  const int Ea = P.Ea;
+  if (GENERIC) {
+    const int f = P.flags;
+    if ((f & F_SKIP_OTHER) && lib_panel != tgt_panel)
+      return MISSING_D;
+    if ((f & F_SKIP_FUTURE) && lib_tt > tgt_pt)
+      return MISSING_D;
+    if (f & F_IDW)
+      acc += P.idw * (double)(lib_panel != tgt_panel);
+    else if (f & F_IDWEIGHTS)
+      acc += (double)P.id_weights[(size_t)lib_panel * P.num_panels + tgt_panel];
+    const bool mae = f & F_MAE;
+    for (int j = 0; j < Ea; j++) {
+      const double x = xs[j], yj = y[j];
+      double d;
+      if ((f & F_MISSING) && (x == MISSING_D || yj == MISSING_D)) {
+        if (P.missing_distance == 0.0)
+          return MISSING_D;
+        d = P.missing_distance;
+      } else if ((f & F_CATEGORICAL) && P.metrics[j] == 1) {
+        d = (double)(x != yj);
+      } else {
+        d = x - yj;
+      }
+      if (mae)
+        acc += fabs(d) / (double)Ea;
+      else
+        acc += d * d;
+    }
+  } else {
+    for (int j = 0; j < Ea; j++) {
+      const double d = xs[j] - y[j];
+      acc += d * d;
+    }
+  }
+  return acc;
+}
+
+template <bool GENERIC>
+__global__ __launch_bounds__(256) void dense_dist_rows(LpParams P, int t0, int nt, int slice_len,
+                                                      double* __restrict__ D, int* __restrict__ out_cnt)
+{
+  __shared__ double lds_x[LDS_CHUNK_DOUBLES];
+  __shared__ int lds_panel[LDS_CHUNK_MAXPTS];
+  __shared__ double lds_tt[LDS_CHUNK_MAXPTS];
+
+  const int tl = blockIdx.x * blockDim.x + threadIdx.x;
+  const int t = t0 + tl;
+  const bool active = tl < nt && t < P.NP;
+  const int lo = blockIdx.y * slice_len;
+  const int hi = min(P.NL, lo + slice_len);
+  const double* y = P.Mp + (size_t)(active ? t : 0) * P.Ea;
+  const int tgt_panel = (GENERIC && active && P.Mppanel) ? P.Mppanel[t] : 0;
+  const double tgt_pt = (GENERIC && active && P.Mppt) ? P.Mppt[t] : 0.0;
+  const bool mae = GENERIC && (P.flags & F_MAE);
+  const int ch = min(LDS_CHUNK_MAXPTS, LDS_CHUNK_DOUBLES / P.Ea);
+  int cnt = 0;
+  double* row = D + (size_t)tl * P.NL;
+  for (int c0 = lo; c0 < hi; c0 += ch) {
+    const int cn = min(ch, hi - c0);
+    __syncthreads();
+    stage_chunk(P, lds_x, lds_panel, lds_tt, c0, cn, GENERIC);
+    __syncthreads();
+    if (active) {
+      for (int c = 0; c < cn; c++) {
+        const double acc = pair_acc_rows<GENERIC>(P, lds_x + c * P.Ea, y, GENERIC ? lds_panel[c] : 0,
+                                                  GENERIC ? lds_tt[c] : 0.0, tgt_panel, tgt_pt);
+        double s = -1.0;
+        if (acc != 0.0 && acc != MISSING_D) {
+          cnt++;
+          s = mae ? acc : sqrt(acc);
+        }
+        row[c0 + c] = s;
+      }
+    }
+  }
+  if (active)
+    out_cnt[(size_t)blockIdx.y * P.NP + t] = cnt;
+}
+
+// ----------------------------------------------------------------------------------------------
 // Merge the per-slice lists, apply the k policy (src/edm.cpp:593-621), record the neighbour set,
 // and make the simplex predictions (src/edm.cpp:707-735).  One thread per target.
 // nn_k[t] = number of neighbours in nn_i/nn_s; -1 means "all valid, in library order, see D";
@@ -447,6 +537,8 @@ template __global__ void dense_dist_f64<8, true>(LpParams, int, int, int, double
 template __global__ void dense_dist_f64<16, true>(LpParams, int, int, int, double*, int*);
 template __global__ void dense_dist_f64<32, true>(LpParams, int, int, int, double*, int*);
 template __global__ void dense_dist_f64<64, true>(LpParams, int, int, int, double*, int*);
+template __global__ void dense_dist_rows<false>(LpParams, int, int, int, double*, int*);
+template __global__ void dense_dist_rows<true>(LpParams, int, int, int, double*, int*);
 template __global__ void finish_lists<4>(FinishParams, const double*, const int*, const int*, int);
 template __global__ void finish_lists<8>(FinishParams, const double*, const int*, const int*, int);
 template __global__ void finish_lists<16>(FinishParams, const double*, const int*, const int*, int);

@@ -965,8 +965,311 @@ __global__ __launch_bounds__(64) void smap_hyb(SmapParams S, const double* __res
 // Launches the S-map kernel for n = E_a + 1 unknowns over targets [t0, t0 + nt) at every theta:
 // register rows for n <= 24, U rows in registers for 33 <= n <= 48, LDS rows otherwise.  A persistent grid of one-wavefront blocks, as many as
 // stay resident.  Returns 0, or -22 when n is beyond 64 (a system needs a lane per unknown).
+// ---------------------------------------------------------------------------------------------
+// Wide systems (65 <= n <= SMAP_BIG_MAX unknowns): one (target, theta) system per 256-thread block, a
+// grid-stride loop over the items.  W lives in LDS; U and V live transposed in the block's slice of
+// global scratch (column i of U at Ut[i n ..], so a rotation's two column updates are contiguous
+// rows).  The Gram entries are summed neighbour by neighbour as in form_group, and the solve makes
+// jacobi_svd_solve_wave's operations in its order, with threads in place of lanes and a barrier
+// where the wavefront version orders its LDS traffic.
+constexpr int SB_THREADS = 256;
+constexpr int SB_CH = 8; // neighbour rows staged per Gram chunk
+
+__host__ __device__ inline size_t smap_big_lds_doubles(int n)
+{
+  // W [n^2] | b, sv, tmp, x [n each] | Xc [SB_CH n] | yc, cd, wts [SB_CH each] | cidx [SB_CH ints] | scalars
+  return (size_t)n * n + 4 * (size_t)n + (size_t)SB_CH * n + 4 * SB_CH + 8;
+}
+
+__device__ __forceinline__ double block_max(double v, double* red)
+{
+  v = wave_max(v);
+  const int w = threadIdx.x >> 6;
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0)
+    red[w] = v;
+  __syncthreads();
+  double m = red[0];
+  for (int i = 1; i < (int)(blockDim.x >> 6); i++)
+    m = fmax(m, red[i]);
+  return m;
+}
+
+__global__ __launch_bounds__(SB_THREADS) void smap_big(SmapParams S, const double* __restrict__ D, int t0, int nt)
+{
+  extern __shared__ double smem[];
+  __shared__ double red[SB_THREADS / 64];
+  const int n = S.Ea + 1, Ea = S.Ea, T = S.T, tid = threadIdx.x;
+  double* W = smem;
+  double* bv = W + (size_t)n * n;
+  double* sv = bv + n;
+  double* tmp = sv + n;
+  double* xs = tmp + n;
+  double* Xc = xs + n;
+  double* yc = Xc + (size_t)SB_CH * n;
+  double* cd = yc + SB_CH;
+  double* wts = cd + SB_CH;
+  int* cidx = (int*)(wts + SB_CH);
+  double* scal = wts + SB_CH + SB_CH / 2; // [0] mean, [1] chunk count, [2] sort position, [3] nonzero
+  double* Ut = S.big + (size_t)blockIdx.x * 2 * n * n;
+  double* Vt = Ut + (size_t)n * n;
+  const int items = nt * T;
+  for (int item = blockIdx.x; item < items; item += gridDim.x) {
+    const int tl = item / T, th = item - (item / T) * T;
+    const int t = t0 + tl;
+    if (t >= S.NP)
+      continue;
+    const int kk = S.nn_k[t];
+    if (kk == 0)
+      continue;
+    const bool dense = kk < 0;
+    const int k = dense ? S.nn_cnt[t] : kk;
+    const double* drow = dense ? D + (size_t)tl * S.NL : nullptr;
+    const int* li = S.nn_i + (size_t)(t - S.nn_t0) * S.nn_stride;
+    const double* lsd = S.nn_s + (size_t)(t - S.nn_t0) * S.nn_stride;
+    __syncthreads();
+    if (tid == 0) {
+      // mean of the selected distances, summed in neighbour order (Eigen mean = sum / size)
+      double sum = 0.0;
+      if (dense) {
+        for (int i = 0; i < S.NL; i++)
+          if (drow[i] >= 0.0)
+            sum += drow[i];
+      } else {
+        for (int j = 0; j < k; j++)
+          sum += lsd[j];
+      }
+      scal[0] = sum / (double)k;
+    }
+    for (int e = tid; e < n * n; e += SB_THREADS)
+      W[e] = 0.0;
+    for (int e = tid; e < n; e += SB_THREADS)
+      bv[e] = 0.0;
+    __syncthreads();
+    const double mean = scal[0];
+    const double theta = S.thetas[th];
+    // X^T X (entries p <= q, accumulated in W[p n + q]) and X^T y, neighbour by neighbour
+    int pos = 0, done = 0;
+    while (done < k) {
+      if (tid == 0) {
+        int c = 0;
+        if (dense) {
+          while (c < SB_CH && pos < S.NL) {
+            if (drow[pos] >= 0.0) {
+              cidx[c] = pos;
+              cd[c] = drow[pos];
+              c++;
+            }
+            pos++;
+          }
+        } else {
+          c = min(SB_CH, k - done);
+          for (int j = 0; j < c; j++) {
+            cidx[j] = li[done + j];
+            cd[j] = lsd[done + j];
+          }
+        }
+        scal[1] = (double)c;
+      }
+      __syncthreads();
+      const int cnum = (int)scal[1];
+      if (cnum == 0)
+        break;
+      if (tid < cnum) {
+        const double w = fdlibm_exp(-theta * (cd[tid] / mean));
+        wts[tid] = w;
+        yc[tid] = w * S.targets[cidx[tid]];
+      }
+      __syncthreads();
+      for (int e = tid; e < cnum * n; e += SB_THREADS) {
+        const int c = e / n, j = e - (e / n) * n;
+        const double w = wts[c];
+        Xc[e] = j == 0 ? w : w * S.M[(size_t)cidx[c] * Ea + (j - 1)];
+      }
+      __syncthreads();
+      for (int e = tid; e < n * n + n; e += SB_THREADS) {
+        if (e < n * n) {
+          const int p = e / n, q = e - (e / n) * n;
+          if (q < p)
+            continue;
+          double a = W[e];
+          for (int c = 0; c < cnum; c++)
+            a += Xc[c * n + p] * Xc[c * n + q];
+          W[e] = a;
+        } else {
+          const int p = e - n * n;
+          double a = bv[p];
+          for (int c = 0; c < cnum; c++)
+            a += Xc[c * n + p] * yc[c];
+          bv[p] = a;
+        }
+      }
+      __syncthreads();
+      done += cnum;
+    }
+    for (int e = tid; e < n * n; e += SB_THREADS) {
+      const int p = e / n, q = e - (e / n) * n;
+      if (q < p)
+        W[e] = W[q * n + p];
+    }
+    __syncthreads();
+
+    // Jacobi SVD solve (jacobi_svd_solve_wave's steps)
+    double cm = 0.0;
+    for (int e = tid; e < n * n; e += SB_THREADS)
+      cm = fmax(cm, fabs(W[e]));
+    double scale = block_max(cm, red);
+    const bool bad = !isfinite(scale);
+    if (scale == 0.0)
+      scale = 1.0;
+    if (!bad) {
+      for (int e = tid; e < n * n; e += SB_THREADS) {
+        W[e] = W[e] / scale;
+        const int r = e / n, c = e - (e / n) * n;
+        Ut[e] = (r == c) ? 1.0 : 0.0;
+        Vt[e] = (r == c) ? 1.0 : 0.0;
+      }
+    }
+    __syncthreads();
+    int rank = 0;
+    if (!bad) {
+      const double precision = 2.0 * DBL_EPSILON;
+      double maxDiag = 0.0;
+      for (int i = 0; i < n; i++)
+        maxDiag = fmax(maxDiag, fabs(W[i * n + i]));
+      bool finished = false;
+      int sweeps = 0;
+      while (!finished && sweeps < 1000) {
+        finished = true;
+        sweeps++;
+        for (int p = 1; p < n; p++) {
+          for (int q = 0; q < p; q++) {
+            const double thr = fmax(DBL_MIN, precision * maxDiag);
+            const double wpq = W[p * n + q], wqp = W[q * n + p];
+            if (fabs(wpq) > thr || fabs(wqp) > thr) {
+              finished = false;
+              Rot jl, jr;
+              real_2x2(W[p * n + p], wpq, wqp, W[q * n + q], &jl, &jr);
+              __syncthreads();
+              if (!(jl.c == 1.0 && jl.s == 0.0))
+                for (int j = tid; j < n; j += SB_THREADS) {
+                  const double xv = W[p * n + j], yv = W[q * n + j];
+                  W[p * n + j] = jl.c * xv + jl.s * yv;
+                  W[q * n + j] = -jl.s * xv + jl.c * yv;
+                  const double xu = Ut[p * n + j], yu = Ut[q * n + j];
+                  Ut[p * n + j] = jl.c * xu + jl.s * yu;
+                  Ut[q * n + j] = -jl.s * xu + jl.c * yu;
+                }
+              __syncthreads();
+              const double tc = jr.c, ts = -jr.s;
+              if (!(tc == 1.0 && ts == 0.0))
+                for (int j = tid; j < n; j += SB_THREADS) {
+                  const double xv = W[j * n + p], yv = W[j * n + q];
+                  W[j * n + p] = tc * xv + ts * yv;
+                  W[j * n + q] = -ts * xv + tc * yv;
+                  const double xw = Vt[p * n + j], yw = Vt[q * n + j];
+                  Vt[p * n + j] = tc * xw + ts * yw;
+                  Vt[q * n + j] = -ts * xw + tc * yw;
+                }
+              __syncthreads();
+              maxDiag = fmax(maxDiag, fmax(fabs(W[p * n + p]), fabs(W[q * n + q])));
+            }
+          }
+        }
+      }
+      // non-negative singular values
+      for (int i = 0; i < n; i++) {
+        const double a = W[i * n + i];
+        if (a < 0.0)
+          for (int j = tid; j < n; j += SB_THREADS)
+            Ut[i * n + j] = -Ut[i * n + j];
+        if (tid == 0)
+          sv[i] = fabs(a) * scale;
+      }
+      __syncthreads();
+      // decreasing order, first maximum wins
+      int nonzero = n;
+      for (int i = 0; i < n; i++) {
+        int sp = i;
+        double mx = sv[i];
+        for (int j = i + 1; j < n; j++)
+          if (sv[j] > mx) {
+            mx = sv[j];
+            sp = j;
+          }
+        if (mx == 0.0) {
+          nonzero = i;
+          break;
+        }
+        if (sp != i) {
+          __syncthreads();
+          if (tid == 0) {
+            const double a = sv[i];
+            sv[i] = sv[sp];
+            sv[sp] = a;
+          }
+          for (int j = tid; j < n; j += SB_THREADS) {
+            double a = Ut[i * n + j];
+            Ut[i * n + j] = Ut[sp * n + j];
+            Ut[sp * n + j] = a;
+            a = Vt[i * n + j];
+            Vt[i * n + j] = Vt[sp * n + j];
+            Vt[sp * n + j] = a;
+          }
+          __syncthreads();
+        }
+      }
+      const double thr = fmax(sv[0] * ((double)n * DBL_EPSILON), DBL_MIN);
+      int r = nonzero - 1;
+      while (r >= 0 && sv[r] < thr)
+        r--;
+      rank = r + 1;
+      for (int i = tid; i < rank; i += SB_THREADS) {
+        double s = 0.0;
+        for (int j = 0; j < n; j++)
+          s += Ut[i * n + j] * bv[j];
+        tmp[i] = (1.0 / sv[i]) * s;
+      }
+      __syncthreads();
+    }
+    for (int l = tid; l < n; l += SB_THREADS) {
+      double s = 0.0;
+      for (int i = 0; i < rank; i++)
+        s += Vt[i * n + l] * tmp[i];
+      xs[l] = s;
+    }
+    __syncthreads();
+    if (tid == 0) {
+      double r = xs[0];
+      const double* yp = S.Mp + (size_t)t * Ea;
+      for (int c = 0; c < Ea; c++)
+        if (yp[c] != MISSING_D)
+          r += yp[c] * xs[c + 1];
+      S.pred[(size_t)th * S.NP + t] = r;
+      S.rc[(size_t)th * S.NP + t] = RC_SUCCESS;
+    }
+    if (S.coeffs && th == T - 1)
+      for (int j = tid; j < n; j += SB_THREADS) {
+        const double c = xs[j];
+        S.coeffs[(size_t)t * n + j] = fabs(c) < 1.0e-11 ? MISSING_D : c;
+      }
+  }
+}
+
 int smap_launch(int n, hipStream_t s, const SmapParams& S, const double* D, int t0, int nt, bool lds_rows)
 {
+  if (n > 64 && n <= SMAP_BIG_MAX) {
+    if (!S.big || S.big_blocks < 1)
+      return -22;
+    const size_t lds = sizeof(double) * smap_big_lds_doubles(n);
+    hipError_t e = hipFuncSetAttribute((const void*)smap_big, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    const long long items = (long long)nt * S.T;
+    const int grid = (int)std::max<long long>(1, std::min<long long>(items, S.big_blocks));
+    void* args[] = { (void*)&S, (void*)&D, (void*)&t0, (void*)&nt };
+    if (e == hipSuccess)
+      e = hipLaunchKernel((const void*)smap_big, dim3(grid), dim3(SB_THREADS), args, lds, s);
+    return e == hipSuccess ? 0 : -(int)e - 1000;
+  }
   if (n > 64 || n < 2)
     return -22;
   const int G = smap_group(n);

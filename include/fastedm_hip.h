@@ -104,7 +104,9 @@ void edm_ctx_destroy(edm_ctx* ctx);
  *   nn_idx [Mp->num_points x nn_cap] or NULL: the chosen library positions in the order the
  *          predictor used them, -1 padded (not part of the reference's seam; used for parity checks)
  *   keep_going: polled between stages, may be NULL.
- * Returns 0, or -22 for invalid arguments (the S-map solve takes E_actual <= 63: a lane per unknown).
+ * Returns 0, -22 for invalid arguments, or -95 (outputs filled, rc 8000) for a width this build does
+ * not take: E_actual above 4096, an S-map with E_actual above 127 (128 unknowns), Wasserstein with
+ * E_x above 32.
  */
 int edm_predict(edm_ctx* ctx, const edm_manifold_view* M, const edm_manifold_view* Mp, const edm_options* o,
                 double* pred, int* rc, double* coeffs, int* k_used, int* nn_idx, int nn_cap, int (*keep_going)(void));

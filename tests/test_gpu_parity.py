@@ -52,6 +52,17 @@ CASES = {
     "wide_missing_panels_k120": dict(NL=2000, NP=100, Ea=4, k=120, missing=0.05, missing_distance=0.3, panels=3,
                                      idw=0.5),
     "wide_invalid_algorithm": dict(NL=300, NP=50, Ea=2, k=70, algorithm=7),
+    # points wider than 64 coordinates: the row-wise distance kernel, exact selection on the dense rows,
+    # and the block-wide S-map solve for 65..128 unknowns
+    "simplex_E80_k4": dict(NL=600, NP=60, Ea=80, k=4, thetas=(1.0, 3.0)),
+    "simplex_E80_all": dict(NL=300, NP=40, Ea=80, k=-1, thetas=(0.5, 2.0)),
+    "simplex_E300_k5": dict(NL=300, NP=30, Ea=300, k=5),
+    "wide_options_E90": dict(NL=500, NP=40, Ea=90, k=6, missing=0.05, missing_distance=0.3, panels=3, idw=0.5),
+    "smap_E64_k70": dict(NL=400, NP=30, Ea=64, k=70, algorithm=1, thetas=(0.5,), save_coeffs=True),
+    "smap_E80_k82": dict(NL=500, NP=30, Ea=80, k=82, algorithm=1, thetas=(0, 2.0), save_coeffs=True),
+    "smap_E80_k82_mixed": dict(NL=500, NP=30, Ea=80, k=82, algorithm=1, thetas=(1.0,), precision=1),
+    "smap_E80_all": dict(NL=200, NP=20, Ea=80, k=-1, algorithm=1, thetas=(1.0,), save_coeffs=True),
+    "smap_E127_k130": dict(NL=400, NP=12, Ea=127, k=130, algorithm=1, thetas=(1.0,), save_coeffs=True),
 }
 
 
@@ -96,12 +107,12 @@ def test_smap_register_rows_match_lds_rows(Ea, monkeypatch):
 
 
 def test_smap_width_limit():
-    """An S-map system needs a lane per unknown: E_a = 64 (65 unknowns) is refused with -22, while the
+    """The S-map solve takes at most 128 unknowns: E_a = 128 is refused with -95 (and rc 8000), while the
     same task with simplex runs."""
     fe = load_product()
-    M, Mp, opts = random_task(7, NL=300, NP=20, Ea=64, k=66, algorithm=1)
-    print("smap width limit: Ea=64", flush=True)
-    with pytest.raises(fe.FastEDMError, match=r"\(-22\)"):
+    M, Mp, opts = random_task(7, NL=300, NP=20, Ea=128, k=131, algorithm=1)
+    print("smap width limit: Ea=128", flush=True)
+    with pytest.raises(fe.FastEDMError, match=r"\(-95\)"):
         fe.predict(M, Mp, opts, nn_cap=0)
     opts["algorithm"] = 0
     opts["k"] = 5
