@@ -36,12 +36,24 @@ struct BatchParams
   int2* redo;            // (task, target) whose squared-distance ranking may differ from the sqrt ranking
   int* redo_count;       // entries appended (may exceed redo_cap: then every target is redone)
   int redo_cap;
+  // spatially pruned direct kernel (batch_simplex_pruned)
+  const int* tperm;      // spatial order of each prediction list: tperm[pred_begin + j] = list position
+  const int2* pitems;    // (task, first target in spatial order)
+  int num_pitems;
+  int* pcounter;         // its work-queue head
 };
+
+// Libraries of PR_MIN..PR_BCH rows take the pruned direct kernel (when the knob allows it).
+constexpr int PR_MIN = 64;
+constexpr int PR_BCH = 512;
+constexpr int PR_EMAX = 8;  // widest point it takes
 
 constexpr int batch_waves(int kcap, int emax) { return kcap <= 4 && emax <= 4 ? 6 : 1; }
 constexpr int batch_waves_alt(int kcap, int emax) { return kcap <= 4 && emax <= 4 ? 5 : 2; }
 template <int KCAP, int EMAX, int WPE>
 __global__ void batch_simplex(BatchParams B);
+template <int KCAP, int EMAX>
+__global__ void batch_simplex_pruned(BatchParams B);
 
 constexpr int RANK_STORE = 1024; // longest stored prefix
 constexpr int RANK_ZCAP = 8;     // rows at distance 0 kept per target

@@ -278,6 +278,7 @@ extern "C" void fedm_default_knobs(fedm_knobs* k)
   // 24 unknowns); the default keeps rows in registers where they fit.  Both give the same bits.
   const char* e = getenv("FEDM_SMAP_LDS_ROWS");
   k->smap_lds = e && e[0] == '1';
+  k->direct_prune = 1;
 }
 
 static fedm_knobs default_knobs()
@@ -1062,6 +1063,7 @@ struct edm_batch
   std::vector<void*> owned;
   BatchParams B;
   int grid = 0;
+  int pgrid = 0; // pruned direct kernel
   // rank-table route (one shared prediction list whose tasks draw libraries from one pool)
   bool rank_route = false;
   RankParams R;
@@ -1150,6 +1152,25 @@ static const void* rank_scan_multi_kernel(int kcap, int words)
     case 8: return rank_scan_multi_k<8>(words);
     case 16: return rank_scan_multi_k<16>(words);
     default: return rank_scan_multi_k<32>(words);
+  }
+}
+template <int KCAP>
+static const void* batch_pruned_k(int demax)
+{
+  switch (demax) {
+    case 2: return (const void*)batch_simplex_pruned<KCAP, 2>;
+    case 3: return (const void*)batch_simplex_pruned<KCAP, 3>;
+    case 4: return (const void*)batch_simplex_pruned<KCAP, 4>;
+    default: return (const void*)batch_simplex_pruned<KCAP, 8>;
+  }
+}
+static const void* batch_pruned_kernel(int kcap, int demax)
+{
+  switch (kcap) {
+    case 4: return batch_pruned_k<4>(demax);
+    case 8: return batch_pruned_k<8>(demax);
+    case 16: return batch_pruned_k<16>(demax);
+    default: return batch_pruned_k<32>(demax);
   }
 }
 static const void* batch_redo_kernel(int kcap)
@@ -1686,10 +1707,15 @@ int edm_batch_create_knobs(edm_ctx* c, const edm_manifold_view* L, const edm_man
   for (int t = 0; t < num_tasks; t++)
     order[t] = t;
   std::stable_sort(order.begin(), order.end(), [&](int a, int q) { return lib_count[a] > lib_count[q]; });
-  std::vector<int2> items, ritems;
-  for (int t : order)
+  std::vector<int2> items, ritems, pitems;
+  // libraries of PR_MIN..PR_BCH rows with points of at most PR_EMAX coordinates: the pruned direct kernel
+  const bool prune_ok = b->kn.direct_prune != 0 && direct_emax(Ea) <= PR_EMAX;
+  std::vector<char> is_pruned(num_tasks, 0);
+  for (int t : order) {
+    is_pruned[t] = prune_ok && !is_rank[t] && lib_count[t] >= PR_MIN && lib_count[t] <= PR_BCH;
     for (int t0 = 0; t0 < pred_count[t]; t0 += 256)
-      (is_rank[t] ? ritems : items).push_back(make_int2(t, t0));
+      (is_rank[t] ? ritems : is_pruned[t] ? pitems : items).push_back(make_int2(t, t0));
+  }
   B.counter = b->dev<int>(nullptr, 1);
   // targets whose squared-distance ranking may differ from the sqrt ranking (rare; see kernels_batch.hip)
   B.redo_cap = (int)std::min<int64_t>(b->total_k, 1 << 16);
@@ -1726,6 +1752,50 @@ int edm_batch_create_knobs(edm_ctx* c, const edm_manifold_view* L, const edm_man
   }
   B.items = b->dev(items.data(), items.size());
   B.num_items = (int)items.size();
+  if (b->rank_route == false)
+    for (int t = 0; t < num_tasks; t++)
+      is_rank[t] = 0;
+  B.pitems = b->dev(pitems.data(), pitems.size());
+  B.num_pitems = (int)pitems.size();
+  B.pcounter = b->dev<int>(nullptr, 1);
+  {
+    // spatial order of every prediction list a pruned task uses: Morton order of the first (up to three)
+    // coordinates, quantised to 21 bits over the list's range
+    std::vector<int> tperm(std::max<size_t>(pred_total, 1), 0);
+    std::map<std::pair<int, int>, int> done;
+    for (int t = 0; t < num_tasks; t++) {
+      if (!is_pruned[t] || !done.emplace(std::make_pair(pred_begin[t], pred_count[t]), t).second)
+        continue;
+      const int pb0 = pred_begin[t], n = pred_count[t], dims = std::min(Ea, 3);
+      double lo[3] = { 0, 0, 0 }, hi[3] = { 0, 0, 0 };
+      for (int d = 0; d < dims; d++) {
+        lo[d] = hi[d] = b->Pdata[(size_t)pred_rows[pb0] * Ea + d];
+        for (int i = 0; i < n; i++) {
+          const double v = b->Pdata[(size_t)pred_rows[pb0 + i] * Ea + d];
+          lo[d] = std::min(lo[d], v);
+          hi[d] = std::max(hi[d], v);
+        }
+      }
+      std::vector<std::pair<uint64_t, int>> key(n);
+      for (int i = 0; i < n; i++) {
+        uint64_t code = 0;
+        unsigned qd[3] = { 0, 0, 0 };
+        for (int d = 0; d < dims; d++) {
+          const double v = b->Pdata[(size_t)pred_rows[pb0 + i] * Ea + d];
+          const double f = hi[d] > lo[d] ? (v - lo[d]) / (hi[d] - lo[d]) : 0.0;
+          qd[d] = (unsigned)std::min(2097151.0, std::max(0.0, f * 2097151.0));
+        }
+        for (int bit = 20; bit >= 0; bit--)
+          for (int d = 0; d < dims; d++)
+            code = (code << 1) | ((qd[d] >> bit) & 1u);
+        key[i] = { code, i };
+      }
+      std::stable_sort(key.begin(), key.end());
+      for (int i = 0; i < n; i++)
+        tperm[pb0 + i] = key[i].second;
+    }
+    B.tperm = b->dev(tperm.data(), tperm.size());
+  }
   {
     // the direct kernel's tasks: library rows and targets gathered contiguously per task, and the
     // prediction rows in list order, so that its loads need no row-list indirection
@@ -1879,6 +1949,11 @@ int edm_batch_create_knobs(edm_ctx* c, const edm_manifold_view* L, const edm_man
   if (b->kn.direct_blocks_per_cu > 0)
     per = std::min(std::max(per, 1), b->kn.direct_blocks_per_cu);
   b->grid = std::max(1, std::min(B.num_items, ncu * std::max(per, 1)));
+  if (B.num_pitems > 0) {
+    int pper = 0;
+    hipOccupancyMaxActiveBlocksPerMultiprocessor(&pper, batch_pruned_kernel(kcap, direct_emax(Ea)), 256, 0);
+    b->pgrid = std::max(1, std::min(B.num_pitems, ncu * std::max(pper, 1)));
+  }
   if (b->rank_route) {
     int sper = 0;
     hipOccupancyMaxActiveBlocksPerMultiprocessor(&sper, rank_scan_kernel(kcap, emax), 256, b->scan_lds);
@@ -1926,6 +2001,8 @@ static int enqueue_device(edm_batch* b, bool timed)
   hipStream_t s = c->stream;
   auto stop = [b, timed] { return timed && b->keep_going && !b->keep_going(); };
   dfill(c, B.counter, 1, 0);
+  if (B.num_pitems > 0)
+    dfill(c, B.pcounter, 1, 0);
   if (!b->outputs_filled) {
     // Every run writes every output of every target (a prediction, or the fill values themselves for a
     // target left unpredicted), so the outputs are filled on the first run only, as a safety net: a
@@ -1967,6 +2044,8 @@ static int enqueue_device(edm_batch* b, bool timed)
   void* args[] = { &B };
   if (B.num_items > 0)
     HIP_TRY(hipLaunchKernel(fn, dim3(b->grid), dim3(256), args, 0, s));
+  if (B.num_pitems > 0)
+    HIP_TRY(hipLaunchKernel(batch_pruned_kernel(b->kcap, direct_emax(b->Ea)), dim3(b->pgrid), dim3(256), args, 0, s));
   {
     // targets handed over by the direct kernel or the rank scan: exact selection
     const int2* none = nullptr;
@@ -1975,9 +2054,9 @@ static int enqueue_device(edm_batch* b, bool timed)
     HIP_TRY(hipLaunchKernel(batch_redo_kernel(b->kcap), dim3(256), dim3(256), rargs, 0, s));
     // more hand-overs than the list holds: every target again with the exact selection (these
     // launches return at once otherwise, so the host does not wait for the count mid-step)
-    const int2* items[2] = { B.items, b->rank_route ? b->S.items : nullptr };
-    int counts[2] = { B.num_items, b->rank_route ? b->S.num_items : 0 };
-    for (int q = 0; q < 2; q++)
+    const int2* items[3] = { B.items, b->rank_route ? b->S.items : nullptr, B.pitems };
+    int counts[3] = { B.num_items, b->rank_route ? b->S.num_items : 0, B.num_pitems };
+    for (int q = 0; q < 3; q++)
       if (counts[q] > 0) {
         void* aargs[] = { &B, &items[q], &counts[q] };
         HIP_TRY(hipLaunchKernel(batch_redo_kernel(b->kcap), dim3(b->grid), dim3(256), aargs, 0, s));

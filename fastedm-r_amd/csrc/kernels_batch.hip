@@ -414,6 +414,366 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
   }
 }
 
+// ---------------------------------------------------------------------------------------------------
+// The direct kernel with spatial pruning, for libraries of PR_MIN..PR_BCH rows and points of at most
+// PR_EMAX coordinates.  The 256 targets of an item are consecutive in a spatial (Morton) order of the
+// prediction list, so they lie close together.  The library rows are staged in PR_NB buckets of
+// increasing distance r from the targets' centroid c (the bucket index is monotone in the computed r,
+// and each bucket keeps the smallest r it holds).  Before each block of rows a lane tests the triangle
+// inequality against the smallest r still to come: |x - y| >= r - |y - c|.  With the FP64 rounding of
+// both distances bounded (relative (E + 3) 2^-53 each; the test takes (E + 6) 2^-52 of their sum) and a margin that covers the
+// rounding of the coordinate-order squared distance, a lane whose bound exceeds sqrt(threshold) can
+// stop: no remaining row can enter its list, be dropped as a zero distance, or lie just above its k-th
+// entry.  A wavefront leaves the block loop when all its lanes can.  Rows no longer arrive in position
+// order, so list entries are compared by (squared distance, position); screening, walks, hand-over and
+// finish are the direct kernel's.
+
+constexpr int PR_NB = 64;
+
+template <int KCAP>
+__device__ __forceinline__ double insert_lex(double (&la)[KCAP], int (&li)[KCAP], double a, int i)
+{
+  bool c[KCAP];
+#pragma unroll
+  for (int j = 0; j < KCAP; j++)
+    c[j] = a < la[j] || (a == la[j] && i < li[j]);
+  const double out = c[KCAP - 1] ? la[KCAP - 1] : a;
+#pragma unroll
+  for (int j = KCAP - 1; j >= 1; j--) {
+    const double v = c[j] ? a : la[j];
+    const int vi = c[j] ? i : li[j];
+    la[j] = c[j - 1] ? la[j - 1] : v;
+    li[j] = c[j - 1] ? li[j - 1] : vi;
+  }
+  la[0] = c[0] ? a : la[0];
+  li[0] = c[0] ? i : li[0];
+  return out;
+}
+
+template <int KCAP>
+__device__ __forceinline__ int kth_pos(const int (&li)[KCAP], int k)
+{
+  int v = INT_MAX;
+#pragma unroll
+  for (int j = 0; j < KCAP; j++)
+    if (j == k - 1)
+      v = li[j];
+  return v;
+}
+
+// walk_rows with each row's library position read from pos[] (rows in bucket order).
+template <int KCAP, int EMAX, bool FULL>
+__device__ __forceinline__ void walk_rows_pos(unsigned mask, const double* x, const int* pos, const double (&y)[EMAX],
+                                              int k, double (&la)[KCAP], int (&li)[KCAP], double& kth, double& thr_g,
+                                              int& zeros, bool& near)
+{
+  constexpr int ST = row_stride<EMAX>();
+  while (mask) {
+    const int r = __builtin_clz(mask);
+    mask ^= 0x80000000u >> r;
+    const double acc = row_dist<EMAX>(x + r * ST, y);
+    const int pr = pos[r];
+    const bool drop = acc == 0.0 || acc == MISSING_D;
+    zeros += drop ? 1 : 0;
+    // a row that does not enter passes (inf, INT_MAX), which compares equal to an empty entry and so
+    // changes nothing
+    if (FULL) {
+      const double out = insert_lex<KCAP>(la, li, drop ? INF_D : acc, drop ? INT_MAX : pr);
+      kth = la[KCAP - 1];
+      near = near | ((out > kth) & (out < kth * SQRT_GUARD));
+    } else {
+      // rows arrive out of position order: one equal to the k-th entry enters when its position is lower
+      const int ikth = kth_pos<KCAP>(li, k);
+      const bool enters = !drop & ((acc < kth) | ((acc == kth) & (pr < ikth)));
+      near = near | (!drop & (acc > kth) & (acc < thr_g));
+      const double out = kth;
+      insert_lex<KCAP>(la, li, enters ? acc : INF_D, enters ? pr : INT_MAX);
+      kth = kth_of<KCAP>(la, k);
+      near = near | ((out > kth) & (out < kth * SQRT_GUARD));
+    }
+    thr_g = kth * SQRT_GUARD;
+  }
+}
+
+template <int KCAP, int EMAX>
+__global__ __launch_bounds__(256) void batch_simplex_pruned(BatchParams B)
+{
+  constexpr int ST = row_stride<EMAX>();
+  constexpr int NBLK = 2 + (PR_BCH - SUB0 + SUB - 1) / SUB; // row blocks of a library, plus one
+  __shared__ double lds64[(PR_BCH + SUB0) * ST];
+  __shared__ int lpos[PR_BCH + SUB0];
+  __shared__ double rr[PR_BCH];
+  __shared__ int bk[PR_BCH], rk[PR_BCH], sb[PR_BCH];
+  __shared__ int bstart[PR_NB + 1];
+  __shared__ unsigned long long bmin[PR_NB];
+  __shared__ double blo[NBLK];
+  __shared__ double wsum[4][EMAX];
+  __shared__ unsigned long long rext[2];
+  __shared__ int item_sh;
+
+  const int Ea = B.Ea;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  // slack factors of the pruning test (see above)
+  const double eta2 = (Ea + 6) * 2.220446049250313e-16, eta3 = (8.0 * Ea + 16.0) * 1.1102230246251565e-16;
+  if (tid == 0)
+    item_sh = atomicAdd(B.pcounter, 1);
+  __syncthreads();
+  int item = item_sh;
+  while (item < B.num_pitems) {
+    __syncthreads();
+    if (tid == 0)
+      item_sh = atomicAdd(B.pcounter, 1);
+    do {
+      const int2 it = B.pitems[item];
+      const int tk = it.x;
+      const int np = B.pred_count[tk];
+      const int nl = B.lib_count[tk];
+      const int ts = it.y + tid;
+      const bool active = ts < np;
+      const int t = active ? B.tperm[B.pred_begin[tk] + ts] : 0;
+      const int* lrows = B.lib_rows + B.lib_begin[tk];
+      const int k = B.k[tk];
+      const double* lg = B.Lg + (size_t)B.gbeg[tk] * Ea;
+      const double* tgt = B.Tg + B.gbeg[tk];
+
+      double y[EMAX];
+      const double* yp = B.Pg + (size_t)(B.pred_begin[tk] + t) * Ea;
+#pragma unroll
+      for (int j = 0; j < EMAX; j++)
+        y[j] = (active && j < Ea) ? yp[j] : 0.0;
+      double yy = 0.0;
+#pragma unroll
+      for (int j = 0; j < EMAX; j++)
+        yy += y[j] * y[j];
+      // centroid of the item's targets (any point serves; only the rounding of the distances matters)
+#pragma unroll
+      for (int j = 0; j < EMAX; j++) {
+        double v = y[j];
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1)
+          v += __shfl_xor(v, o);
+        if (lane == 0)
+          wsum[w][j] = v;
+      }
+      if (tid == 0) {
+        rext[0] = ~0ull;
+        rext[1] = 0ull;
+      }
+      for (int b = tid; b < PR_NB; b += blockDim.x) {
+        bstart[b] = 0;
+        bmin[b] = ~0ull;
+      }
+      __syncthreads();
+      const double na = (double)min(256, np - it.y);
+      double cen[EMAX];
+#pragma unroll
+      for (int j = 0; j < EMAX; j++)
+        cen[j] = ((wsum[0][j] + wsum[1][j]) + (wsum[2][j] + wsum[3][j])) / na;
+      double rho2 = 0.0;
+#pragma unroll
+      for (int j = 0; j < EMAX; j++) {
+        const double d = y[j] - cen[j];
+        rho2 += d * d;
+      }
+      const double rho = sqrt(rho2);
+      // distances of the library rows from the centroid, and their range
+      for (int c = tid; c < nl; c += blockDim.x) {
+        const double* src = lg + (size_t)c * Ea;
+        double a = 0.0;
+#pragma unroll
+        for (int j = 0; j < EMAX; j++) {
+          const double d = (j < Ea ? src[j] : 0.0) - cen[j];
+          a += d * d;
+        }
+        rr[c] = sqrt(a);
+      }
+      {
+        // the range of r: per-thread, then per-wavefront, then one atomic per wavefront
+        double mn = INF_D, mx = 0.0;
+        for (int c = tid; c < nl; c += blockDim.x) {
+          mn = fmin(mn, rr[c]);
+          mx = fmax(mx, rr[c]);
+        }
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+          mn = fmin(mn, __shfl_xor(mn, o));
+          mx = fmax(mx, __shfl_xor(mx, o));
+        }
+        if (lane == 0) {
+          atomicMin(&rext[0], (unsigned long long)__double_as_longlong(mn));
+          atomicMax(&rext[1], (unsigned long long)__double_as_longlong(mx));
+        }
+      }
+      __syncthreads();
+      const double rmin = __longlong_as_double((long long)rext[0]);
+      const double rmax = __longlong_as_double((long long)rext[1]);
+      const double width = (rmax - rmin) / (double)PR_NB;
+      for (int c = tid; c < nl; c += blockDim.x) {
+        const double r = rr[c];
+        int b = 0;
+        if (width > 0.0) {
+          const double q = (r - rmin) / width;
+          b = q >= (double)(PR_NB - 1) ? PR_NB - 1 : (int)q;
+        }
+        bk[c] = b;
+        rk[c] = atomicAdd(&bstart[b], 1);
+        atomicMin(&bmin[b], (unsigned long long)__double_as_longlong(r));
+      }
+      __syncthreads();
+      if (w == 0) {
+        // exclusive scan of the PR_NB (= 64) bucket counts on one wavefront
+        const int cnt = bstart[lane];
+        int v = cnt;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+          const int u = __shfl_up(v, o);
+          if (lane >= o)
+            v += u;
+        }
+        bstart[lane] = v - cnt;
+        if (lane == 63)
+          bstart[PR_NB] = v;
+      }
+      __syncthreads();
+      const int cpad = SUB0 + (nl - SUB0 + SUB - 1) / SUB * SUB;
+      for (int c = tid; c < cpad; c += blockDim.x) {
+        if (c < nl) {
+          const int slot = bstart[bk[c]] + rk[c];
+          double* dst = lds64 + slot * ST;
+          const double* src = lg + (size_t)c * Ea;
+          double xx = 0.0;
+#pragma unroll
+          for (int j = 0; j < EMAX; j++) {
+            const double v = j < Ea ? src[j] : 0.0;
+            dst[j] = v;
+            xx += v * v;
+          }
+          dst[EMAX] = -0.5 * xx;
+          lpos[slot] = c;
+          sb[slot] = bk[c];
+        } else {
+          double* dst = lds64 + c * ST;
+#pragma unroll
+          for (int j = 0; j < EMAX; j++)
+            dst[j] = j == 0 ? INF_D : 0.0;
+          dst[EMAX] = -INF_D;
+          lpos[c] = INT_MAX;
+        }
+      }
+      __syncthreads();
+      // smallest centroid distance among the rows from each block start on: the bmin of the bucket that
+      // holds the block's first slot (buckets are in increasing r)
+      for (int q = tid; q < NBLK; q += blockDim.x) {
+        const int s0 = q == 0 ? 0 : SUB0 + (q - 1) * SUB;
+        blo[q] = s0 < nl ? __longlong_as_double((long long)bmin[sb[s0]]) : INF_D;
+      }
+      __syncthreads();
+      if (!active)
+        continue;
+
+      double la[KCAP];
+      int li[KCAP];
+#pragma unroll
+      for (int j = 0; j < KCAP; j++) {
+        la[j] = INF_D;
+        li[j] = INT_MAX;
+      }
+      double kth = INF_D, thr_g = INF_D;
+      int zeros = 0;
+      bool near = false;
+      for (int s0 = 0, q = 0; s0 < cpad; q++) {
+        unsigned mask = 0;
+        int nb;
+        if (s0 == 0) {
+          nb = SUB0;
+          if (KCAP * 4 <= SUB0) {
+            double dr[SUB0], m[KCAP];
+#pragma unroll
+            for (int j = 0; j < KCAP; j++)
+              m[j] = INF_D;
+#pragma unroll
+            for (int r = 0; r < SUB0; r++) {
+              dr[r] = row_dist<EMAX>(lds64 + r * ST, y);
+              double v = dr[r] == 0.0 ? INF_D : dr[r];
+#pragma unroll
+              for (int j = 0; j < KCAP; j++) {
+                const double lo = fmin(m[j], v);
+                v = fmax(m[j], v);
+                m[j] = lo;
+              }
+            }
+            const double t0 = m[KCAP - 1] == INF_D ? INF_D : m[KCAP - 1] * SQRT_GUARD;
+#pragma unroll
+            for (int r = 0; r < SUB0; r++)
+              mask = shift_in(mask, dr[r] < t0);
+          } else {
+#pragma unroll
+            for (int r = 0; r < SUB0; r++)
+              mask = shift_in(mask, row_dist<EMAX>(lds64 + r * ST, y) < thr_g);
+          }
+          mask <<= 32 - SUB0;
+        } else {
+          // the triangle-inequality exit: no row from here on can matter to this lane
+          const double lo = blo[q];
+          const bool done = lo == INF_D || lo - rho - (lo + rho) * eta2 > sqrt(thr_g) * (1.0 + eta3);
+          if (__ballot(!done) == 0ull)
+            break;
+          nb = SUB;
+          if (!done) {
+            const double lo2 = ((yy - thr_g) - B.c4) * 0.5;
+#pragma unroll
+            for (int r = 0; r < SUB; r++) {
+              const double* x = lds64 + (s0 + r) * ST;
+              double e = x[EMAX];
+#pragma unroll
+              for (int j = 0; j < EMAX; j++)
+                e = __builtin_fma(x[j], y[j], e);
+              mask = shift_in(mask, e > lo2);
+            }
+          }
+        }
+        if (k == KCAP)
+          walk_rows_pos<KCAP, EMAX, true>(mask, lds64 + s0 * ST, lpos + s0, y, k, la, li, kth, thr_g, zeros, near);
+        else
+          walk_rows_pos<KCAP, EMAX, false>(mask, lds64 + s0 * ST, lpos + s0, y, k, la, li, kth, thr_g, zeros, near);
+        s0 += nb;
+      }
+      if (near) {
+        const int r = atomicAdd(B.redo_count, 1);
+        if (r < B.redo_cap)
+          B.redo[r] = make_int2(tk, t);
+        continue;
+      }
+      double ls[KCAP];
+      bool tie = false;
+#pragma unroll
+      for (int j = 0; j < KCAP; j++)
+        ls[j] = la[j] == INF_D ? INF_D : sqrt(la[j]);
+#pragma unroll
+      for (int j = 0; j + 1 < KCAP; j++)
+        tie = tie || (ls[j] == ls[j + 1] && li[j] > li[j + 1]);
+      if (tie) {
+#pragma unroll
+        for (int pass = 0; pass < KCAP; pass++) {
+#pragma unroll
+          for (int j = pass & 1; j + 1 < KCAP; j += 2) {
+            const bool sw = ls[j + 1] < ls[j] || (ls[j + 1] == ls[j] && li[j + 1] < li[j]);
+            const double a = ls[j], b = ls[j + 1];
+            const int ia = li[j], ib = li[j + 1];
+            ls[j] = sw ? b : a;
+            ls[j + 1] = sw ? a : b;
+            li[j] = sw ? ib : ia;
+            li[j + 1] = sw ? ia : ib;
+          }
+        }
+      }
+      finish_target<KCAP>(B, tk, t, np, lrows, nl, zeros, k, ls, li, tgt);
+    } while (0);
+    __syncthreads();
+    item = item_sh;
+  }
+}
+
 // The CPU path's selection for one (task, target): FP64 distances from the point table, list ordered
 // by (sqrt distance, position) as they arrive.
 template <int KCAP>
@@ -608,5 +968,14 @@ FEDM_BATCH_E(4)
 FEDM_BATCH_E(8)
 FEDM_BATCH_E(16)
 FEDM_BATCH_E(32)
+#define FEDM_PRUNED(KC)                                                                                          \
+  template __global__ void batch_simplex_pruned<KC, 2>(BatchParams);                                           \
+  template __global__ void batch_simplex_pruned<KC, 3>(BatchParams);                                           \
+  template __global__ void batch_simplex_pruned<KC, 4>(BatchParams);                                           \
+  template __global__ void batch_simplex_pruned<KC, 8>(BatchParams);
+FEDM_PRUNED(4)
+FEDM_PRUNED(8)
+FEDM_PRUNED(16)
+FEDM_PRUNED(32)
 
 } // namespace fedm

@@ -156,6 +156,8 @@ typedef struct
   int mfma_blocks;          /* matrix-core list route: the same (2048) */
   int use_mfma;             /* mixed form may screen on the matrix cores (E_actual >= 8) (1) */
   int smap_lds;             /* S-map solve keeps U and V in LDS for every width (0) */
+  int direct_prune;         /* persistent route: libraries of 64..512 rows (points of <= 8 coordinates) take the
+                               spatially pruned direct kernel (1) */
 } fedm_knobs;
 void fedm_default_knobs(fedm_knobs* k);
 
