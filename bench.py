@@ -72,14 +72,15 @@ def pair_count(libs, reps, n_pred=19998):
 
 
 def timed_steps(plan, fe, steps, warmup):
-    """Wall time per step and the mean HIP-event stage times [batch, rank build, rank scan, direct] (ms)."""
+    """Wall time per step and the mean HIP-event stage times [batch, rank build, rank scan, direct stage,
+    direct kernel, pruned direct kernel, exact re-selection, -] (ms)."""
     for _ in range(warmup):
         plan.run()
     stages = []
     t0 = time.perf_counter()
     for _ in range(steps):
         plan.run()
-        stages.append(fe.timings())
+        stages.append(fe.timings_ex())
     wall = time.perf_counter() - t0
     return wall / steps, np.mean(np.asarray(stages), axis=0)
 
@@ -113,43 +114,49 @@ def rank_route_rates(st, libs, reps):
                     "pair count of the step (215.8e9) overstates the work this route does"}
 
 
-def roofline(st, libs, reps):
-    """Roofline of the stage that dominates the device time, with that stage's algorithmic work.
+PR_MIN, PR_BCH = 64, 512  # library sizes the pruned direct kernel takes (kernels' constants)
 
-    direct     (batch_simplex): 3 E_a = 9 flops per library-target pair scanned (FP64 for short libraries);
-    rank build (rank_build):    9 flops per pool-target pair and histogram/filter pass (2 passes);
-    rank scan  (rank_scan_multi + rank_scatter): HBM bytes of the ranked-prefix entries the tasks' k-NN
-                                selections need (k pool / L entries of 4 B plus k entries of 8 B per (task,
-                                target)) and of their outputs (16 B per prediction)."""
+
+def roofline(st, libs, reps):
+    """Roofline of the kernel with the largest device time in the step, with that kernel's algorithmic work.
+
+    batch_simplex          3 E_a = 9 FP64 flops per (target, library row) pair of its tasks (libraries below
+                           64 rows);
+    batch_simplex_pruned   the same unit over every pair of its tasks (libraries of 64..423 rows), although it
+                           computes only the pairs the triangle bound does not exclude;
+    rank_build             9 FP32 flops per pool-target pair and pass (2 passes);
+    rank_scan              HBM bytes of the ranked-prefix entries the tasks' selections need (k pool / L entries
+                           of 4 B plus k entries of 8 B per (task, target)) and of their outputs (16 B per
+                           prediction)."""
     pool = N_TARGETS
     lthr = rank_threshold(K, pool)
     direct_libs = libs[libs < lthr]
+    pruned = direct_libs[(direct_libs >= PR_MIN) & (direct_libs <= PR_BCH)]
+    plain = direct_libs[(direct_libs < PR_MIN) | (direct_libs > PR_BCH)]
     rank_libs = libs[libs >= lthr]
     work = {
-        # libraries below the rank threshold are short: the direct kernel runs its FP64 variant; both
-        # kernels run on the vector units (no matrix-core instruction), hence "valu"
-        "direct": ("valu", 9.0 * reps * float(np.sum(direct_libs)) * N_TARGETS, "TFLOP/s", FP64_PEAK_TFLOPS,
-                   "batch_simplex<4,3>"),
-        "rank_build": ("valu", 2 * 9.0 * pool * N_TARGETS, "TFLOP/s", FP32_PEAK_TFLOPS, "rank_build<4>"),
+        "direct": ("valu", 9.0 * reps * float(np.sum(plain)) * N_TARGETS, "TFLOP/s", FP64_PEAK_TFLOPS,
+                   "batch_simplex<4,3,6>", "fp64 VALU"),
+        "direct_pruned": ("valu", 9.0 * reps * float(np.sum(pruned)) * N_TARGETS, "TFLOP/s", FP64_PEAK_TFLOPS,
+                          "batch_simplex_pruned<4,3>", "fp64 VALU"),
+        "rank_build": ("valu", 2 * 9.0 * pool * N_TARGETS, "TFLOP/s", FP32_PEAK_TFLOPS, "rank_build<4>", "fp32 VALU"),
         "rank_scan": ("hbm", reps * float(np.sum(K * pool / rank_libs * 4.0 + K * 8.0 + 16.0)) * N_TARGETS, "GB/s",
-                      HBM_PEAK_GBS, "rank_scan_multi<4,1> + rank_scatter"),
+                      HBM_PEAK_GBS, "rank_scan_multi<4,1> + rank_scatter", "HBM3E"),
     }
-    # HBM bytes per launch from rocprofv3 FETCH_SIZE (x2.0) and WRITE_SIZE (x1.0) passes over a steady-state
-    # step, scaled by the same counters on a 1 GiB copy (profiles/ccm_sweep_E3_k4.md); the direct kernel's
-    # writes include its register spills (0.56 GB) beside the 0.64 GB of outputs
-    traffic = {"direct": 1.40e9, "rank_build": 0.16e9, "rank_scan": 1.03e9 + 0.98e9}
-    times = {"rank_build": st[1], "rank_scan": st[2], "direct": st[3]}
+    # HBM bytes per launch from rocprofv3 FETCH_SIZE and WRITE_SIZE passes over a steady-state step,
+    # scaled by the same counters on a 1 GiB copy (profiles/ccm_sweep_E3_k4.md)
+    traffic = {"direct": 0.11e9, "direct_pruned": 0.52e9, "rank_build": 0.16e9, "rank_scan": 1.03e9 + 0.98e9}
+    times = {"rank_build": st[1], "rank_scan": st[2], "direct": st[4], "direct_pruned": st[5]}
     name = max(times, key=lambda n: times[n])
-    bound, amount, unit, peak, kernel = work[name]
+    bound, amount, unit, peak, kernel, engine = work[name]
     ms = float(times[name])
     achieved = amount / (ms * 1e-3) / (1e12 if unit == "TFLOP/s" else 1e9)
     return {"bound": bound, "achieved": achieved, "peak": peak, "unit": unit, "frac": achieved / peak,
             "traffic": traffic[name],
             "traffic_source": "not measured by this run: rocprofv3 FETCH_SIZE/WRITE_SIZE passes over the same "
                               "command, scaled by a 1 GiB copy (profiles/ccm_sweep_E3_k4.md)",
-            "kernel": kernel, "kernel_ms": ms,
-            "algorithmic_work_per_launch": amount,
-            "engine": {"direct": "fp64 VALU", "rank_build": "fp32 VALU", "rank_scan": "HBM3E"}[name]}
+            "kernel": kernel, "kernel_ms": ms, "algorithmic_work_per_launch": amount, "engine": engine,
+            "kernel_ms_all": {k: float(v) for k, v in times.items()}}
 
 
 SMAP_UNIT = "2k(E_a+1)^2 + 2k(E_a+1) + k flops per S-map prediction (Gram X^T X, X^T y and weights; the SVD is not counted)"
@@ -473,11 +480,12 @@ def main():
                       "replicates": args.reps, "tasks": int(args.reps * len(libs)), "predictions_per_step": n_pred,
                       "library_target_pairs_per_step": pairs, "k": 4, "algorithm": "simplex",
                       "form": "batched: rank-table route (FP32-screened exact ranking, one pass per target "
-                              "over every long-library task) for libraries >= 1.5 sqrt(k n), direct exact FP64 "
-                              "kernel for shorter ones"}}
+                              "over every long-library task) for libraries >= 1.5 sqrt(k n), exact FP64 direct "
+                              "kernels for shorter ones (spatially pruned from 64 rows)"}}
 
     out["stages_ms"] = {"device_total": float(st[0]), "rank_build": float(st[1]), "rank_scan": float(st[2]),
-                        "direct": float(st[3])}
+                        "direct": float(st[3]), "direct_kernel": float(st[4]), "direct_pruned_kernel": float(st[5]),
+                        "exact_reselection": float(st[6])}
     out["exact_reselections"] = redos  # (task, target) pairs of the last step handed to the exact selection
     out["roofline"] = roofline(st, libs, args.reps)
     out["rank_route"] = rank_route_rates(st, libs, args.reps)

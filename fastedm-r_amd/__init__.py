@@ -72,7 +72,8 @@ EXPORTS = ["edm_ctx_create", "edm_ctx_destroy", "edm_predict", "edm_predict_batc
            "fedm_plan_num_tasks", "fedm_plan_task", "fedm_plan_fetch", "edm_batch_set_keep_going",
            "edm_batch_route", "edm_batch_set_graph", "edm_batch_fetch_coeffs", "fedm_plan_set_graph",
            "fedm_plan_resident_batches", "edm_predict_noalloc", "edm_selftest_exp", "fedm_default_knobs",
-           "edm_batch_create_knobs", "edm_batch_knobs", "fedm_plan_tune", "fedm_plan_group_knobs", "edm_batch_on_device", "edm_selftest_mfma", "fedm_rank_threshold", "fedm_run_command", "fedm_run_task_group", "fedm_result_rc", "fedm_result_num_stats", "fedm_result_stats", "fedm_result_k",
+           "edm_batch_create_knobs", "edm_batch_knobs", "fedm_plan_tune", "fedm_plan_group_knobs",
+           "edm_ctx_timings_ex", "edm_batch_on_device", "edm_selftest_mfma", "fedm_rank_threshold", "fedm_run_command", "fedm_run_task_group", "fedm_result_rc", "fedm_result_num_stats", "fedm_result_stats", "fedm_result_k",
            "fedm_result_scalar", "fedm_result_matrix_shape", "fedm_result_matrix", "fedm_result_num_manifolds",
            "fedm_result_manifold_shape", "fedm_result_manifold", "fedm_result_free"]
 
@@ -421,6 +422,16 @@ def timings():
     """Device time of the last call's stages in ms: [distance/select, merge/simplex, S-map, total]."""
     out = np.zeros(4)
     load_library().edm_ctx_timings(context(), out.ctypes.data_as(_dp))
+    return out
+
+
+def timings_ex():
+    """timings() followed by the persistent route's direct stage split: [4] direct kernel, [5] pruned
+    direct kernel, [6] exact re-selection, [7] unused."""
+    out = np.zeros(8)
+    f = load_library().edm_ctx_timings_ex
+    f.argtypes = [C.c_void_p, _dp]
+    f(context(), out.ctypes.data_as(_dp))
     return out
 
 

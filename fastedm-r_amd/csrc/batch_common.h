@@ -38,13 +38,22 @@ struct BatchParams
   int redo_cap;
   // spatially pruned direct kernel (batch_simplex_pruned)
   const int* tperm;      // spatial order of each prediction list: tperm[pred_begin + j] = list position
-  const int2* pitems;    // (task, first target in spatial order)
+  const int2* pitems;    // (task, first target in spatial order), grouped by XCD queue
   int num_pitems;
-  int* pcounter;         // its work-queue head
+  int* pcounter;         // work-queue heads, one per queue, PR_CSTRIDE ints apart
+  const int* pq_begin;   // [queues + 1] item range of each queue
+  int pq_n;              // queues (1 .. PR_QUEUES)
 };
+
+// The pruned kernel's items are dealt to one queue per XCD, a task's items all to one queue, so that a
+// task's scattered output writes meet in one L2 (blocks are dispatched to the XCDs in turn, so block b
+// serves queue b % PR_QUEUES); a block whose queue is empty takes items from the others.
+constexpr int PR_QUEUES = 8;
+constexpr int PR_CSTRIDE = 32; // ints between queue heads (128 bytes: separate cache lines)
 
 // Libraries of PR_MIN..PR_BCH rows take the pruned direct kernel (when the knob allows it).
 constexpr int PR_MIN = 64;
+constexpr int SUB0_ROWS = 16; // rows of a library's first block in the direct kernels
 constexpr int PR_BCH = 512;
 constexpr int PR_EMAX = 8;  // widest point it takes
 

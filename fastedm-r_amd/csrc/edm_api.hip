@@ -60,9 +60,9 @@ struct edm_ctx
 {
   int device = 0;
   hipStream_t stream = nullptr;
-  hipEvent_t ev[8] = {};
+  hipEvent_t ev[10] = {};
   std::vector<DevBuf> bufs;
-  double timings[4] = { 0, 0, 0, 0 };
+  double timings[8] = { 0, 0, 0, 0, 0, 0, 0, 0 };
   int last_error = 0;
   int last_redo = 0;
   unsigned char* scratch = nullptr; // this device's copy of the no-allocation scratch array
@@ -356,6 +356,12 @@ void edm_ctx_timings(edm_ctx* c, double* ms4)
 {
   for (int i = 0; i < 4; i++)
     ms4[i] = c->timings[i];
+}
+
+void edm_ctx_timings_ex(edm_ctx* c, double* ms8)
+{
+  for (int i = 0; i < 8; i++)
+    ms8[i] = c->timings[i];
 }
 
 int edm_ctx_exact_redos(const edm_ctx* c)
@@ -1710,9 +1716,10 @@ int edm_batch_create_knobs(edm_ctx* c, const edm_manifold_view* L, const edm_man
   std::vector<int2> items, ritems, pitems;
   // libraries of PR_MIN..PR_BCH rows with points of at most PR_EMAX coordinates: the pruned direct kernel
   const bool prune_ok = b->kn.direct_prune != 0 && direct_emax(Ea) <= PR_EMAX;
+  const int pr_min = PR_MIN;
   std::vector<char> is_pruned(num_tasks, 0);
   for (int t : order) {
-    is_pruned[t] = prune_ok && !is_rank[t] && lib_count[t] >= PR_MIN && lib_count[t] <= PR_BCH;
+    is_pruned[t] = prune_ok && !is_rank[t] && lib_count[t] >= pr_min && lib_count[t] <= PR_BCH;
     for (int t0 = 0; t0 < pred_count[t]; t0 += 256)
       (is_rank[t] ? ritems : is_pruned[t] ? pitems : items).push_back(make_int2(t, t0));
   }
@@ -1755,9 +1762,34 @@ int edm_batch_create_knobs(edm_ctx* c, const edm_manifold_view* L, const edm_man
   if (b->rank_route == false)
     for (int t = 0; t < num_tasks; t++)
       is_rank[t] = 0;
+  {
+    // deal the pruned tasks to PR_QUEUES queues (largest work first, to the lightest queue), keeping the
+    // items of a task together and in the order above within each queue
+    const int nq = PR_QUEUES;
+    B.pq_n = nq;
+    std::vector<double> load(nq, 0.0);
+    std::vector<int> qof(num_tasks, 0);
+    for (int t : order)
+      if (is_pruned[t]) {
+        const int q = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+        qof[t] = q;
+        load[q] += (double)lib_count[t] * pred_count[t];
+      }
+    std::vector<int2> sorted;
+    std::vector<int> qb(nq + 1, 0);
+    for (int q = 0; q < nq; q++) {
+      qb[q] = (int)sorted.size();
+      for (const int2& it : pitems)
+        if (qof[it.x] == q)
+          sorted.push_back(it);
+    }
+    qb[nq] = (int)sorted.size();
+    pitems.swap(sorted);
+    B.pq_begin = b->dev(qb.data(), qb.size());
+  }
   B.pitems = b->dev(pitems.data(), pitems.size());
   B.num_pitems = (int)pitems.size();
-  B.pcounter = b->dev<int>(nullptr, 1);
+  B.pcounter = b->dev<int>(nullptr, (size_t)PR_QUEUES * PR_CSTRIDE);
   {
     // spatial order of every prediction list a pruned task uses: Morton order of the first (up to three)
     // coordinates, quantised to 21 bits over the list's range
@@ -2002,7 +2034,7 @@ static int enqueue_device(edm_batch* b, bool timed)
   auto stop = [b, timed] { return timed && b->keep_going && !b->keep_going(); };
   dfill(c, B.counter, 1, 0);
   if (B.num_pitems > 0)
-    dfill(c, B.pcounter, 1, 0);
+    dfill(c, B.pcounter, (size_t)PR_QUEUES * PR_CSTRIDE, 0);
   if (!b->outputs_filled) {
     // Every run writes every output of every target (a prediction, or the fill values themselves for a
     // target left unpredicted), so the outputs are filled on the first run only, as a safety net: a
@@ -2044,8 +2076,12 @@ static int enqueue_device(edm_batch* b, bool timed)
   void* args[] = { &B };
   if (B.num_items > 0)
     HIP_TRY(hipLaunchKernel(fn, dim3(b->grid), dim3(256), args, 0, s));
+  if (timed)
+    HIP_TRY(hipEventRecord(c->ev[8], s));
   if (B.num_pitems > 0)
     HIP_TRY(hipLaunchKernel(batch_pruned_kernel(b->kcap, direct_emax(b->Ea)), dim3(b->pgrid), dim3(256), args, 0, s));
+  if (timed)
+    HIP_TRY(hipEventRecord(c->ev[9], s));
   {
     // targets handed over by the direct kernel or the rank scan: exact selection
     const int2* none = nullptr;
@@ -2138,6 +2174,10 @@ int edm_batch_run(edm_batch* b)
   c->timings[1] = graphed ? 0.0 : ev_ms(c->ev[4], c->ev[6]);
   c->timings[2] = graphed ? 0.0 : ev_ms(c->ev[6], c->ev[7]);
   c->timings[3] = graphed ? 0.0 : ev_ms(c->ev[7], c->ev[5]);
+  // within the direct stage: the unpruned kernel, the pruned kernel, the exact re-selection
+  c->timings[4] = graphed ? 0.0 : ev_ms(c->ev[7], c->ev[8]);
+  c->timings[5] = graphed ? 0.0 : ev_ms(c->ev[8], c->ev[9]);
+  c->timings[6] = graphed ? 0.0 : ev_ms(c->ev[9], c->ev[5]);
   return 0;
 }
 

@@ -218,6 +218,9 @@ int edm_selftest_exp(const double* x, double* out, int n);
 /* Last stage timings of the most recent call on ctx (milliseconds, HIP events on the ctx stream):
  * [0] distance/selection kernel, [1] merge/predict kernel, [2] S-map kernel, [3] whole device section. */
 void edm_ctx_timings(edm_ctx* ctx, double* ms4);
+/* The same with the persistent route's direct stage split: [4] the direct kernel, [5] the spatially
+ * pruned direct kernel, [6] the exact re-selection ([7] unused). */
+void edm_ctx_timings_ex(edm_ctx* ctx, double* ms8);
 
 /* Diagnostic: how many (task, target) pairs the last batched run on ctx re-selected with the
  * sqrt-ordered list because two different squared distances near the k-th place could share a
