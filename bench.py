@@ -159,11 +159,13 @@ def roofline(st, libs, reps):
             "kernel_ms_all": {k: float(v) for k, v in times.items()}}
 
 
-SMAP_UNIT = "2k(E_a+1)^2 + 2k(E_a+1) + k flops per S-map prediction (Gram X^T X, X^T y and weights; the SVD is not counted)"
+SMAP_UNIT = ("k(n+1)n + 2kn flops per S-map prediction, n = E_a + 1: one multiply and one add per neighbour for each "
+             "entry of the upper triangle of X^T X and of X^T y (the weights' exps and the SVD are not counted)")
 
 
 def smap_flops(k, Ea):
-    return 2.0 * k * (Ea + 1) ** 2 + 2.0 * k * (Ea + 1) + k
+    n = Ea + 1
+    return 2.0 * k * (n * (n + 1) / 2 + n)
 
 
 def stage_roofline(stages, work):
@@ -237,7 +239,8 @@ def cfg2_leg(fe, oracle, cores, steps):
     st = [out["mixed"]["select_ms"], out["mixed"]["merge_ms"], out["mixed"]["smap_ms"]]
     out["roofline"] = stage_roofline(st, {
         "select": ("dense_dist_f64<8>", 3.0 * Ea * 2498 * 2499, "fp64 VALU", FP64_PEAK_TFLOPS, "valu"),
-        "smap": ("smap_rows<8,0> (Gram + solve)", smap_flops(k, Ea) * n_pred, "fp64 VALU", FP64_PEAK_TFLOPS, "valu")})
+        "smap": ("dense_means + gram_dense<1,3> (systems) + smap_rows<8,0> (solve)", smap_flops(k, Ea) * n_pred,
+                 "fp64 VALU", FP64_PEAK_TFLOPS, "valu")})
     out["roofline"]["unit_of_work"] = SMAP_UNIT
     if oracle is not None:
         dt, times, b = median_time(lambda: oracle.edm(t, x, numThreads=cores, **kw))

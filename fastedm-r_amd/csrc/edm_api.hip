@@ -33,6 +33,8 @@ __global__ void wasserstein_dense(WassParams P, int t0, int nt, int slice_len, d
 __global__ void select_wide(FinishParams F, const double* D, int t0, int nt, unsigned long long* gkey, int* gidx,
                             int gstride);
 int smap_launch(int n, hipStream_t s, const SmapParams& S, const double* D, int t0, int nt, bool lds_rows);
+size_t gram_dense_bytes(int n, int nt, int T);
+int gram_dense_launch(int n, hipStream_t s, const SmapParams& S, const double* D, int t0, int nt, double* gw);
 
 template <int KCAP, int KP>
 __global__ void knn_mfma(MfmaParams P, int k, int slice_len, double* out_s, int* out_i, int* out_cnt);
@@ -124,6 +126,7 @@ enum
   B_MIX3,
   B_MIX4,
   B_SMAPBIG,
+  B_GRAM,
   B_NUM
 };
 } // namespace fedm
@@ -228,9 +231,11 @@ static void launch_mfma(int kcap, int kp, dim3 g, hipStream_t s, const MfmaParam
   }
 }
 
-static void launch_dense(int emax, bool generic, dim3 g, hipStream_t s, const LpParams& P, int t0, int nt, int sl,
+// Dense distance rows of targets t0 .. t0 + nt - 1 over S library slices of sl points.
+static void launch_dense(int emax, bool generic, int S, hipStream_t s, const LpParams& P, int t0, int nt, int sl,
                          double* D, int* pc)
 {
+  const dim3 g = emax == 0 ? dim3((nt + 255) / 256, S) : dim3(S, (nt + DT_TGT - 1) / DT_TGT);
 #define FEDM_DENSE(EM)                                                                                           \
   if (generic)                                                                                                   \
     hipLaunchKernelGGL((dense_dist_f64<EM, true>), g, dim3(256), 0, s, P, t0, nt, sl, D, pc);                    \
@@ -436,6 +441,7 @@ static BufRole buf_role(int slot)
     case B_WKEY:
     case B_WIDX:
     case B_SMAPBIG:
+    case B_GRAM:
       return ROLE_SCRATCH;
     case B_PRED:
     case B_RC:
@@ -669,21 +675,33 @@ static int plan_task(const edm_manifold_view* M, const edm_manifold_view* Mp, co
     const int n2 =
       tp.wide ? (int)std::min<long long>(1ll << 30, 1ll << (64 - __builtin_clzll((unsigned long long)k - 1))) : 0;
     tp.gstride = tp.wide && n2 > 4096 ? n2 : 0;
+    // all-neighbour S-map systems of up to 64 unknowns are formed ahead of the solve by gram_dense
+    const bool gram = tp.smap && !tp.wide && tp.n <= 64;
     const size_t per_target = sizeof(double) * (size_t)std::max(NL, 1) +
                               (tp.wide ? (sizeof(int) + sizeof(double)) * (size_t)k : 0) +
-                              (sizeof(unsigned long long) + sizeof(int)) * (size_t)tp.gstride;
+                              (sizeof(unsigned long long) + sizeof(int)) * (size_t)tp.gstride +
+                              (gram ? gram_dense_bytes(tp.n, 1, T) : 0);
     tp.nt = (int)std::max<size_t>(1, std::min<size_t>((size_t)std::max(NP, 1), dense_budget / per_target));
-    const int bxmax = tp.wass ? tp.nt : (tp.nt + 255) / 256;
-    int S = std::max(1, std::min(MAX_SLICES, (1024 + bxmax - 1) / bxmax));
-    S = std::max(1, std::min(S, NL / (tp.wass ? 64 : 256)));
-    tp.S = std::max(1, std::min(S, max_slices));
-    tp.sl = (NL + tp.S - 1) / tp.S;
+    if (!tp.wass && tp.emax > 0) {
+      // dense_dist_f64: library slices of whole 256-point tiles, one block per slice and DT_TGT targets
+      const int tiles = std::max(1, (NL + 255) / 256);
+      tp.S = std::max(1, std::min(std::min(MAX_SLICES, max_slices), tiles));
+      tp.sl = (tiles + tp.S - 1) / tp.S * 256;
+      tp.S = std::max(1, (NL + tp.sl - 1) / tp.sl);
+    } else {
+      const int bxmax = tp.wass ? tp.nt : (tp.nt + 255) / 256;
+      int S = std::max(1, std::min(MAX_SLICES, (1024 + bxmax - 1) / bxmax));
+      S = std::max(1, std::min(S, NL / (tp.wass ? 64 : 256)));
+      tp.S = std::max(1, std::min(S, max_slices));
+      tp.sl = (NL + tp.S - 1) / tp.S;
+    }
     by[B_D] = sizeof(double) * (size_t)tp.nt * std::max(NL, 1);
     by[B_PCNT] = sizeof(int) * (size_t)tp.S * NP;
     by[B_NNI] = tp.wide ? sizeof(int) * (size_t)tp.nt * k : 0;
     by[B_NNS] = tp.wide ? sizeof(double) * (size_t)tp.nt * k : 0;
     by[B_WKEY] = tp.gstride ? sizeof(unsigned long long) * (size_t)tp.nt * tp.gstride : 0;
     by[B_WIDX] = tp.gstride ? sizeof(int) * (size_t)tp.nt * tp.gstride : 0;
+    by[B_GRAM] = gram ? gram_dense_bytes(tp.n, tp.nt, T) : 0;
   }
   return 0;
 }
@@ -779,6 +797,7 @@ static int bind_task(TaskPlan& tp, const edm_options* o, Get&& get)
   Sm.nn_stride = F.nn_stride;
   Sm.big = (double*)tp.ptr[B_SMAPBIG];
   Sm.big_blocks = Sm.big ? SMAP_BIG_BLOCKS : 0;
+  Sm.gram = nullptr;
 
   if (tp.route == R_MFMA) {
     MfmaParams& MP = tp.MP;
@@ -882,22 +901,35 @@ static int launch_task(TaskPlan& tp, hipStream_t s, hipEvent_t* ev, int (*keep_g
   double* D = (double*)tp.ptr[B_D];
   unsigned long long* gkey = (unsigned long long*)tp.ptr[B_WKEY];
   int* gidx = (int*)tp.ptr[B_WIDX];
-  if (ev)
+  // stage events: with one block of targets, [0, 1] the distances, [1, 2] the neighbour policy (and
+  // the wide selection), [2, 3] the S-map systems and solves; with several, [1, 2] spans the loop
+  const bool one = NP <= tp.nt;
+  if (ev && !one)
     HIP_TRY(hipEventRecord(ev[1], s));
   for (int t0 = 0; t0 < NP; t0 += tp.nt) {
     const int cnt = std::min(tp.nt, NP - t0);
     if (tp.wass)
       hipLaunchKernelGGL(wasserstein_dense, dim3(cnt, tp.S), dim3(256), 0, s, tp.W, t0, cnt, tp.sl, D, pc);
     else
-      launch_dense(tp.emax, tp.generic, dim3((cnt + 255) / 256, tp.S), s, tp.P, t0, cnt, tp.sl, D, pc);
+      launch_dense(tp.emax, tp.generic, tp.S, s, tp.P, t0, cnt, tp.sl, D, pc);
     HIP_TRY(hipGetLastError());
+    if (ev && one)
+      HIP_TRY(hipEventRecord(ev[1], s));
     hipLaunchKernelGGL(finish_dense, dim3((cnt + 255) / 256), dim3(256), 0, s, F, D, t0, cnt, pc, tp.S);
     HIP_TRY(hipGetLastError());
     if (tp.wide) {
       hipLaunchKernelGGL(select_wide, dim3(cnt), dim3(256), 0, s, F, D, t0, cnt, gkey, gidx, tp.gstride);
       HIP_TRY(hipGetLastError());
     }
+    if (ev && one)
+      HIP_TRY(hipEventRecord(ev[2], s));
     Sm.nn_t0 = t0;
+    if (tp.smap && tp.ptr[B_GRAM]) {
+      Sm.gram = (const double*)tp.ptr[B_GRAM];
+      const int r = gram_dense_launch(tp.n, s, Sm, D, t0, cnt, (double*)tp.ptr[B_GRAM]);
+      if (r)
+        return r;
+    }
     if (tp.smap) {
       const int r = smap_launch(tp.n, s, Sm, D, t0, cnt, tp.smap_lds);
       if (r)
@@ -908,7 +940,8 @@ static int launch_task(TaskPlan& tp, hipStream_t s, hipEvent_t* ev, int (*keep_g
       break;
   }
   if (ev) {
-    HIP_TRY(hipEventRecord(ev[2], s));
+    if (!one)
+      HIP_TRY(hipEventRecord(ev[2], s));
     HIP_TRY(hipEventRecord(ev[3], s));
   }
   return 0;

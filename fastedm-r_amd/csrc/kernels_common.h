@@ -17,6 +17,7 @@ constexpr double SQRT_GUARD = 1.0 + 1.7763568394002505e-15;
 constexpr int LDS_CHUNK_DOUBLES = 4096; // 32 KiB library chunk per workgroup
 constexpr int LDS_CHUNK_MAXPTS = 1024;
 constexpr int MAX_SLICES = 64;
+constexpr int DT_TGT = 64; // targets per block of the dense distance kernel
 constexpr int SMAP_BIG_MAX = 128;    // widest S-map system (unknowns = E_actual + 1) the solve takes
 constexpr int SMAP_BIG_BLOCKS = 256; // blocks of the wide S-map kernel (one system each at a time)
 
@@ -125,6 +126,8 @@ struct SmapParams
   double* coeffs; // [NP x (Ea+1)] or null
   double* big;    // smap_big: [blocks x 2 n^2] scratch for the transposed U and V of each block's system
   int big_blocks; // blocks that scratch holds
+  const double* gram; // gram_dense: [items x (n^2 + n)] X^T X and X^T y of the dense block's all-neighbour
+                      // items (nn_k = -1), or null; the solve kernels copy those instead of forming them
 };
 
 // Matrix-core k-NN (knn_mfma): FP32 screening operands, the FP64 rows of the exact re-rank and the

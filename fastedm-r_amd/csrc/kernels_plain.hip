@@ -16,12 +16,12 @@
 namespace fedm {
 
 // ----------------------------------------------------------------------------------------------
-// Pair distance.  The library point comes from LDS (xs), the target from registers (y).
-// Returns the accumulated distance, or MISSING_D when the pair is skipped or has a missing
-// coordinate without missingdistance (both are dropped by the caller's keep rule).
-template <int EMAX, bool GENERIC>
-__device__ __forceinline__ double pair_acc(const LpParams& P, const double* __restrict__ xs, const double (&y)[EMAX],
-                                           int i, int lib_panel, double lib_tt, int tgt_panel, double tgt_pt)
+// Pair distance of library point x and target y, coordinate j read by lib(j) and tgt(j).  Returns the
+// accumulated distance, or MISSING_D when the pair is skipped or has a missing coordinate without
+// missingdistance (both are dropped by the caller's keep rule).
+template <int EMAX, bool GENERIC, class LIB, class TGT>
+__device__ __forceinline__ double pair_acc_core(const LpParams& P, LIB lib, TGT tgt, int lib_panel, double lib_tt,
+                                                int tgt_panel, double tgt_pt)
 {
   double acc = 0.0;
   if (GENERIC) {
@@ -38,16 +38,16 @@ __device__ __forceinline__ double pair_acc(const LpParams& P, const double* __re
 #pragma unroll
     for (int j = 0; j < EMAX; j++) {
       if (j < P.Ea) {
-        const double x = xs[j];
+        const double x = lib(j), y = tgt(j);
         double d;
-        if ((f & F_MISSING) && (x == MISSING_D || y[j] == MISSING_D)) {
+        if ((f & F_MISSING) && (x == MISSING_D || y == MISSING_D)) {
           if (P.missing_distance == 0.0)
             return MISSING_D;
           d = P.missing_distance;
         } else if ((f & F_CATEGORICAL) && P.metrics[j] == 1) {
-          d = (double)(x != y[j]);
+          d = (double)(x != y);
         } else {
-          d = x - y[j];
+          d = x - y;
         }
         if (mae)
           acc += fabs(d) / (double)P.Ea;
@@ -59,12 +59,21 @@ __device__ __forceinline__ double pair_acc(const LpParams& P, const double* __re
 #pragma unroll
     for (int j = 0; j < EMAX; j++) {
       if (j < P.Ea) {
-        const double d = xs[j] - y[j];
+        const double d = lib(j) - tgt(j);
         acc += d * d;
       }
     }
   }
   return acc;
+}
+
+// The library point from LDS (xs), the target from registers (y).
+template <int EMAX, bool GENERIC>
+__device__ __forceinline__ double pair_acc(const LpParams& P, const double* __restrict__ xs, const double (&y)[EMAX],
+                                           int i, int lib_panel, double lib_tt, int tgt_panel, double tgt_pt)
+{
+  return pair_acc_core<EMAX, GENERIC>(
+    P, [&](int j) { return xs[j]; }, [&](int j) { return y[j]; }, lib_panel, lib_tt, tgt_panel, tgt_pt);
 }
 
 // Lexicographic (s, idx) order of the reference's k-NN (src/edm.cpp:663-668).
@@ -189,56 +198,65 @@ __global__ __launch_bounds__(256) void knn_slices_f64(LpParams P, int k, int sli
 }
 
 // ----------------------------------------------------------------------------------------------
-// Dense distances (k = all): D[t][i] = s, or -1 where the pair is not a valid neighbour.
+// Dense distances (k = all, or k wider than the register lists): D[t][i] = the distance s, or -1 where
+// the pair is not a valid neighbour.  Library-major: lane = library point (its coordinates in
+// registers), the block's DT_TGT targets staged in LDS, where every lane reads the same target at once.  A wavefront's stores of one target's row are 64 consecutive doubles.
+// Block (x, y) takes library slice x (slice_len, a multiple of 256, points in tiles of 256) and targets
+// y * DT_TGT .. + DT_TGT of the block of nt; out_cnt[x][t] = the slice's valid pairs of target t.
 template <int EMAX, bool GENERIC>
-__global__ __launch_bounds__(256) void dense_dist_f64(LpParams P, int t0, int nt, int slice_len, double* __restrict__ D,
-                                                     int* __restrict__ out_cnt)
+__global__ __launch_bounds__(256) void dense_dist_f64(LpParams P, int t0, int nt, int slice_len,
+                                                      double* __restrict__ D, int* __restrict__ out_cnt)
 {
-  __shared__ double lds_x[LDS_CHUNK_DOUBLES];
-  __shared__ int lds_panel[LDS_CHUNK_MAXPTS];
-  __shared__ double lds_tt[LDS_CHUNK_MAXPTS];
-
-  const int tl = blockIdx.x * blockDim.x + threadIdx.x;
-  const int t = t0 + tl;
-  const bool active = tl < nt && t < P.NP;
-  const int lo = blockIdx.y * slice_len;
-  const int hi = min(P.NL, lo + slice_len);
-  double y[EMAX];
-#pragma unroll
-  for (int j = 0; j < EMAX; j++)
-    y[j] = (active && j < P.Ea) ? P.Mp[(size_t)t * P.Ea + j] : 0.0;
-  const int tgt_panel = (GENERIC && active && P.Mppanel) ? P.Mppanel[t] : 0;
-  const double tgt_pt = (GENERIC && active && P.Mppt) ? P.Mppt[t] : 0.0;
+  __shared__ double ys[DT_TGT * EMAX];
+  __shared__ int tpanel[DT_TGT];
+  __shared__ double tpt[DT_TGT];
+  __shared__ int wcnt[4][DT_TGT];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int tb = blockIdx.y * DT_TGT;
+  const int ntb = min(DT_TGT, nt - tb);
+  const int Ea = P.Ea;
+  for (int e = tid; e < ntb * Ea; e += 256)
+    ys[e] = P.Mp[(size_t)(t0 + tb) * Ea + e];
+  if (tid < DT_TGT) {
+    tpanel[tid] = (GENERIC && tid < ntb && P.Mppanel) ? P.Mppanel[t0 + tb + tid] : 0;
+    tpt[tid] = (GENERIC && tid < ntb && P.Mppt) ? P.Mppt[t0 + tb + tid] : 0.0;
+  }
+  for (int e = tid; e < 4 * DT_TGT; e += 256)
+    wcnt[e / DT_TGT][e % DT_TGT] = 0;
+  __syncthreads();
   const bool mae = GENERIC && (P.flags & F_MAE);
-  const int ch = min(LDS_CHUNK_MAXPTS, LDS_CHUNK_DOUBLES / P.Ea);
-  int cnt = 0;
-  double* row = D + (size_t)tl * P.NL;
-  for (int c0 = lo; c0 < hi; c0 += ch) {
-    const int cn = min(ch, hi - c0);
-    __syncthreads();
-    stage_chunk(P, lds_x, lds_panel, lds_tt, c0, cn, GENERIC);
-    __syncthreads();
-    if (active) {
-      for (int c = 0; c < cn; c++) {
-        const double acc = pair_acc<EMAX, GENERIC>(P, lds_x + c * P.Ea, y, c0 + c, GENERIC ? lds_panel[c] : 0,
-                                                   GENERIC ? lds_tt[c] : 0.0, tgt_panel, tgt_pt);
-        double s = -1.0;
-        if (acc != 0.0 && acc != MISSING_D) {
-          cnt++;
-          s = mae ? acc : sqrt(acc);
-        }
-        row[c0 + c] = s;
-      }
+  const int lo = blockIdx.x * slice_len, hi = min(P.NL, lo + slice_len);
+  for (int i0 = lo; i0 < hi; i0 += 256) {
+    const int i = i0 + tid;
+    const bool inlib = i < hi;
+    double x[EMAX];
+#pragma unroll
+    for (int j = 0; j < EMAX; j++)
+      x[j] = (inlib && j < Ea) ? P.M[(size_t)i * Ea + j] : 0.0;
+    const int lib_panel = (GENERIC && inlib && P.Mpanel) ? P.Mpanel[i] : 0;
+    const double lib_tt = (GENERIC && inlib && P.Mtt) ? P.Mtt[i] : 0.0;
+    for (int tl = 0; tl < ntb; tl++) {
+      const double* yt = ys + tl * Ea;
+      const double acc = pair_acc_core<EMAX, GENERIC>(
+        P, [&](int j) { return x[j]; }, [&](int j) { return yt[j]; }, lib_panel, lib_tt, tpanel[tl], tpt[tl]);
+      const bool v = inlib && acc != 0.0 && acc != MISSING_D;
+      if (inlib)
+        D[(size_t)(tb + tl) * P.NL + i] = v ? (mae ? acc : sqrt(acc)) : -1.0;
+      const unsigned long long bb = __ballot(v);
+      if (lane == 0)
+        wcnt[wv][tl] += __popcll(bb);
     }
   }
-  if (active)
-    out_cnt[(size_t)blockIdx.y * P.NP + t] = cnt;
+  __syncthreads();
+  if (tid < ntb)
+    out_cnt[(size_t)blockIdx.x * P.NP + t0 + tb + tid] = wcnt[0][tid] + wcnt[1][tid] + wcnt[2][tid] + wcnt[3][tid];
 }
 
 // ----------------------------------------------------------------------------------------------
 // Dense distances for points wider than the register rows (E_actual > 64): pair_acc's arithmetic with
 // the target's coordinates read from memory one at a time (they are shared by the block's threads only
-// through the cache) and the library rows staged in LDS in chunks.  Same output as dense_dist_f64.
+// through the cache) and the library rows staged in LDS in chunks, one target per thread (grid: target
+// blocks of 256 x library slices).  Same output as dense_dist_f64.
 template <bool GENERIC>
 __device__ __forceinline__ double pair_acc_rows(const LpParams& P, const double* __restrict__ xs,
                                                 const double* __restrict__ y, int lib_panel, double lib_tt,

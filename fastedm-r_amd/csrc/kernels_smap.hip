@@ -223,6 +223,17 @@ __device__ unsigned form_group(const SmapParams& S, const double* __restrict__ D
     if (kk == 0)
       continue;
     valid |= 1u << g;
+    if (kk < 0 && S.gram) {
+      // formed by gram_dense
+      const double* src = S.gram + (size_t)item * (n * n + n);
+      double* Wg = W + (size_t)g * n * n;
+      for (int e = lane; e < n * n; e += 64)
+        Wg[e] = src[e];
+      for (int e = lane; e < n; e += 64)
+        bvec[g * n + e] = src[n * n + e];
+      __syncthreads();
+      continue;
+    }
     const bool dense = kk < 0;
     const int k = dense ? S.nn_cnt[t] : kk;
     const double* drow = dense ? D + (size_t)tl * S.NL : nullptr;
@@ -965,6 +976,312 @@ __global__ __launch_bounds__(64) void smap_hyb(SmapParams S, const double* __res
 // Launches the S-map kernel for n = E_a + 1 unknowns over targets [t0, t0 + nt) at every theta:
 // register rows for n <= 24, U rows in registers for 33 <= n <= 48, LDS rows otherwise.  A persistent grid of one-wavefront blocks, as many as
 // stay resident.  Returns 0, or -22 when n is beyond 64 (a system needs a lane per unknown).
+// ---------------------------------------------------------------------------------------------
+// Means of the valid distances of the targets of a dense block, each summed in library order (the sum
+// form_group makes for an all-neighbour item).  A block takes DM_ROWS targets; their rows go through LDS
+// DM_COLS columns at a time (coalesced; the next tile's loads are in flight while lane r adds row r's
+// entries of the current one, one by one).
+constexpr int DM_ROWS = 8, DM_COLS = 512, DM_THREADS = 256, DM_PER = DM_ROWS * DM_COLS / DM_THREADS;
+
+__global__ __launch_bounds__(DM_THREADS) void dense_means(SmapParams S, const double* __restrict__ D, int t0,
+                                                          int nt, double* __restrict__ mean)
+{
+  __shared__ double tile[DM_ROWS * (DM_COLS + 1) + 32];
+  const int tid = threadIdx.x, r0 = blockIdx.x * DM_ROWS;
+  const int rows = min(DM_ROWS, nt - r0);
+  auto load = [&](int c0, double (&v)[DM_PER]) {
+#pragma unroll
+    for (int u = 0; u < DM_PER; u++) {
+      const int e = tid + u * DM_THREADS, r = e / DM_COLS, c = c0 + e - r * DM_COLS;
+      v[u] = r < rows && c < S.NL ? D[(size_t)(r0 + r) * S.NL + c] : -1.0;
+    }
+  };
+  double v[DM_PER];
+  load(0, v);
+  double sum = 0.0;
+  for (int c0 = 0; c0 < S.NL; c0 += DM_COLS) {
+    // an entry that is not a neighbour (negative) enters the tile as +0.0: adding it leaves the sum as
+    // it is (the sum starts at +0.0 and so is never -0.0), and the chain of adds needs no selects
+#pragma unroll
+    for (int u = 0; u < DM_PER; u++) {
+      const int e = tid + u * DM_THREADS, r = e / DM_COLS, c = e - r * DM_COLS;
+      tile[r * (DM_COLS + 1) + c] = v[u] >= 0.0 ? v[u] : 0.0;
+    }
+    __syncthreads();
+    if (c0 + DM_COLS < S.NL)
+      load(c0 + DM_COLS, v);
+    if (tid < rows) {
+      // the reads of the next 16 entries are in flight while the current 16 are added (the tile's tail
+      // pads the last reads)
+      const double* tr = tile + tid * (DM_COLS + 1);
+      double a[16], b[16];
+#pragma unroll
+      for (int u = 0; u < 16; u++)
+        a[u] = tr[u];
+      for (int c = 0; c < DM_COLS; c += 32) {
+#pragma unroll
+        for (int u = 0; u < 16; u++)
+          b[u] = tr[c + 16 + u];
+#pragma unroll
+        for (int u = 0; u < 16; u++)
+          sum += a[u];
+#pragma unroll
+        for (int u = 0; u < 16; u++)
+          a[u] = tr[c + 32 + u];
+#pragma unroll
+        for (int u = 0; u < 16; u++)
+          sum += b[u];
+      }
+    }
+    __syncthreads();
+  }
+  if (tid < rows)
+    mean[r0 + tid] = sum / (double)S.nn_cnt[t0 + r0 + tid];
+}
+
+// X^T X and X^T y of the all-neighbour items of a dense block (nn_k = -1: every valid library point, in
+// library order), for n <= 64.  One wavefront takes GI consecutive (target, theta) items (gd_items), so
+// that every lane owns PER fixed entries of them over the whole library and keeps their sums in
+// registers.  Library points go by in chunks of 64, one per lane: each lane weights its point for every
+// item and stores the weighted row [w, w x_1 .. w x_E, w y] coordinate-major in LDS (zeros for a point
+// that is not a neighbour); then every lane adds the chunk's 64 products to each of its entries, in
+// library order.  Entry (p, q) is the sum over the neighbours of (w x_p)(w x_q), w = exp(-theta d /
+// mean(d)), with the products and sums of form_group in its order, so the solve sees the same bits: a
+// zero row adds +0.0, which leaves a sum unchanged (it starts at +0.0 and so is never -0.0).
+// Output gw[item] = [W (n x n, both triangles) | b (n)].
+constexpr int GD_MAXG = 16; // items a wavefront takes at most
+constexpr int GD_RS = 66;   // LDS row stride (doubles): paired reads of up to 16 rows fall in different banks
+
+__host__ __device__ constexpr int gd_nacc(int n)
+{
+  return n * (n + 1) / 2 + n;
+}
+__host__ __device__ constexpr int gd_items(int n)
+{
+  return gd_nacc(n) >= 64 ? 1 : (64 / gd_nacc(n) < GD_MAXG ? 64 / gd_nacc(n) : GD_MAXG);
+}
+__host__ __device__ constexpr int gd_per(int n)
+{
+  return (gd_items(n) * gd_nacc(n) + 63) / 64;
+}
+
+// Stores the lane's weighted row [w, w x_1 .. w x_E, w y] for each of the GI items; with SEL, zeros for
+// an item whose distance d is negative (not a neighbour: its row may hold NaNs).  mrow is the point's
+// row of the library (row 0 for a lane past its end).
+template <int GI, bool SEL>
+__device__ __forceinline__ void gd_stage(double* smem, int n, int lane, const double (&w)[GI], const double (&d)[GI],
+                                         const double (&m0)[8], const double* mrow, double y)
+{
+  const int Ea = n - 1;
+#pragma unroll
+  for (int g = 0; g < GI; g++)
+    smem[(g * (n + 1)) * GD_RS + lane] = !SEL || d[g] >= 0.0 ? w[g] : 0.0;
+  for (int j0 = 0; j0 < Ea; j0 += 8) {
+    // coordinates 0 .. 7 come fetched (m0); the others are read here
+    double m[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++)
+      m[u] = j0 == 0 ? m0[u] : mrow[min(j0 + u, Ea - 1)]; // in bounds: a row of the library
+#pragma unroll
+    for (int u = 0; u < 8; u++)
+      if (j0 + u < Ea)
+#pragma unroll
+        for (int g = 0; g < GI; g++)
+          smem[(g * (n + 1) + 1 + j0 + u) * GD_RS + lane] = !SEL || d[g] >= 0.0 ? w[g] * m[u] : 0.0;
+  }
+#pragma unroll
+  for (int g = 0; g < GI; g++)
+    smem[(g * (n + 1) + n) * GD_RS + lane] = !SEL || d[g] >= 0.0 ? w[g] * y : 0.0;
+}
+
+template <int PER, int GI>
+__global__ __launch_bounds__(64) void gram_dense(SmapParams S, const double* __restrict__ D, int t0, int nt,
+                                                 const double* __restrict__ mean, double* __restrict__ gw)
+{
+  extern __shared__ double smem[]; // [GI][n + 1][GD_RS]: rows 0 .. n - 1 weighted coordinates, row n weighted y
+  const int n = S.Ea + 1, Ea = S.Ea, T = S.T, lane = threadIdx.x;
+  const int ntri = n * (n + 1) / 2, nacc = ntri + n;
+  __shared__ double s_mean[GD_MAXG], s_theta[GD_MAXG];
+  __shared__ int s_row[GD_MAXG];
+  // this lane's entries: item g, rows p and q of the staged chunk (q = n: X^T y); oa/ob their LDS rows
+  int ent[PER], oa[PER], ob[PER];
+#pragma unroll
+  for (int u = 0; u < PER; u++) {
+    ent[u] = -1;
+    oa[u] = ob[u] = 0;
+    const int e = lane + 64 * u;
+    if (e < GI * nacc) {
+      const int g = e / nacc, r = e - g * nacc;
+      int p = 0, q = n;
+      if (r < ntri) {
+        int rem = r;
+        while (rem >= n - p) {
+          rem -= n - p;
+          p++;
+        }
+        q = p + rem;
+      } else {
+        p = r - ntri;
+      }
+      ent[u] = g << 16 | p << 8 | q;
+      oa[u] = (g * (n + 1) + p) * GD_RS;
+      ob[u] = (g * (n + 1) + q) * GD_RS;
+    }
+  }
+  const int items = nt * T;
+  for (int i0 = blockIdx.x * GI; i0 < items; i0 += gridDim.x * GI) {
+    // per item: its distance row (-1: no all-neighbour system), mean and theta
+    if (lane < GI) {
+      const int item = i0 + lane;
+      int row = -1;
+      if (item < items) {
+        const int tl = item / T, th = item - (item / T) * T, t = t0 + tl;
+        if (S.nn_k[t] < 0) {
+          row = tl;
+          s_mean[lane] = mean[tl];
+          s_theta[lane] = S.thetas[th];
+        }
+      }
+      s_row[lane] = row;
+    }
+    __syncthreads();
+    double acc[PER];
+#pragma unroll
+    for (int u = 0; u < PER; u++)
+      acc[u] = 0.0;
+    // the inputs of this lane's point of the next chunk, fetched a chunk ahead: its distance to each
+    // item's target (items of one target share the load), its first 8 coordinates and its target value
+    double nd[GI], nm[8], ny;
+    auto fetch = [&](int c0) {
+      const int pos = c0 + lane;
+      const bool inlib = pos < S.NL;
+#pragma unroll
+      for (int g = 0; g < GI; g++)
+        nd[g] = (g == 0 || s_row[g] != s_row[g - 1]) && inlib && s_row[g] >= 0
+                  ? D[(size_t)s_row[g] * S.NL + pos]
+                  : -1.0;
+      const double* mrow = S.M + (size_t)(inlib ? pos : 0) * Ea;
+#pragma unroll
+      for (int u = 0; u < 8; u++)
+        nm[u] = mrow[min(u, Ea - 1)]; // a row of the library (row 0 past its end): in bounds
+      ny = inlib ? S.targets[pos] : 0.0;
+    };
+    fetch(0);
+    for (int c0 = 0; c0 < S.NL; c0 += 64) {
+      const int pos = c0 + lane;
+      const bool inlib = pos < S.NL;
+      double d[GI], m[8], q[GI], w[GI];
+#pragma unroll
+      for (int g = 0; g < GI; g++)
+        d[g] = nd[g];
+#pragma unroll
+      for (int u = 0; u < 8; u++)
+        m[u] = nm[u];
+      const double y = ny;
+      if (c0 + 64 < S.NL)
+        fetch(c0 + 64);
+      // the weight of the point for each item (items of one target share the quotient d / mean)
+#pragma unroll
+      for (int g = 0; g < GI; g++) {
+        if (g > 0 && s_row[g] == s_row[g - 1]) {
+          d[g] = d[g - 1];
+          q[g] = q[g - 1];
+        } else {
+          q[g] = fmax(d[g], 0.0) / s_mean[g];
+        }
+      }
+      bool ok = inlib;
+#pragma unroll
+      for (int g = 0; g < GI; g++) {
+        w[g] = fdlibm_exp(-s_theta[g] * q[g]);
+        ok = ok && d[g] >= 0.0;
+      }
+      const double* mrow = S.M + (size_t)(inlib ? pos : 0) * Ea;
+      if (__all(ok)) {
+        gd_stage<GI, false>(smem, n, lane, w, d, m, mrow, y);
+      } else {
+        gd_stage<GI, true>(smem, n, lane, w, d, m, mrow, y);
+      }
+      __syncthreads();
+#pragma unroll
+      for (int u = 0; u < PER; u++) {
+        const double* xa = smem + oa[u];
+        const double* xb = smem + ob[u];
+        double a = acc[u];
+#pragma unroll 16
+        for (int s2 = 0; s2 < 64; s2++)
+          a += xa[s2] * xb[s2];
+        acc[u] = a;
+      }
+      __syncthreads();
+    }
+#pragma unroll
+    for (int u = 0; u < PER; u++) {
+      const int en = ent[u];
+      if (en < 0)
+        continue;
+      const int g = en >> 16, p = (en >> 8) & 255, q = en & 255;
+      if (s_row[g] < 0)
+        continue;
+      double* out = gw + (size_t)(i0 + g) * (n * n + n);
+      if (q < n) {
+        out[p * n + q] = acc[u];
+        out[q * n + p] = acc[u];
+      } else {
+        out[n * n + p] = acc[u];
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// Forms the all-neighbour systems of targets t0 .. t0 + nt - 1 into gw (the scratch gram_dense_bytes
+// gives, whose tail holds the means); the solve kernels then copy them (S.gram = gw).
+size_t gram_dense_bytes(int n, int nt, int T)
+{
+  return sizeof(double) * ((size_t)nt * T * ((size_t)n * n + n) + (size_t)nt);
+}
+
+int gram_dense_launch(int n, hipStream_t s, const SmapParams& S, const double* D, int t0, int nt, double* gw)
+{
+  if (n > 64 || n < 2)
+    return -22;
+  double* mean = gw + (size_t)nt * S.T * ((size_t)n * n + n);
+  hipLaunchKernelGGL(dense_means, dim3((nt + DM_ROWS - 1) / DM_ROWS), dim3(DM_THREADS), 0, s, S, D, t0, nt, mean);
+  hipError_t e = hipGetLastError();
+  const int GI = gd_items(n), per = gd_per(n);
+  const void* fn = GI == 12   ? (const void*)gram_dense<1, 12>
+                   : GI == 7  ? (const void*)gram_dense<1, 7>
+                   : GI == 4  ? (const void*)gram_dense<1, 4>
+                   : GI == 3  ? (const void*)gram_dense<1, 3>
+                   : GI == 2  ? (const void*)gram_dense<1, 2>
+                   : GI != 1  ? nullptr
+                   : per <= 1 ? (const void*)gram_dense<1, 1>
+                   : per <= 2 ? (const void*)gram_dense<2, 1>
+                   : per <= 3 ? (const void*)gram_dense<3, 1>
+                   : per <= 6 ? (const void*)gram_dense<6, 1>
+                   : per <= 12 ? (const void*)gram_dense<12, 1>
+                               : (const void*)gram_dense<36, 1>;
+  if (!fn)
+    return -22;
+  const size_t lds = sizeof(double) * (size_t)GI * (n + 1) * GD_RS;
+  const long long groups = ((long long)nt * S.T + GI - 1) / GI;
+  int dev = 0, ncu = 0, occ = 0;
+  if (e == hipSuccess)
+    e = hipGetDevice(&dev);
+  if (e == hipSuccess)
+    e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+  if (e == hipSuccess)
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fn, 64, lds);
+  if (e != hipSuccess)
+    return -(int)e - 1000;
+  const int grid = (int)std::max<long long>(1, std::min<long long>(groups, (long long)ncu * std::max(occ, 1)));
+  const double* cmean = mean;
+  void* args[] = { (void*)&S, (void*)&D, (void*)&t0, (void*)&nt, (void*)&cmean, (void*)&gw };
+  e = hipLaunchKernel(fn, dim3(grid), dim3(64), args, lds, s);
+  return e == hipSuccess ? 0 : -(int)e - 1000;
+}
+
 // ---------------------------------------------------------------------------------------------
 // Wide systems (65 <= n <= SMAP_BIG_MAX unknowns): one (target, theta) system per 256-thread block, a
 // grid-stride loop over the items.  W lives in LDS; U and V live transposed in the block's slice of

@@ -216,7 +216,10 @@ int edm_selftest_mfma(float* out256);
 int edm_selftest_exp(const double* x, double* out, int n);
 
 /* Last stage timings of the most recent call on ctx (milliseconds, HIP events on the ctx stream):
- * [0] distance/selection kernel, [1] merge/predict kernel, [2] S-map kernel, [3] whole device section. */
+ * [0] distance/selection kernel, [1] merge/predict kernel, [2] S-map kernel, [3] whole device section.
+ * On the dense route (k = all, or k > 64) with one block of targets: [0] the distance rows, [1] the
+ * neighbour policy and wide selection, [2] the S-map systems and solves; with several blocks, [1]
+ * spans them all. */
 void edm_ctx_timings(edm_ctx* ctx, double* ms4);
 /* The same with the persistent route's direct stage split: [4] the direct kernel, [5] the spatially
  * pruned direct kernel, [6] the exact re-selection ([7] unused). */

@@ -63,6 +63,16 @@ CASES = {
     "smap_E80_k82_mixed": dict(NL=500, NP=30, Ea=80, k=82, algorithm=1, thetas=(1.0,), precision=1),
     "smap_E80_all": dict(NL=200, NP=20, Ea=80, k=-1, algorithm=1, thetas=(1.0,), save_coeffs=True),
     "smap_E127_k130": dict(NL=400, NP=12, Ea=127, k=130, algorithm=1, thetas=(1.0,), save_coeffs=True),
+    # all-neighbour S-map systems formed ahead of the solve (one size class of entries per lane each)
+    "smap_all_E1_8thetas": dict(NL=700, NP=90, Ea=1, k=-1, algorithm=1, thetas=(0, .1, .5, 1, 2, 3, 4, 8),
+                                save_coeffs=True),
+    "smap_all_E8_missing_panels": dict(NL=1000, NP=150, Ea=8, k=-1, algorithm=1, thetas=(0, 1, 3), missing=0.05,
+                                       missing_distance=0.3, panels=3, save_coeffs=True),
+    "smap_all_E12": dict(NL=500, NP=60, Ea=12, k=-1, algorithm=1, thetas=(0.5, 2.0), save_coeffs=True),
+    "smap_all_E20": dict(NL=400, NP=40, Ea=20, k=-1, algorithm=1, thetas=(1.0, 3.0), save_coeffs=True),
+    "smap_all_E36": dict(NL=300, NP=30, Ea=36, k=-1, algorithm=1, thetas=(1.0,), save_coeffs=True),
+    "smap_all_E50": dict(NL=300, NP=20, Ea=50, k=-1, algorithm=1, thetas=(0.5,), save_coeffs=True),
+    "smap_all_E63": dict(NL=260, NP=20, Ea=63, k=-1, algorithm=1, thetas=(1.0,), save_coeffs=True),
 }
 
 
