@@ -62,7 +62,9 @@ struct edm_ctx
 {
   int device = 0;
   hipStream_t stream = nullptr;
-  hipEvent_t ev[10] = {};
+  hipStream_t stream2 = nullptr; // the persistent route's direct kernels, beside the rank-table kernels
+  hipEvent_t fork = nullptr, join = nullptr;
+  hipEvent_t ev[12] = {};
   std::vector<DevBuf> bufs;
   double timings[8] = { 0, 0, 0, 0, 0, 0, 0, 0 };
   int last_error = 0;
@@ -284,6 +286,9 @@ extern "C" void fedm_default_knobs(fedm_knobs* k)
   const char* e = getenv("FEDM_SMAP_LDS_ROWS");
   k->smap_lds = e && e[0] == '1';
   k->direct_prune = 1;
+  // measured: the two routes each fill the device, so running them side by side gains nothing (5.66 ms
+  // against 5.55 ms on the 40-library CCM sweep); the tuner can still choose it
+  k->concurrent_routes = 0;
 }
 
 static fedm_knobs default_knobs()
@@ -334,8 +339,15 @@ int edm_ctx_create(edm_ctx** out, int device)
     delete c;
     return -1;
   }
+  if (hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking) != hipSuccess) {
+    hipStreamDestroy(c->stream);
+    delete c;
+    return -1;
+  }
   for (auto& e : c->ev)
     hipEventCreate(&e);
+  hipEventCreateWithFlags(&c->fork, hipEventDisableTiming);
+  hipEventCreateWithFlags(&c->join, hipEventDisableTiming);
   c->bufs.resize(B_NUM);
   c->scratch_err = scratch_address(device, &c->scratch);
   *out = c;
@@ -351,8 +363,12 @@ void edm_ctx_destroy(edm_ctx* c)
   for (auto& b : c->bufs)
     if (b.p)
       hipFree(b.p);
+  hipStreamSynchronize(c->stream2);
   for (auto& e : c->ev)
     hipEventDestroy(e);
+  hipEventDestroy(c->fork);
+  hipEventDestroy(c->join);
+  hipStreamDestroy(c->stream2);
   hipStreamDestroy(c->stream);
   delete c;
 }
@@ -1116,6 +1132,7 @@ struct edm_batch
   int (*keep_going)(void) = nullptr; // polled before each task or stage launch
   int redo_last = 0; // targets handed to the exact selection in the last run
   bool outputs_filled = false; // the outputs have held their fill values once (first run)
+  bool split_last = false;     // the last run put the direct kernels on the second stream
   // host copies for the task-by-task route
   edm_manifold_view Lv, Pv;
   edm_options o;
@@ -2064,7 +2081,17 @@ static int enqueue_device(edm_batch* b, bool timed)
   edm_ctx* c = b->ctx;
   BatchParams& B = b->B;
   hipStream_t s = c->stream;
+  // the direct kernels (short libraries, VALU-bound) on a second stream beside the rank-table kernels
+  // (long libraries, HBM-bound): the two task sets write disjoint outputs and meet at the re-selection
+  const bool split = b->kn.concurrent_routes != 0 && b->rank_route && (B.num_items > 0 || B.num_pitems > 0);
+  hipStream_t sd = split ? c->stream2 : s;
+  b->split_last = split;
   auto stop = [b, timed] { return timed && b->keep_going && !b->keep_going(); };
+  auto brk = [&] {
+    if (split)
+      hipStreamWaitEvent(s, c->join, 0);
+    return batch_break(b);
+  };
   dfill(c, B.counter, 1, 0);
   if (B.num_pitems > 0)
     dfill(c, B.pcounter, (size_t)PR_QUEUES * PR_CSTRIDE, 0);
@@ -2080,6 +2107,29 @@ static int enqueue_device(edm_batch* b, bool timed)
   dfill(c, B.redo_count, 1, 0);
   if (timed)
     HIP_TRY(hipEventRecord(c->ev[4], s));
+  const void* fn = batch_kernel(b->kcap, b->Ea, b->kn.direct_alt_waves != 0);
+  void* args[] = { &B };
+  auto direct = [&]() -> int {
+    if (B.num_items > 0)
+      HIP_TRY(hipLaunchKernel(fn, dim3(b->grid), dim3(256), args, 0, sd));
+    if (timed)
+      HIP_TRY(hipEventRecord(c->ev[8], sd));
+    if (B.num_pitems > 0)
+      HIP_TRY(hipLaunchKernel(batch_pruned_kernel(b->kcap, direct_emax(b->Ea)), dim3(b->pgrid), dim3(256), args, 0,
+                              sd));
+    if (timed)
+      HIP_TRY(hipEventRecord(c->ev[9], sd));
+    return 0;
+  };
+  if (split) {
+    HIP_TRY(hipEventRecord(c->fork, s));
+    HIP_TRY(hipStreamWaitEvent(sd, c->fork, 0));
+    if (timed)
+      HIP_TRY(hipEventRecord(c->ev[10], sd));
+    if (const int r = direct())
+      return r;
+    HIP_TRY(hipEventRecord(c->join, sd));
+  }
   if (b->rank_route) {
     dfill(c, b->S.counter, 1, 0);
     void* rargs[] = { &b->R };
@@ -2087,7 +2137,7 @@ static int enqueue_device(edm_batch* b, bool timed)
     if (timed)
       HIP_TRY(hipEventRecord(c->ev[6], s));
     if (stop())
-      return batch_break(b);
+      return brk();
     void* sargs[] = { &b->S };
     if (b->multi_scan) {
       HIP_TRY(hipLaunchKernel(rank_scan_multi_kernel(b->kcap, b->S.words),
@@ -2104,17 +2154,14 @@ static int enqueue_device(edm_batch* b, bool timed)
   if (timed)
     HIP_TRY(hipEventRecord(c->ev[7], s));
   if (stop())
-    return batch_break(b);
-  const void* fn = batch_kernel(b->kcap, b->Ea, b->kn.direct_alt_waves != 0);
-  void* args[] = { &B };
-  if (B.num_items > 0)
-    HIP_TRY(hipLaunchKernel(fn, dim3(b->grid), dim3(256), args, 0, s));
-  if (timed)
-    HIP_TRY(hipEventRecord(c->ev[8], s));
-  if (B.num_pitems > 0)
-    HIP_TRY(hipLaunchKernel(batch_pruned_kernel(b->kcap, direct_emax(b->Ea)), dim3(b->pgrid), dim3(256), args, 0, s));
-  if (timed)
-    HIP_TRY(hipEventRecord(c->ev[9], s));
+    return brk();
+  if (split) {
+    HIP_TRY(hipStreamWaitEvent(s, c->join, 0));
+    if (timed)
+      HIP_TRY(hipEventRecord(c->ev[11], s));
+  } else if (const int r = direct()) {
+    return r;
+  }
   {
     // targets handed over by the direct kernel or the rank scan: exact selection
     const int2* none = nullptr;
@@ -2202,15 +2249,18 @@ int edm_batch_run(edm_batch* b)
   }
   b->runs++;
   b->last_ms = ev_ms(c->ev[4], c->ev[5]);
-  // [0] whole batch, [1] rank-table build, [2] rank scan, [3] direct kernel (stages: eager runs only)
+  // [0] whole batch, [1] rank-table build, [2] rank scan, [3] direct kernels, [4] the unpruned direct
+  // kernel, [5] the pruned one, [6] the exact re-selection (stages: eager runs only); with the routes
+  // concurrent, [1]-[2] and [3]-[5] overlap in time and each kernel shares the device with the others
+  const bool sp = b->split_last;
   c->timings[0] = b->last_ms;
   c->timings[1] = graphed ? 0.0 : ev_ms(c->ev[4], c->ev[6]);
   c->timings[2] = graphed ? 0.0 : ev_ms(c->ev[6], c->ev[7]);
-  c->timings[3] = graphed ? 0.0 : ev_ms(c->ev[7], c->ev[5]);
-  // within the direct stage: the unpruned kernel, the pruned kernel, the exact re-selection
-  c->timings[4] = graphed ? 0.0 : ev_ms(c->ev[7], c->ev[8]);
+  c->timings[3] = graphed ? 0.0 : ev_ms(c->ev[sp ? 10 : 7], c->ev[9]);
+  c->timings[4] = graphed ? 0.0 : ev_ms(c->ev[sp ? 10 : 7], c->ev[8]);
   c->timings[5] = graphed ? 0.0 : ev_ms(c->ev[8], c->ev[9]);
-  c->timings[6] = graphed ? 0.0 : ev_ms(c->ev[9], c->ev[5]);
+  c->timings[6] = graphed ? 0.0 : ev_ms(c->ev[sp ? 11 : 9], c->ev[5]);
+  c->timings[7] = graphed ? 0.0 : (sp ? 1.0 : 0.0);
   return 0;
 }
 

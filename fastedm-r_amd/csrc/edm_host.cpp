@@ -942,9 +942,9 @@ static std::string knobs_text(const fedm_knobs& k)
   char buf[256];
   snprintf(buf, sizeof buf,
            "rank_scale=%.3f direct_blocks_per_cu=%d direct_alt_waves=%d list_blocks=%d mfma_blocks=%d use_mfma=%d "
-           "smap_lds=%d direct_prune=%d",
+           "smap_lds=%d direct_prune=%d concurrent_routes=%d",
            k.rank_scale, k.direct_blocks_per_cu, k.direct_alt_waves, k.list_blocks, k.mfma_blocks, k.use_mfma,
-           k.smap_lds, k.direct_prune);
+           k.smap_lds, k.direct_prune, k.concurrent_routes);
   return buf;
 }
 
@@ -979,6 +979,8 @@ static bool parse_knobs(const std::string& line, fedm_knobs* k)
       k->smap_lds = (int)v;
     else if (name == "direct_prune")
       k->direct_prune = (int)v;
+    else if (name == "concurrent_routes")
+      k->concurrent_routes = (int)v;
     else
       got--;
   }
@@ -1105,6 +1107,7 @@ static int tune_group(fedm_plan* P, fedm_plan::Group& g, int trials, std::string
   const int Ea = P->s.E_actual(g.E);
   if (route == 2) {
     axes.push_back({ "direct_prune", { 1, 0 } });
+    axes.push_back({ "concurrent_routes", { 1, 0 } });
     axes.push_back({ "rank_scale", { 1.0, 1.25, 1.5, 2.0, 3.0 } });
     axes.push_back({ "direct_alt_waves", { 0, 1 } });
     axes.push_back({ "direct_blocks_per_cu", { 0, 4, 2 } });
@@ -1135,6 +1138,8 @@ static int tune_group(fedm_plan* P, fedm_plan::Group& g, int trials, std::string
       k.smap_lds = (int)v;
     else if (n == "direct_prune")
       k.direct_prune = (int)v;
+    else if (n == "concurrent_routes")
+      k.concurrent_routes = (int)v;
     return k;
   };
   for (const auto& ax : axes) {

@@ -158,6 +158,8 @@ typedef struct
   int smap_lds;             /* S-map solve keeps U and V in LDS for every width (0) */
   int direct_prune;         /* persistent route: libraries of 64..512 rows (points of <= 8 coordinates) take the
                                spatially pruned direct kernel (1) */
+  int concurrent_routes;    /* persistent route: the direct kernels run on a second stream beside the
+                               rank-table kernels (0) */
 } fedm_knobs;
 void fedm_default_knobs(fedm_knobs* k);
 
@@ -215,14 +217,17 @@ int edm_selftest_mfma(float* out256);
  * CPU oracle evaluates the same expression).  Returns 0 when the probe ran. */
 int edm_selftest_exp(const double* x, double* out, int n);
 
-/* Last stage timings of the most recent call on ctx (milliseconds, HIP events on the ctx stream):
- * [0] distance/selection kernel, [1] merge/predict kernel, [2] S-map kernel, [3] whole device section.
- * On the dense route (k = all, or k > 64) with one block of targets: [0] the distance rows, [1] the
- * neighbour policy and wide selection, [2] the S-map systems and solves; with several blocks, [1]
- * spans them all. */
+/* Last stage timings of the most recent call on ctx (milliseconds, HIP events on the ctx streams).
+ * After edm_predict or a resident batch run: [0] distance/selection kernel, [1] merge/predict kernel,
+ * [2] S-map kernel, [3] whole device section.  On the dense route (k = all, or k > 64) with one block
+ * of targets: [0] the distance rows, [1] the neighbour policy and wide selection, [2] the S-map
+ * systems and solves; with several blocks, [1] spans them all.
+ * After a persistent batch run: [0] the whole run, [1] the rank-table build, [2] the rank scan, [3] the
+ * direct kernels. */
 void edm_ctx_timings(edm_ctx* ctx, double* ms4);
-/* The same with the persistent route's direct stage split: [4] the direct kernel, [5] the spatially
- * pruned direct kernel, [6] the exact re-selection ([7] unused). */
+/* The same with the persistent run's direct stage split: [4] the direct kernel, [5] the spatially
+ * pruned direct kernel, [6] the exact re-selection, [7] 1 when the direct kernels ran on the second
+ * stream beside the rank-table kernels (their spans then overlap), else 0. */
 void edm_ctx_timings_ex(edm_ctx* ctx, double* ms8);
 
 /* Diagnostic: how many (task, target) pairs the last batched run on ctx re-selected with the
