@@ -129,13 +129,16 @@ struct Series
   std::vector<std::vector<double>> extras;
   std::vector<int> panel, obs;
   int tau = 1, p = 0;
+  int n_lagged = 0; // the first n_lagged extras enter a point with E lags each (src/manifold.h:152)
   bool dt = false, reldt = false, allow_missing = false, has_panel = false, has_xmap = false;
   double w_dt = 0;
 
   int n() const { return (int)t.size(); }
   int E_dt(int E) const { return (dt || reldt) ? E : 0; }
-  int E_extras() const { return (int)extras.size(); }
-  int E_actual(int E) const { return E + E_dt(E) + E_extras(); }
+  int num_extras() const { return (int)extras.size(); }
+  int E_extras(int E) const { return num_extras() + n_lagged * (E - 1); } // extra columns of a point
+  int E_lagged_extras(int E) const { return n_lagged * E; }
+  int E_actual(int E) const { return E + E_dt(E) + E_extras(E); }
 
   static double fgcd(double a, double b) // src/manifold.cpp:7-18
   {
@@ -255,9 +258,13 @@ struct Series
           }
         }
       }
-      if (j == 0)
-        for (int e = 0; e < E_extras(); e++)
-          out[E + Edt + e] = extras[e][i];
+      // extras: the lagged ones at every lag, the others once (src/manifold.cpp:225-233)
+      for (int e = 0; e < num_extras(); e++) {
+        if (e < n_lagged)
+          out[E + Edt + e * E + j] = r < 0 ? MISS : extras[e][r];
+        else if (j == 0)
+          out[E + Edt + n_lagged * E + (e - n_lagged)] = extras[e][i];
+      }
       prev = lagged;
     }
   }
@@ -299,7 +306,7 @@ struct Gathered
 {
   std::vector<double> data, targets, ptimes, ttimes;
   std::vector<int> panel;
-  int n = 0, E = 0, Edt = 0, Eex = 0, Ea = 0;
+  int n = 0, E = 0, Edt = 0, Eex = 0, Elag = 0, Ea = 0;
   edm_manifold_view view() const
   {
     edm_manifold_view v;
@@ -312,7 +319,7 @@ struct Gathered
     v.E_x = E;
     v.E_dt = Edt;
     v.E_extras = Eex;
-    v.E_lagged_extras = 0;
+    v.E_lagged_extras = Elag;
     v.E_actual = Ea;
     return v;
   }
@@ -323,7 +330,8 @@ static Gathered gather(const Series& s, int E, const std::vector<bool>& filter, 
   Gathered g;
   g.E = E;
   g.Edt = s.E_dt(E);
-  g.Eex = s.E_extras();
+  g.Eex = s.E_extras(E);
+  g.Elag = s.E_lagged_extras(E);
   g.Ea = s.E_actual(E);
   const bool co = copred && prediction_set;
   std::vector<int> rows;
@@ -857,12 +865,16 @@ static int plan_tasks(edm_ctx* ctx, fedm_plan* P, const TaskGrid& grid)
     metrics.assign(Ea, 0);
     if (!P->extra_metrics.empty()) {
       if (o.distance == FEDM_DIST_WASSERSTEIN) {
+        // one per variable: x, dt, then the extras
         const int rows = 1 + (s.E_dt(g.E) > 0);
-        for (int e = 0; e < s.E_extras() && rows + e < Ea; e++)
+        for (int e = 0; e < s.num_extras() && rows + e < Ea; e++)
           metrics[rows + e] = P->extra_metrics[e];
       } else {
-        for (int e = 0; e < s.E_extras(); e++)
-          metrics[g.E + s.E_dt(g.E) + e] = P->extra_metrics[e];
+        // one per coordinate: a lagged extra's metric at each of its E lags
+        int c = g.E + s.E_dt(g.E);
+        for (int e = 0; e < s.num_extras(); e++)
+          for (int l = 0; l < (e < s.n_lagged ? g.E : 1); l++)
+            metrics[c++] = P->extra_metrics[e];
       }
     }
     edm_options& go = g.go;
@@ -889,9 +901,11 @@ static int create_group_batch(fedm_plan* P, fedm_plan::Group& g, const fedm_knob
   const Series& s = P->s;
   const int n = s.n(), Ea = s.E_actual(g.E);
   edm_manifold_view Lv{ g.table.data(), g.targets.data(), g.ptimes.data(), g.ttimes.data(),
-                        g.panel.empty() ? nullptr : g.panel.data(), n, g.E, s.E_dt(g.E), s.E_extras(), 0, Ea };
+                        g.panel.empty() ? nullptr : g.panel.data(), n, g.E, s.E_dt(g.E), s.E_extras(g.E),
+                        s.E_lagged_extras(g.E), Ea };
   edm_manifold_view Pv{ g.ptable.data(), g.targets.data(), g.ptimes.data(), g.ttimes.data(),
-                        g.panel.empty() ? nullptr : g.panel.data(), n, g.E, s.E_dt(g.E), s.E_extras(), 0, Ea };
+                        g.panel.empty() ? nullptr : g.panel.data(), n, g.E, s.E_dt(g.E), s.E_extras(g.E),
+                        s.E_lagged_extras(g.E), Ea };
   return edm_batch_create_knobs(P->ctx, &Lv, &Pv, &g.go, (int)g.members.size(), g.lb.data(), g.lc.data(),
                                 g.lr.data(), g.pb.data(), g.pc.data(), g.pr.data(), g.kk.data(), g.out_begin.data(),
                                 kn, out);
@@ -1528,8 +1542,8 @@ int fedm_run_command(edm_ctx* ctx, const fedm_command* c, fedm_result** out)
 int fedm_run_task_group(edm_ctx* ctx, const fedm_task_group* g, fedm_result** out)
 {
   *out = nullptr;
-  if (!ctx || !g || g->n <= 0 || g->num_E <= 0 || g->num_extras_lagged != 0 || !g->t || !g->x || !g->obs ||
-      !g->usable || (g->num_extras > 0 && !g->extras))
+  if (!ctx || !g || g->n <= 0 || g->num_E <= 0 || g->num_extras_lagged < 0 || g->num_extras_lagged > g->num_extras ||
+      !g->t || !g->x || !g->obs || !g->usable || (g->num_extras > 0 && !g->extras))
     return -22;
   auto P = std::make_unique<fedm_plan>();
   P->ctx = ctx;
@@ -1545,6 +1559,7 @@ int fedm_run_task_group(edm_ctx* ctx, const fedm_task_group* g, fedm_result** ou
     s.co_x.assign(g->co_x, g->co_x + n);
   for (int e = 0; e < g->num_extras; e++)
     s.extras.emplace_back(g->extras + (size_t)e * n, g->extras + (size_t)(e + 1) * n);
+  s.n_lagged = g->num_extras_lagged;
   s.obs.assign(g->obs, g->obs + n);
   if (g->panel) {
     s.panel.assign(g->panel, g->panel + n);

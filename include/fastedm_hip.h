@@ -335,7 +335,8 @@ typedef struct
   const int* obs;           /* _observation_number */
   const int* panel;         /* _panelIDs (when _panel_mode), or NULL */
   const double* extras;     /* _extras, [num_extras x n] */
-  int num_extras, num_extras_lagged; /* lagged extras are not supported (-22) */
+  int num_extras, num_extras_lagged; /* the first num_extras_lagged extras enter a point with E lags each
+                                        (src/manifold.cpp:225-233) */
   int dt, reldt;            /* _dt, _reldt */
   double dt_weight;         /* _dtWeight */
   int tau, p;               /* _tau, _p */

@@ -1235,7 +1235,7 @@ static void build_generator(const orc_command& c, Generator& g, bool& explore)
     g.extras.push_back(replace_nan(c.extras + (size_t)e * n, n));
   g.tau = c.tau;
   g.p = c.p;
-  g.num_extras_lagged = 0;
+  g.num_extras_lagged = std::max(0, std::min(c.num_extras_lagged, c.num_extras));
   g.allow_missing = c.allow_missing != 0;
   g.init(c.dt != 0, c.reldt != 0, c.dt_weight);
 }

@@ -75,6 +75,8 @@ typedef struct
   int distance;
   double aspect_ratio;
   int precision;
+  int num_extras_lagged; /* the generator's numExtrasLagged: the first ones of the extras get E lags; the R
+                            front end fixes it at 0 (src/r.cpp:246), task-group dumps may carry more */
 } orc_command;
 
 /* Task-level predictor: the edm_task fan-out over every target of Mp (reference src/edm.cpp:367-432).

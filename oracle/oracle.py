@@ -46,7 +46,8 @@ class Command(C.Structure):
                 ("dt_weight", C.c_double), ("allow_missing", C.c_int), ("missing_distance", C.c_double),
                 ("panel_weight", C.c_double), ("panel_weights", _dp), ("panel_weights_dim", C.c_int),
                 ("num_threads", C.c_int), ("low_memory", C.c_int), ("predict_with_past", C.c_int),
-                ("distance", C.c_int), ("aspect_ratio", C.c_double), ("precision", C.c_int)]
+                ("distance", C.c_int), ("aspect_ratio", C.c_double), ("precision", C.c_int),
+                ("num_extras_lagged", C.c_int)]
 
 
 _lib = None
@@ -135,8 +136,10 @@ def make_command(t, x, y=None, panel=None, E=2, tau=1, theta=1, library=None, k=
                  saveCoPredictions=False, saveManifolds=False, saveSMAPCoeffs=False, extras=None,
                  allowMissing=False, missingDistance=0.0, dt=False, reldt=False, dtWeight=0.0, numReps=1,
                  panelWeight=0, panelWeights=None, numThreads=1, lowMemory=False, predictWithPast=False,
-                 distance="euclidean", aspectRatio=1.0):
-    """Argument normalisation of R/edm.R:241-338 and the fixed choices of src/r.cpp:123-159."""
+                 distance="euclidean", aspectRatio=1.0, extrasLagged=0):
+    """Argument normalisation of R/edm.R:241-338 and the fixed choices of src/r.cpp:123-159.  extrasLagged
+    (no R argument: src/r.cpp:246 fixes it at 0) is the generator's numExtrasLagged, which task-group dumps
+    record: the first extrasLagged extras then enter the points with E lags each."""
     n = len(t)
     if len(x) != n:
         raise ValueError("The time and x variables should be the same length")
@@ -211,6 +214,7 @@ def make_command(t, x, y=None, panel=None, E=2, tau=1, theta=1, library=None, k=
     c.distance = dist
     c.aspect_ratio = float(aspectRatio)
     c.precision = 0
+    c.num_extras_lagged = int(extrasLagged)
     return c, b, explore
 
 

@@ -54,6 +54,10 @@ def calls():
         "crossfold": dict(t=ts, x=xs, E=2, crossfold=5, seed=3),
         "full_k": dict(t=t, x=x, y=y, E=2, k=5, full=True),
         "reldt_explore": dict(t=tm, x=xm, E=3, reldt=True),
+        # a lagged extra (numExtrasLagged = 1: the R front end never sets it, src/r.cpp:246, but the
+        # generator and its dumps carry it) beside an unlagged one, with missing values
+        "extras_lagged": dict(t=tm, x=xm, y=ym, E=[2, 3], extras=[u, u1], extrasLagged=1, allowMissing=True,
+                              savePredictions=True),
     }
 
 
@@ -80,7 +84,7 @@ def task_group(kw):
     generator = {
         "_dt": st["dt"], "_reldt": st["reldt"], "_dtWeight": st["dt_weight"], "_panel_mode": st["panel_mode"],
         "_xmap_mode": st["xmap"] is not None, "_tau": int(c.tau), "_p": int(c.p),
-        "_num_extras": int(c.num_extras), "_num_extras_lagged": 0,
+        "_num_extras": int(c.num_extras), "_num_extras_lagged": int(c.num_extras_lagged),
         "_x": _list(st["x"]), "_xmap": [] if st["xmap"] is None else _list(st["xmap"]),
         "_co_x": [] if st["co_x"] is None else _list(st["co_x"]), "_t": _list(st["t"]),
         "_observation_number": [int(v) for v in st["obs"]], "_extras": [_list(e) for e in st["extras"]],

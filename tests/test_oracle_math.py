@@ -286,6 +286,25 @@ def test_manifold_lags_and_targets_by_hand():
     assert tg[9] == MISSING
 
 
+def test_manifold_lagged_extras_by_hand():
+    """Point layout [x lags | lagged extras, E lags each | unlagged extras] (src/manifold.cpp:225-233): with
+    numExtrasLagged = 1 the first extra enters at every lag, the second once, a missing lag missing in both."""
+    t = np.arange(1.0, 9.0)
+    x = np.arange(1.0, 9.0)
+    a = 100.0 + np.arange(8.0)
+    b = 200.0 + np.arange(8.0)
+    c, keep = command(t, x, E=3, p=1, extras=[a, b], extrasLagged=1)
+    n, Ea = 8, 3 + 3 + 1
+    f = np.ones(n, np.int32)
+    out = np.zeros(n * Ea)
+    tg = np.zeros(n)
+    oracle.lib().orc_manifold(oracle.C.byref(c), 3, oracle.iptr(f), 1, 0, oracle.dptr(out), oracle.dptr(tg), n)
+    pts = out.reshape(n, Ea)
+    np.testing.assert_array_equal(pts[5], [6.0, 5.0, 4.0, 105.0, 104.0, 103.0, 205.0])
+    np.testing.assert_array_equal(pts[1][[2, 5]], [MISSING, MISSING])  # lag 2 of row 1 is absent
+    assert pts[1][6] == 201.0
+
+
 def test_manifold_panel_boundary_and_gap():
     t = np.array([1.0, 2, 3, 4, 1, 2, 3, 5])
     x = np.arange(1.0, 9.0)
