@@ -29,6 +29,7 @@ __global__ void dense_dist_rows(LpParams P, int t0, int nt, int slice_len, doubl
 template <int KCAP>
 __global__ void finish_lists(FinishParams F, const double* ps, const int* pi, const int* pcnt, int S);
 __global__ void finish_dense(FinishParams F, const double* D, int t0, int nt, const int* pcnt, int S);
+template <int NPL, int W_WAVES>
 __global__ void wasserstein_dense(WassParams P, int t0, int nt, int slice_len, double* D, int* out_cnt);
 __global__ void select_wide(FinishParams F, const double* D, int t0, int nt, unsigned long long* gkey, int* gidx,
                             int gstride);
@@ -519,7 +520,7 @@ static int plan_task(const edm_manifold_view* M, const edm_manifold_view* Mp, co
   const int wE = M->E_x;
   const int wrows = wE > 0 ? 1 + (M->E_dt > 0) + M->E_lagged_extras / wE : 0;
   const int wunl = M->E_extras - M->E_lagged_extras;
-  bool wass_ok = wE >= 1 && wE <= 32 && (M->E_dt == 0 || M->E_dt == wE) && M->E_lagged_extras % wE == 0 &&
+  bool wass_ok = wE >= 1 && wE <= 64 && (M->E_dt == 0 || M->E_dt == wE) && M->E_lagged_extras % wE == 0 &&
                  wunl >= 0 && wrows * wE + wunl == Ea && o->missing_distance >= 0.0;
   if (tp.wass && wass_ok && o->panel_mode && o->idw <= 0 && o->id_weights)
     for (int a = 0; a < o->num_panels * o->num_panels; a++)
@@ -925,7 +926,10 @@ static int launch_task(TaskPlan& tp, hipStream_t s, hipEvent_t* ev, int (*keep_g
   for (int t0 = 0; t0 < NP; t0 += tp.nt) {
     const int cnt = std::min(tp.nt, NP - t0);
     if (tp.wass)
-      hipLaunchKernelGGL(wasserstein_dense, dim3(cnt, tp.S), dim3(256), 0, s, tp.W, t0, cnt, tp.sl, D, pc);
+      if (tp.wE <= 32)
+        hipLaunchKernelGGL((wasserstein_dense<1, 4>), dim3(cnt, tp.S), dim3(256), 0, s, tp.W, t0, cnt, tp.sl, D, pc);
+      else
+        hipLaunchKernelGGL((wasserstein_dense<2, 2>), dim3(cnt, tp.S), dim3(128), 0, s, tp.W, t0, cnt, tp.sl, D, pc);
     else
       launch_dense(tp.emax, tp.generic, tp.S, s, tp.P, t0, cnt, tp.sl, D, pc);
     HIP_TRY(hipGetLastError());

@@ -19,8 +19,8 @@
 
 namespace fedm {
 
-constexpr int W_EMAX = 32; // len_i + len_j <= 64 lanes
-constexpr int W_WAVES = 4;
+// NPL transport nodes per lane: len_i + len_j <= 64 NPL, so series of up to 32 NPL observations.  NPL = 1
+// takes E_x <= 32 with four wavefronts per block; NPL = 2 takes E_x <= 64 with two (48 KB of LDS each).
 
 __device__ __forceinline__ void wave_sync()
 {
@@ -53,71 +53,123 @@ __device__ __forceinline__ double wave_min(double v)
 }
 
 // Exact optimal transport value of the li x lj cost C (row-major, LDS) with uniform marginals.
-// F: li x lj integer flows (LDS, zeroed by the caller).  Returns NaN if no augmenting path is found
-// (impossible for finite costs), so the pair is not kept.
+// F: li x lj integer flows (LDS, zeroed by the caller).  Node u (rows 0 .. li - 1, then columns) lives on
+// lane u % 64, slot u / 64.  Returns NaN if no augmenting path is found (impossible for finite costs), so
+// the pair is not kept.
+template <int NPL>
 __device__ double transport_wave(const double* C, int* F, int li, int lj, int lane)
 {
   const int nn = li + lj;
-  const bool isrow = lane < li;
-  const bool iscol = lane >= li && lane < nn;
-  const int jc = lane - li;
-  double pot = 0.0;
-  if (iscol) {
-    double mn = INF_D;
-    for (int i = 0; i < li; i++)
-      mn = fmin(mn, C[i * lj + jc]);
-    pot = mn < 0.0 ? -mn : 0.0; // reduced costs C - pot_row + pot_col start non-negative
+  bool isrow[NPL], iscol[NPL];
+  int jc[NPL], sup[NPL];
+  double pot[NPL];
+#pragma unroll
+  for (int s = 0; s < NPL; s++) {
+    const int u = lane + 64 * s;
+    isrow[s] = u < li;
+    iscol[s] = u >= li && u < nn;
+    jc[s] = u - li;
+    pot[s] = 0.0;
+    if (iscol[s]) {
+      double mn = INF_D;
+      for (int i = 0; i < li; i++)
+        mn = fmin(mn, C[i * lj + jc[s]]);
+      pot[s] = mn < 0.0 ? -mn : 0.0; // reduced costs C - pot_row + pot_col start non-negative
+    }
+    sup[s] = isrow[s] ? lj : (iscol[s] ? li : 0);
   }
-  int sup = isrow ? lj : (iscol ? li : 0);
+  // a node's value from its owner lane (u wave-uniform)
+  auto at = [&](const auto (&v)[NPL], int u) {
+    auto x = v[0];
+#pragma unroll
+    for (int s = 1; s < NPL; s++)
+      if (u >> 6 == s)
+        x = v[s];
+    return __shfl(x, u & 63);
+  };
   int remaining = li * lj;
   while (remaining > 0) {
-    const unsigned long long sm = __ballot(isrow && sup > 0);
-    const int src = __ffsll((long long)sm) - 1;
-    double d = lane == src ? 0.0 : INF_D;
-    bool done = !(isrow || iscol);
-    int pred = -1, tgt = -1;
+    // the first row node with supply left
+    int src = -1;
+#pragma unroll
+    for (int s = NPL - 1; s >= 0; s--) {
+      const unsigned long long sm = __ballot(isrow[s] && sup[s] > 0);
+      if (sm)
+        src = 64 * s + __ffsll((long long)sm) - 1;
+    }
+    double d[NPL];
+    bool done[NPL];
+    int pred[NPL];
+#pragma unroll
+    for (int s = 0; s < NPL; s++) {
+      d[s] = lane + 64 * s == src ? 0.0 : INF_D;
+      done[s] = !(isrow[s] || iscol[s]);
+      pred[s] = -1;
+    }
+    int tgt = -1;
     for (int it = 0; it < nn; it++) {
-      double v = done ? INF_D : d;
-      int u = lane;
+      // (value, node) minimum over the not yet settled nodes; ties go to the lower node
+      double v = INF_D;
+      int u = 0x7fffffff;
+#pragma unroll
+      for (int s = 0; s < NPL; s++) {
+        const double vs = done[s] ? INF_D : d[s];
+        if (vs < v) {
+          v = vs;
+          u = lane + 64 * s;
+        }
+      }
+      if (v == INF_D)
+        u = lane; // an empty lane takes its own id, as in the one-node case
       wave_argmin(v, u);
       if (v == INF_D)
         break;
-      if (lane == u)
-        done = true;
-      if (u >= li && __shfl(sup, u) > 0) {
+#pragma unroll
+      for (int s = 0; s < NPL; s++)
+        if (lane + 64 * s == u)
+          done[s] = true;
+      if (u >= li && at(sup, u) > 0) {
         tgt = u;
         break;
       }
-      const double pu = __shfl(pot, u);
+      const double pu = at(pot, u);
       if (u < li) {
-        if (iscol && !done) {
-          const double cand = v + (C[u * lj + jc] - pu + pot);
-          if (cand < d) {
-            d = cand;
-            pred = u;
+#pragma unroll
+        for (int s = 0; s < NPL; s++)
+          if (iscol[s] && !done[s]) {
+            const double cand = v + (C[u * lj + jc[s]] - pu + pot[s]);
+            if (cand < d[s]) {
+              d[s] = cand;
+              pred[s] = u;
+            }
           }
-        }
       } else {
         const int ju = u - li;
-        if (isrow && !done && F[lane * lj + ju] > 0) {
-          const double cand = v + (-C[lane * lj + ju] - pu + pot);
-          if (cand < d) {
-            d = cand;
-            pred = u;
+#pragma unroll
+        for (int s = 0; s < NPL; s++) {
+          const int r = lane + 64 * s;
+          if (isrow[s] && !done[s] && F[r * lj + ju] > 0) {
+            const double cand = v + (-C[r * lj + ju] - pu + pot[s]);
+            if (cand < d[s]) {
+              d[s] = cand;
+              pred[s] = u;
+            }
           }
         }
       }
     }
     if (tgt < 0)
       return __builtin_nan("");
-    const double dt = __shfl(d, tgt);
-    if (isrow || iscol)
-      pot -= fmin(d, dt);
+    const double dt = at(d, tgt);
+#pragma unroll
+    for (int s = 0; s < NPL; s++)
+      if (isrow[s] || iscol[s])
+        pot[s] -= fmin(d[s], dt);
     // amount: remaining supply at src, remaining demand at tgt, flow on the backward arcs of the path
-    int delta = min(__shfl(sup, src), __shfl(sup, tgt));
+    int delta = min(at(sup, src), at(sup, tgt));
     int v = tgt;
     for (int s = 0; s < nn && v != src; s++) {
-      const int p = __shfl(pred, v);
+      const int p = at(pred, v);
       if (p < 0)
         return __builtin_nan("");
       if (p >= li)
@@ -128,7 +180,7 @@ __device__ double transport_wave(const double* C, int* F, int li, int lj, int la
       return __builtin_nan("");
     v = tgt;
     for (int s = 0; s < nn && v != src; s++) {
-      const int p = __shfl(pred, v);
+      const int p = at(pred, v);
       if (lane == 0) {
         if (p < li)
           F[p * lj + (v - li)] += delta;
@@ -138,10 +190,13 @@ __device__ double transport_wave(const double* C, int* F, int li, int lj, int la
       v = p;
     }
     wave_sync();
-    if (lane == src)
-      sup -= delta;
-    if (lane == tgt)
-      sup -= delta;
+#pragma unroll
+    for (int s = 0; s < NPL; s++) {
+      if (lane + 64 * s == src)
+        sup[s] -= delta;
+      if (lane + 64 * s == tgt)
+        sup[s] -= delta;
+    }
     remaining -= delta;
   }
   double cost = 0.0;
@@ -158,9 +213,11 @@ __device__ double transport_wave(const double* C, int* F, int li, int lj, int la
 
 // Block (target t0 + blockIdx.x, library slice blockIdx.y); wave w takes library points lo + w, lo + w + 4, ...
 // Writes the distance row (kept distances, -1 otherwise) and the slice's kept count.
+template <int NPL, int W_WAVES>
 __global__ __launch_bounds__(64 * W_WAVES) void wasserstein_dense(WassParams P, int t0, int nt, int slice_len,
                                                                  double* __restrict__ D, int* __restrict__ out_cnt)
 {
+  constexpr int W_EMAX = 32 * NPL; // longest series: len_i + len_j <= 64 NPL nodes
   __shared__ double Cs[W_WAVES][W_EMAX * W_EMAX];
   __shared__ int Fs[W_WAVES][W_EMAX * W_EMAX];
   __shared__ int cmi[W_WAVES][W_EMAX];
@@ -265,7 +322,7 @@ __global__ __launch_bounds__(64 * W_WAVES) void wasserstein_dense(WassParams P, 
           Fs[w][e] = 0;
         }
         wave_sync();
-        const double cost = transport_wave(Cs[w], Fs[w], li, lj, lane);
+        const double cost = transport_wave<NPL>(Cs[w], Fs[w], li, lj, lane);
         if (cost != 0.0 && fabs(cost) >= DBL_MIN && fabs(cost) < INF_D)
           dist = cost;
         wave_sync();
@@ -282,5 +339,8 @@ __global__ __launch_bounds__(64 * W_WAVES) void wasserstein_dense(WassParams P, 
   if (threadIdx.x == 0)
     out_cnt[(size_t)blockIdx.y * P.NP + t] = kept_s;
 }
+
+template __global__ void wasserstein_dense<1, 4>(WassParams, int, int, int, double*, int*);
+template __global__ void wasserstein_dense<2, 2>(WassParams, int, int, int, double*, int*);
 
 } // namespace fedm

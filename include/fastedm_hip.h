@@ -106,7 +106,7 @@ void edm_ctx_destroy(edm_ctx* ctx);
  *   keep_going: polled between stages, may be NULL.
  * Returns 0, -22 for invalid arguments, or -95 (outputs filled, rc 8000) for a width this build does
  * not take: E_actual above 4096, an S-map with E_actual above 127 (128 unknowns), Wasserstein with
- * E_x above 32.
+ * E_x above 64.
  */
 int edm_predict(edm_ctx* ctx, const edm_manifold_view* M, const edm_manifold_view* Mp, const edm_options* o,
                 double* pred, int* rc, double* coeffs, int* k_used, int* nn_idx, int nn_cap, int (*keep_going)(void));

@@ -145,6 +145,10 @@ WASS_CASES = {
     "wass_wide_k80": dict(NL=400, NP=40, E=5, k=80),
     "wass_E16": dict(NL=200, NP=40, E=16, k=5),
     "wass_E32": dict(NL=60, NP=10, E=32, k=3),
+    # two transport nodes per lane (series of 33..64 observations)
+    "wass_E33": dict(NL=40, NP=8, E=33, k=3),
+    "wass_E48_dt_missing": dict(NL=40, NP=8, E=48, k=3, dt=True, missing=0.1, thetas=(0.5, 2.0)),
+    "wass_E64_missing_distance": dict(NL=30, NP=6, E=64, k=3, missing=0.1, missing_distance=0.4, unlagged=1),
 }
 
 
