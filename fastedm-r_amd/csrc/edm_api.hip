@@ -33,7 +33,8 @@ template <int NPL, int W_WAVES>
 __global__ void wasserstein_dense(WassParams P, int t0, int nt, int slice_len, double* D, int* out_cnt);
 __global__ void select_wide(FinishParams F, const double* D, int t0, int nt, unsigned long long* gkey, int* gidx,
                             int gstride);
-int smap_launch(int n, hipStream_t s, const SmapParams& S, const double* D, int t0, int nt, bool lds_rows);
+int smap_launch(int n, hipStream_t s, const SmapParams& S, const double* D, int t0, int nt, bool lds_rows,
+                bool block_solve);
 size_t gram_dense_bytes(int n, int nt, int T);
 int gram_dense_launch(int n, hipStream_t s, const SmapParams& S, const double* D, int t0, int nt, double* gw);
 
@@ -290,6 +291,11 @@ extern "C" void fedm_default_knobs(fedm_knobs* k)
   // measured: the two routes each fill the device, so running them side by side gains nothing (5.66 ms
   // against 5.55 ms on the 40-library CCM sweep); the tuner can still choose it
   k->concurrent_routes = 0;
+  // FEDM_SMAP_BLOCK=1 solves S-map systems of up to 24 unknowns block-wide (same bits).  Measured slower
+  // on the panel workload (n = 17: 115.8 against 63.8 ms): wavefront 0's 2x2 chain sits on the critical
+  // path of every rotation of the block, behind three block barriers, at two wavefronts per SIMD
+  const char* b = getenv("FEDM_SMAP_BLOCK");
+  k->smap_block = b && b[0] == '1';
 }
 
 static fedm_knobs default_knobs()
@@ -479,6 +485,7 @@ struct TaskPlan
   int kcap = 0, kp = 0, S = 1, sl = 0, bx = 0, nt = 0, gstride = 0;
   int wE = 0, wE_dt = 0, wrows = 0, wunl = 0; // Wasserstein layout
   bool smap_lds = false;                      // S-map solve with U and V in LDS
+  bool smap_blk = false;                      // S-map solve of n <= 24 block-wide (2x2 steps once per system)
   size_t bytes[B_NUM] = {};
   void* ptr[B_NUM] = {};
   // host copies the upload needs beyond the views
@@ -502,6 +509,7 @@ static int plan_task(const edm_manifold_view* M, const edm_manifold_view* Mp, co
   const fedm_knobs knobs = kn ? *kn : default_knobs();
   mfma_ok = mfma_ok && knobs.use_mfma;
   tp.smap_lds = knobs.smap_lds != 0;
+  tp.smap_blk = knobs.smap_block != 0;
   const int NL = M->num_points, NP = Mp->num_points, Ea = M->E_actual, T = o->num_thetas;
   if (Ea != Mp->E_actual || T <= 0 || Ea <= 0 || NL < 0 || NP < 0)
     return -22;
@@ -906,7 +914,7 @@ static int launch_task(TaskPlan& tp, hipStream_t s, hipEvent_t* ev, int (*keep_g
     if (ev)
       HIP_TRY(hipEventRecord(ev[2], s));
     if (tp.smap) {
-      const int r = smap_launch(tp.n, s, Sm, nullptr, 0, NP, tp.smap_lds);
+      const int r = smap_launch(tp.n, s, Sm, nullptr, 0, NP, tp.smap_lds, tp.smap_blk);
       if (r)
         return r;
       HIP_TRY(hipGetLastError());
@@ -951,7 +959,7 @@ static int launch_task(TaskPlan& tp, hipStream_t s, hipEvent_t* ev, int (*keep_g
         return r;
     }
     if (tp.smap) {
-      const int r = smap_launch(tp.n, s, Sm, D, t0, cnt, tp.smap_lds);
+      const int r = smap_launch(tp.n, s, Sm, D, t0, cnt, tp.smap_lds, tp.smap_blk);
       if (r)
         return r;
       HIP_TRY(hipGetLastError());

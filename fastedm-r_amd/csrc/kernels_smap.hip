@@ -199,10 +199,26 @@ __device__ void jacobi_svd_solve_multi(int n, int G, double* W, double* U, doubl
   __syncthreads();
 }
 
+// A barrier over the group's buffers: the block's, or only the wavefront's when the block holds several
+// wavefronts with buffers of their own (LDS operations of one wavefront complete in order).
+template <bool WAVE>
+__device__ __forceinline__ void group_sync()
+{
+  if (WAVE) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+  } else {
+    __syncthreads();
+  }
+}
+
 // Forms the systems of the group of G (target, theta) items that starts at item i0: X^T X into the slot
 // W_g (both triangles) and X^T y into b_g, each entry summed neighbour by neighbour in neighbour order.
 // Returns bit g set when item i0 + g has a system.  Weighted neighbour rows are staged sch at a time in
-// stage, or, when stage is null, in the slot W_g of the system being formed (sch (n + 1) <= n^2).
+// stage, or, when stage is null, in the slot W_g of the system being formed (sch (n + 1) <= n^2).  With
+// WAVE_SYNC the group's buffers belong to one wavefront of a larger block, which syncs on its own.
+template <bool WAVE_SYNC = false>
 __device__ unsigned form_group(const SmapParams& S, const double* __restrict__ D, int t0, int nt, int i0, int n,
                                int G, double* W, double* bvec, double* stage, int sch, double* acc, double* cd,
                                int* cidx, int lane)
@@ -231,7 +247,7 @@ __device__ unsigned form_group(const SmapParams& S, const double* __restrict__ D
         Wg[e] = src[e];
       for (int e = lane; e < n; e += 64)
         bvec[g * n + e] = src[n * n + e];
-      __syncthreads();
+      group_sync<WAVE_SYNC>();
       continue;
     }
     const bool dense = kk < 0;
@@ -284,7 +300,7 @@ __device__ unsigned form_group(const SmapParams& S, const double* __restrict__ D
       }
       if (cnum == 0)
         break;
-      __syncthreads();
+      group_sync<WAVE_SYNC>();
       if (lane < cnum) {
         const int ci = cidx[lane];
         const double w = fdlibm_exp(-theta * (cd[lane] / mean));
@@ -294,7 +310,7 @@ __device__ unsigned form_group(const SmapParams& S, const double* __restrict__ D
           Xc[lane * n + j] = w * row[j - 1];
         yc[lane] = w * S.targets[ci];
       }
-      __syncthreads();
+      group_sync<WAVE_SYNC>();
       for (int e = lane; e < nacc; e += 64) {
         double a = acc[e];
         if (e < ntri) {
@@ -313,7 +329,7 @@ __device__ unsigned form_group(const SmapParams& S, const double* __restrict__ D
         }
         acc[e] = a;
       }
-      __syncthreads();
+      group_sync<WAVE_SYNC>();
       done += cnum;
     }
     double* Wg = W + (size_t)g * n * n;
@@ -331,7 +347,7 @@ __device__ unsigned form_group(const SmapParams& S, const double* __restrict__ D
         bvec[g * n + e - ntri] = acc[e];
       }
     }
-    __syncthreads();
+    group_sync<WAVE_SYNC>();
   }
   return valid;
 }
@@ -720,6 +736,327 @@ __global__ __launch_bounds__(64) void smap_rows(SmapParams S, const double* __re
     const int gl = lane / n;
     const bool mine = gl < G && ((valid >> gl) & 1u);
     jacobi_svd_solve_rows<NM, NT>(n, G, W, bvec, sv, tmp, xsol, perm, !mine, lane);
+    write_group(S, t0, i0, n, G, valid, xsol, lane);
+    __syncthreads();
+  }
+}
+
+// The rotations of one Jacobi step of one system of a block, made by the block's 2x2 lanes.
+struct BlkRot
+{
+  double lc, ls, rc, rs;
+  int rot, pad;
+};
+
+// jacobi_svd_solve_rows over a block of WB wavefronts, each with G systems in lane segments (U and V rows in
+// registers, W in LDS), for up to 8 WB systems.  The 2x2 step of each rotation, a chain of IEEE divides
+// and square roots that every lane of a segment would otherwise repeat, is made once per system: lane s
+// of wavefront 0 takes system s of the block (its threshold test, its 2x2 rotations, its largest
+// diagonal entry) and leaves the rotations in LDS; then every lane applies them to its column of W and
+// its rows of U and V.  The block goes through the sweeps together (a system that has finished makes no
+// rotations, as in the wavefront solve), so every system sees the operations of jacobi_svd_solve_multi in
+// its order.  The scaling before and the sort and products after are the wavefront's own.
+template <int NM, int NT, int WB>
+__device__ void jacobi_svd_solve_blk(int n, int G, double* Wall, const double* b, double* sv, double* tmp,
+                                     double* x, int* perm, bool skip, int lane, int w, BlkRot* R, int* sbad)
+{
+  typedef double VM __attribute__((ext_vector_type(NM)));
+  typedef double VT __attribute__((ext_vector_type(NT > 0 ? NT : 1)));
+  constexpr int TM = NT > 0 ? NT - 1 : 0;
+  const int g = lane / n, j = lane - (lane / n) * n;
+  const bool act = g < G;
+  const int gg = act ? g : 0;
+  double* Wg = Wall + (size_t)(w * G + gg) * n * n;
+  const BlkRot* Rg = R + w * G + gg;
+  const double* bg = b + gg * n;
+  double* svg = sv + gg * n;
+  double* tmpg = tmp + gg * n;
+  double* xg = x + gg * n;
+  int* pg = perm + gg * n;
+
+  // scale by the largest |w| of the system
+  double cm = 0.0;
+  if (act)
+    for (int i = 0; i < n; i++)
+      cm = fmax(cm, fabs(Wg[i * n + j]));
+  double scale = 0.0;
+  for (int i = 0; i < n; i++)
+    scale = fmax(scale, __shfl(cm, gg * n + i));
+  const bool bad = skip || !act || !isfinite(scale);
+  if (scale == 0.0)
+    scale = 1.0;
+  if (!bad)
+    for (int i = 0; i < n; i++)
+      Wg[i * n + j] = Wg[i * n + j] / scale;
+  if (act && j == 0)
+    sbad[w * G + g] = bad ? 1 : 0;
+  int jo = j;
+  asm volatile("" : "+v"(jo));
+  VM um, vm;
+  VT ut, vt;
+#pragma unroll
+  for (int i = 0; i < NM; i++) {
+    um[i] = i == jo ? 1.0 : 0.0;
+    vm[i] = um[i];
+  }
+#pragma unroll
+  for (int i = 0; i < TM + 1; i++) {
+    ut[i] = NM + i == jo ? 1.0 : 0.0;
+    vt[i] = ut[i];
+  }
+  __syncthreads();
+  // the 2x2 lanes: lane s of wavefront 0 owns system s of the block
+  const bool A = w == 0 && lane < WB * G;
+  double* Ws = Wall + (size_t)(A ? lane : 0) * n * n;
+  const double precision = 2.0 * DBL_EPSILON;
+  double maxDiag = 0.0;
+  bool finished = true, run = false;
+  if (A) {
+    finished = sbad[lane] != 0;
+    if (!finished)
+      for (int i = 0; i < n; i++)
+        maxDiag = fmax(maxDiag, fabs(Ws[i * n + i]));
+  }
+  // sweeps and their p loop as one loop (one home for the register rows), at most 1000 sweeps
+  int sweeps = 0;
+  bool go = __syncthreads_or(A && !finished) != 0;
+  if (go) {
+    sweeps = 1;
+    run = !finished;
+    if (run)
+      finished = true;
+  }
+  int p = 1;
+  while (go) {
+    {
+      double up = row_get<NM, NT>(um, ut, p), vp = row_get<NM, NT>(vm, vt, p);
+      auto step = [&](int q, auto tail) {
+        constexpr bool TAIL = decltype(tail)::value;
+        int rot = 0;
+        if (A) {
+          double wpq = 0.0, wqp = 0.0;
+          if (run) {
+            const double thr = fmax(DBL_MIN, precision * maxDiag);
+            wpq = Ws[p * n + q];
+            wqp = Ws[q * n + p];
+            rot = fabs(wpq) > thr || fabs(wqp) > thr;
+          }
+          Rot jl{ 1.0, 0.0 }, jr{ 1.0, 0.0 };
+          if (rot) {
+            finished = false;
+            real_2x2(Ws[p * n + p], wpq, wqp, Ws[q * n + q], &jl, &jr);
+          }
+          BlkRot* r = R + lane;
+          r->lc = jl.c;
+          r->ls = jl.s;
+          r->rc = jr.c;
+          r->rs = jr.s;
+          r->rot = rot;
+        }
+        if (!__syncthreads_or(rot))
+          return;
+        const bool me = act && Rg->rot != 0;
+        const double lc = Rg->lc, ls = Rg->ls, rc = Rg->rc, rs = Rg->rs;
+        // rows p, q of W and columns p, q of U by jl
+        const bool dl = me && !(lc == 1.0 && ls == 0.0);
+        if (dl) {
+          const double xv = Wg[p * n + j], yv = Wg[q * n + j];
+          Wg[p * n + j] = lc * xv + ls * yv;
+          Wg[q * n + j] = -ls * xv + lc * yv;
+        }
+        {
+          double uq;
+          if constexpr (TAIL)
+            uq = ut[(q - NM) & TM];
+          else
+            uq = um[q & (NM - 1)];
+          const double np = lc * up + ls * uq, nq = -ls * up + lc * uq;
+          up = dl ? np : up;
+          if constexpr (TAIL)
+            ut[(q - NM) & TM] = dl ? nq : uq;
+          else
+            um[q & (NM - 1)] = dl ? nq : uq;
+        }
+        __syncthreads();
+        // columns p, q of W and V by jr (applyOnTheRight: rotation (c, -s))
+        const double tc = rc, ts = -rs;
+        const bool dr = me && !(tc == 1.0 && ts == 0.0);
+        if (dr) {
+          const double xv = Wg[j * n + p], yv = Wg[j * n + q];
+          Wg[j * n + p] = tc * xv + ts * yv;
+          Wg[j * n + q] = -ts * xv + tc * yv;
+        }
+        {
+          double vq;
+          if constexpr (TAIL)
+            vq = vt[(q - NM) & TM];
+          else
+            vq = vm[q & (NM - 1)];
+          const double np = tc * vp + ts * vq, nq = -ts * vp + tc * vq;
+          vp = dr ? np : vp;
+          if constexpr (TAIL)
+            vt[(q - NM) & TM] = dr ? nq : vq;
+          else
+            vm[q & (NM - 1)] = dr ? nq : vq;
+        }
+        __syncthreads();
+        if (A && rot)
+          maxDiag = fmax(maxDiag, fmax(fabs(Ws[p * n + p]), fabs(Ws[q * n + q])));
+      };
+      const int qm = p < NM ? p : NM;
+      for (int q = 0; q < qm; q++)
+        step(q, std::false_type{});
+      if constexpr (NT > 0)
+        for (int q = NM; q < p; q++)
+          step(q, std::true_type{});
+      row_set<NM, NT>(um, ut, p, up);
+      row_set<NM, NT>(vm, vt, p, vp);
+    }
+    if (++p == n) {
+      p = 1;
+      go = __syncthreads_or(A && !finished) != 0 && sweeps < 1000;
+      if (go) {
+        sweeps++;
+        run = !finished;
+        if (run)
+          finished = true;
+      }
+    }
+  }
+  // from here on each wavefront works on its own systems
+  unsigned flip = 0;
+  for (int i = 0; i < n; i++) {
+    const double a = bad ? 0.0 : Wg[i * n + i];
+    if (!bad && a < 0.0)
+      flip |= 1u << i;
+    if (!bad && j == 0)
+      svg[i] = fabs(a) * scale;
+  }
+  if (act)
+    pg[j] = j;
+  group_sync<true>();
+  int nonzero = n;
+  bool stop = bad;
+  for (int i = 0; i < n; i++) {
+    int pos = i;
+    if (!stop) {
+      double mx = svg[i];
+      for (int c = i + 1; c < n; c++)
+        if (svg[c] > mx) {
+          mx = svg[c];
+          pos = c;
+        }
+      if (mx == 0.0) {
+        nonzero = i;
+        stop = true;
+      }
+    }
+    group_sync<true>();
+    if (!stop && pos != i && j == 0) {
+      const double a = svg[i];
+      svg[i] = svg[pos];
+      svg[pos] = a;
+      const int c = pg[i];
+      pg[i] = pg[pos];
+      pg[pos] = c;
+    }
+    group_sync<true>();
+  }
+  int rank = 0;
+  if (!bad) {
+    const double thr = fmax(svg[0] * ((double)n * DBL_EPSILON), DBL_MIN);
+    int r = nonzero - 1;
+    while (r >= 0 && svg[r] < thr)
+      r--;
+    rank = r + 1;
+  }
+  if (act) {
+#pragma unroll
+    for (int k = 0; k < NM; k++)
+      if (k < n)
+        Wg[j * n + k] = ((flip >> k) & 1u) ? -um[k] : um[k];
+#pragma unroll
+    for (int k = 0; k < NT; k++)
+      if (NM + k < n)
+        Wg[j * n + NM + k] = ((flip >> (NM + k)) & 1u) ? -ut[k] : ut[k];
+  }
+  group_sync<true>();
+  if (!bad && j < rank) {
+    const int cj = pg[j];
+    double s = 0.0;
+    for (int c = 0; c < n; c++)
+      s += Wg[c * n + cj] * bg[c];
+    tmpg[j] = (1.0 / svg[j]) * s;
+  }
+  group_sync<true>();
+  if (act) {
+#pragma unroll
+    for (int k = 0; k < NM; k++)
+      if (k < n)
+        Wg[j * n + k] = vm[k];
+#pragma unroll
+    for (int k = 0; k < NT; k++)
+      if (NM + k < n)
+        Wg[j * n + NM + k] = vt[k];
+  }
+  group_sync<true>();
+  if (act) {
+    double s = 0.0;
+    for (int i = 0; i < rank; i++)
+      s += Wg[j * n + pg[i]] * tmpg[i];
+    xg[j] = s;
+  }
+  group_sync<true>();
+}
+
+// LDS of smap_blk, in doubles: the block's W [WB G n^2], the step rotations and flags, then per wavefront
+// smap_rows' buffers without W.
+constexpr int BLK_WAVES = 4;
+__host__ __device__ inline size_t smap_blk_wave_doubles(int n, int G)
+{
+  return smap_rows_lds_doubles(n, G) - (size_t)G * n * n;
+}
+__host__ __device__ inline size_t smap_blk_lds_doubles(int n, int G, int WB)
+{
+  const size_t rot = ((size_t)WB * G * sizeof(BlkRot) + (size_t)WB * G * sizeof(int) + 7) / 8;
+  return (size_t)WB * G * n * n + rot + (size_t)WB * smap_blk_wave_doubles(n, G);
+}
+
+// smap_rows with the block solve: WB wavefronts of G systems each; wavefront w forms and finishes items
+// base + w G .. base + w G + G - 1 of each block group on its own, and the solve runs block-wide.
+template <int NM, int NT, int WB>
+__global__ __launch_bounds__(64 * WB) void smap_blk(SmapParams S, const double* __restrict__ D, int t0, int nt)
+{
+  extern __shared__ double smem[];
+  const int n = S.Ea + 1;
+  const int G = smap_group(n);
+  const int nacc = n * (n + 1) / 2 + n;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int sin = smap_rows_stage(n);
+  double* Wall = smem;
+  BlkRot* R = (BlkRot*)(Wall + (size_t)WB * G * n * n);
+  int* sbad = (int*)(R + WB * G);
+  double* base_w = smem + (smap_blk_lds_doubles(n, G, WB) - (size_t)WB * smap_blk_wave_doubles(n, G)) +
+                   (size_t)w * smap_blk_wave_doubles(n, G);
+  double* bvec = base_w;
+  double* sv = bvec + G * n;
+  double* tmp = sv + G * n;
+  double* xsol = tmp + G * n;
+  double* acc = xsol + G * n;
+  double* cd = acc + nacc;
+  double* stage = cd + SM_CH;
+  int* cidx = (int*)(stage + (sin > 0 ? 0 : SM_CH * (n + 1)));
+  int* perm = cidx + SM_CH;
+  double* W = Wall + (size_t)w * G * n * n;
+
+  const int items = nt * S.T;
+  for (int base = blockIdx.x * WB * G; base < items; base += gridDim.x * WB * G) {
+    const int i0 = base + w * G;
+    const unsigned valid = form_group<true>(S, D, t0, nt, i0, n, G, W, bvec, sin > 0 ? nullptr : stage,
+                                            sin > 0 ? sin : SM_CH, acc, cd, cidx, lane);
+    const int gl = lane / n;
+    const bool mine = gl < G && ((valid >> gl) & 1u);
+    jacobi_svd_solve_blk<NM, NT, WB>(n, G, Wall, bvec, sv, tmp, xsol, perm, !mine, lane, w, R, sbad);
     write_group(S, t0, i0, n, G, valid, xsol, lane);
     __syncthreads();
   }
@@ -1573,7 +1910,8 @@ __global__ __launch_bounds__(SB_THREADS) void smap_big(SmapParams S, const doubl
   }
 }
 
-int smap_launch(int n, hipStream_t s, const SmapParams& S, const double* D, int t0, int nt, bool lds_rows)
+int smap_launch(int n, hipStream_t s, const SmapParams& S, const double* D, int t0, int nt, bool lds_rows,
+                bool block_solve)
 {
   if (n > 64 && n <= SMAP_BIG_MAX) {
     if (!S.big || S.big_blocks < 1)
@@ -1592,7 +1930,17 @@ int smap_launch(int n, hipStream_t s, const SmapParams& S, const double* D, int 
   const int G = smap_group(n);
   const void* fn = (const void*)smap_multi;
   size_t lds = sizeof(double) * smap_lds_doubles(n, G);
-  if (!lds_rows && n <= 24) {
+  int waves = 1; // wavefronts per block (each with G systems)
+  if (!lds_rows && n <= 24 && block_solve) {
+    constexpr int WB = BLK_WAVES;
+    fn = n <= 8    ? (const void*)smap_blk<8, 0, WB>
+         : n <= 16 ? (const void*)smap_blk<16, 0, WB>
+         : n <= 17 ? (const void*)smap_blk<16, 1, WB>
+         : n <= 20 ? (const void*)smap_blk<16, 4, WB>
+                   : (const void*)smap_blk<16, 8, WB>;
+    lds = sizeof(double) * smap_blk_lds_doubles(n, G, WB);
+    waves = WB;
+  } else if (!lds_rows && n <= 24) {
     fn = n <= 8    ? (const void*)smap_rows<8, 0>
          : n <= 16 ? (const void*)smap_rows<16, 0>
          : n <= 17 ? (const void*)smap_rows<16, 1>
@@ -1607,19 +1955,19 @@ int smap_launch(int n, hipStream_t s, const SmapParams& S, const double* D, int 
   hipError_t e = hipSuccess;
   if (lds > 64 * 1024)
     e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-  const long long groups = ((long long)nt * S.T + G - 1) / G;
+  const long long groups = ((long long)nt * S.T + G * waves - 1) / (G * waves);
   int dev = 0, ncu = 0, occ = 0;
   if (e == hipSuccess)
     e = hipGetDevice(&dev);
   if (e == hipSuccess)
     e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
   if (e == hipSuccess)
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fn, 64, lds);
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fn, 64 * waves, lds);
   if (e != hipSuccess)
     return -(int)e - 1000;
   const int grid = (int)std::max<long long>(1, std::min<long long>(groups, (long long)ncu * std::max(occ, 1)));
   void* args[] = { (void*)&S, (void*)&D, (void*)&t0, (void*)&nt };
-  e = hipLaunchKernel(fn, dim3(grid), dim3(64), args, lds, s);
+  e = hipLaunchKernel(fn, dim3(grid), dim3(64 * waves), args, lds, s);
   return e == hipSuccess ? 0 : -(int)e - 1000;
 }
 

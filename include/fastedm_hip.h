@@ -160,6 +160,8 @@ typedef struct
                                spatially pruned direct kernel (1) */
   int concurrent_routes;    /* persistent route: the direct kernels run on a second stream beside the
                                rank-table kernels (0) */
+  int smap_block;           /* S-map solve of up to 24 unknowns over blocks of four wavefronts, each
+                               rotation's 2x2 step made once per system (1), or per wavefront (0) (0) */
 } fedm_knobs;
 void fedm_default_knobs(fedm_knobs* k);
 

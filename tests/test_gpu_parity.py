@@ -96,24 +96,32 @@ def test_task_parity(name):
 
 @pytest.mark.parametrize("Ea", [1, 3, 7, 8, 12, 15, 16, 17, 19, 22, 23, 32, 40, 47])
 def test_smap_register_rows_match_lds_rows(Ea, monkeypatch):
-    """For n = E_a + 1 <= 24 the S-map solve keeps U and V rows in registers, for 33 <= n <= 48 U's rows;
-    FEDM_SMAP_LDS_ROWS=1 keeps them in LDS.  Both run the same floating-point operations in the same
-    order, so every prediction and coefficient must agree to the bit (k near n gives rank-deficient and
-    ill-conditioned systems)."""
+    """For n = E_a + 1 <= 24 the S-map solve keeps U and V rows in registers (one wavefront at a time;
+    FEDM_SMAP_BLOCK=1: block-wide, each rotation's 2x2 step made once per system), for 33 <= n <= 48
+    U's rows; FEDM_SMAP_LDS_ROWS=1 keeps them in LDS.  All run the same floating-point operations in the
+    same order, so every prediction and coefficient must agree to the bit (k near n gives rank-deficient
+    and ill-conditioned systems)."""
     fe = load_product()
     for k in (Ea + 1, Ea + 3):
         M, Mp, opts = random_task(100 + 7 * Ea + k, NL=700, NP=150, Ea=Ea, k=k, algorithm=1, thetas=(0, 1, 4),
                                   save_coeffs=True, integer=(k == Ea + 1))
         print(f"smap rows: Ea={Ea} k={k}", flush=True)
         monkeypatch.delenv("FEDM_SMAP_LDS_ROWS", raising=False)
+        monkeypatch.delenv("FEDM_SMAP_BLOCK", raising=False)
         gp, gr, gc, gk, gn, status = fe.predict(M, Mp, opts, nn_cap=64)
-        monkeypatch.setenv("FEDM_SMAP_LDS_ROWS", "1")
-        lp, lr, lc, lk, ln, lstatus = fe.predict(M, Mp, opts, nn_cap=64)
-        monkeypatch.delenv("FEDM_SMAP_LDS_ROWS")
-        assert status == 0 and lstatus == 0
-        np.testing.assert_array_equal(gr, lr)
-        np.testing.assert_array_equal(np.ascontiguousarray(gp).view(np.uint64), np.ascontiguousarray(lp).view(np.uint64))
-        np.testing.assert_array_equal(np.ascontiguousarray(gc).view(np.uint64), np.ascontiguousarray(lc).view(np.uint64))
+        runs = []
+        for var, val in (("FEDM_SMAP_LDS_ROWS", "1"), ("FEDM_SMAP_BLOCK", "1")):
+            monkeypatch.setenv(var, val)
+            runs.append(fe.predict(M, Mp, opts, nn_cap=64))
+            monkeypatch.delenv(var)
+        assert status == 0
+        for lp, lr, lc, lk, ln, lstatus in runs:
+            assert lstatus == 0
+            np.testing.assert_array_equal(gr, lr)
+            np.testing.assert_array_equal(np.ascontiguousarray(gp).view(np.uint64),
+                                          np.ascontiguousarray(lp).view(np.uint64))
+            np.testing.assert_array_equal(np.ascontiguousarray(gc).view(np.uint64),
+                                          np.ascontiguousarray(lc).view(np.uint64))
 
 
 def test_smap_width_limit():
