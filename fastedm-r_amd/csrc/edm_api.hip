@@ -5,6 +5,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <numeric>
 #include <cstdio>
 #include <cstdlib>
 #include <array>
@@ -34,12 +35,14 @@ __global__ void wasserstein_dense(WassParams P, int t0, int nt, int slice_len, d
 __global__ void select_wide(FinishParams F, const double* D, int t0, int nt, unsigned long long* gkey, int* gidx,
                             int gstride);
 int smap_launch(int n, hipStream_t s, const SmapParams& S, const double* D, int t0, int nt, bool lds_rows,
-                bool block_solve);
+                int solve_mode);
 size_t gram_dense_bytes(int n, int nt, int T);
 int gram_dense_launch(int n, hipStream_t s, const SmapParams& S, const double* D, int t0, int nt, double* gw);
 
 template <int KCAP, int KP>
 __global__ void knn_mfma(MfmaParams P, int k, int slice_len, double* out_s, int* out_i, int* out_cnt);
+template <int KP>
+__global__ void knn_seed(MfmaParams P, const int* kdp, const int* wst, const int* tord, int win, int k, double* seed);
 __global__ void mfma_layout_probe(float* out);
 
 
@@ -131,6 +134,10 @@ enum
   B_MIX4,
   B_SMAPBIG,
   B_GRAM,
+  B_KDP,
+  B_WST,
+  B_TORD,
+  B_SEED,
   B_NUM
 };
 } // namespace fedm
@@ -223,6 +230,18 @@ static void launch_mfma_k(int kp, dim3 g, hipStream_t s, const MfmaParams& P, in
     hipLaunchKernelGGL((knn_mfma<KCAP, 64>), g, dim3(256), 0, s, P, k, sl, ps, pi, pc);
 }
 
+static void launch_seed(int kp, hipStream_t s, const MfmaParams& P, const int* kdp, const int* wst, const int* tord,
+                        int win, int k, double* seed)
+{
+  const dim3 g((P.NP + 3) / 4);
+  if (kp == 16)
+    hipLaunchKernelGGL((knn_seed<16>), g, dim3(256), 0, s, P, kdp, wst, tord, win, k, seed);
+  else if (kp == 32)
+    hipLaunchKernelGGL((knn_seed<32>), g, dim3(256), 0, s, P, kdp, wst, tord, win, k, seed);
+  else
+    hipLaunchKernelGGL((knn_seed<64>), g, dim3(256), 0, s, P, kdp, wst, tord, win, k, seed);
+}
+
 static void launch_mfma(int kcap, int kp, dim3 g, hipStream_t s, const MfmaParams& P, int k, int sl, double* ps,
                         int* pi, int* pc)
 {
@@ -291,11 +310,16 @@ extern "C" void fedm_default_knobs(fedm_knobs* k)
   // measured: the two routes each fill the device, so running them side by side gains nothing (5.66 ms
   // against 5.55 ms on the 40-library CCM sweep); the tuner can still choose it
   k->concurrent_routes = 0;
-  // FEDM_SMAP_BLOCK=1 solves S-map systems of up to 24 unknowns block-wide (same bits).  Measured slower
-  // on the panel workload (n = 17: 115.8 against 63.8 ms): wavefront 0's 2x2 chain sits on the critical
-  // path of every rotation of the block, behind three block barriers, at two wavefronts per SIMD
+  // FEDM_SMAP_BLOCK=0..4 picks the S-map solve of up to 24 unknowns (same bits; SMAP_SOLVE_*).  Measured
+  // on the panel workload (n = 17, 796 995 systems): one row per lane 63.8 ms, block-wide 115.8 ms (the
+  // 2x2 chain of wavefront 0 sits on the critical path of every rotation of the block, behind three
+  // block barriers), two rows per lane 56.0 ms; at n = 11 and 7 500 systems two rows per lane is slower
+  // (0.55 against 0.45 ms: two wavefronts per SIMD and too few groups to fill the device)
   const char* b = getenv("FEDM_SMAP_BLOCK");
-  k->smap_block = b && b[0] == '1';
+  k->smap_block = b && b[0] >= '0' && b[0] <= '4' ? b[0] - '0' : SMAP_SOLVE_AUTO;
+  // FEDM_MFMA_SEED=0 starts the matrix-core screening unbounded (same results)
+  const char* sd = getenv("FEDM_MFMA_SEED");
+  k->mfma_seed = !(sd && sd[0] == '0');
 }
 
 static fedm_knobs default_knobs()
@@ -465,6 +489,7 @@ static BufRole buf_role(int slot)
     case B_WIDX:
     case B_SMAPBIG:
     case B_GRAM:
+    case B_SEED:
       return ROLE_SCRATCH;
     case B_PRED:
     case B_RC:
@@ -485,19 +510,112 @@ struct TaskPlan
   int kcap = 0, kp = 0, S = 1, sl = 0, bx = 0, nt = 0, gstride = 0;
   int wE = 0, wE_dt = 0, wrows = 0, wunl = 0; // Wasserstein layout
   bool smap_lds = false;                      // S-map solve with U and V in LDS
-  bool smap_blk = false;                      // S-map solve of n <= 24 block-wide (2x2 steps once per system)
+  int smap_mode = SMAP_SOLVE_ROWS;            // S-map solve of n <= 24: SMAP_SOLVE_*
   size_t bytes[B_NUM] = {};
   void* ptr[B_NUM] = {};
   // host copies the upload needs beyond the views
   std::vector<int> metrics;
   std::vector<float> m32, mn, p32, pn;
   std::vector<double> ynorm;
+  std::vector<int> kdp, wst, tord; // matrix-core seed: library rows in k-d order, window starts, target order
+  int seed_win = 0;                // rows per seed window (0: no seed)
   LpParams P;
   WassParams W;
   FinishParams F;
   SmapParams Sm;
   MfmaParams MP;
 };
+
+constexpr int SEED_WIN = 256; // library rows per seed window of the matrix-core route (knn_seed: 4 per lane)
+
+// Seed windows of the matrix-core route.  A k-d order of the library rows: a node splits at the median of
+// its coordinate of largest spread, with every row equal to the median on one side (so a coordinate that
+// is constant over most rows, such as a dt column, separates only the rows that differ from it and does
+// not scatter equal rows over both sides); leaves hold at most 32 rows.  Each target descends the tree to
+// a leaf and takes the `win` rows of the order centred on it.  The windows only tighten the screening's
+// starting threshold; any window gives the same results.
+static void kd_seed_windows(const double* L, int NL, const double* P, int NP, int Ea, int win, std::vector<int>& kdp,
+                            std::vector<int>& wst, std::vector<int>& tord)
+{
+  constexpr int LEAF = 32;
+  struct Node
+  {
+    int dim; // -1: leaf
+    double val;
+    int le;   // 1: rows <= val go left, 0: rows < val go left
+    int a, b; // rows kdp[a .. b)
+    int left, right;
+  };
+  kdp.resize(NL);
+  std::iota(kdp.begin(), kdp.end(), 0);
+  std::vector<Node> nodes;
+  nodes.push_back({ -1, 0.0, 0, 0, NL, -1, -1 });
+  std::vector<int> stack{ 0 };
+  std::vector<double> lo(Ea), hi(Ea);
+  std::vector<int> dims(Ea);
+  while (!stack.empty()) {
+    const int ni = stack.back();
+    stack.pop_back();
+    const int a = nodes[ni].a, b = nodes[ni].b;
+    if (b - a <= LEAF)
+      continue;
+    std::fill(lo.begin(), lo.end(), INFINITY);
+    std::fill(hi.begin(), hi.end(), -INFINITY);
+    for (int i = a; i < b; i++) {
+      const double* r = L + (size_t)kdp[i] * Ea;
+      for (int d = 0; d < Ea; d++) {
+        lo[d] = std::min(lo[d], r[d]);
+        hi[d] = std::max(hi[d], r[d]);
+      }
+    }
+    std::iota(dims.begin(), dims.end(), 0);
+    std::stable_sort(dims.begin(), dims.end(), [&](int x, int y) { return hi[x] - lo[x] > hi[y] - lo[y]; });
+    for (int d : dims) {
+      if (!(hi[d] > lo[d]))
+        break;
+      auto key = [&](int r) { return L[(size_t)r * Ea + d]; };
+      const int m = a + (b - a) / 2;
+      std::nth_element(kdp.begin() + a, kdp.begin() + m, kdp.begin() + b, [&](int x, int y) { return key(x) < key(y); });
+      const double v = key(kdp[m]);
+      auto it = std::partition(kdp.begin() + a, kdp.begin() + b, [&](int r) { return key(r) < v; });
+      int le = 0;
+      if (it == kdp.begin() + a) {
+        it = std::partition(kdp.begin() + a, kdp.begin() + b, [&](int r) { return key(r) <= v; });
+        le = 1;
+      }
+      const int s = (int)(it - kdp.begin());
+      if (s == a || s == b)
+        continue;
+      const int l = (int)nodes.size();
+      nodes.push_back({ -1, 0.0, 0, a, s, -1, -1 });
+      nodes.push_back({ -1, 0.0, 0, s, b, -1, -1 });
+      nodes[ni].dim = d;
+      nodes[ni].val = v;
+      nodes[ni].le = le;
+      nodes[ni].left = l;
+      nodes[ni].right = l + 1;
+      stack.push_back(l);
+      stack.push_back(l + 1);
+      break;
+    }
+  }
+  const int w = std::min(win, NL);
+  wst.resize(NP);
+  for (int t = 0; t < NP; t++) {
+    const double* y = P + (size_t)t * Ea;
+    int ni = 0;
+    while (nodes[ni].dim >= 0) {
+      const Node& nd = nodes[ni];
+      const double v = y[nd.dim];
+      ni = (nd.le ? v <= nd.val : v < nd.val) ? nd.left : nd.right;
+    }
+    const int c = (nodes[ni].a + nodes[ni].b) / 2;
+    wst[t] = std::max(0, std::min(NL - w, c - w / 2));
+  }
+  tord.resize(NP);
+  std::iota(tord.begin(), tord.end(), 0);
+  std::stable_sort(tord.begin(), tord.end(), [&](int x, int y) { return wst[x] < wst[y]; });
+}
 
 // Chooses the route and sizes every buffer.  Returns 0, -22 (invalid arguments) or -95 (a width or
 // option this build does not take; the caller fills the outputs).  `mfma_ok` allows the matrix-core
@@ -509,7 +627,7 @@ static int plan_task(const edm_manifold_view* M, const edm_manifold_view* Mp, co
   const fedm_knobs knobs = kn ? *kn : default_knobs();
   mfma_ok = mfma_ok && knobs.use_mfma;
   tp.smap_lds = knobs.smap_lds != 0;
-  tp.smap_blk = knobs.smap_block != 0;
+  tp.smap_mode = knobs.smap_block;
   const int NL = M->num_points, NP = Mp->num_points, Ea = M->E_actual, T = o->num_thetas;
   if (Ea != Mp->E_actual || T <= 0 || Ea <= 0 || NL < 0 || NP < 0)
     return -22;
@@ -674,6 +792,14 @@ static int plan_task(const edm_manifold_view* M, const edm_manifold_view* Mp, co
     by[B_MIX2] = sizeof(float) * tp.p32.size();
     by[B_MIX3] = sizeof(float) * tp.pn.size();
     by[B_MIX4] = sizeof(double) * tp.ynorm.size();
+    if (knobs.mfma_seed && NL > 0 && NP > 0) {
+      kd_seed_windows(M->data, NL, Mp->data, NP, Ea, SEED_WIN, tp.kdp, tp.wst, tp.tord);
+      tp.seed_win = std::min(NL, SEED_WIN);
+      by[B_KDP] = sizeof(int) * tp.kdp.size();
+      by[B_WST] = sizeof(int) * tp.wst.size();
+      by[B_TORD] = sizeof(int) * tp.tord.size();
+      by[B_SEED] = sizeof(double) * (size_t)NP;
+    }
     by[B_PS] = sizeof(double) * (size_t)tp.S * NP * tp.kcap;
     by[B_PI] = sizeof(int) * (size_t)tp.S * NP * tp.kcap;
     by[B_PCNT] = sizeof(int) * (size_t)tp.S * NP;
@@ -833,6 +959,7 @@ static int bind_task(TaskPlan& tp, const edm_options* o, Get&& get)
     MP.ynorm = (const double*)tp.ptr[B_MIX4];
     MP.M = P.M;
     MP.Mp = P.Mp;
+    MP.seed = tp.seed_win > 0 ? (const double*)tp.ptr[B_SEED] : nullptr;
   }
   return 0;
 }
@@ -871,6 +998,12 @@ static int upload_task(const TaskPlan& tp, const edm_manifold_view* M, const edm
       e = cp(B_MIX3, tp.pn.data());
     if (e == hipSuccess)
       e = cp(B_MIX4, tp.ynorm.data());
+    if (e == hipSuccess)
+      e = cp(B_KDP, tp.kdp.data());
+    if (e == hipSuccess)
+      e = cp(B_WST, tp.wst.data());
+    if (e == hipSuccess)
+      e = cp(B_TORD, tp.tord.data());
   }
   return e == hipSuccess ? 0 : -3;
 }
@@ -902,8 +1035,12 @@ static int launch_task(TaskPlan& tp, hipStream_t s, hipEvent_t* ev, int (*keep_g
   int* pi = (int*)tp.ptr[B_PI];
   int* pc = (int*)tp.ptr[B_PCNT];
   if (tp.route != R_DENSE) {
-    if (tp.route == R_MFMA)
+    if (tp.route == R_MFMA) {
+      if (tp.MP.seed)
+        launch_seed(tp.kp, s, tp.MP, (const int*)tp.ptr[B_KDP], (const int*)tp.ptr[B_WST],
+                    (const int*)tp.ptr[B_TORD], tp.seed_win, tp.k, (double*)tp.ptr[B_SEED]);
       launch_mfma(tp.kcap, tp.kp, dim3(tp.bx, tp.S), s, tp.MP, tp.k, tp.sl, ps, pi, pc);
+    }
     else
       launch_knn(tp.kcap, tp.emax, tp.generic, dim3(tp.bx, tp.S), s, tp.P, tp.k, tp.sl, ps, pi, pc);
     HIP_TRY(hipGetLastError());
@@ -914,7 +1051,7 @@ static int launch_task(TaskPlan& tp, hipStream_t s, hipEvent_t* ev, int (*keep_g
     if (ev)
       HIP_TRY(hipEventRecord(ev[2], s));
     if (tp.smap) {
-      const int r = smap_launch(tp.n, s, Sm, nullptr, 0, NP, tp.smap_lds, tp.smap_blk);
+      const int r = smap_launch(tp.n, s, Sm, nullptr, 0, NP, tp.smap_lds, tp.smap_mode);
       if (r)
         return r;
       HIP_TRY(hipGetLastError());
@@ -959,7 +1096,7 @@ static int launch_task(TaskPlan& tp, hipStream_t s, hipEvent_t* ev, int (*keep_g
         return r;
     }
     if (tp.smap) {
-      const int r = smap_launch(tp.n, s, Sm, D, t0, cnt, tp.smap_lds, tp.smap_blk);
+      const int r = smap_launch(tp.n, s, Sm, D, t0, cnt, tp.smap_lds, tp.smap_mode);
       if (r)
         return r;
       HIP_TRY(hipGetLastError());

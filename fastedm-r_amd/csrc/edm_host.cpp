@@ -956,9 +956,9 @@ static std::string knobs_text(const fedm_knobs& k)
   char buf[256];
   snprintf(buf, sizeof buf,
            "rank_scale=%.3f direct_blocks_per_cu=%d direct_alt_waves=%d list_blocks=%d mfma_blocks=%d use_mfma=%d "
-           "smap_lds=%d direct_prune=%d concurrent_routes=%d smap_block=%d",
+           "smap_lds=%d direct_prune=%d concurrent_routes=%d smap_block=%d mfma_seed=%d",
            k.rank_scale, k.direct_blocks_per_cu, k.direct_alt_waves, k.list_blocks, k.mfma_blocks, k.use_mfma,
-           k.smap_lds, k.direct_prune, k.concurrent_routes, k.smap_block);
+           k.smap_lds, k.direct_prune, k.concurrent_routes, k.smap_block, k.mfma_seed);
   return buf;
 }
 
@@ -997,6 +997,8 @@ static bool parse_knobs(const std::string& line, fedm_knobs* k)
       k->concurrent_routes = (int)v;
     else if (name == "smap_block")
       k->smap_block = (int)v;
+    else if (name == "mfma_seed")
+      k->mfma_seed = (int)v;
     else
       got--;
   }
@@ -1131,10 +1133,11 @@ static int tune_group(fedm_plan* P, fedm_plan::Group& g, int trials, std::string
     axes.push_back({ "list_blocks", { 512, 1024, 2048, 4096 } });
     if (mixed && Ea >= 8) {
       axes.push_back({ "use_mfma", { 1, 0 } });
+      axes.push_back({ "mfma_seed", { 1, 0 } });
       axes.push_back({ "mfma_blocks", { 1024, 2048, 4096 } });
     }
     if (g.go.algorithm == FEDM_SMAP) {
-      axes.push_back({ "smap_block", { 1, 0 } });
+      axes.push_back({ "smap_block", { 4, 0, 1, 2, 3 } });
       axes.push_back({ "smap_lds", { 0, 1 } });
     }
   }
@@ -1160,6 +1163,8 @@ static int tune_group(fedm_plan* P, fedm_plan::Group& g, int trials, std::string
       k.concurrent_routes = (int)v;
     else if (n == "smap_block")
       k.smap_block = (int)v;
+    else if (n == "mfma_seed")
+      k.mfma_seed = (int)v;
     return k;
   };
   for (const auto& ax : axes) {

@@ -18,6 +18,14 @@ constexpr int LDS_CHUNK_DOUBLES = 4096; // 32 KiB library chunk per workgroup
 constexpr int LDS_CHUNK_MAXPTS = 1024;
 constexpr int MAX_SLICES = 64;
 constexpr int DT_TGT = 64; // targets per block of the dense distance kernel
+// S-map solve of up to 24 unknowns (fedm_knobs.smap_block): per wavefront with one row slot per lane,
+// block-wide with the 2x2 steps made once per system, per wavefront with two row slots per lane and
+// the 2x2 steps made once per system (as many systems as leave the register-bound occupancy, or as fit),
+// or the two-slot solve for 9..24 unknowns when a launch has at least SMAP_PACK_MIN_ITEMS systems and
+// the one-slot solve otherwise (the two-slot kernel runs two wavefronts per SIMD, so it needs many
+// groups to fill the device)
+enum { SMAP_SOLVE_ROWS = 0, SMAP_SOLVE_BLOCK = 1, SMAP_SOLVE_PACK = 2, SMAP_SOLVE_PACK_MAX = 3, SMAP_SOLVE_AUTO = 4 };
+constexpr long long SMAP_PACK_MIN_ITEMS = 65536;
 constexpr int SMAP_BIG_MAX = 128;    // widest S-map system (unknowns = E_actual + 1) the solve takes
 constexpr int SMAP_BIG_BLOCKS = 256; // blocks of the wide S-map kernel (one system each at a time)
 
@@ -144,6 +152,8 @@ struct MfmaParams
   double R;            // largest |centred library row|
   double k1, k3;       // bound: coefficient of sqrt(D) per unit rr, constant term per unit rr^2
   double c2;           // bound: coefficient of D
+  const double* seed;  // [NP] an exact upper bound on each target's k-th smallest nonzero squared distance
+                       // (+inf: none), from knn_seed; null: the screening starts unbounded
   int NL, NP, Ea;
 };
 

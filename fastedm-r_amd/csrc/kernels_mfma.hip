@@ -126,7 +126,21 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(mfma_waves(
   int zeros = 0;
   const double yn = owner ? P.ynorm[ot] : 0.0;
   if (lane < 16) {
-    thr_s[w * 16 + lane] = __builtin_huge_valf();
+    // the screening threshold starts from the seed's bound when there is one: every pair that can
+    // enter the list has sqrt(D) <= sqrt(seed), so D <= seed' = sqrt(seed)^2 (1 + 2^-49), and the
+    // FP32 value of such a pair is within B(seed') of D (the bound of the threshold updates below)
+    float ts = __builtin_huge_valf();
+    if (owner && P.seed) {
+      const double a = P.seed[ot];
+      if (a < INF_D) {
+        const double s0 = sqrt(a);
+        const double T = s0 * s0 * SQRT_GUARD;
+        const double sq = sqrt(T);
+        const double rr = fmin(P.R + yn, 2.0 * yn + sq) * (1.0 + 1e-6);
+        ts = up_f32(T + 2.0 * (P.k1 * rr * sq + P.c2 * T + P.k3 * rr * rr));
+      }
+    }
+    thr_s[w * 16 + lane] = ts;
     qcnt[w * 16 + lane] = 0;
   }
 
@@ -317,6 +331,72 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(mfma_waves(
     out_cnt[(size_t)blockIdx.y * P.NP + ot] = (hi > lo ? hi - lo : 0) - zeros;
   }
 }
+
+// Seed of knn_mfma's screening threshold: for each target, the k-th smallest nonzero exact squared distance
+// over a window of `win` (<= 256) library rows that lie near the target in a k-d order of the library
+// (rows kdp[wst[t]] .. kdp[wst[t] + win - 1]), or +inf when the window has fewer than k.  The k-th over
+// a subset is never below the k-th over the whole library, so the seed bounds the true k-th distance
+// from above.  Distances are summed in the reference's order (src/distances.cpp:186-216), as in the
+// re-rank.  One wavefront per target, four window rows per lane; targets are taken in window order
+// (tord), so consecutive wavefronts read overlapping rows.
+template <int KP>
+__global__ __launch_bounds__(256) void knn_seed(MfmaParams P, const int* __restrict__ kdp, const int* __restrict__ wst,
+                                               const int* __restrict__ tord, int win, int k,
+                                               double* __restrict__ seed)
+{
+  const int lane = threadIdx.x & 63;
+  const int wi = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (wi >= P.NP)
+    return;
+  const int t = tord[wi];
+  const int a = wst[t];
+  const double* y = P.Mp + (size_t)t * P.Ea;
+  double v[4];
+#pragma unroll
+  for (int r = 0; r < 4; r++) {
+    const int j = lane + 64 * r;
+    double acc = INF_D;
+    if (j < win) {
+      const double d = dist_f64<KP>(P.M + (size_t)kdp[a + j] * P.Ea, y, P.Ea);
+      acc = (d != 0.0 && d != MISSING_D) ? d : INF_D;
+    }
+    v[r] = acc;
+  }
+  // sort the lane's four values, then take k wave-wide minima of the lanes' heads
+  auto cs = [](double& x, double& z) {
+    const double lo = fmin(x, z), hi = fmax(x, z);
+    x = lo;
+    z = hi;
+  };
+  cs(v[0], v[1]);
+  cs(v[2], v[3]);
+  cs(v[0], v[2]);
+  cs(v[1], v[3]);
+  cs(v[1], v[2]);
+  double kth = INF_D;
+  int h = 0;
+  for (int i = 0; i < k; i++) {
+    const double head = h == 0 ? v[0] : h == 1 ? v[1] : h == 2 ? v[2] : h == 3 ? v[3] : INF_D;
+    double m = head;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+      m = fmin(m, __shfl_xor(m, off));
+    kth = m;
+    if (m == INF_D)
+      break;
+    const unsigned long long b = __ballot(head == m);
+    if (lane == (int)__ffsll((long long)b) - 1)
+      h++;
+  }
+  if (lane == 0)
+    seed[t] = kth;
+}
+
+#define FEDM_SEED(KPv)                                                                                           \
+  template __global__ void knn_seed<KPv>(MfmaParams, const int*, const int*, const int*, int, int, double*);
+FEDM_SEED(16)
+FEDM_SEED(32)
+FEDM_SEED(64)
 
 // Lane-layout self-test of v_mfma_f32_16x16x4f32 as knn_mfma uses it: A[i][k] = i + 1 for k = 0,
 // B[k][j] = 100 (j + 1) for k = 0, zero elsewhere; out[lane * 4 + r] must be D[4 g + r][lane % 16].

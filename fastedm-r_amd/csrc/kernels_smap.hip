@@ -741,6 +741,403 @@ __global__ __launch_bounds__(64) void smap_rows(SmapParams S, const double* __re
   }
 }
 
+// Systems per wavefront of smap_pack: two row slots per lane (at most 32 systems, the width of the
+// group's validity mask).
+__host__ __device__ inline int smap_pack_group(int n)
+{
+  return 128 / n > 32 ? 32 : 128 / n;
+}
+
+// LDS doubles of smap_pack: smap_rows' buffers for G systems, then the step rotations [G x 4], the
+// scales [G], and the ranks and sign masks [G ints each].
+__host__ __device__ inline size_t smap_pack_lds_doubles(int n, int G)
+{
+  return smap_rows_lds_doubles(n, G) + 5 * (size_t)G + (size_t)G;
+}
+
+// The solve of jacobi_svd_solve_rows for up to 128 / n systems per wavefront.  Slot r (0, 1) of lane L is
+// row j = (64 r + L) mod n of system g = (64 r + L) / n: the lane keeps row j of U_g and of V_g in
+// registers and applies the rotations to column j (row update) and row j (column update) of W_g.  The
+// scalar part of each rotation step, the threshold test and the 2x2 rotations, is made once per system:
+// lane g < G owns system g, leaves the two rotations in LDS (the identity when the system does not
+// rotate), and keeps the system's largest diagonal entry.  The scaling, the singular values, their sort
+// and the rank are the owner lane's; the two products of the solve are the slot lanes'.  Every
+// floating-point operation and its order are those of jacobi_svd_solve_multi.
+template <int NM, int NT>
+__device__ void jacobi_svd_solve_pack(int n, int G, double* W, const double* b, double* sv, double* tmp,
+                                      double* x, int* perm, double* Rt, double* scl, int* rk, int* fl,
+                                      unsigned live, int lane)
+{
+  typedef double VM __attribute__((ext_vector_type(NM)));
+  typedef double VT __attribute__((ext_vector_type(NT > 0 ? NT : 1)));
+  constexpr int TM = NT > 0 ? NT - 1 : 0;
+  const int s1 = 64 + lane;
+  const int g0 = lane / n, j0 = lane - (lane / n) * n;
+  const int g1 = s1 / n, j1 = s1 - (s1 / n) * n;
+  const bool act0 = g0 < G, act1 = g1 < G;
+  const int gg0 = act0 ? g0 : 0, gg1 = act1 ? g1 : 0;
+  double* W0 = W + (size_t)gg0 * n * n;
+  double* W1 = W + (size_t)gg1 * n * n;
+
+  // owner lanes: scale by the largest |w| of the system (the order of an exact maximum is immaterial)
+  const bool A = lane < G;
+  const int ga = A ? lane : 0;
+  double* Wa = W + (size_t)ga * n * n;
+  double* sva = sv + ga * n;
+  int* pa = perm + ga * n;
+  double scale = 0.0;
+  if (A)
+    for (int e = 0; e < n * n; e++)
+      scale = fmax(scale, fabs(Wa[e]));
+  const bool badA = !A || !((live >> lane) & 1u) || !isfinite(scale);
+  if (scale == 0.0)
+    scale = 1.0;
+  if (A) {
+    scl[lane] = scale;
+    rk[lane] = badA ? -1 : 0;
+  }
+  group_sync<true>();
+  const bool bad0 = !act0 || rk[gg0] < 0, bad1 = !act1 || rk[gg1] < 0;
+  {
+    const double sc0 = scl[gg0], sc1 = scl[gg1];
+    if (!bad0)
+      for (int i = 0; i < n; i++)
+        W0[i * n + j0] = W0[i * n + j0] / sc0;
+    if (!bad1)
+      for (int i = 0; i < n; i++)
+        W1[i * n + j1] = W1[i * n + j1] / sc1;
+  }
+  int jo0 = j0, jo1 = j1;
+  asm volatile("" : "+v"(jo0), "+v"(jo1));
+  VM um0, vm0, um1, vm1;
+  VT ut0, vt0, ut1, vt1;
+#pragma unroll
+  for (int i = 0; i < NM; i++) {
+    um0[i] = i == jo0 ? 1.0 : 0.0;
+    vm0[i] = um0[i];
+    um1[i] = i == jo1 ? 1.0 : 0.0;
+    vm1[i] = um1[i];
+  }
+#pragma unroll
+  for (int i = 0; i < TM + 1; i++) {
+    ut0[i] = NM + i == jo0 ? 1.0 : 0.0;
+    vt0[i] = ut0[i];
+    ut1[i] = NM + i == jo1 ? 1.0 : 0.0;
+    vt1[i] = ut1[i];
+  }
+  group_sync<true>();
+  const double precision = 2.0 * DBL_EPSILON;
+  double maxDiag = 0.0;
+  if (!badA)
+    for (int i = 0; i < n; i++)
+      maxDiag = fmax(maxDiag, fabs(Wa[i * n + i]));
+
+  // One rotation's update of a slot: rows p, q of W and columns p, q of U by (lc, ls), after a wavefront
+  // sync columns p, q of W and V by (rc, -rs).
+  auto left = [&](double* Ws, int j, bool act, double lc, double ls, int p, int q, double& up, double uq,
+                  double& nq) {
+    const bool dl = act && !(lc == 1.0 && ls == 0.0);
+    if (dl) {
+      const double xv = Ws[p * n + j], yv = Ws[q * n + j];
+      Ws[p * n + j] = lc * xv + ls * yv;
+      Ws[q * n + j] = -ls * xv + lc * yv;
+    }
+    const double np = lc * up + ls * uq, nn = -ls * up + lc * uq;
+    up = dl ? np : up;
+    nq = dl ? nn : uq;
+  };
+  auto right = [&](double* Ws, int j, bool act, double rc, double rs, int p, int q, double& vp, double vq,
+                   double& nq) {
+    const double tc = rc, ts = -rs;
+    const bool dr = act && !(tc == 1.0 && ts == 0.0);
+    if (dr) {
+      const double xv = Ws[j * n + p], yv = Ws[j * n + q];
+      Ws[j * n + p] = tc * xv + ts * yv;
+      Ws[j * n + q] = -ts * xv + tc * yv;
+    }
+    const double np = tc * vp + ts * vq, nn = -ts * vp + tc * vq;
+    vp = dr ? np : vp;
+    nq = dr ? nn : vq;
+  };
+
+  // the sweeps and their p loop as one loop (one home for the register rows), at most 1000 sweeps
+  bool finished = badA, run = false;
+  int sweeps = 0;
+  bool go = __ballot(!finished) != 0ull;
+  if (go) {
+    sweeps = 1;
+    run = !finished;
+    if (run)
+      finished = true;
+  }
+  int p = 1;
+  while (go) {
+    {
+      double up0 = row_get<NM, NT>(um0, ut0, p), vp0 = row_get<NM, NT>(vm0, vt0, p);
+      double up1 = row_get<NM, NT>(um1, ut1, p), vp1 = row_get<NM, NT>(vm1, vt1, p);
+      auto step = [&](int q, auto tail) {
+        constexpr bool TAIL = decltype(tail)::value;
+        bool rot = false;
+        double wpq = 0.0, wqp = 0.0;
+        if (run) {
+          const double thr = fmax(DBL_MIN, precision * maxDiag);
+          wpq = Wa[p * n + q];
+          wqp = Wa[q * n + p];
+          rot = fabs(wpq) > thr || fabs(wqp) > thr;
+        }
+        if (!__ballot(rot))
+          return;
+        if (A) {
+          Rot jl{ 1.0, 0.0 }, jr{ 1.0, 0.0 };
+          if (rot) {
+            finished = false;
+            real_2x2(Wa[p * n + p], wpq, wqp, Wa[q * n + q], &jl, &jr);
+          }
+          Rt[4 * lane + 0] = jl.c;
+          Rt[4 * lane + 1] = jl.s;
+          Rt[4 * lane + 2] = jr.c;
+          Rt[4 * lane + 3] = jr.s;
+        }
+        group_sync<true>();
+        const double lc0 = Rt[4 * gg0 + 0], ls0 = Rt[4 * gg0 + 1], rc0 = Rt[4 * gg0 + 2], rs0 = Rt[4 * gg0 + 3];
+        const double lc1 = Rt[4 * gg1 + 0], ls1 = Rt[4 * gg1 + 1], rc1 = Rt[4 * gg1 + 2], rs1 = Rt[4 * gg1 + 3];
+        if constexpr (TAIL) {
+          double n0, n1;
+          left(W0, j0, act0, lc0, ls0, p, q, up0, ut0[(q - NM) & TM], n0);
+          ut0[(q - NM) & TM] = n0;
+          left(W1, j1, act1, lc1, ls1, p, q, up1, ut1[(q - NM) & TM], n1);
+          ut1[(q - NM) & TM] = n1;
+        } else {
+          double n0, n1;
+          left(W0, j0, act0, lc0, ls0, p, q, up0, um0[q & (NM - 1)], n0);
+          um0[q & (NM - 1)] = n0;
+          left(W1, j1, act1, lc1, ls1, p, q, up1, um1[q & (NM - 1)], n1);
+          um1[q & (NM - 1)] = n1;
+        }
+        group_sync<true>();
+        if constexpr (TAIL) {
+          double n0, n1;
+          right(W0, j0, act0, rc0, rs0, p, q, vp0, vt0[(q - NM) & TM], n0);
+          vt0[(q - NM) & TM] = n0;
+          right(W1, j1, act1, rc1, rs1, p, q, vp1, vt1[(q - NM) & TM], n1);
+          vt1[(q - NM) & TM] = n1;
+        } else {
+          double n0, n1;
+          right(W0, j0, act0, rc0, rs0, p, q, vp0, vm0[q & (NM - 1)], n0);
+          vm0[q & (NM - 1)] = n0;
+          right(W1, j1, act1, rc1, rs1, p, q, vp1, vm1[q & (NM - 1)], n1);
+          vm1[q & (NM - 1)] = n1;
+        }
+        group_sync<true>();
+        if (rot)
+          maxDiag = fmax(maxDiag, fmax(fabs(Wa[p * n + p]), fabs(Wa[q * n + q])));
+      };
+      const int qm = p < NM ? p : NM;
+      for (int q = 0; q < qm; q++)
+        step(q, std::false_type{});
+      if constexpr (NT > 0)
+        for (int q = NM; q < p; q++)
+          step(q, std::true_type{});
+      row_set<NM, NT>(um0, ut0, p, up0);
+      row_set<NM, NT>(vm0, vt0, p, vp0);
+      row_set<NM, NT>(um1, ut1, p, up1);
+      row_set<NM, NT>(vm1, vt1, p, vp1);
+    }
+    if (++p == n) {
+      p = 1;
+      go = __ballot(!finished) != 0ull && sweeps < 1000;
+      if (go) {
+        sweeps++;
+        run = !finished;
+        if (run)
+          finished = true;
+      }
+    }
+  }
+  // owner lanes: non-negative singular values (bit i of the sign mask negates column i of U), their
+  // decreasing order (first maximum wins; perm[i] is the column now at position i) and the rank
+  if (A) {
+    unsigned flip = 0;
+    for (int i = 0; i < n; i++) {
+      const double a = badA ? 0.0 : Wa[i * n + i];
+      if (!badA && a < 0.0)
+        flip |= 1u << i;
+      if (!badA)
+        sva[i] = fabs(a) * scale;
+      pa[i] = i;
+    }
+    int nonzero = n;
+    if (!badA)
+      for (int i = 0; i < n; i++) {
+        int pos = i;
+        double mx = sva[i];
+        for (int c = i + 1; c < n; c++)
+          if (sva[c] > mx) {
+            mx = sva[c];
+            pos = c;
+          }
+        if (mx == 0.0) {
+          nonzero = i;
+          break;
+        }
+        if (pos != i) {
+          const double a = sva[i];
+          sva[i] = sva[pos];
+          sva[pos] = a;
+          const int c = pa[i];
+          pa[i] = pa[pos];
+          pa[pos] = c;
+        }
+      }
+    int rank = 0;
+    if (!badA) {
+      const double thr = fmax(sva[0] * ((double)n * DBL_EPSILON), DBL_MIN);
+      int r = nonzero - 1;
+      while (r >= 0 && sva[r] < thr)
+        r--;
+      rank = r + 1;
+    }
+    rk[lane] = rank;
+    fl[lane] = (int)flip;
+  }
+  group_sync<true>();
+  // rows of U (signs applied) into W's slot, then tmp = S^-1 (U^T b) over the rank
+  const unsigned f0 = (unsigned)fl[gg0], f1 = (unsigned)fl[gg1];
+  if (act0) {
+#pragma unroll
+    for (int k = 0; k < NM; k++)
+      if (k < n)
+        W0[j0 * n + k] = ((f0 >> k) & 1u) ? -um0[k] : um0[k];
+#pragma unroll
+    for (int k = 0; k < NT; k++)
+      if (NM + k < n)
+        W0[j0 * n + NM + k] = ((f0 >> (NM + k)) & 1u) ? -ut0[k] : ut0[k];
+  }
+  if (act1) {
+#pragma unroll
+    for (int k = 0; k < NM; k++)
+      if (k < n)
+        W1[j1 * n + k] = ((f1 >> k) & 1u) ? -um1[k] : um1[k];
+#pragma unroll
+    for (int k = 0; k < NT; k++)
+      if (NM + k < n)
+        W1[j1 * n + NM + k] = ((f1 >> (NM + k)) & 1u) ? -ut1[k] : ut1[k];
+  }
+  group_sync<true>();
+  const int rank0 = act0 ? rk[gg0] : 0, rank1 = act1 ? rk[gg1] : 0;
+  auto utb = [&](const double* Ws, int g, int j, int rank) {
+    if (j < rank) {
+      const int cj = perm[g * n + j];
+      const double* bg = b + g * n;
+      double s = 0.0;
+      for (int c = 0; c < n; c++)
+        s += Ws[c * n + cj] * bg[c];
+      tmp[g * n + j] = (1.0 / sv[g * n + j]) * s;
+    }
+  };
+  utb(W0, gg0, j0, rank0);
+  utb(W1, gg1, j1, rank1);
+  group_sync<true>();
+  // rows of V into W's slot, then x = V tmp over the rank
+  if (act0) {
+#pragma unroll
+    for (int k = 0; k < NM; k++)
+      if (k < n)
+        W0[j0 * n + k] = vm0[k];
+#pragma unroll
+    for (int k = 0; k < NT; k++)
+      if (NM + k < n)
+        W0[j0 * n + NM + k] = vt0[k];
+  }
+  if (act1) {
+#pragma unroll
+    for (int k = 0; k < NM; k++)
+      if (k < n)
+        W1[j1 * n + k] = vm1[k];
+#pragma unroll
+    for (int k = 0; k < NT; k++)
+      if (NM + k < n)
+        W1[j1 * n + NM + k] = vt1[k];
+  }
+  group_sync<true>();
+  auto vx = [&](const double* Ws, int g, int j, int rank) {
+    double s = 0.0;
+    for (int i = 0; i < rank; i++)
+      s += Ws[j * n + perm[g * n + i]] * tmp[g * n + i];
+    x[g * n + j] = s;
+  };
+  if (act0)
+    vx(W0, gg0, j0, rank0);
+  if (act1)
+    vx(W1, gg1, j1, rank1);
+  group_sync<true>();
+}
+
+// Prediction (and, at the last theta, coefficient j) of system g of the group that starts at item i0.
+__device__ void write_slot(const SmapParams& S, int t0, int i0, int n, int g, int j, const double* xsol)
+{
+  const int Ea = n - 1, T = S.T;
+  const int item = i0 + g;
+  const int tl = item / T, th = item - (item / T) * T;
+  const int t = t0 + tl;
+  const double* xg = xsol + g * n;
+  if (j == 0) {
+    double r = xg[0];
+    const double* yp = S.Mp + (size_t)t * Ea;
+    for (int c = 0; c < Ea; c++)
+      if (yp[c] != MISSING_D)
+        r += yp[c] * xg[c + 1];
+    S.pred[(size_t)th * S.NP + t] = r;
+    S.rc[(size_t)th * S.NP + t] = RC_SUCCESS;
+  }
+  if (S.coeffs && th == T - 1) {
+    const double c = xg[j];
+    S.coeffs[(size_t)t * n + j] = fabs(c) < 1.0e-11 ? MISSING_D : c;
+  }
+}
+
+// Persistent grid over groups of G (target, theta) items (G <= smap_pack_group(n), chosen by the host),
+// two row slots per lane.
+template <int NM, int NT>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void smap_pack(SmapParams S, const double* __restrict__ D, int t0, int nt, int G)
+{
+  extern __shared__ double smem[];
+  const int n = S.Ea + 1;
+  const int nacc = n * (n + 1) / 2 + n;
+  const int lane = threadIdx.x;
+  const int sin = smap_rows_stage(n);
+  double* W = smem;
+  double* bvec = W + (size_t)G * n * n;
+  double* sv = bvec + G * n;
+  double* tmp = sv + G * n;
+  double* xsol = tmp + G * n;
+  double* acc = xsol + G * n;
+  double* cd = acc + nacc;
+  double* stage = cd + SM_CH;
+  int* cidx = (int*)(stage + (sin > 0 ? 0 : SM_CH * (n + 1)));
+  int* perm = cidx + SM_CH;
+  double* Rt = smem + smap_rows_lds_doubles(n, G);
+  double* scl = Rt + 4 * G;
+  int* rk = (int*)(scl + G);
+  int* fl = rk + G;
+
+  const int s1 = 64 + lane;
+  const int g0 = lane / n, j0 = lane - (lane / n) * n;
+  const int g1 = s1 / n, j1 = s1 - (s1 / n) * n;
+  const int items = nt * S.T;
+  for (int i0 = blockIdx.x * G; i0 < items; i0 += gridDim.x * G) {
+    const unsigned valid =
+        form_group(S, D, t0, nt, i0, n, G, W, bvec, sin > 0 ? nullptr : stage, sin > 0 ? sin : SM_CH, acc, cd,
+                   cidx, lane);
+    jacobi_svd_solve_pack<NM, NT>(n, G, W, bvec, sv, tmp, xsol, perm, Rt, scl, rk, fl, valid, lane);
+    if (g0 < G && ((valid >> g0) & 1u))
+      write_slot(S, t0, i0, n, g0, j0, xsol);
+    if (g1 < G && ((valid >> g1) & 1u))
+      write_slot(S, t0, i0, n, g1, j1, xsol);
+    __syncthreads();
+  }
+}
+
 // The rotations of one Jacobi step of one system of a block, made by the block's 2x2 lanes.
 struct BlkRot
 {
@@ -1910,9 +2307,45 @@ __global__ __launch_bounds__(SB_THREADS) void smap_big(SmapParams S, const doubl
   }
 }
 
-int smap_launch(int n, hipStream_t s, const SmapParams& S, const double* D, int t0, int nt, bool lds_rows,
-                bool block_solve)
+// smap_pack over groups of G systems: the largest G whose LDS leaves the blocks per CU that the kernel's
+// registers allow (or, with max_group, the largest G that fits).
+static int smap_pack_launch(int n, hipStream_t s, const SmapParams& S, const double* D, int t0, int nt, bool max_group)
 {
+  const void* fn = n <= 8    ? (const void*)smap_pack<8, 0>
+                   : n <= 16 ? (const void*)smap_pack<16, 0>
+                   : n <= 17 ? (const void*)smap_pack<16, 1>
+                   : n <= 20 ? (const void*)smap_pack<16, 4>
+                             : (const void*)smap_pack<16, 8>;
+  hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+  int dev = 0, ncu = 0, occ1 = 0, occ = 0;
+  if (e == hipSuccess)
+    e = hipGetDevice(&dev);
+  if (e == hipSuccess)
+    e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+  if (e == hipSuccess)
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ1, fn, 64, sizeof(double) * smap_pack_lds_doubles(n, 1));
+  int G = smap_pack_group(n);
+  while (e == hipSuccess) {
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fn, 64, sizeof(double) * smap_pack_lds_doubles(n, G));
+    if (e != hipSuccess || G == 1 || (max_group ? occ >= 1 : occ >= occ1))
+      break;
+    G--;
+  }
+  if (e != hipSuccess)
+    return -(int)e - 1000;
+  const size_t lds = sizeof(double) * smap_pack_lds_doubles(n, G);
+  const long long groups = ((long long)nt * S.T + G - 1) / G;
+  const int grid = (int)std::max<long long>(1, std::min<long long>(groups, (long long)ncu * std::max(occ, 1)));
+  void* args[] = { (void*)&S, (void*)&D, (void*)&t0, (void*)&nt, (void*)&G };
+  e = hipLaunchKernel(fn, dim3(grid), dim3(64), args, lds, s);
+  return e == hipSuccess ? 0 : -(int)e - 1000;
+}
+
+int smap_launch(int n, hipStream_t s, const SmapParams& S, const double* D, int t0, int nt, bool lds_rows,
+                int solve_mode)
+{
+  if (solve_mode == SMAP_SOLVE_AUTO)
+    solve_mode = n >= 9 && n <= 24 && (long long)nt * S.T >= SMAP_PACK_MIN_ITEMS ? SMAP_SOLVE_PACK : SMAP_SOLVE_ROWS;
   if (n > 64 && n <= SMAP_BIG_MAX) {
     if (!S.big || S.big_blocks < 1)
       return -22;
@@ -1927,11 +2360,13 @@ int smap_launch(int n, hipStream_t s, const SmapParams& S, const double* D, int 
   }
   if (n > 64 || n < 2)
     return -22;
+  if (!lds_rows && n >= 2 && n <= 24 && (solve_mode == SMAP_SOLVE_PACK || solve_mode == SMAP_SOLVE_PACK_MAX))
+    return smap_pack_launch(n, s, S, D, t0, nt, solve_mode == SMAP_SOLVE_PACK_MAX);
   const int G = smap_group(n);
   const void* fn = (const void*)smap_multi;
   size_t lds = sizeof(double) * smap_lds_doubles(n, G);
   int waves = 1; // wavefronts per block (each with G systems)
-  if (!lds_rows && n <= 24 && block_solve) {
+  if (!lds_rows && n <= 24 && solve_mode == SMAP_SOLVE_BLOCK) {
     constexpr int WB = BLK_WAVES;
     fn = n <= 8    ? (const void*)smap_blk<8, 0, WB>
          : n <= 16 ? (const void*)smap_blk<16, 0, WB>

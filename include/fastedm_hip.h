@@ -160,8 +160,15 @@ typedef struct
                                spatially pruned direct kernel (1) */
   int concurrent_routes;    /* persistent route: the direct kernels run on a second stream beside the
                                rank-table kernels (0) */
-  int smap_block;           /* S-map solve of up to 24 unknowns over blocks of four wavefronts, each
-                               rotation's 2x2 step made once per system (1), or per wavefront (0) (0) */
+  int smap_block;           /* S-map solve of up to 24 unknowns: one row per lane, every lane of a system
+                               making the 2x2 steps (0); blocks of four wavefronts, each 2x2 step made once
+                               per system (1); two rows per lane, each 2x2 step made once per system, as
+                               many systems per wavefront as keep the register-bound occupancy (2) or as
+                               fit (3); (2) for 9..24 unknowns and launches of at least 65 536 systems,
+                               else (0) (4).  All give the same bits. (4) */
+  int mfma_seed;            /* matrix-core route: each target's screening threshold starts from the exact k-th
+                               distance over 256 library rows near it in a k-d order of the library (1), or
+                               unbounded (0) (1) */
 } fedm_knobs;
 void fedm_default_knobs(fedm_knobs* k);
 

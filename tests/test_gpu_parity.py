@@ -96,11 +96,12 @@ def test_task_parity(name):
 
 @pytest.mark.parametrize("Ea", [1, 3, 7, 8, 12, 15, 16, 17, 19, 22, 23, 32, 40, 47])
 def test_smap_register_rows_match_lds_rows(Ea, monkeypatch):
-    """For n = E_a + 1 <= 24 the S-map solve keeps U and V rows in registers (one wavefront at a time;
-    FEDM_SMAP_BLOCK=1: block-wide, each rotation's 2x2 step made once per system), for 33 <= n <= 48
-    U's rows; FEDM_SMAP_LDS_ROWS=1 keeps them in LDS.  All run the same floating-point operations in the
-    same order, so every prediction and coefficient must agree to the bit (k near n gives rank-deficient
-    and ill-conditioned systems)."""
+    """For n = E_a + 1 <= 24 the S-map solve keeps U and V rows in registers: FEDM_SMAP_BLOCK=0, one row
+    per lane; 1, block-wide with each rotation's 2x2 step made once per system; 2 and 3, two rows per lane
+    with the 2x2 steps made once per system (fewer or more systems per wavefront); 4, the default choice.
+    For 33 <= n <= 48 U's rows are in registers; FEDM_SMAP_LDS_ROWS=1 keeps them in LDS.  All run the same
+    floating-point operations in the same order, so every prediction and coefficient must agree to the
+    bit (k near n gives rank-deficient and ill-conditioned systems)."""
     fe = load_product()
     for k in (Ea + 1, Ea + 3):
         M, Mp, opts = random_task(100 + 7 * Ea + k, NL=700, NP=150, Ea=Ea, k=k, algorithm=1, thetas=(0, 1, 4),
@@ -110,7 +111,8 @@ def test_smap_register_rows_match_lds_rows(Ea, monkeypatch):
         monkeypatch.delenv("FEDM_SMAP_BLOCK", raising=False)
         gp, gr, gc, gk, gn, status = fe.predict(M, Mp, opts, nn_cap=64)
         runs = []
-        for var, val in (("FEDM_SMAP_LDS_ROWS", "1"), ("FEDM_SMAP_BLOCK", "1")):
+        for var, val in (("FEDM_SMAP_LDS_ROWS", "1"), ("FEDM_SMAP_BLOCK", "0"), ("FEDM_SMAP_BLOCK", "1"),
+                         ("FEDM_SMAP_BLOCK", "2"), ("FEDM_SMAP_BLOCK", "3")):
             monkeypatch.setenv(var, val)
             runs.append(fe.predict(M, Mp, opts, nn_cap=64))
             monkeypatch.delenv(var)
@@ -296,3 +298,42 @@ def test_device_exp_matches_oracle_bitwise():
     O.orc_weight_exp.argtypes = [C.c_double]
     ref = np.array([O.orc_weight_exp(float(v)) for v in x])
     assert np.array_equal(out.view(np.int64), ref.view(np.int64))
+
+
+@pytest.mark.parametrize("case", ["clustered_dt", "ties"])
+def test_mfma_seed_leaves_results_unchanged(case, monkeypatch):
+    """The matrix-core route starts each target's FP32 screening threshold from the exact k-th distance
+    over a window of library rows near it in a k-d order (FEDM_MFMA_SEED=1, the default).  The seed only
+    removes candidates that cannot enter the list, so indices, k_used, codes and predictions must equal
+    the unseeded run's bit for bit and the oracle's.  Cases: a dt-like layout (a column equal to a
+    constant on most rows), and an integer table with many tied and zero distances."""
+    fe = load_product()
+    rng = np.random.default_rng(11 if case == "ties" else 12)
+    NL, NP, Ea, k = 3000, 400, 12, 9
+    if case == "ties":
+        data = rng.integers(0, 4, size=(NL + NP, Ea)).astype(float)
+    else:
+        base = rng.random((NL + NP, Ea))
+        base[:, 6:] = 1.25
+        gaps = rng.random((NL + NP, 6)) < 0.05
+        base[:, 6:][gaps] = 2.5
+        data = base
+    M = {"data": data[:NL].copy(), "targets": rng.random(NL)}
+    Mp = {"data": data[NL:].copy()}
+    opts = {"algorithm": 0, "k": k, "thetas": [0.0, 1.0], "precision": fe.MIXED}
+    print(f"mfma seed: {case}", flush=True)
+    outs = []
+    for seed in ("0", "1"):
+        monkeypatch.setenv("FEDM_MFMA_SEED", seed)
+        outs.append(fe.predict(M, Mp, opts, nn_cap=k))
+    monkeypatch.delenv("FEDM_MFMA_SEED")
+    (p0, r0, c0, k0, n0, s0), (p1, r1, c1, k1, n1, s1) = outs
+    assert s0 == 0 and s1 == 0
+    np.testing.assert_array_equal(n0, n1)
+    np.testing.assert_array_equal(k0, k1)
+    np.testing.assert_array_equal(r0, r1)
+    np.testing.assert_array_equal(np.ascontiguousarray(p0).view(np.uint64), np.ascontiguousarray(p1).view(np.uint64))
+    rp, rr, rc_, rk, rn = oracle.predict(M, Mp, dict(opts, precision=0), nn_cap=k)
+    np.testing.assert_array_equal(n1, rn)
+    np.testing.assert_array_equal(k1, rk)
+    assert_predictions_close(p1, rp)
