@@ -40,9 +40,19 @@ size_t gram_dense_bytes(int n, int nt, int T);
 int gram_dense_launch(int n, hipStream_t s, const SmapParams& S, const double* D, int t0, int nt, double* gw);
 
 template <int KCAP, int KP>
-__global__ void knn_mfma(MfmaParams P, int k, int slice_len, double* out_s, int* out_i, int* out_cnt);
+__global__ void knn_mfma(MfmaParams P, int k, int slice_len, double* out_s, int* out_i, int* out_cnt, const int* tl,
+                         const int* tn);
 template <int KP>
-__global__ void knn_seed(MfmaParams P, const int* kdp, const int* wst, const int* tord, int win, int k, double* seed);
+__global__ void knn_seed(MfmaParams P, const double* kdm, const int* wst, const int* tord, int win, int k,
+                         double* seed);
+template <int KP>
+__global__ void knn_screen(MfmaParams P, int slice_len, int* cand, int* ccnt);
+template <int KCAP, int KP>
+__global__ void knn_rerank(MfmaParams P, int k, int slice_len, const int* cand, const int* ccnt, double* out_s,
+                           int* out_i, int* out_cnt, int* fflag, int* flist, int* fcnt);
+template <int KCAP>
+__global__ void merge_fallback(int NP, int S, int SF, const int* flist, const int* fcnt, const double* fs, const int* fi,
+                               const int* fc, double* out_s, int* out_i, int* out_cnt);
 __global__ void mfma_layout_probe(float* out);
 
 
@@ -138,6 +148,14 @@ enum
   B_WST,
   B_TORD,
   B_SEED,
+  B_CAND,
+  B_CCNT,
+  B_FLIST,
+  B_FCNT,
+  B_FFLAG,
+  B_FPS,
+  B_FPI,
+  B_FPC,
   B_NUM
 };
 } // namespace fedm
@@ -220,18 +238,18 @@ static void launch_knn(int kcap, int emax, bool generic, dim3 g, hipStream_t s, 
 
 template <int KCAP>
 static void launch_mfma_k(int kp, dim3 g, hipStream_t s, const MfmaParams& P, int k, int sl, double* ps, int* pi,
-                          int* pc)
+                          int* pc, const int* tl, const int* tn)
 {
   if (kp == 16)
-    hipLaunchKernelGGL((knn_mfma<KCAP, 16>), g, dim3(256), 0, s, P, k, sl, ps, pi, pc);
+    hipLaunchKernelGGL((knn_mfma<KCAP, 16>), g, dim3(256), 0, s, P, k, sl, ps, pi, pc, tl, tn);
   else if (kp == 32)
-    hipLaunchKernelGGL((knn_mfma<KCAP, 32>), g, dim3(256), 0, s, P, k, sl, ps, pi, pc);
+    hipLaunchKernelGGL((knn_mfma<KCAP, 32>), g, dim3(256), 0, s, P, k, sl, ps, pi, pc, tl, tn);
   else
-    hipLaunchKernelGGL((knn_mfma<KCAP, 64>), g, dim3(256), 0, s, P, k, sl, ps, pi, pc);
+    hipLaunchKernelGGL((knn_mfma<KCAP, 64>), g, dim3(256), 0, s, P, k, sl, ps, pi, pc, tl, tn);
 }
 
-static void launch_seed(int kp, hipStream_t s, const MfmaParams& P, const int* kdp, const int* wst, const int* tord,
-                        int win, int k, double* seed)
+static void launch_seed(int kp, hipStream_t s, const MfmaParams& P, const double* kdp, const int* wst,
+                        const int* tord, int win, int k, double* seed)
 {
   const dim3 g((P.NP + 3) / 4);
   if (kp == 16)
@@ -242,15 +260,57 @@ static void launch_seed(int kp, hipStream_t s, const MfmaParams& P, const int* k
     hipLaunchKernelGGL((knn_seed<64>), g, dim3(256), 0, s, P, kdp, wst, tord, win, k, seed);
 }
 
+// Phases one and two of the two-phase matrix-core route: the seeded screen of every slice, then the exact
+// re-rank of each (slice, target) whose candidates fit; the others land in (flist, fcnt).
+template <int KP>
+static void launch_two_phase_p(int kcap, hipStream_t s, const MfmaParams& P, int k, int S, int sl, int* cand,
+                               int* ccnt, double* ps, int* pi, int* pc, int* ff, int* fl, int* fc)
+{
+  hipLaunchKernelGGL((knn_screen<KP>), dim3((P.NP + 127) / 128, S), dim3(256), 0, s, P, sl, cand, ccnt);
+  const dim3 g((P.NP + 3) / 4, S);
+  switch (kcap) {
+    case 4: hipLaunchKernelGGL((knn_rerank<4, KP>), g, dim3(256), 0, s, P, k, sl, cand, ccnt, ps, pi, pc, ff, fl, fc); break;
+    case 8: hipLaunchKernelGGL((knn_rerank<8, KP>), g, dim3(256), 0, s, P, k, sl, cand, ccnt, ps, pi, pc, ff, fl, fc); break;
+    case 16: hipLaunchKernelGGL((knn_rerank<16, KP>), g, dim3(256), 0, s, P, k, sl, cand, ccnt, ps, pi, pc, ff, fl, fc); break;
+    case 32: hipLaunchKernelGGL((knn_rerank<32, KP>), g, dim3(256), 0, s, P, k, sl, cand, ccnt, ps, pi, pc, ff, fl, fc); break;
+    default: hipLaunchKernelGGL((knn_rerank<64, KP>), g, dim3(256), 0, s, P, k, sl, cand, ccnt, ps, pi, pc, ff, fl, fc); break;
+  }
+}
+
+static void launch_two_phase(int kcap, int kp, hipStream_t s, const MfmaParams& P, int k, int S, int sl, int* cand,
+                             int* ccnt, double* ps, int* pi, int* pc, int* ff, int* fl, int* fc)
+{
+  if (kp == 16)
+    launch_two_phase_p<16>(kcap, s, P, k, S, sl, cand, ccnt, ps, pi, pc, ff, fl, fc);
+  else if (kp == 32)
+    launch_two_phase_p<32>(kcap, s, P, k, S, sl, cand, ccnt, ps, pi, pc, ff, fl, fc);
+  else
+    launch_two_phase_p<64>(kcap, s, P, k, S, sl, cand, ccnt, ps, pi, pc, ff, fl, fc);
+}
+
+static void launch_merge_fallback(int kcap, hipStream_t s, int NP, int S, int SF, const int* fl, const int* fc,
+                                  const double* fs, const int* fi, const int* fcn, double* ps, int* pi, int* pc)
+{
+  const dim3 g((NP + 255) / 256);
+  switch (kcap) {
+    case 4: hipLaunchKernelGGL((merge_fallback<4>), g, dim3(256), 0, s, NP, S, SF, fl, fc, fs, fi, fcn, ps, pi, pc); break;
+    case 8: hipLaunchKernelGGL((merge_fallback<8>), g, dim3(256), 0, s, NP, S, SF, fl, fc, fs, fi, fcn, ps, pi, pc); break;
+    case 16: hipLaunchKernelGGL((merge_fallback<16>), g, dim3(256), 0, s, NP, S, SF, fl, fc, fs, fi, fcn, ps, pi, pc); break;
+    case 32: hipLaunchKernelGGL((merge_fallback<32>), g, dim3(256), 0, s, NP, S, SF, fl, fc, fs, fi, fcn, ps, pi, pc); break;
+    default: hipLaunchKernelGGL((merge_fallback<64>), g, dim3(256), 0, s, NP, S, SF, fl, fc, fs, fi, fcn, ps, pi, pc); break;
+  }
+}
+
+// The streaming matrix-core k-NN over every target, or (tl, tn) over the targets listed per slice.
 static void launch_mfma(int kcap, int kp, dim3 g, hipStream_t s, const MfmaParams& P, int k, int sl, double* ps,
-                        int* pi, int* pc)
+                        int* pi, int* pc, const int* tl = nullptr, const int* tn = nullptr)
 {
   switch (kcap) {
-    case 4: launch_mfma_k<4>(kp, g, s, P, k, sl, ps, pi, pc); break;
-    case 8: launch_mfma_k<8>(kp, g, s, P, k, sl, ps, pi, pc); break;
-    case 16: launch_mfma_k<16>(kp, g, s, P, k, sl, ps, pi, pc); break;
-    case 32: launch_mfma_k<32>(kp, g, s, P, k, sl, ps, pi, pc); break;
-    default: launch_mfma_k<64>(kp, g, s, P, k, sl, ps, pi, pc); break;
+    case 4: launch_mfma_k<4>(kp, g, s, P, k, sl, ps, pi, pc, tl, tn); break;
+    case 8: launch_mfma_k<8>(kp, g, s, P, k, sl, ps, pi, pc, tl, tn); break;
+    case 16: launch_mfma_k<16>(kp, g, s, P, k, sl, ps, pi, pc, tl, tn); break;
+    case 32: launch_mfma_k<32>(kp, g, s, P, k, sl, ps, pi, pc, tl, tn); break;
+    default: launch_mfma_k<64>(kp, g, s, P, k, sl, ps, pi, pc, tl, tn); break;
   }
 }
 
@@ -317,9 +377,10 @@ extern "C" void fedm_default_knobs(fedm_knobs* k)
   // (0.55 against 0.45 ms: two wavefronts per SIMD and too few groups to fill the device)
   const char* b = getenv("FEDM_SMAP_BLOCK");
   k->smap_block = b && b[0] >= '0' && b[0] <= '4' ? b[0] - '0' : SMAP_SOLVE_AUTO;
-  // FEDM_MFMA_SEED=0 starts the matrix-core screening unbounded (same results)
+  // FEDM_MFMA_SEED=0 starts the matrix-core screening unbounded, 1 seeds the streaming kernel, 2 runs the
+  // two-phase form, 3 chooses between 1 and 2 (same results)
   const char* sd = getenv("FEDM_MFMA_SEED");
-  k->mfma_seed = !(sd && sd[0] == '0');
+  k->mfma_seed = sd && sd[0] >= '0' && sd[0] <= '3' ? sd[0] - '0' : 3;
 }
 
 static fedm_knobs default_knobs()
@@ -490,6 +551,14 @@ static BufRole buf_role(int slot)
     case B_SMAPBIG:
     case B_GRAM:
     case B_SEED:
+    case B_CAND:
+    case B_CCNT:
+    case B_FLIST:
+    case B_FCNT:
+    case B_FFLAG:
+    case B_FPS:
+    case B_FPI:
+    case B_FPC:
       return ROLE_SCRATCH;
     case B_PRED:
     case B_RC:
@@ -518,7 +587,10 @@ struct TaskPlan
   std::vector<float> m32, mn, p32, pn;
   std::vector<double> ynorm;
   std::vector<int> kdp, wst, tord; // matrix-core seed: library rows in k-d order, window starts, target order
+  std::vector<double> kdm;         // the library rows in that order (FP64)
   int seed_win = 0;                // rows per seed window (0: no seed)
+  bool two_phase = false;          // seeded screen, exact re-rank, streaming kernel for what overflows
+  int sf = 1;                      // two-phase: library sub-slices of the streaming kernel on the overflow
   LpParams P;
   WassParams W;
   FinishParams F;
@@ -779,7 +851,14 @@ static int plan_task(const edm_manifold_view* M, const edm_manifold_view* Mp, co
     const double u = 5.9604644775390625e-08;
     MfmaParams& MP = tp.MP;
     std::memset(&MP, 0, sizeof(MP));
-    // the screening bound per target (kernel): 2 (k1 rr sqrt(D) + c2 D + k3 rr^2), rr >= |x| + |y|
+    // the screening bound per target (kernel): 2 (k1 rr sqrt(D) + c2 D + k3 rr^2), rr >= |x| + |y|:
+    //   k1 rr sqrt(D) + k1^2 rr^2 / 4: the FP32 rounding of the centred coordinates (|dx| <= u rr);
+    //   c2 D: the rounding of the centring and the FP64 evaluation of the floor;
+    //   k3 rr^2: the two FP32 norms (gamma_Ea (|x|^2 + |y|^2)) and the accumulation of the KP rounded
+    //   products onto -||x||^2 / 2 (gamma_(KP+1) (||x||^2 / 2 + |x| |y|), doubled in the distance), together
+    //   within gamma_(KP+1) (2 |x|^2 + |y|^2 + 2 |x| |y|) <= 2 gamma_(KP+1) rr^2 (Ea <= KP), with
+    //   gamma_m = m u / (1 - m u) <= 1.02 m u, plus u^2 rr^2 from the inputs; B = 2 (...) covers it with
+    //   k3 = (KP + 3) 1.02 u + u^2, as 2 (KP + 3) 1.02 >= 2.04 (KP + 1)
     MP.R = R;
     MP.k1 = 2.02 * u;
     MP.k3 = (kp + 3) * 1.02 * u + u * u;
@@ -795,10 +874,34 @@ static int plan_task(const edm_manifold_view* M, const edm_manifold_view* Mp, co
     if (knobs.mfma_seed && NL > 0 && NP > 0) {
       kd_seed_windows(M->data, NL, Mp->data, NP, Ea, SEED_WIN, tp.kdp, tp.wst, tp.tord);
       tp.seed_win = std::min(NL, SEED_WIN);
-      by[B_KDP] = sizeof(int) * tp.kdp.size();
+      tp.kdm.resize((size_t)NL * Ea);
+      for (int i = 0; i < NL; i++)
+        std::copy(M->data + (size_t)tp.kdp[i] * Ea, M->data + (size_t)(tp.kdp[i] + 1) * Ea, tp.kdm.data() + (size_t)i * Ea);
+      by[B_KDP] = sizeof(double) * tp.kdm.size();
       by[B_WST] = sizeof(int) * tp.wst.size();
       by[B_TORD] = sizeof(int) * tp.tord.size();
       by[B_SEED] = sizeof(double) * (size_t)NP;
+      // auto (3): the two-phase form where the seeds are near the final k-th distance, that is for points of
+      // up to 16 coordinates and up to 32 neighbours (panel workload, E_a = 16, k = 18: select 39.6 -> 35.5
+      // ms); with 40 coordinates and k = 42 (Lorenz-96) nearly every target overflows the candidate space
+      // and the seeded streaming kernel is 3x faster
+      const bool two = knobs.mfma_seed == 2 || (knobs.mfma_seed == 3 && tp.kp == 16 && tp.kcap <= 32);
+      if (two) {
+        tp.two_phase = true;
+        by[B_CAND] = sizeof(int) * (size_t)tp.S * NP * SCR_CAP;
+        by[B_CCNT] = sizeof(int) * (size_t)tp.S * NP;
+        by[B_FLIST] = sizeof(int) * (size_t)NP;
+        by[B_FCNT] = sizeof(int);
+        by[B_FFLAG] = sizeof(int) * (size_t)NP;
+        // the overflowing targets are few: split the library into up to 16 sub-slices for them, within a
+        // 1 GiB budget for their lists
+        const size_t per = (size_t)NP * tp.kcap * 12;
+        tp.sf = (int)std::max<size_t>(1, std::min<size_t>({ 16, ((size_t)1 << 30) / std::max<size_t>(per, 1),
+                                                            (size_t)std::max(1, NL / 256) }));
+        by[B_FPS] = sizeof(double) * (size_t)tp.sf * NP * tp.kcap;
+        by[B_FPI] = sizeof(int) * (size_t)tp.sf * NP * tp.kcap;
+        by[B_FPC] = sizeof(int) * (size_t)tp.sf * NP;
+      }
     }
     by[B_PS] = sizeof(double) * (size_t)tp.S * NP * tp.kcap;
     by[B_PI] = sizeof(int) * (size_t)tp.S * NP * tp.kcap;
@@ -999,7 +1102,7 @@ static int upload_task(const TaskPlan& tp, const edm_manifold_view* M, const edm
     if (e == hipSuccess)
       e = cp(B_MIX4, tp.ynorm.data());
     if (e == hipSuccess)
-      e = cp(B_KDP, tp.kdp.data());
+      e = cp(B_KDP, tp.kdm.data());
     if (e == hipSuccess)
       e = cp(B_WST, tp.wst.data());
     if (e == hipSuccess)
@@ -1037,9 +1140,26 @@ static int launch_task(TaskPlan& tp, hipStream_t s, hipEvent_t* ev, int (*keep_g
   if (tp.route != R_DENSE) {
     if (tp.route == R_MFMA) {
       if (tp.MP.seed)
-        launch_seed(tp.kp, s, tp.MP, (const int*)tp.ptr[B_KDP], (const int*)tp.ptr[B_WST],
+        launch_seed(tp.kp, s, tp.MP, (const double*)tp.ptr[B_KDP], (const int*)tp.ptr[B_WST],
                     (const int*)tp.ptr[B_TORD], tp.seed_win, tp.k, (double*)tp.ptr[B_SEED]);
-      launch_mfma(tp.kcap, tp.kp, dim3(tp.bx, tp.S), s, tp.MP, tp.k, tp.sl, ps, pi, pc);
+      if (tp.MP.seed && tp.two_phase) {
+        int* fc = (int*)tp.ptr[B_FCNT];
+        int* fl = (int*)tp.ptr[B_FLIST];
+        int* ff = (int*)tp.ptr[B_FFLAG];
+        fill(fc, (size_t)1, 0);
+        fill(ff, (size_t)NP, 0);
+        launch_two_phase(tp.kcap, tp.kp, s, tp.MP, tp.k, tp.S, tp.sl, (int*)tp.ptr[B_CAND], (int*)tp.ptr[B_CCNT],
+                         ps, pi, pc, ff, fl, fc);
+        // the targets that overflowed: the streaming kernel over sf sub-slices of the library, then merged
+        // into the route's per-slice lists
+        double* fs = (double*)tp.ptr[B_FPS];
+        int* fi = (int*)tp.ptr[B_FPI];
+        int* fcn = (int*)tp.ptr[B_FPC];
+        launch_mfma(tp.kcap, tp.kp, dim3(tp.bx, tp.sf), s, tp.MP, tp.k, (tp.NL + tp.sf - 1) / tp.sf, fs, fi, fcn, fl, fc);
+        launch_merge_fallback(tp.kcap, s, NP, tp.S, tp.sf, fl, fc, fs, fi, fcn, ps, pi, pc);
+      } else {
+        launch_mfma(tp.kcap, tp.kp, dim3(tp.bx, tp.S), s, tp.MP, tp.k, tp.sl, ps, pi, pc);
+      }
     }
     else
       launch_knn(tp.kcap, tp.emax, tp.generic, dim3(tp.bx, tp.S), s, tp.P, tp.k, tp.sl, ps, pi, pc);

@@ -1133,7 +1133,7 @@ static int tune_group(fedm_plan* P, fedm_plan::Group& g, int trials, std::string
     axes.push_back({ "list_blocks", { 512, 1024, 2048, 4096 } });
     if (mixed && Ea >= 8) {
       axes.push_back({ "use_mfma", { 1, 0 } });
-      axes.push_back({ "mfma_seed", { 1, 0 } });
+      axes.push_back({ "mfma_seed", { 3, 2, 1, 0 } });
       axes.push_back({ "mfma_blocks", { 1024, 2048, 4096 } });
     }
     if (g.go.algorithm == FEDM_SMAP) {

@@ -1,10 +1,12 @@
 // Mixed-precision k-NN for one task: FP32 matrix-core distance tiles with an exact FP64 re-rank.
 //
-// Distances are screened with the expansion ||x||^2 + ||y||^2 - 2 x.y on centred FP32 coordinates;
-// the dot products of a 16-target x 16-library tile come from v_mfma_f32_16x16x4f32 (KP/4
-// instructions per tile).  A pair is a candidate when its FP32 distance is within a rigorous bound of
-// the target's current k-th distance (bound: FP32 rounding of the centred inputs, of the norms and dot
-// products, and of the final combination).  Candidates are queued per target in LDS and recomputed in
+// Distances are screened with the expansion ||x||^2 + ||y||^2 - 2 x.y on centred FP32 coordinates:
+// a 16-target x 16-library tile of x.y - ||x||^2 / 2 comes from v_mfma_f32_16x16x4f32 (KP/4 instructions
+// per tile, the accumulator started at -||x||^2 / 2 of the tile's library point), and a pair is a
+// candidate when that value is at least the target's screening floor (||y||^2 - th) / 2, rounded down,
+// where th is the target's current k-th squared distance plus a rigorous bound (FP32 rounding of the
+// centred inputs, of the norms, and of the accumulation of KP products and the starting value); one
+// compare per pair.  Candidates are queued per target in LDS and recomputed in
 // FP64 in the reference's operation order (src/distances.cpp:186-216) by the target's owner lane,
 // which keeps the exact (sqrt distance, position) list (src/edm.cpp:652-681).  The per-slice lists have
 // the layout of knn_slices_f64, so the merge, simplex and S-map stages are shared with the plain route.
@@ -14,12 +16,28 @@ namespace fedm {
 
 typedef float v4f __attribute__((ext_vector_type(4)));
 
-__device__ __forceinline__ float up_f32(double x)
+__device__ __forceinline__ float down_f32(double x)
 {
   float f = (float)x;
-  if ((double)f < x)
-    f = __int_as_float(__float_as_int(f) + 1);
+  if ((double)f > x)
+    f = nextafterf(f, -__builtin_huge_valf());
   return f;
+}
+
+// Screening floor of a target with FP32 squared norm ny, for squared distances up to T: a pair whose
+// exact squared distance is at most T has x.y - ||x||^2 / 2 (as the matrix cores accumulate it) at least
+// (ny - T - B(T)) / 2, B(T) = 2 (k1 rr sqrt(T) + c2 T + k3 rr^2) for any rr >= |x| + |y| of the centred rows
+// (rr is taken at least 1e-12, so that the flush of subnormal FP32 products stays inside the bound).
+__device__ __forceinline__ float screen_floor(const MfmaParams& P, double T, double yn, float ny)
+{
+  if (!(T < INF_D))
+    return -__builtin_huge_valf();
+  const double sq = sqrt(T);
+  const double rr = fmax(fmin(P.R + yn, 2.0 * yn + sq) * (1.0 + 1e-6), 1e-12);
+  const double th = T + 2.0 * (P.k1 * rr * sq + P.c2 * T + P.k3 * rr * rr);
+  // (ny - th) / 2, lowered past the rounding of its FP64 evaluation, then rounded down to FP32
+  const double v = 0.5 * ((double)ny - th);
+  return down_f32(v - fabs(v) * 0x1p-50 - 0x1p-1000);
 }
 
 // FP64 squared distance in the reference's order (src/distances.cpp:186-216); the loads of 8 coordinates
@@ -69,13 +87,14 @@ constexpr int mfma_waves(int kcap, int kp)
 
 template <int KCAP, int KP>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(mfma_waves(KCAP, KP)))) void knn_mfma(MfmaParams P, int k, int slice_len, double* __restrict__ out_s,
-                                               int* __restrict__ out_i, int* __restrict__ out_cnt)
+                                               int* __restrict__ out_i, int* __restrict__ out_cnt,
+                                               const int* __restrict__ tl, const int* __restrict__ tn)
 {
   constexpr int MCH = 64; // library points per LDS chunk (16-point tiles); LDS for four blocks per CU
   constexpr int F4 = MCH * KP / 4 / 256;    // float4 of the next chunk each thread prefetches
   __shared__ float Bs[MCH * KP];
-  __shared__ float Bn[MCH];
-  __shared__ float thr_s[64];
+  __shared__ float Bc[MCH];  // -||x||^2 / 2 of the chunk's library points (the accumulators' start)
+  __shared__ float flo_s[64]; // screening floor of each target of the block
   __shared__ int qcnt[64];
   constexpr int QCAP = 2 * MCH;             // queue slots per target: the candidates of two chunks
   __shared__ unsigned char queue[64 * QCAP];
@@ -89,32 +108,34 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(mfma_waves(
   const int w = threadIdx.x >> 6;
   const int g = lane >> 4;  // lane group: supplies K index g of each 4-wide MFMA step
   const int l16 = lane & 15;
-  const int tbase = blockIdx.x * 64 + w * 16;
   const int lo = blockIdx.y * slice_len;
   const int hi = min(P.NL, lo + slice_len);
+  // Targets: block row x of the wavefront is target blockIdx.x * 64 + 16 w + x, or, with a target list
+  // (tl, *tn entries, the same for every slice), that entry of the list.
+  const int nlist = tl ? *tn : P.NP;
+  if (tl && blockIdx.x * 64 >= nlist)
+    return;
+  auto target_of = [&](int x) {
+    const int e = blockIdx.x * 64 + w * 16 + x;
+    return e < nlist ? (tl ? tl[e] : e) : -1;
+  };
 
-  // A operand: target row (tbase + l16), coordinates 16q + 4g .. +3
+  // A operand: target row target_of(l16), coordinates 16q + 4g .. +3
   v4f a[KP / 16];
   {
-    const int t = tbase + l16;
+    const int t = target_of(l16);
 #pragma unroll
     for (int q = 0; q < KP / 16; q++) {
-      if (t < P.NP)
+      if (t >= 0)
         a[q] = *(const v4f*)(P.P32 + (size_t)t * KP + 16 * q + 4 * g);
       else
         a[q] = v4f{ 0.f, 0.f, 0.f, 0.f };
     }
   }
-  // output rows of this lane: targets tbase + 4g + r
-  float ny[4];
-#pragma unroll
-  for (int r = 0; r < 4; r++) {
-    const int t = tbase + 4 * g + r;
-    ny[r] = t < P.NP ? P.Pn[t] : 0.f;
-  }
-  // owner lanes 0..15 keep the exact lists of targets tbase + lane (LDS rows)
-  const bool owner = lane < 16 && (tbase + lane) < P.NP;
-  const int ot = tbase + lane;
+  // output rows of this lane: block rows 4g + r
+  // owner lanes 0..15 keep the exact lists of the targets of block rows lane (LDS rows)
+  const int ot = lane < 16 ? target_of(lane) : -1;
+  const bool owner = ot >= 0;
   double* ls = Ls + (w * 16 + (lane & 15));
   int* li = Li + (w * 16 + (lane & 15));
   if (lane < 16)
@@ -125,29 +146,27 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(mfma_waves(
   double s_kth = INF_D, thr = INF_D;
   int zeros = 0;
   const double yn = owner ? P.ynorm[ot] : 0.0;
+  const float nyo = owner ? P.Pn[ot] : 0.f;
   if (lane < 16) {
-    // the screening threshold starts from the seed's bound when there is one: every pair that can
-    // enter the list has sqrt(D) <= sqrt(seed), so D <= seed' = sqrt(seed)^2 (1 + 2^-49), and the
-    // FP32 value of such a pair is within B(seed') of D (the bound of the threshold updates below)
-    float ts = __builtin_huge_valf();
+    // the screening starts from the seed's bound when there is one: every pair that can enter the list
+    // has sqrt(D) <= sqrt(seed), so D <= sqrt(seed)^2 (1 + 2^-49)
+    float fl = -__builtin_huge_valf();
     if (owner && P.seed) {
       const double a = P.seed[ot];
       if (a < INF_D) {
         const double s0 = sqrt(a);
-        const double T = s0 * s0 * SQRT_GUARD;
-        const double sq = sqrt(T);
-        const double rr = fmin(P.R + yn, 2.0 * yn + sq) * (1.0 + 1e-6);
-        ts = up_f32(T + 2.0 * (P.k1 * rr * sq + P.c2 * T + P.k3 * rr * rr));
+        fl = screen_floor(P, s0 * s0 * SQRT_GUARD, yn, nyo);
       }
     }
-    thr_s[w * 16 + lane] = ts;
+    flo_s[w * 16 + lane] = fl;
     qcnt[w * 16 + lane] = 0;
   }
 
-  // the next chunk's FP32 rows and norms, loaded while the current chunk is processed
-  v4f nb[F4];
-  float nn = 0.f;
-  auto prefetch = [&](int c0) {
+  // Library chunks are staged through registers two chunks ahead (the rows of chunk c + 2 are loaded
+  // while chunk c is screened), in two register sets that alternate with the chunks.
+  v4f nbA[F4], nbB[F4];
+  float nnA = 0.f, nnB = 0.f;
+  auto prefetch = [&](v4f(&nb)[F4], float& nn, int c0) {
     const int cn = min(MCH, hi - c0);
 #pragma unroll
     for (int v = 0; v < F4; v++) {
@@ -155,33 +174,35 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(mfma_waves(
       const int j = E / (KP / 4), sub = E - j * (KP / 4);
       nb[v] = j < cn ? *(const v4f*)(P.M32 + (size_t)(c0 + j) * KP + 4 * sub) : v4f{ 0.f, 0.f, 0.f, 0.f };
     }
-    nn = (threadIdx.x < MCH && (int)threadIdx.x < cn) ? P.Mn[c0 + threadIdx.x] : 0.f;
+    nn = (threadIdx.x < MCH && (int)threadIdx.x < cn) ? -0.5f * P.Mn[c0 + threadIdx.x] : 0.f;
   };
   if (lo < hi)
-    prefetch(lo);
+    prefetch(nbA, nnA, lo);
+  if (lo + MCH < hi)
+    prefetch(nbB, nnB, lo + MCH);
 
   // The queues are drained every second chunk (and after the last): a queued entry is the position
   // in the pair of chunks that starts at cbase.
   int half = 0, cbase = lo;
-  for (int c0 = lo; c0 < hi; c0 += MCH) {
+  auto chunk = [&](int c0, v4f(&nb)[F4], float& nn) {
     const int cn = min(MCH, hi - c0);
     __syncthreads();
 #pragma unroll
     for (int v = 0; v < F4; v++)
       *(v4f*)(Bs + 4 * (threadIdx.x + 256 * v)) = nb[v];
     if (threadIdx.x < MCH)
-      Bn[threadIdx.x] = nn;
-    if (c0 + MCH < hi)
-      prefetch(c0 + MCH);
+      Bc[threadIdx.x] = nn;
+    if (c0 + 2 * MCH < hi)
+      prefetch(nb, nn, c0 + 2 * MCH);
     __syncthreads();
-    float th[4];
+    float fl[4];
 #pragma unroll
     for (int r = 0; r < 4; r++)
-      th[r] = thr_s[w * 16 + 4 * g + r];
+      fl[r] = flo_s[w * 16 + 4 * g + r];
     bool tok[4]; // this lane's output rows are real targets
 #pragma unroll
     for (int r = 0; r < 4; r++)
-      tok[r] = tbase + 4 * g + r < P.NP;
+      tok[r] = target_of(4 * g + r) >= 0;
     // TB tiles at a time: their operand loads and their independent MFMA chains are issued together,
     // then the 4 TB results of each lane are screened without branches; only a wavefront in which
     // some pair passes takes the (divergent) queue appends
@@ -195,8 +216,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(mfma_waves(
           b[u][q] = *(const v4f*)(Bs + ((jt0 + u) * 16 + l16) * KP + 16 * q + 4 * g);
       v4f acc[TB];
 #pragma unroll
-      for (int u = 0; u < TB; u++)
-        acc[u] = v4f{ 0.f, 0.f, 0.f, 0.f };
+      for (int u = 0; u < TB; u++) {
+        const float c = Bc[(jt0 + u) * 16 + l16];
+        acc[u] = v4f{ c, c, c, c };
+      }
 #pragma unroll
       for (int q = 0; q < KP / 16; q++)
 #pragma unroll
@@ -204,16 +227,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(mfma_waves(
 #pragma unroll
           for (int u = 0; u < TB; u++)
             acc[u] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[q][kk], b[u][q][kk], acc[u], 0, 0, 0);
-      float d2[TB][4];
-      bool any = false; // some value passes the threshold (validity checked on the slow path)
+      // some value reaches its floor (validity is checked on the slow path)
+      bool any = false;
 #pragma unroll
-      for (int u = 0; u < TB; u++) {
-        const float nx = Bn[(jt0 + u) * 16 + l16];
+      for (int r = 0; r < 4; r++) {
+        float m = acc[0][r];
 #pragma unroll
-        for (int r = 0; r < 4; r++) {
-          d2[u][r] = __builtin_fmaf(-2.0f, acc[u][r], ny[r] + nx);
-          any = any | (d2[u][r] <= th[r]);
-        }
+        for (int u = 1; u < TB; u++)
+          m = fmaxf(m, acc[u][r]);
+        any = any | (m >= fl[r]);
       }
       if (__ballot(any)) {
 #pragma unroll
@@ -221,7 +243,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(mfma_waves(
           const int j = (jt0 + u) * 16 + l16;
 #pragma unroll
           for (int r = 0; r < 4; r++)
-            if ((d2[u][r] <= th[r]) & tok[r] & (j < cn)) {
+            if ((acc[u][r] >= fl[r]) & tok[r] & (j < cn)) {
               const int ti = 4 * g + r;
               const int slot = atomicAdd(&qcnt[w * 16 + ti], 1);
               queue[slot * 64 + w * 16 + ti] = (unsigned char)(half * MCH + j);
@@ -232,7 +254,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(mfma_waves(
     const bool drain = half == 1 || c0 + MCH >= hi;
     half ^= 1;
     if (!drain)
-      continue;
+      return;
     half = 0;
     const int cb = cbase;
     cbase = c0 + MCH;
@@ -243,9 +265,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(mfma_waves(
     {
       const int ti = lane & 15, sl = lane >> 4;
       const int qi = w * 16 + ti;
-      if (tbase + ti < P.NP && sl < qcnt[qi]) {
+      const int tt = target_of(ti);
+      if (tt >= 0 && sl < qcnt[qi]) {
         const int j = queue[sl * 64 + qi];
-        accq[sl * 64 + qi] = dist_f64<KP>(P.M + (size_t)(cb + j) * P.Ea, P.Mp + (size_t)(tbase + ti) * P.Ea, P.Ea);
+        accq[sl * 64 + qi] = dist_f64<KP>(P.M + (size_t)(cb + j) * P.Ea, P.Mp + (size_t)tt * P.Ea, P.Ea);
       }
     }
     lds_wave_sync();
@@ -311,16 +334,20 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(mfma_waves(
       }
       qcnt[w * 16 + lane] = 0;
       if (moved_kth && thr != INF_D) {
-        // FP32 screening threshold: thr plus the bound B(thr) = 2 (c1 sqrt(thr) + c2 thr + c3) on
-        // |FP32 d2 - D| for pairs with D <= thr, where c1 = k1 rr and c3 = k3 rr^2 for any rr >=
-        // |x| + |y| (centred rows).  Such a pair has |x| <= |y| + sqrt(D), so rr = 2 |y| + sqrt(thr) bounds
-        // it as well as R + |y| does, and more tightly once thr is small.
-        const double sq = sqrt(thr);
-        const double rr = fmin(P.R + yn, 2.0 * yn + sq) * (1.0 + 1e-6);
-        thr_s[w * 16 + lane] = up_f32(thr + 2.0 * (P.k1 * rr * sq + P.c2 * thr + P.k3 * rr * rr));
+        // screening floor for pairs with D <= thr: a pair that can still enter has |x| <= |y| + sqrt(D),
+        // so rr = 2 |y| + sqrt(thr) bounds |x| + |y| as well as R + |y| does, and more tightly once thr
+        // is small (screen_floor)
+        const float f = screen_floor(P, thr, yn, nyo);
+        flo_s[w * 16 + lane] = fmaxf(flo_s[w * 16 + lane], f);
       }
     }
+    };
+  for (int c0 = lo; c0 < hi; c0 += 2 * MCH) {
+    chunk(c0, nbA, nnA);
+    if (c0 + MCH < hi)
+      chunk(c0 + MCH, nbB, nnB);
   }
+
   if (owner) {
     const size_t base = ((size_t)blockIdx.y * P.NP + ot) * KCAP;
 #pragma unroll
@@ -334,15 +361,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(mfma_waves(
 
 // Seed of knn_mfma's screening threshold: for each target, the k-th smallest nonzero exact squared distance
 // over a window of `win` (<= 256) library rows that lie near the target in a k-d order of the library
-// (rows kdp[wst[t]] .. kdp[wst[t] + win - 1]), or +inf when the window has fewer than k.  The k-th over
+// (rows wst[t] .. wst[t] + win - 1 of that order), or +inf when the window has fewer than k.  The k-th over
 // a subset is never below the k-th over the whole library, so the seed bounds the true k-th distance
 // from above.  Distances are summed in the reference's order (src/distances.cpp:186-216), as in the
-// re-rank.  One wavefront per target, four window rows per lane; targets are taken in window order
-// (tord), so consecutive wavefronts read overlapping rows.
+// re-rank.  kdm holds the library rows in the k-d order, so a window is contiguous.  One wavefront per
+// target, four window rows per lane; targets are taken in window order (tord), so consecutive
+// wavefronts read overlapping rows.
 template <int KP>
-__global__ __launch_bounds__(256) void knn_seed(MfmaParams P, const int* __restrict__ kdp, const int* __restrict__ wst,
-                                               const int* __restrict__ tord, int win, int k,
-                                               double* __restrict__ seed)
+__global__ __launch_bounds__(256) void knn_seed(MfmaParams P, const double* __restrict__ kdm,
+                                               const int* __restrict__ wst, const int* __restrict__ tord, int win,
+                                               int k, double* __restrict__ seed)
 {
   const int lane = threadIdx.x & 63;
   const int wi = blockIdx.x * 4 + (threadIdx.x >> 6);
@@ -357,7 +385,7 @@ __global__ __launch_bounds__(256) void knn_seed(MfmaParams P, const int* __restr
     const int j = lane + 64 * r;
     double acc = INF_D;
     if (j < win) {
-      const double d = dist_f64<KP>(P.M + (size_t)kdp[a + j] * P.Ea, y, P.Ea);
+      const double d = dist_f64<KP>(kdm + (size_t)(a + j) * P.Ea, y, P.Ea);
       acc = (d != 0.0 && d != MISSING_D) ? d : INF_D;
     }
     v[r] = acc;
@@ -393,10 +421,343 @@ __global__ __launch_bounds__(256) void knn_seed(MfmaParams P, const int* __restr
 }
 
 #define FEDM_SEED(KPv)                                                                                           \
-  template __global__ void knn_seed<KPv>(MfmaParams, const int*, const int*, const int*, int, int, double*);
+  template __global__ void knn_seed<KPv>(MfmaParams, const double*, const int*, const int*, int, int, double*);
 FEDM_SEED(16)
 FEDM_SEED(32)
 FEDM_SEED(64)
+
+// Two-phase form of the seeded matrix-core route.  Phase one (knn_screen) screens every (target, library
+// point) pair of a slice against the target's fixed seeded floor and records the positions that pass, up
+// to SCR_CAP per (slice, target); phase two (knn_rerank) recomputes the recorded pairs in FP64 and keeps
+// the exact (sqrt distance, position) list.  The seed bounds the target's k-th distance over the whole
+// library, so every pair of the final list, and every zero-distance pair, passes the screen: the list of
+// a slice holds its pairs up to the seed, which is all the merge of the slices needs, and the count of
+// valid pairs is the slice size less the zeros among the candidates.  A (slice, target) with more than
+// SCR_CAP candidates, or without a seed, is handed to the streaming kernel (knn_mfma on a target list).
+
+// Phase one: blocks of four wavefronts, each wavefront two 16-target tiles against the 64-point chunks of
+// the slice, double-buffered in LDS (one barrier per chunk).  A pair passes when x.y - ||x||^2 / 2 (the
+// matrix cores' accumulation, started at -||x||^2 / 2) reaches the target's floor.  ccnt receives the
+// candidate count of each (slice, target), SCR_CAP + 1 for a target without a seed.
+// Largest of four FP32 values with v_max3_f32 and v_max_f32 (the values are finite or infinite, never NaN,
+// so no canonicalisation is needed).
+__device__ __forceinline__ float max4(float a, float b, float c, float d)
+{
+  float m;
+  asm("v_max3_f32 %0, %1, %2, %3\n\tv_max_f32 %0, %0, %4" : "=&v"(m) : "v"(a), "v"(b), "v"(c), "v"(d));
+  return m;
+}
+
+template <int KP>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void knn_screen(MfmaParams P, int slice_len,
+                                                                                     int* __restrict__ cand,
+                                                                                     int* __restrict__ ccnt)
+{
+  constexpr int MCH = 64, TW = 2;
+  constexpr int F4 = MCH * KP / 4 / 256;
+  constexpr int TB = KP == 64 ? 2 : 4; // library tiles per MFMA batch
+  __shared__ float Bs[2][MCH * KP];
+  __shared__ float Bc[2][MCH];
+  __shared__ int lcnt[64 * TW];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, g = lane >> 4, l16 = lane & 15;
+  const int tblk = blockIdx.x * 64 * TW; // first target of the block
+  const int tw0 = tblk + w * 16 * TW;    // first target of the wavefront
+  const int lo = blockIdx.y * slice_len;
+  const int hi = min(P.NL, lo + slice_len);
+  const size_t sbase = (size_t)blockIdx.y * P.NP;
+
+  v4f a[TW][KP / 16];
+#pragma unroll
+  for (int i = 0; i < TW; i++) {
+    const int t = tw0 + 16 * i + l16;
+#pragma unroll
+    for (int q = 0; q < KP / 16; q++)
+      a[i][q] = t < P.NP ? *(const v4f*)(P.P32 + (size_t)t * KP + 16 * q + 4 * g) : v4f{ 0.f, 0.f, 0.f, 0.f };
+  }
+  // floors of this lane's output rows: tile i, row 4g + r
+  float fl[TW][4];
+  bool tok[TW][4];
+#pragma unroll
+  for (int i = 0; i < TW; i++)
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const int t = tw0 + 16 * i + 4 * g + r;
+      float f = -__builtin_huge_valf();
+      if (t < P.NP) {
+        const double sd = P.seed[t];
+        if (sd < INF_D) {
+          const double s0 = sqrt(sd);
+          f = screen_floor(P, s0 * s0 * SQRT_GUARD, P.ynorm[t], P.Pn[t]);
+        }
+      }
+      fl[i][r] = f;
+      tok[i][r] = f > -__builtin_huge_valf(); // targets without a seed go to the streaming kernel
+    }
+  if (threadIdx.x < 64 * TW)
+    lcnt[threadIdx.x] = 0;
+
+  v4f nb[F4];
+  float nn = 0.f;
+  auto load = [&](int c0) {
+    const int cn = min(MCH, hi - c0);
+#pragma unroll
+    for (int v = 0; v < F4; v++) {
+      const int E = threadIdx.x + 256 * v;
+      const int j = E / (KP / 4), sub = E - j * (KP / 4);
+      nb[v] = j < cn ? *(const v4f*)(P.M32 + (size_t)(c0 + j) * KP + 4 * sub) : v4f{ 0.f, 0.f, 0.f, 0.f };
+    }
+    nn = (threadIdx.x < MCH && (int)threadIdx.x < cn) ? -0.5f * P.Mn[c0 + threadIdx.x] : 0.f;
+  };
+  auto store = [&](int buf) {
+#pragma unroll
+    for (int v = 0; v < F4; v++)
+      *(v4f*)(&Bs[buf][4 * (threadIdx.x + 256 * v)]) = nb[v];
+    if (threadIdx.x < MCH)
+      Bc[buf][threadIdx.x] = nn;
+  };
+  if (lo < hi) {
+    load(lo);
+    store(0);
+  }
+  __syncthreads();
+  int buf = 0;
+  for (int c0 = lo; c0 < hi; c0 += MCH) {
+    const int cn = min(MCH, hi - c0);
+    const bool more = c0 + MCH < hi;
+    if (more)
+      load(c0 + MCH);
+    for (int jt0 = 0; jt0 < MCH / 16; jt0 += TB) {
+      v4f b[TB][KP / 16];
+#pragma unroll
+      for (int u = 0; u < TB; u++)
+#pragma unroll
+        for (int q = 0; q < KP / 16; q++)
+          b[u][q] = *(const v4f*)(&Bs[buf][((jt0 + u) * 16 + l16) * KP + 16 * q + 4 * g]);
+      v4f acc[TW][TB];
+#pragma unroll
+      for (int u = 0; u < TB; u++) {
+        const float c = Bc[buf][(jt0 + u) * 16 + l16];
+#pragma unroll
+        for (int i = 0; i < TW; i++)
+          acc[i][u] = v4f{ c, c, c, c };
+      }
+#pragma unroll
+      for (int q = 0; q < KP / 16; q++)
+#pragma unroll
+        for (int kk = 0; kk < 4; kk++)
+#pragma unroll
+          for (int i = 0; i < TW; i++)
+#pragma unroll
+            for (int u = 0; u < TB; u++)
+              acc[i][u] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[i][q][kk], b[u][q][kk], acc[i][u], 0, 0, 0);
+      bool any = false;
+#pragma unroll
+      for (int i = 0; i < TW; i++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          float m;
+          if constexpr (TB == 4)
+            m = max4(acc[i][0][r], acc[i][1][r], acc[i][2][r], acc[i][3][r]);
+          else
+            m = acc[i][0][r] > acc[i][1][r] ? acc[i][0][r] : acc[i][1][r];
+          any = any | (m >= fl[i][r]);
+        }
+      if (__ballot(any)) {
+#pragma unroll
+        for (int i = 0; i < TW; i++)
+#pragma unroll
+          for (int u = 0; u < TB; u++) {
+            const int j = (jt0 + u) * 16 + l16;
+#pragma unroll
+            for (int r = 0; r < 4; r++)
+              if ((acc[i][u][r] >= fl[i][r]) & tok[i][r] & (j < cn)) {
+                const int lt = w * 16 * TW + 16 * i + 4 * g + r;
+                const int slot = atomicAdd(&lcnt[lt], 1);
+                if (slot < SCR_CAP)
+                  cand[(sbase + tblk + lt) * SCR_CAP + slot] = c0 + j;
+              }
+          }
+      }
+    }
+    if (more)
+      store(buf ^ 1);
+    __syncthreads();
+    buf ^= 1;
+  }
+  if (threadIdx.x < 64 * TW) {
+    const int t = tblk + threadIdx.x;
+    if (t < P.NP)
+      ccnt[sbase + t] = P.seed[t] < INF_D ? min(lcnt[threadIdx.x], SCR_CAP + 1) : SCR_CAP + 1;
+  }
+}
+
+// Phase two: one wavefront per (slice, target) whose candidates all fit.  Each lane recomputes up to four
+// candidates in FP64 in the reference's order (src/distances.cpp:186-216) and sorts them; k rounds of a
+// wavefront minimum by (sqrt distance, position) give the exact list (src/edm.cpp:652-681), written in the
+// layout of knn_mfma's per-slice lists (KCAP entries, +inf / INT_MAX after the k-th).  A target with a
+// (slice, target) that does not fit is appended once (fflag) to the list for the streaming kernel (flist,
+// fcnt).
+template <int KCAP, int KP>
+__global__ __launch_bounds__(256) void knn_rerank(MfmaParams P, int k, int slice_len, const int* __restrict__ cand,
+                                                 const int* __restrict__ ccnt, double* __restrict__ out_s,
+                                                 int* __restrict__ out_i, int* __restrict__ out_cnt,
+                                                 int* __restrict__ fflag, int* __restrict__ flist,
+                                                 int* __restrict__ fcnt)
+{
+  static_assert(SCR_CAP == 256, "four candidates per lane");
+  const int lane = threadIdx.x & 63;
+  const int t = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (t >= P.NP)
+    return;
+  const size_t st = (size_t)blockIdx.y * P.NP + t;
+  const int n = ccnt[st];
+  if (n > SCR_CAP) {
+    if (lane == 0 && atomicExch(&fflag[t], 1) == 0)
+      flist[atomicAdd(fcnt, 1)] = t;
+    return;
+  }
+  const int lo = blockIdx.y * slice_len;
+  const int hi = min(P.NL, lo + slice_len);
+  const double* y = P.Mp + (size_t)t * P.Ea;
+  double s[4];
+  int ix[4];
+  int zeros = 0;
+#pragma unroll
+  for (int v = 0; v < 4; v++) {
+    const int e = lane + 64 * v;
+    s[v] = INF_D;
+    ix[v] = INT_MAX;
+    if (e < n) {
+      const int pos = cand[st * SCR_CAP + e];
+      const double acc = dist_f64<KP>(P.M + (size_t)pos * P.Ea, y, P.Ea);
+      if (acc == 0.0 || acc == MISSING_D) {
+        zeros++;
+      } else {
+        s[v] = sqrt(acc);
+        ix[v] = pos;
+      }
+    }
+  }
+  auto cs = [&](int x, int z) {
+    const bool sw = key_less(s[z], ix[z], s[x], ix[x]);
+    const double ts = s[x];
+    const int ti = ix[x];
+    s[x] = sw ? s[z] : s[x];
+    ix[x] = sw ? ix[z] : ix[x];
+    s[z] = sw ? ts : s[z];
+    ix[z] = sw ? ti : ix[z];
+  };
+  cs(0, 1);
+  cs(2, 3);
+  cs(0, 2);
+  cs(1, 3);
+  cs(1, 2);
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1)
+    zeros += __shfl_xor(zeros, off);
+  const size_t base = st * KCAP;
+  int h = 0;
+  for (int o = 0; o < KCAP; o++) {
+    double bs = INF_D;
+    int bi = INT_MAX;
+    if (o < k) {
+      const double hs = h == 0 ? s[0] : h == 1 ? s[1] : h == 2 ? s[2] : h == 3 ? s[3] : INF_D;
+      const int hi_ = h == 0 ? ix[0] : h == 1 ? ix[1] : h == 2 ? ix[2] : h == 3 ? ix[3] : INT_MAX;
+      bs = hs;
+      bi = hi_;
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) {
+        const double os = __shfl_xor(bs, off);
+        const int oi = __shfl_xor(bi, off);
+        if (key_less(os, oi, bs, bi)) {
+          bs = os;
+          bi = oi;
+        }
+      }
+      if (bi != INT_MAX && hi_ == bi)
+        h++; // positions are distinct: only the lane that holds the minimum advances
+    }
+    if (lane == 0) {
+      out_s[base + o] = bs;
+      out_i[base + o] = bi;
+    }
+  }
+  if (lane == 0)
+    out_cnt[st] = (hi > lo ? hi - lo : 0) - zeros;
+}
+
+#define FEDM_SCREEN(KPv) template __global__ void knn_screen<KPv>(MfmaParams, int, int*, int*);
+FEDM_SCREEN(16)
+FEDM_SCREEN(32)
+FEDM_SCREEN(64)
+// The streaming kernel's lists of the listed targets over SF sub-slices of the library (fs, fi, fc: SF x NP
+// lists and counts) merged into the per-slice lists of the main route: the whole list in slice 0, empty
+// lists in the others (the merge of the slices then sees the same pairs).  One thread per listed target.
+template <int KCAP>
+__global__ __launch_bounds__(256) void merge_fallback(int NP, int S, int SF, const int* __restrict__ flist,
+                                                     const int* __restrict__ fcnt, const double* __restrict__ fs,
+                                                     const int* __restrict__ fi, const int* __restrict__ fc,
+                                                     double* __restrict__ out_s, int* __restrict__ out_i,
+                                                     int* __restrict__ out_cnt)
+{
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= *fcnt)
+    return;
+  const int t = flist[e];
+  int head[MAX_SLICES];
+  int cnt = 0;
+  for (int f = 0; f < SF; f++) {
+    head[f] = 0;
+    cnt += fc[(size_t)f * NP + t];
+  }
+  for (int o = 0; o < KCAP; o++) {
+    int best = -1;
+    double bs = INF_D;
+    int bi = INT_MAX;
+    for (int f = 0; f < SF; f++) {
+      if (head[f] >= KCAP)
+        continue;
+      const size_t b = ((size_t)f * NP + t) * KCAP + head[f];
+      if (fi[b] != INT_MAX && key_less(fs[b], fi[b], bs, bi)) {
+        bs = fs[b];
+        bi = fi[b];
+        best = f;
+      }
+    }
+    if (best >= 0)
+      head[best]++;
+    out_s[(size_t)t * KCAP + o] = bs;
+    out_i[(size_t)t * KCAP + o] = bi;
+  }
+  out_cnt[t] = cnt;
+  for (int s = 1; s < S; s++) {
+    for (int o = 0; o < KCAP; o++) {
+      out_s[((size_t)s * NP + t) * KCAP + o] = INF_D;
+      out_i[((size_t)s * NP + t) * KCAP + o] = INT_MAX;
+    }
+    out_cnt[(size_t)s * NP + t] = 0;
+  }
+}
+
+#define FEDM_RERANK(KC, KPv)                                                                                     \
+  template __global__ void knn_rerank<KC, KPv>(MfmaParams, int, int, const int*, const int*, double*, int*, int*, \
+                                               int*, int*, int*);
+#define FEDM_RERANK_K(KC) FEDM_RERANK(KC, 16) FEDM_RERANK(KC, 32) FEDM_RERANK(KC, 64)
+FEDM_RERANK_K(4)
+FEDM_RERANK_K(8)
+FEDM_RERANK_K(16)
+FEDM_RERANK_K(32)
+FEDM_RERANK_K(64)
+template __global__ void merge_fallback<4>(int, int, int, const int*, const int*, const double*, const int*, const int*,
+                                           double*, int*, int*);
+template __global__ void merge_fallback<8>(int, int, int, const int*, const int*, const double*, const int*, const int*,
+                                           double*, int*, int*);
+template __global__ void merge_fallback<16>(int, int, int, const int*, const int*, const double*, const int*,
+                                            const int*, double*, int*, int*);
+template __global__ void merge_fallback<32>(int, int, int, const int*, const int*, const double*, const int*,
+                                            const int*, double*, int*, int*);
+template __global__ void merge_fallback<64>(int, int, int, const int*, const int*, const double*, const int*,
+                                            const int*, double*, int*, int*);
 
 // Lane-layout self-test of v_mfma_f32_16x16x4f32 as knn_mfma uses it: A[i][k] = i + 1 for k = 0,
 // B[k][j] = 100 (j + 1) for k = 0, zero elsewhere; out[lane * 4 + r] must be D[4 g + r][lane % 16].
@@ -413,7 +774,8 @@ __global__ void mfma_layout_probe(float* out)
     out[lane * 4 + r] = acc[r];
 }
 
-#define FEDM_MFMA(KC, KPv) template __global__ void knn_mfma<KC, KPv>(MfmaParams, int, int, double*, int*, int*);
+#define FEDM_MFMA(KC, KPv)                                                                                       \
+  template __global__ void knn_mfma<KC, KPv>(MfmaParams, int, int, double*, int*, int*, const int*, const int*);
 #define FEDM_MFMA_K(KC) FEDM_MFMA(KC, 16) FEDM_MFMA(KC, 32) FEDM_MFMA(KC, 64)
 FEDM_MFMA_K(4)
 FEDM_MFMA_K(8)

@@ -76,12 +76,6 @@ __device__ __forceinline__ double pair_acc(const LpParams& P, const double* __re
     P, [&](int j) { return xs[j]; }, [&](int j) { return y[j]; }, lib_panel, lib_tt, tgt_panel, tgt_pt);
 }
 
-// Lexicographic (s, idx) order of the reference's k-NN (src/edm.cpp:663-668).
-__device__ __forceinline__ bool key_less(double s1, int i1, double s2, int i2)
-{
-  return s1 < s2 || (s1 == s2 && i1 < i2);
-}
-
 // Sorted insertion of (s, i) into a register-resident list of KCAP entries.
 template <int KCAP>
 __device__ __forceinline__ void list_insert(double (&ls)[KCAP], int (&li)[KCAP], double s, int i)

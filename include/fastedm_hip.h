@@ -166,9 +166,12 @@ typedef struct
                                many systems per wavefront as keep the register-bound occupancy (2) or as
                                fit (3); (2) for 9..24 unknowns and launches of at least 65 536 systems,
                                else (0) (4).  All give the same bits. (4) */
-  int mfma_seed;            /* matrix-core route: each target's screening threshold starts from the exact k-th
-                               distance over 256 library rows near it in a k-d order of the library (1), or
-                               unbounded (0) (1) */
+  int mfma_seed;            /* matrix-core route: each target's screening threshold starts unbounded (0), or
+                               from the exact k-th distance over 256 library rows near it in a k-d order of
+                               the library (1); or (2) a screen against that fixed bound records candidates,
+                               one exact FP64 re-rank per target follows, and a target with more than 256
+                               candidates takes the streaming kernel; (3) (2) for points of up to 16
+                               coordinates and up to 32 neighbours, else (1).  All give the same results. (3) */
 } fedm_knobs;
 void fedm_default_knobs(fedm_knobs* k);
 

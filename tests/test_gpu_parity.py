@@ -303,10 +303,13 @@ def test_device_exp_matches_oracle_bitwise():
 @pytest.mark.parametrize("case", ["clustered_dt", "ties"])
 def test_mfma_seed_leaves_results_unchanged(case, monkeypatch):
     """The matrix-core route starts each target's FP32 screening threshold from the exact k-th distance
-    over a window of library rows near it in a k-d order (FEDM_MFMA_SEED=1, the default).  The seed only
-    removes candidates that cannot enter the list, so indices, k_used, codes and predictions must equal
-    the unseeded run's bit for bit and the oracle's.  Cases: a dt-like layout (a column equal to a
-    constant on most rows), and an integer table with many tied and zero distances."""
+    over a window of library rows near it in a k-d order (FEDM_MFMA_SEED=1), or screens every pair against
+    that fixed bound, re-ranks the recorded candidates once and hands targets with more than 256
+    candidates to the streaming kernel (FEDM_MFMA_SEED=2; 3, the default, chooses).  Each only removes candidates that
+    cannot enter the list, so indices, k_used, codes and predictions must equal the unseeded run's
+    (FEDM_MFMA_SEED=0) bit for bit and the oracle's.  Cases: a dt-like layout (a column equal to a
+    constant on most rows), and an integer table with many tied and zero distances (targets whose
+    candidates overflow)."""
     fe = load_product()
     rng = np.random.default_rng(11 if case == "ties" else 12)
     NL, NP, Ea, k = 3000, 400, 12, 9
@@ -323,16 +326,19 @@ def test_mfma_seed_leaves_results_unchanged(case, monkeypatch):
     opts = {"algorithm": 0, "k": k, "thetas": [0.0, 1.0], "precision": fe.MIXED}
     print(f"mfma seed: {case}", flush=True)
     outs = []
-    for seed in ("0", "1"):
+    for seed in ("0", "1", "2", "3"):
         monkeypatch.setenv("FEDM_MFMA_SEED", seed)
         outs.append(fe.predict(M, Mp, opts, nn_cap=k))
     monkeypatch.delenv("FEDM_MFMA_SEED")
-    (p0, r0, c0, k0, n0, s0), (p1, r1, c1, k1, n1, s1) = outs
-    assert s0 == 0 and s1 == 0
-    np.testing.assert_array_equal(n0, n1)
-    np.testing.assert_array_equal(k0, k1)
-    np.testing.assert_array_equal(r0, r1)
-    np.testing.assert_array_equal(np.ascontiguousarray(p0).view(np.uint64), np.ascontiguousarray(p1).view(np.uint64))
+    p0, r0, c0, k0, n0, s0 = outs[0]
+    assert s0 == 0
+    for p1, r1, c1, k1, n1, s1 in outs[1:]:
+        assert s1 == 0
+        np.testing.assert_array_equal(n0, n1)
+        np.testing.assert_array_equal(k0, k1)
+        np.testing.assert_array_equal(r0, r1)
+        np.testing.assert_array_equal(np.ascontiguousarray(p0).view(np.uint64),
+                                      np.ascontiguousarray(p1).view(np.uint64))
     rp, rr, rc_, rk, rn = oracle.predict(M, Mp, dict(opts, precision=0), nn_cap=k)
     np.testing.assert_array_equal(n1, rn)
     np.testing.assert_array_equal(k1, rk)
