@@ -294,9 +294,11 @@ def cfg4_leg(fe, oracle, cores, steps):
     Ea, k, NL = 16, 18, n_pred // 3
     st = [out["mixed"]["select_ms"], out["mixed"]["merge_ms"], out["mixed"]["smap_ms"]]
     out["roofline"] = stage_roofline(st, {
-        "select": ("knn_mfma<32,16> (FP32 matrix cores, FP64 re-rank)", 3.0 * Ea * float(NL) * NL,
+        "select": ("knn_seed + knn_screen<16> (FP32 matrix cores, seeded floor) + knn_rerank<32,16> (FP64) + "
+                   "knn_mfma<32,16> on the targets that overflow", 3.0 * Ea * float(NL) * NL,
                    "fp32 MFMA", FP32_PEAK_TFLOPS, "mfma"),
-        "smap": ("smap_rows<16,1> (Gram + Eigen-order Jacobi SVD)", smap_flops(k, Ea) * n_pred, "fp64 VALU",
+        "smap": ("smap_pack<16,1> (Gram + Eigen-order Jacobi SVD, two rows per lane)", smap_flops(k, Ea) * n_pred,
+                 "fp64 VALU",
                  FP64_PEAK_TFLOPS, "valu")})
     out["roofline"]["unit_of_work"] = SMAP_UNIT
     if oracle is not None:
@@ -318,7 +320,7 @@ def cfg4_leg(fe, oracle, cores, steps):
 
         M, Mt = manifold(rows[:nu // 2], False)
         Mp, _ = manifold(rows[nu // 2:], True)
-        m = 60
+        m = 20000
         pi = panel.astype(np.int32)
         Md = {"data": M, "targets": Mt, "panel": pi[rows[:nu // 2]]}
         Mpd = {"data": Mp[:m], "panel": pi[rows[nu // 2:]][:m]}
@@ -362,17 +364,19 @@ def cfg5_leg(fe, oracle, cores, steps):
     Ea, k = 40, 42
     st = [out["mixed"]["select_ms"], out["mixed"]["merge_ms"], out["mixed"]["smap_ms"]]
     out["roofline"] = stage_roofline(st, {
-        "select": ("knn_mfma<64,64> (FP32 matrix cores, FP64 re-rank)", 3.0 * Ea * float(nl) * npd, "fp32 MFMA",
+        "select": ("knn_seed + knn_mfma<64,64> (FP32 matrix cores from a seeded floor, FP64 re-rank)",
+                   3.0 * Ea * float(nl) * npd, "fp32 MFMA",
                    FP32_PEAK_TFLOPS, "mfma"),
         "smap": ("smap_hyb (Gram + Eigen-order Jacobi SVD, n = 41)", smap_flops(k, Ea) * npd, "fp64 VALU",
                  FP64_PEAK_TFLOPS, "valu")})
     out["roofline"]["unit_of_work"] = SMAP_UNIT
     if oracle is not None:
-        m = 300
+        m = npd
         dt, times, r = median_time(lambda: oracle.predict(lib, {"data": pred["data"][:m]}, dict(opts, nthreads=cores)))
         out["cpu_baseline"] = {"value": m / dt, "unit": "predictions/s", "cores": cores, "kind": "port",
                                "runs_s": [round(v, 3) for v in times],
-                               "sample": f"the first {m} targets against the full library, median of 3"}
+                               "sample": f"every target ({m}) against the full library (the whole task, manifold "
+                                         f"build not timed), median of 3"}
         got = raw["mixed"][0][0, :m]
         ref = r[0][0]
         out["max_abs_d_yhat_mixed_vs_oracle_sample"] = float(np.max(np.abs(got - ref)))
@@ -534,6 +538,13 @@ def main():
         out["plain"] = {"value": n_pred / sec_pt, "ms_per_step": sec_pt * 1e3,
                         "route": "one edm_predict per task (host gather + upload per task, as the drop-in seam)"}
         out["ratio_mixed_vs_plain"] = out["value"] / out["plain"]["value"]
+        out["ratio_mixed_vs_plain_batched"] = out["value"] / out["plain_batched"]["value"]
+        out["ratio_note"] = ("plain is the drop-in (one edm_predict per task, host gather and upload per task); "
+                             "plain_batched runs every library through the FP64 direct kernel in the persistent "
+                             "launch.  Most of the mixed route's lead comes from its algorithm: one exact ranked "
+                             "prefix per target serves all 2 000 long-library tasks, and an FP64 ranking would "
+                             "give the plain form the same reuse; the FP32 screen saves only the ranking's "
+                             "distance work")
         out["parity"]["max_abs_d_rho_plain_taskwise_vs_plain_batched"] = float(
             np.nanmax(np.abs(res_pt["stats"]["rho"] - res_pb["stats"]["rho"])))
         pt.close()

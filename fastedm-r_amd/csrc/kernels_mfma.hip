@@ -436,7 +436,9 @@ FEDM_SEED(64)
 // SCR_CAP candidates, or without a seed, is handed to the streaming kernel (knn_mfma on a target list).
 
 // Phase one: blocks of four wavefronts, each wavefront two 16-target tiles against the 64-point chunks of
-// the slice, double-buffered in LDS (one barrier per chunk).  A pair passes when x.y - ||x||^2 / 2 (the
+// the slice, double-buffered in LDS (one barrier per chunk), five wavefronts per SIMD (on the panel
+// workload 34.6 ms of select against 35.2 at four; eight per SIMD, loads two chunks ahead, 128-point
+// chunks and accumulators started from LDS copies were all slower).  A pair passes when x.y - ||x||^2 / 2 (the
 // matrix cores' accumulation, started at -||x||^2 / 2) reaches the target's floor.  ccnt receives the
 // candidate count of each (slice, target), SCR_CAP + 1 for a target without a seed.
 // Largest of four FP32 values with v_max3_f32 and v_max_f32 (the values are finite or infinite, never NaN,
@@ -449,7 +451,7 @@ __device__ __forceinline__ float max4(float a, float b, float c, float d)
 }
 
 template <int KP>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void knn_screen(MfmaParams P, int slice_len,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void knn_screen(MfmaParams P, int slice_len,
                                                                                      int* __restrict__ cand,
                                                                                      int* __restrict__ ccnt)
 {
