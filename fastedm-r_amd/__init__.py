@@ -34,7 +34,8 @@ class Knobs(C.Structure):
     _fields_ = [("rank_scale", C.c_double), ("direct_blocks_per_cu", C.c_int), ("direct_alt_waves", C.c_int),
                 ("list_blocks", C.c_int), ("mfma_blocks", C.c_int), ("use_mfma", C.c_int), ("smap_lds", C.c_int),
                 ("direct_prune", C.c_int), ("concurrent_routes", C.c_int), ("smap_block", C.c_int),
-                ("mfma_seed", C.c_int)]
+                ("mfma_seed", C.c_int), ("mfma_prune", C.c_int),
+                ("mfma_split", C.c_int)]
 
     def as_dict(self):
         return {f[0]: getattr(self, f[0]) for f in self._fields_}

@@ -46,7 +46,7 @@ template <int KP>
 __global__ void knn_seed(MfmaParams P, const double* kdm, const int* wst, const int* tord, int win, int k,
                          double* seed);
 template <int KP>
-__global__ void knn_screen(MfmaParams P, int slice_len, int* cand, int* ccnt);
+__global__ void knn_screen(MfmaParams P, KdScreen K, int split, int* cand, int* ccnt);
 template <int KCAP, int KP>
 __global__ void knn_rerank(MfmaParams P, int k, int slice_len, const int* cand, const int* ccnt, double* out_s,
                            int* out_i, int* out_cnt, int* fflag, int* flist, int* fcnt);
@@ -156,6 +156,11 @@ enum
   B_FPS,
   B_FPI,
   B_FPC,
+  B_KD32,
+  B_KDN,
+  B_KDPOS,
+  B_CBOX,
+  B_TBOX,
   B_NUM
 };
 } // namespace fedm
@@ -263,10 +268,11 @@ static void launch_seed(int kp, hipStream_t s, const MfmaParams& P, const double
 // Phases one and two of the two-phase matrix-core route: the seeded screen of every slice, then the exact
 // re-rank of each (slice, target) whose candidates fit; the others land in (flist, fcnt).
 template <int KP>
-static void launch_two_phase_p(int kcap, hipStream_t s, const MfmaParams& P, int k, int S, int sl, int* cand,
-                               int* ccnt, double* ps, int* pi, int* pc, int* ff, int* fl, int* fc)
+static void launch_two_phase_p(int kcap, hipStream_t s, const MfmaParams& P, const KdScreen& K, int split, int k,
+                               int S, int sl, int* cand, int* ccnt, double* ps, int* pi, int* pc, int* ff, int* fl,
+                               int* fc)
 {
-  hipLaunchKernelGGL((knn_screen<KP>), dim3((P.NP + 127) / 128, S), dim3(256), 0, s, P, sl, cand, ccnt);
+  hipLaunchKernelGGL((knn_screen<KP>), dim3((P.NP + 127) / 128, split), dim3(256), 0, s, P, K, split, cand, ccnt);
   const dim3 g((P.NP + 3) / 4, S);
   switch (kcap) {
     case 4: hipLaunchKernelGGL((knn_rerank<4, KP>), g, dim3(256), 0, s, P, k, sl, cand, ccnt, ps, pi, pc, ff, fl, fc); break;
@@ -277,15 +283,16 @@ static void launch_two_phase_p(int kcap, hipStream_t s, const MfmaParams& P, int
   }
 }
 
-static void launch_two_phase(int kcap, int kp, hipStream_t s, const MfmaParams& P, int k, int S, int sl, int* cand,
-                             int* ccnt, double* ps, int* pi, int* pc, int* ff, int* fl, int* fc)
+static void launch_two_phase(int kcap, int kp, hipStream_t s, const MfmaParams& P, const KdScreen& K, int split,
+                             int k, int S, int sl, int* cand, int* ccnt, double* ps, int* pi, int* pc, int* ff,
+                             int* fl, int* fc)
 {
   if (kp == 16)
-    launch_two_phase_p<16>(kcap, s, P, k, S, sl, cand, ccnt, ps, pi, pc, ff, fl, fc);
+    launch_two_phase_p<16>(kcap, s, P, K, split, k, S, sl, cand, ccnt, ps, pi, pc, ff, fl, fc);
   else if (kp == 32)
-    launch_two_phase_p<32>(kcap, s, P, k, S, sl, cand, ccnt, ps, pi, pc, ff, fl, fc);
+    launch_two_phase_p<32>(kcap, s, P, K, split, k, S, sl, cand, ccnt, ps, pi, pc, ff, fl, fc);
   else
-    launch_two_phase_p<64>(kcap, s, P, k, S, sl, cand, ccnt, ps, pi, pc, ff, fl, fc);
+    launch_two_phase_p<64>(kcap, s, P, K, split, k, S, sl, cand, ccnt, ps, pi, pc, ff, fl, fc);
 }
 
 static void launch_merge_fallback(int kcap, hipStream_t s, int NP, int S, int SF, const int* fl, const int* fc,
@@ -381,6 +388,11 @@ extern "C" void fedm_default_knobs(fedm_knobs* k)
   // two-phase form, 3 chooses between 1 and 2 (same results)
   const char* sd = getenv("FEDM_MFMA_SEED");
   k->mfma_seed = sd && sd[0] >= '0' && sd[0] <= '3' ? sd[0] - '0' : 3;
+  // FEDM_MFMA_PRUNE=0 screens every chunk of the library in the two-phase form (same results)
+  const char* pr = getenv("FEDM_MFMA_PRUNE");
+  k->mfma_prune = !(pr && pr[0] == '0');
+  const char* sp = getenv("FEDM_MFMA_SPLIT");
+  k->mfma_split = sp ? atoi(sp) : 0;
 }
 
 static fedm_knobs default_knobs()
@@ -588,14 +600,18 @@ struct TaskPlan
   std::vector<double> ynorm;
   std::vector<int> kdp, wst, tord; // matrix-core seed: library rows in k-d order, window starts, target order
   std::vector<double> kdm;         // the library rows in that order (FP64)
+  std::vector<float> kd32, kdn;    // two-phase screen: the centred FP32 rows and their norms in that order
+  std::vector<float> cbox, tbox;   // two-phase screen: boxes of the library chunks and the target blocks
   int seed_win = 0;                // rows per seed window (0: no seed)
   bool two_phase = false;          // seeded screen, exact re-rank, streaming kernel for what overflows
   int sf = 1;                      // two-phase: library sub-slices of the streaming kernel on the overflow
+  int split = 1;                   // two-phase: blocks sharing each target block's chunks
   LpParams P;
   WassParams W;
   FinishParams F;
   SmapParams Sm;
   MfmaParams MP;
+  KdScreen KS{};
 };
 
 constexpr int SEED_WIN = 256; // library rows per seed window of the matrix-core route (knn_seed: 4 per lane)
@@ -687,6 +703,47 @@ static void kd_seed_windows(const double* L, int NL, const double* P, int NP, in
   tord.resize(NP);
   std::iota(tord.begin(), tord.end(), 0);
   std::stable_sort(tord.begin(), tord.end(), [&](int x, int y) { return wst[x] < wst[y]; });
+}
+
+// The two-phase screen's operands in the k-d orders (kdp for the library, tord for the targets): the
+// centred FP32 rows and norms of the library in its order, and the boxes of its 64-row chunks and of the
+// 128-target blocks of the target order, from the FP64 coordinates rounded outward to FP32.
+static void build_screen_order(TaskPlan& tp, const double* L, const double* P, bool prune)
+{
+  const int NL = tp.NL, NP = tp.NP, Ea = tp.Ea, kp = tp.kp;
+  tp.kd32.resize((size_t)NL * kp);
+  tp.kdn.resize(NL);
+  for (int i = 0; i < NL; i++) {
+    std::copy(tp.m32.begin() + (size_t)tp.kdp[i] * kp, tp.m32.begin() + (size_t)(tp.kdp[i] + 1) * kp,
+              tp.kd32.begin() + (size_t)i * kp);
+    tp.kdn[i] = tp.mn[tp.kdp[i]];
+  }
+  auto boxes = [Ea](const double* X, const int* order, int n, int per, std::vector<float>& out) {
+    const int nb = (n + per - 1) / per;
+    out.assign((size_t)nb * 2 * Ea, 0.f);
+    for (int b = 0; b < nb; b++) {
+      float* lo = out.data() + (size_t)b * 2 * Ea;
+      float* hi = lo + Ea;
+      for (int d = 0; d < Ea; d++) {
+        double l = INFINITY, h = -INFINITY;
+        for (int i = b * per; i < std::min(n, (b + 1) * per); i++) {
+          const double v = X[(size_t)order[i] * Ea + d];
+          l = std::min(l, v);
+          h = std::max(h, v);
+        }
+        float fl = (float)l, fh = (float)h;
+        if ((double)fl > l)
+          fl = std::nextafter(fl, -INFINITY);
+        if ((double)fh < h)
+          fh = std::nextafter(fh, INFINITY);
+        lo[d] = fl;
+        hi[d] = fh;
+      }
+    }
+  };
+  boxes(L, tp.kdp.data(), NL, 64, tp.cbox);
+  boxes(P, tp.tord.data(), NP, 128, tp.tbox);
+  tp.KS.prune = prune ? 1 : 0;
 }
 
 // Chooses the route and sizes every buffer.  Returns 0, -22 (invalid arguments) or -95 (a width or
@@ -888,8 +945,20 @@ static int plan_task(const edm_manifold_view* M, const edm_manifold_view* Mp, co
       const bool two = knobs.mfma_seed == 2 || (knobs.mfma_seed == 3 && tp.kp == 16 && tp.kcap <= 32);
       if (two) {
         tp.two_phase = true;
-        by[B_CAND] = sizeof(int) * (size_t)tp.S * NP * SCR_CAP;
-        by[B_CCNT] = sizeof(int) * (size_t)tp.S * NP;
+        // one slice: the screen shares the library's chunks out over `split` blocks per target block instead
+        tp.S = 1;
+        tp.sl = NL;
+        const int nb = (NP + 127) / 128;
+        tp.split = knobs.mfma_split > 0 ? std::min(knobs.mfma_split, 64)
+                                         : std::max(1, std::min(16, (16384 + nb - 1) / nb));
+        build_screen_order(tp, M->data, Mp->data, knobs.mfma_prune != 0);
+        by[B_KD32] = sizeof(float) * tp.kd32.size();
+        by[B_KDN] = sizeof(float) * tp.kdn.size();
+        by[B_KDPOS] = sizeof(int) * tp.kdp.size();
+        by[B_CBOX] = sizeof(float) * tp.cbox.size();
+        by[B_TBOX] = sizeof(float) * tp.tbox.size();
+        by[B_CAND] = sizeof(int) * (size_t)NP * SCR_CAP;
+        by[B_CCNT] = sizeof(int) * (size_t)NP;
         by[B_FLIST] = sizeof(int) * (size_t)NP;
         by[B_FCNT] = sizeof(int);
         by[B_FFLAG] = sizeof(int) * (size_t)NP;
@@ -1063,6 +1132,13 @@ static int bind_task(TaskPlan& tp, const edm_options* o, Get&& get)
     MP.M = P.M;
     MP.Mp = P.Mp;
     MP.seed = tp.seed_win > 0 ? (const double*)tp.ptr[B_SEED] : nullptr;
+    KdScreen& KS = tp.KS;
+    KS.m32 = (const float*)tp.ptr[B_KD32];
+    KS.mn = (const float*)tp.ptr[B_KDN];
+    KS.pos = (const int*)tp.ptr[B_KDPOS];
+    KS.cbox = (const float*)tp.ptr[B_CBOX];
+    KS.tbox = (const float*)tp.ptr[B_TBOX];
+    KS.tord = (const int*)tp.ptr[B_TORD];
   }
   return 0;
 }
@@ -1107,6 +1183,16 @@ static int upload_task(const TaskPlan& tp, const edm_manifold_view* M, const edm
       e = cp(B_WST, tp.wst.data());
     if (e == hipSuccess)
       e = cp(B_TORD, tp.tord.data());
+    if (e == hipSuccess)
+      e = cp(B_KD32, tp.kd32.data());
+    if (e == hipSuccess)
+      e = cp(B_KDN, tp.kdn.data());
+    if (e == hipSuccess)
+      e = cp(B_KDPOS, tp.kdp.data());
+    if (e == hipSuccess)
+      e = cp(B_CBOX, tp.cbox.data());
+    if (e == hipSuccess)
+      e = cp(B_TBOX, tp.tbox.data());
   }
   return e == hipSuccess ? 0 : -3;
 }
@@ -1148,7 +1234,9 @@ static int launch_task(TaskPlan& tp, hipStream_t s, hipEvent_t* ev, int (*keep_g
         int* ff = (int*)tp.ptr[B_FFLAG];
         fill(fc, (size_t)1, 0);
         fill(ff, (size_t)NP, 0);
-        launch_two_phase(tp.kcap, tp.kp, s, tp.MP, tp.k, tp.S, tp.sl, (int*)tp.ptr[B_CAND], (int*)tp.ptr[B_CCNT],
+        fill((int*)tp.ptr[B_CCNT], (size_t)NP, 0);
+        launch_two_phase(tp.kcap, tp.kp, s, tp.MP, tp.KS, tp.split, tp.k, tp.S, tp.sl, (int*)tp.ptr[B_CAND],
+                         (int*)tp.ptr[B_CCNT],
                          ps, pi, pc, ff, fl, fc);
         // the targets that overflowed: the streaming kernel over sf sub-slices of the library, then merged
         // into the route's per-slice lists

@@ -953,12 +953,14 @@ static std::string tune_cache_path()
 
 static std::string knobs_text(const fedm_knobs& k)
 {
-  char buf[256];
+  char buf[384];
   snprintf(buf, sizeof buf,
            "rank_scale=%.3f direct_blocks_per_cu=%d direct_alt_waves=%d list_blocks=%d mfma_blocks=%d use_mfma=%d "
-           "smap_lds=%d direct_prune=%d concurrent_routes=%d smap_block=%d mfma_seed=%d",
+           "smap_lds=%d direct_prune=%d concurrent_routes=%d smap_block=%d mfma_seed=%d mfma_prune=%d "
+           "mfma_split=%d",
            k.rank_scale, k.direct_blocks_per_cu, k.direct_alt_waves, k.list_blocks, k.mfma_blocks, k.use_mfma,
-           k.smap_lds, k.direct_prune, k.concurrent_routes, k.smap_block, k.mfma_seed);
+           k.smap_lds, k.direct_prune, k.concurrent_routes, k.smap_block, k.mfma_seed, k.mfma_prune,
+           k.mfma_split);
   return buf;
 }
 
@@ -999,6 +1001,10 @@ static bool parse_knobs(const std::string& line, fedm_knobs* k)
       k->smap_block = (int)v;
     else if (name == "mfma_seed")
       k->mfma_seed = (int)v;
+    else if (name == "mfma_prune")
+      k->mfma_prune = (int)v;
+    else if (name == "mfma_split")
+      k->mfma_split = (int)v;
     else
       got--;
   }
@@ -1134,6 +1140,8 @@ static int tune_group(fedm_plan* P, fedm_plan::Group& g, int trials, std::string
     if (mixed && Ea >= 8) {
       axes.push_back({ "use_mfma", { 1, 0 } });
       axes.push_back({ "mfma_seed", { 3, 2, 1, 0 } });
+      axes.push_back({ "mfma_prune", { 1, 0 } });
+      axes.push_back({ "mfma_split", { 0, 1, 4, 16, 32 } });
       axes.push_back({ "mfma_blocks", { 1024, 2048, 4096 } });
     }
     if (g.go.algorithm == FEDM_SMAP) {
@@ -1165,6 +1173,10 @@ static int tune_group(fedm_plan* P, fedm_plan::Group& g, int trials, std::string
       k.smap_block = (int)v;
     else if (n == "mfma_seed")
       k.mfma_seed = (int)v;
+    else if (n == "mfma_prune")
+      k.mfma_prune = (int)v;
+    else if (n == "mfma_split")
+      k.mfma_split = (int)v;
     return k;
   };
   for (const auto& ax : axes) {

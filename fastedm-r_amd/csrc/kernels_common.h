@@ -164,6 +164,20 @@ struct MfmaParams
   int NL, NP, Ea;
 };
 
+// The library and targets in the k-d order of the seeded matrix-core route, as the two-phase screen
+// (knn_screen) walks them: library rows in 64-row chunks, targets in 128-target blocks, each with a box.
+struct KdScreen
+{
+  const float* m32;  // [NL x KP] the centred FP32 library rows in k-d order
+  const float* mn;   // [NL] their FP32 squared norms
+  const int* pos;    // [NL] library position of each row of the order
+  const float* cbox; // [ceil(NL / 64) x 2 Ea] per chunk of the order: lower then upper bounds of the FP64
+                     // coordinates, rounded outward to FP32
+  const float* tbox; // [ceil(NP / 128) x 2 Ea] the same per block of the target order
+  const int* tord;   // [NP] targets in k-d window order
+  int prune;         // 1: a chunk whose box lies beyond every threshold of a block is not screened
+};
+
 } // namespace fedm
 
 #include "jacobi_wave.h"

@@ -426,21 +426,28 @@ FEDM_SEED(16)
 FEDM_SEED(32)
 FEDM_SEED(64)
 
-// Two-phase form of the seeded matrix-core route.  Phase one (knn_screen) screens every (target, library
-// point) pair of a slice against the target's fixed seeded floor and records the positions that pass, up
-// to SCR_CAP per (slice, target); phase two (knn_rerank) recomputes the recorded pairs in FP64 and keeps
-// the exact (sqrt distance, position) list.  The seed bounds the target's k-th distance over the whole
-// library, so every pair of the final list, and every zero-distance pair, passes the screen: the list of
-// a slice holds its pairs up to the seed, which is all the merge of the slices needs, and the count of
-// valid pairs is the slice size less the zeros among the candidates.  A (slice, target) with more than
-// SCR_CAP candidates, or without a seed, is handed to the streaming kernel (knn_mfma on a target list).
+// Two-phase form of the seeded matrix-core route.  Phase one (knn_screen) screens the (target, library
+// point) pairs against the target's fixed seeded floor and records the positions that pass, up to SCR_CAP
+// per target; phase two (knn_rerank) recomputes the recorded pairs in FP64 and keeps the exact (sqrt
+// distance, position) list.  The seed bounds the target's k-th distance over the whole library, so every
+// pair of the final list, and every zero-distance pair, passes the screen, and the count of valid pairs is
+// the library size less the zeros among the candidates.  A target with more than SCR_CAP candidates, or
+// without a seed, is handed to the streaming kernel (knn_mfma on a target list).
 
-// Phase one: blocks of four wavefronts, each wavefront two 16-target tiles against the 64-point chunks of
-// the slice, double-buffered in LDS (one barrier per chunk), five wavefronts per SIMD (on the panel
-// workload 34.6 ms of select against 35.2 at four; eight per SIMD, loads two chunks ahead, 128-point
-// chunks and accumulators started from LDS copies were all slower).  A pair passes when x.y - ||x||^2 / 2 (the
-// matrix cores' accumulation, started at -||x||^2 / 2) reaches the target's floor.  ccnt receives the
-// candidate count of each (slice, target), SCR_CAP + 1 for a target without a seed.
+// Phase one.  A block of four wavefronts takes 128 consecutive targets of the k-d target order and one of
+// `split` interleaved parts of the library's 64-row chunks in its k-d order (chunk c belongs to part c mod
+// split, so a block whose targets need most of the library shares it out over `split` blocks).  It first
+// lists the chunks of its part (1 024 at a time) whose box may hold a pair within the largest threshold of
+// its targets: the squared gap between the chunk's box and the block's box, evaluated in FP64 on the
+// outward-rounded FP32 bounds and lowered by 2^-40 of itself (more than the rounding of that sum and of the
+// re-rank's FP64 distance), bounds every pair of the chunk from below, so a chunk left out holds no pair
+// that can pass the exact test of any target of the block (no list pair, no zero distance).  Each wavefront
+// then screens two 16-target tiles against the listed chunks, double-buffered in LDS (one barrier per
+// chunk), five wavefronts per SIMD (four at 64 coordinates, where the LDS of five blocks would not fit).  A
+// pair passes when x.y - ||x||^2 / 2 (the matrix cores' accumulation, started at -||x||^2 / 2) reaches the
+// target's floor.  The passing pairs of one target row and one 64-row step take their slots with one
+// atomic add on the target's count ccnt (zeroed before the launch), which ends as the number of pairs that
+// passed.
 // Largest of four FP32 values with v_max3_f32 and v_max_f32 (the values are finite or infinite, never NaN,
 // so no canonicalisation is needed).
 __device__ __forceinline__ float max4(float a, float b, float c, float d)
@@ -451,41 +458,46 @@ __device__ __forceinline__ float max4(float a, float b, float c, float d)
 }
 
 template <int KP>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void knn_screen(MfmaParams P, int slice_len,
-                                                                                     int* __restrict__ cand,
-                                                                                     int* __restrict__ ccnt)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(KP == 64 ? 4 : 5))) void knn_screen(MfmaParams P,
+                                                                                                KdScreen K,
+                                                                                                int split,
+                                                                                                int* __restrict__ cand,
+                                                                                                int* __restrict__ ccnt)
 {
-  constexpr int MCH = 64, TW = 2;
+  constexpr int MCH = 64, TW = 2, CLW = 1024;
   constexpr int F4 = MCH * KP / 4 / 256;
   constexpr int TB = KP == 64 ? 2 : 4; // library tiles per MFMA batch
   __shared__ float Bs[2][MCH * KP];
   __shared__ float Bc[2][MCH];
-  __shared__ int lcnt[64 * TW];
+  __shared__ int tid[64 * TW];
+  __shared__ int clist[CLW];
+  __shared__ float tb[2 * KP];
+  __shared__ int ncl;
+  __shared__ unsigned long long tmax;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, g = lane >> 4, l16 = lane & 15;
-  const int tblk = blockIdx.x * 64 * TW; // first target of the block
-  const int tw0 = tblk + w * 16 * TW;    // first target of the wavefront
-  const int lo = blockIdx.y * slice_len;
-  const int hi = min(P.NL, lo + slice_len);
-  const size_t sbase = (size_t)blockIdx.y * P.NP;
+  const int tblk = blockIdx.x * 64 * TW; // first slot of the block in the target order
+  const int tw0 = tblk + w * 16 * TW;    // first slot of the wavefront
+  const int Ea = P.Ea;
+  auto target = [&](int x) { return x < P.NP ? K.tord[x] : -1; };
 
   v4f a[TW][KP / 16];
 #pragma unroll
   for (int i = 0; i < TW; i++) {
-    const int t = tw0 + 16 * i + l16;
+    const int t = target(tw0 + 16 * i + l16);
 #pragma unroll
     for (int q = 0; q < KP / 16; q++)
-      a[i][q] = t < P.NP ? *(const v4f*)(P.P32 + (size_t)t * KP + 16 * q + 4 * g) : v4f{ 0.f, 0.f, 0.f, 0.f };
+      a[i][q] = t >= 0 ? *(const v4f*)(P.P32 + (size_t)t * KP + 16 * q + 4 * g) : v4f{ 0.f, 0.f, 0.f, 0.f };
   }
-  // floors of this lane's output rows: tile i, row 4g + r
+  // floors of this lane's output rows: tile i, row 4g + r (-inf and no record for a target without a seed)
   float fl[TW][4];
   bool tok[TW][4];
 #pragma unroll
   for (int i = 0; i < TW; i++)
 #pragma unroll
     for (int r = 0; r < 4; r++) {
-      const int t = tw0 + 16 * i + 4 * g + r;
+      const int t = target(tw0 + 16 * i + 4 * g + r);
       float f = -__builtin_huge_valf();
-      if (t < P.NP) {
+      if (t >= 0) {
         const double sd = P.seed[t];
         if (sd < INF_D) {
           const double s0 = sqrt(sd);
@@ -493,22 +505,34 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
         }
       }
       fl[i][r] = f;
-      tok[i][r] = f > -__builtin_huge_valf(); // targets without a seed go to the streaming kernel
+      tok[i][r] = f > -__builtin_huge_valf();
     }
-  if (threadIdx.x < 64 * TW)
-    lcnt[threadIdx.x] = 0;
+  if (threadIdx.x == 0)
+    tmax = 0ull;
+  if (threadIdx.x < 2 * Ea)
+    tb[threadIdx.x] = K.tbox[(size_t)blockIdx.x * 2 * Ea + threadIdx.x];
+  __syncthreads();
+  if (threadIdx.x < 64 * TW) {
+    const int t = target(tblk + threadIdx.x);
+    tid[threadIdx.x] = t;
+    if (t >= 0 && P.seed[t] < INF_D) {
+      const double s0 = sqrt(P.seed[t]);
+      // the largest threshold of the block (non-negative doubles order as their bit patterns)
+      atomicMax(&tmax, (unsigned long long)__double_as_longlong(s0 * s0 * SQRT_GUARD));
+    }
+  }
 
   v4f nb[F4];
   float nn = 0.f;
   auto load = [&](int c0) {
-    const int cn = min(MCH, hi - c0);
+    const int cn = min(MCH, P.NL - c0);
 #pragma unroll
     for (int v = 0; v < F4; v++) {
       const int E = threadIdx.x + 256 * v;
       const int j = E / (KP / 4), sub = E - j * (KP / 4);
-      nb[v] = j < cn ? *(const v4f*)(P.M32 + (size_t)(c0 + j) * KP + 4 * sub) : v4f{ 0.f, 0.f, 0.f, 0.f };
+      nb[v] = j < cn ? *(const v4f*)(K.m32 + (size_t)(c0 + j) * KP + 4 * sub) : v4f{ 0.f, 0.f, 0.f, 0.f };
     }
-    nn = (threadIdx.x < MCH && (int)threadIdx.x < cn) ? -0.5f * P.Mn[c0 + threadIdx.x] : 0.f;
+    nn = (threadIdx.x < MCH && (int)threadIdx.x < cn) ? -0.5f * K.mn[c0 + threadIdx.x] : 0.f;
   };
   auto store = [&](int buf) {
 #pragma unroll
@@ -517,79 +541,123 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
     if (threadIdx.x < MCH)
       Bc[buf][threadIdx.x] = nn;
   };
-  if (lo < hi) {
-    load(lo);
+  const int nch = (P.NL + MCH - 1) / MCH;
+  for (int cw = blockIdx.y; cw < nch; cw += split * CLW) {
+    // the chunks cw, cw + split, ... (up to CLW of them) that the block screens
+    __syncthreads();
+    if (threadIdx.x == 0)
+      ncl = 0;
+    __syncthreads();
+    const double tm = __longlong_as_double((long long)tmax);
+    for (int m = threadIdx.x; m < CLW; m += 256) {
+      const int c = cw + m * split;
+      if (c >= nch)
+        break;
+      bool keep = true;
+      if (K.prune) {
+        const float* bx = K.cbox + (size_t)c * 2 * Ea;
+        double lb = 0.0;
+        for (int d = 0; d < Ea; d++) {
+          const double gp = fmax(0.0, fmax((double)bx[d] - (double)tb[Ea + d], (double)tb[d] - (double)bx[Ea + d]));
+          lb += gp * gp;
+        }
+        keep = lb - lb * 0x1p-40 <= tm;
+      }
+      if (keep)
+        clist[atomicAdd(&ncl, 1)] = c;
+    }
+    __syncthreads();
+    const int n = ncl;
+    if (n == 0)
+      continue;
+    load(clist[0] * MCH);
     store(0);
-  }
-  __syncthreads();
-  int buf = 0;
-  for (int c0 = lo; c0 < hi; c0 += MCH) {
-    const int cn = min(MCH, hi - c0);
-    const bool more = c0 + MCH < hi;
-    if (more)
-      load(c0 + MCH);
-    for (int jt0 = 0; jt0 < MCH / 16; jt0 += TB) {
-      v4f b[TB][KP / 16];
+    __syncthreads();
+    int buf = 0;
+    for (int li = 0; li < n; li++) {
+      const int c0 = clist[li] * MCH;
+      const int cn = min(MCH, P.NL - c0);
+      const bool more = li + 1 < n;
+      if (more)
+        load(clist[li + 1] * MCH);
+      for (int jt0 = 0; jt0 < MCH / 16; jt0 += TB) {
+        v4f b[TB][KP / 16];
 #pragma unroll
-      for (int u = 0; u < TB; u++)
+        for (int u = 0; u < TB; u++)
+#pragma unroll
+          for (int q = 0; q < KP / 16; q++)
+            b[u][q] = *(const v4f*)(&Bs[buf][((jt0 + u) * 16 + l16) * KP + 16 * q + 4 * g]);
+        v4f acc[TW][TB];
+#pragma unroll
+        for (int u = 0; u < TB; u++) {
+          const float c = Bc[buf][(jt0 + u) * 16 + l16];
+#pragma unroll
+          for (int i = 0; i < TW; i++)
+            acc[i][u] = v4f{ c, c, c, c };
+        }
 #pragma unroll
         for (int q = 0; q < KP / 16; q++)
-          b[u][q] = *(const v4f*)(&Bs[buf][((jt0 + u) * 16 + l16) * KP + 16 * q + 4 * g]);
-      v4f acc[TW][TB];
 #pragma unroll
-      for (int u = 0; u < TB; u++) {
-        const float c = Bc[buf][(jt0 + u) * 16 + l16];
+          for (int kk = 0; kk < 4; kk++)
+#pragma unroll
+            for (int i = 0; i < TW; i++)
+#pragma unroll
+              for (int u = 0; u < TB; u++)
+                acc[i][u] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[i][q][kk], b[u][q][kk], acc[i][u], 0, 0, 0);
+        bool any = false;
 #pragma unroll
         for (int i = 0; i < TW; i++)
-          acc[i][u] = v4f{ c, c, c, c };
-      }
 #pragma unroll
-      for (int q = 0; q < KP / 16; q++)
-#pragma unroll
-        for (int kk = 0; kk < 4; kk++)
+          for (int r = 0; r < 4; r++) {
+            float m;
+            if constexpr (TB == 4)
+              m = max4(acc[i][0][r], acc[i][1][r], acc[i][2][r], acc[i][3][r]);
+            else
+              m = acc[i][0][r] > acc[i][1][r] ? acc[i][0][r] : acc[i][1][r];
+            any = any | ((m >= fl[i][r]) & tok[i][r]);
+          }
+        if (__ballot(any)) {
+          // lanes 16g .. 16g + 15 hold target row 4g + r of tile i against the 16 rows of each library tile:
+          // one atomic add per (tile, row, group) reserves the slots of that target's passing pairs, then
+          // each passing lane writes at the reserved base plus the passing pairs before it
+          auto pass = [&](int i, int u, int r) {
+            return (acc[i][u][r] >= fl[i][r]) & tok[i][r] & ((jt0 + u) * 16 + l16 < cn);
+          };
+          int base[TW][4];
 #pragma unroll
           for (int i = 0; i < TW; i++)
 #pragma unroll
-            for (int u = 0; u < TB; u++)
-              acc[i][u] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[i][q][kk], b[u][q][kk], acc[i][u], 0, 0, 0);
-      bool any = false;
+            for (int r = 0; r < 4; r++) {
+              int cntg = 0;
 #pragma unroll
-      for (int i = 0; i < TW; i++)
+              for (int u = 0; u < TB; u++)
+                cntg += __builtin_popcount((unsigned)(__ballot(pass(i, u, r)) >> (16 * g)) & 0xffffu);
+              base[i][r] = 0;
+              if (l16 == 0 && cntg > 0)
+                base[i][r] = atomicAdd(&ccnt[tid[w * 16 * TW + 16 * i + 4 * g + r]], cntg);
+            }
 #pragma unroll
-        for (int r = 0; r < 4; r++) {
-          float m;
-          if constexpr (TB == 4)
-            m = max4(acc[i][0][r], acc[i][1][r], acc[i][2][r], acc[i][3][r]);
-          else
-            m = acc[i][0][r] > acc[i][1][r] ? acc[i][0][r] : acc[i][1][r];
-          any = any | (m >= fl[i][r]);
-        }
-      if (__ballot(any)) {
+          for (int i = 0; i < TW; i++)
 #pragma unroll
-        for (int i = 0; i < TW; i++)
+            for (int r = 0; r < 4; r++) {
+              int off = __shfl(base[i][r], lane & ~15);
 #pragma unroll
-          for (int u = 0; u < TB; u++) {
-            const int j = (jt0 + u) * 16 + l16;
-#pragma unroll
-            for (int r = 0; r < 4; r++)
-              if ((acc[i][u][r] >= fl[i][r]) & tok[i][r] & (j < cn)) {
-                const int lt = w * 16 * TW + 16 * i + 4 * g + r;
-                const int slot = atomicAdd(&lcnt[lt], 1);
-                if (slot < SCR_CAP)
-                  cand[(sbase + tblk + lt) * SCR_CAP + slot] = c0 + j;
+              for (int u = 0; u < TB; u++) {
+                const unsigned gm = (unsigned)(__ballot(pass(i, u, r)) >> (16 * g)) & 0xffffu;
+                const int slot = off + __builtin_popcount(gm & ((1u << l16) - 1u));
+                if (((gm >> l16) & 1u) && slot < SCR_CAP)
+                  cand[(size_t)tid[w * 16 * TW + 16 * i + 4 * g + r] * SCR_CAP + slot] =
+                    K.pos[c0 + (jt0 + u) * 16 + l16];
+                off += __builtin_popcount(gm);
               }
-          }
+            }
+        }
       }
+      if (more)
+        store(buf ^ 1);
+      __syncthreads();
+      buf ^= 1;
     }
-    if (more)
-      store(buf ^ 1);
-    __syncthreads();
-    buf ^= 1;
-  }
-  if (threadIdx.x < 64 * TW) {
-    const int t = tblk + threadIdx.x;
-    if (t < P.NP)
-      ccnt[sbase + t] = P.seed[t] < INF_D ? min(lcnt[threadIdx.x], SCR_CAP + 1) : SCR_CAP + 1;
   }
 }
 
@@ -613,7 +681,7 @@ __global__ __launch_bounds__(256) void knn_rerank(MfmaParams P, int k, int slice
     return;
   const size_t st = (size_t)blockIdx.y * P.NP + t;
   const int n = ccnt[st];
-  if (n > SCR_CAP) {
+  if (n > SCR_CAP || !(P.seed[t] < INF_D)) {
     if (lane == 0 && atomicExch(&fflag[t], 1) == 0)
       flist[atomicAdd(fcnt, 1)] = t;
     return;
@@ -688,7 +756,7 @@ __global__ __launch_bounds__(256) void knn_rerank(MfmaParams P, int k, int slice
     out_cnt[st] = (hi > lo ? hi - lo : 0) - zeros;
 }
 
-#define FEDM_SCREEN(KPv) template __global__ void knn_screen<KPv>(MfmaParams, int, int*, int*);
+#define FEDM_SCREEN(KPv) template __global__ void knn_screen<KPv>(MfmaParams, KdScreen, int, int*, int*);
 FEDM_SCREEN(16)
 FEDM_SCREEN(32)
 FEDM_SCREEN(64)

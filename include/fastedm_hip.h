@@ -172,6 +172,11 @@ typedef struct
                                one exact FP64 re-rank per target follows, and a target with more than 256
                                candidates takes the streaming kernel; (3) (2) for points of up to 16
                                coordinates and up to 32 neighbours, else (1).  All give the same results. (3) */
+  int mfma_prune;           /* two-phase form: the screen of a block of targets skips the 64-row chunks of the
+                               library's k-d order whose bounding box lies beyond every target's threshold
+                               (1), or screens every chunk (0).  Same results. (1) */
+  int mfma_split;           /* two-phase form: blocks that share out the library chunks of each block of 128
+                               targets (0: about 16 384 blocks in all, at most 16).  Same results. (0) */
 } fedm_knobs;
 void fedm_default_knobs(fedm_knobs* k);
 

@@ -66,4 +66,5 @@ def test_default_knobs():
     L.fedm_default_knobs(ctypes.byref(k))
     assert k.as_dict() == {"rank_scale": 1.5, "direct_blocks_per_cu": 0, "direct_alt_waves": 0, "list_blocks": 1024,
                            "mfma_blocks": 2048, "use_mfma": 1, "smap_lds": 0, "direct_prune": 1,
-                           "concurrent_routes": 0, "smap_block": 4, "mfma_seed": 3}
+                           "concurrent_routes": 0, "smap_block": 4, "mfma_seed": 3,
+                           "mfma_prune": 1, "mfma_split": 0}

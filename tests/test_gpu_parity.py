@@ -300,21 +300,34 @@ def test_device_exp_matches_oracle_bitwise():
     assert np.array_equal(out.view(np.int64), ref.view(np.int64))
 
 
-@pytest.mark.parametrize("case", ["clustered_dt", "ties"])
+@pytest.mark.parametrize("case", ["clustered_dt", "ties", "logistic"])
 def test_mfma_seed_leaves_results_unchanged(case, monkeypatch):
     """The matrix-core route starts each target's FP32 screening threshold from the exact k-th distance
     over a window of library rows near it in a k-d order (FEDM_MFMA_SEED=1), or screens every pair against
     that fixed bound, re-ranks the recorded candidates once and hands targets with more than 256
-    candidates to the streaming kernel (FEDM_MFMA_SEED=2; 3, the default, chooses).  Each only removes candidates that
-    cannot enter the list, so indices, k_used, codes and predictions must equal the unseeded run's
-    (FEDM_MFMA_SEED=0) bit for bit and the oracle's.  Cases: a dt-like layout (a column equal to a
-    constant on most rows), and an integer table with many tied and zero distances (targets whose
-    candidates overflow)."""
+    candidates to the streaming kernel (FEDM_MFMA_SEED=2; 3, the default, chooses); the screen of the
+    two-phase form skips the library chunks whose box lies beyond its targets' thresholds unless
+    FEDM_MFMA_PRUNE=0, and shares each target block's chunks out over FEDM_MFMA_SPLIT blocks (default:
+    chosen from the target count).  Each only removes candidates that cannot enter the list, so indices,
+    k_used, codes and predictions must equal the unseeded run's (FEDM_MFMA_SEED=0) bit for bit and the
+    oracle's.  Cases: a dt-like layout (a column equal to a constant on most rows), an integer table with
+    many tied and zero distances (targets whose candidates overflow), and the lag embedding of a chaotic
+    logistic map with dt columns (a low-dimensional manifold, where most chunks are skipped; targets not a
+    multiple of the 128-target blocks, library not a multiple of the 64-row chunks)."""
     fe = load_product()
     rng = np.random.default_rng(11 if case == "ties" else 12)
     NL, NP, Ea, k = 3000, 400, 12, 9
     if case == "ties":
         data = rng.integers(0, 4, size=(NL + NP, Ea)).astype(float)
+    elif case == "logistic":
+        NL, NP, Ea, k = 20000, 3001, 16, 18
+        x = np.empty(NL + NP + 8)
+        x[0] = 0.4
+        for t in range(x.size - 1):
+            x[t + 1] = 3.9 * x[t] * (1.0 - x[t])
+        lags = np.stack([x[7 - j:x.size - 1 - j] for j in range(8)], axis=1)
+        dt = rng.choice([1.0, 1.25, 2.5], size=(NL + NP, 8), p=[0.9, 0.05, 0.05])
+        data = np.hstack([lags, dt])
     else:
         base = rng.random((NL + NP, Ea))
         base[:, 6:] = 1.25
@@ -326,10 +339,14 @@ def test_mfma_seed_leaves_results_unchanged(case, monkeypatch):
     opts = {"algorithm": 0, "k": k, "thetas": [0.0, 1.0], "precision": fe.MIXED}
     print(f"mfma seed: {case}", flush=True)
     outs = []
-    for seed in ("0", "1", "2", "3"):
+    for seed, prune, split in (("0", "1", "0"), ("1", "1", "0"), ("2", "1", "0"), ("3", "1", "0"), ("2", "0", "0"),
+                               ("2", "1", "1"), ("2", "1", "5")):
         monkeypatch.setenv("FEDM_MFMA_SEED", seed)
+        monkeypatch.setenv("FEDM_MFMA_PRUNE", prune)
+        monkeypatch.setenv("FEDM_MFMA_SPLIT", split)
         outs.append(fe.predict(M, Mp, opts, nn_cap=k))
-    monkeypatch.delenv("FEDM_MFMA_SEED")
+    for v in ("FEDM_MFMA_SEED", "FEDM_MFMA_PRUNE", "FEDM_MFMA_SPLIT"):
+        monkeypatch.delenv(v)
     p0, r0, c0, k0, n0, s0 = outs[0]
     assert s0 == 0
     for p1, r1, c1, k1, n1, s1 in outs[1:]:
