@@ -26,8 +26,8 @@ constexpr int DT_TGT = 64; // targets per block of the dense distance kernel
 // groups to fill the device)
 enum { SMAP_SOLVE_ROWS = 0, SMAP_SOLVE_BLOCK = 1, SMAP_SOLVE_PACK = 2, SMAP_SOLVE_PACK_MAX = 3, SMAP_SOLVE_AUTO = 4 };
 constexpr long long SMAP_PACK_MIN_ITEMS = 65536;
-constexpr int SMAP_BIG_MAX = 128;
-constexpr int SCR_CAP = 256; // candidates recorded per (slice, target) by the matrix-core screen (knn_screen)    // widest S-map system (unknowns = E_actual + 1) the solve takes
+constexpr int SMAP_BIG_MAX = 128;   // widest S-map system (unknowns = E_actual + 1) the solve takes
+constexpr int SCR_CAP = 256;        // candidates recorded per target by the matrix-core screen (knn_screen)
 constexpr int SMAP_BIG_BLOCKS = 256; // blocks of the wide S-map kernel (one system each at a time)
 
 enum
