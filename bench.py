@@ -294,8 +294,10 @@ def cfg4_leg(fe, oracle, cores, steps):
     Ea, k, NL = 16, 18, n_pred // 3
     st = [out["mixed"]["select_ms"], out["mixed"]["merge_ms"], out["mixed"]["smap_ms"]]
     out["roofline"] = stage_roofline(st, {
-        "select": ("knn_seed + knn_screen<16> (FP32 matrix cores, seeded floor) + knn_rerank<32,16> (FP64) + "
-                   "knn_mfma<32,16> on the targets that overflow", 3.0 * Ea * float(NL) * NL,
+        "select": ("knn_seed + knn_screen<16> (FP32 matrix cores, seeded floor, k-d ordered library chunks "
+                   "skipped by box bounds) + knn_rerank<32,16> (FP64) + knn_mfma<32,16> on the targets that "
+                   "overflow; the work unit counts every pair, most of which the boxes skip",
+                   3.0 * Ea * float(NL) * NL,
                    "fp32 MFMA", FP32_PEAK_TFLOPS, "mfma"),
         "smap": ("smap_pack<16,1> (Gram + Eigen-order Jacobi SVD, two rows per lane)", smap_flops(k, Ea) * n_pred,
                  "fp64 VALU",
