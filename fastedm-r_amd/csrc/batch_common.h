@@ -2,6 +2,7 @@
 // host dispatcher and the kernels.
 #pragma once
 #include "kernels_common.h"
+#include "fdlibm_exp.h"
 
 namespace fedm {
 
@@ -124,5 +125,16 @@ struct ScanParams
   int* redo_count;
   int redo_cap;
 };
+
+// The simplex weight of a neighbour at exactly the smallest distance is exp(-theta * 1) (src/edm.cpp:
+// 707-735: its distance over the smallest is x / x = 1 exactly), the same for every prediction of a call:
+// a block makes exp(-theta) of the call's first ETH_N thetas once, into LDS (before a barrier), with the
+// exp of the weights.
+constexpr int ETH_N = 8;
+__device__ __forceinline__ void theta_exps(const double* thetas, int T, double* eth)
+{
+  if ((int)threadIdx.x < min(T, ETH_N))
+    eth[threadIdx.x] = fdlibm_exp(-thetas[threadIdx.x]);
+}
 
 } // namespace fedm

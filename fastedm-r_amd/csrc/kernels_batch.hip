@@ -94,9 +94,12 @@ __device__ __forceinline__ float float_up(double x)
 // k policy, neighbour order and simplex prediction of target t of task tk.  ls/li: the neighbour list
 // sorted by (sqrt distance, position); zeros: pairs dropped by the keep rule.
 // tgt: the task's library targets gathered by position, or null to read them through the row list.
+// eth: theta_exps of the call in LDS, or null: the weight of the first entry while the list stays in
+// distance order.
 template <int KCAP>
 __device__ void finish_target(const BatchParams& B, int tk, int t, int np, const int* lrows, int nl, int zeros,
-                              int k, double (&ls)[KCAP], int (&li)[KCAP], const double* tgt = nullptr)
+                              int k, double (&ls)[KCAP], int (&li)[KCAP], const double* tgt = nullptr,
+                              const double* eth = nullptr)
 {
   const int cnt = nl - zeros;
   const int T = B.T;
@@ -140,14 +143,21 @@ __device__ void finish_target(const BatchParams& B, int tk, int t, int np, const
 #pragma unroll
   for (int j = 0; j < KCAP; j++)
     tg[j] = j < ke ? (tgt ? tgt[li[j]] : B.Ltgt[lrows[li[j]]]) : 0.0;
+  // entry 0 holds the smallest distance unless the list was put in library order
+  const bool first_min = eth && ke != cnt;
   for (int th = 0; th < T; th++) {
     const double theta = B.thetas[th];
+    double w0;
+    if (first_min && th < ETH_N)
+      w0 = eth[th];
+    else
+      w0 = fdlibm_exp(-theta * (ls[0] / minDist));
     double w[KCAP];
     double sumw = 0.0;
 #pragma unroll
     for (int j = 0; j < KCAP; j++)
       if (j < ke) {
-        w[j] = fdlibm_exp(-theta * (ls[j] / minDist));
+        w[j] = j == 0 ? w0 : fdlibm_exp(-theta * (ls[j] / minDist));
         sumw = sumw + w[j];
       }
     double r = 0.0;
@@ -239,8 +249,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
   // the first chunk's blocks start at SUB0, SUB0 + SUB, ...: its last block may reach SUB0 rows past BCH
   __shared__ double lds64[(BCH + SUB0) * ST];
   __shared__ int item_sh;
+  __shared__ double eth[ETH_N];
 
   const int Ea = B.Ea;
+  theta_exps(B.thetas, B.T, eth);
   if (threadIdx.x == 0)
     item_sh = atomicAdd(B.counter, 1);
   __syncthreads();
@@ -407,7 +419,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
           }
         }
       }
-      finish_target<KCAP>(B, tk, t, np, lrows, nl, zeros, k, ls, li, tgt);
+      finish_target<KCAP>(B, tk, t, np, lrows, nl, zeros, k, ls, li, tgt, eth);
     } while (0);
     __syncthreads();
     item = item_sh;
@@ -513,9 +525,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PR_WAVES)))
   __shared__ double wsum[4][EMAX];
   __shared__ unsigned long long rext[2];
   __shared__ int item_sh;
+  __shared__ double eth[ETH_N];
 
   const int Ea = B.Ea;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  theta_exps(B.thetas, B.T, eth);
   // slack factors of the pruning test (see above)
   const double eta2 = (Ea + 6) * 2.220446049250313e-16, eta3 = (8.0 * Ea + 16.0) * 1.1102230246251565e-16;
   // the next item: from this block's XCD queue, then from the others in turn (-1 when all are empty)
@@ -785,7 +799,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PR_WAVES)))
           }
         }
       }
-      finish_target<KCAP>(B, tk, t, np, lrows, nl, zeros, k, ls, li, tgt);
+      finish_target<KCAP>(B, tk, t, np, lrows, nl, zeros, k, ls, li, tgt, eth);
     } while (0);
     __syncthreads();
     item = item_sh;

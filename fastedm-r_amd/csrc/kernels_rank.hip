@@ -687,6 +687,9 @@ __global__ __launch_bounds__(64 * MS_WAVES) void rank_scan_multi(ScanParams S)
   ms_wave_sync();
   constexpr int U = KCAP <= 8 ? 2 : 1; // more side by side costs registers, and so occupancy
   const double* rs = S.rank_s + (size_t)t * S.tstride;
+  // the weight of entry 0 of a prefix-ordered list (the smallest distance, so a ratio of exactly 1) at the
+  // first theta, made once per lane: exp(-theta_0) (src/edm.cpp:707-735)
+  const double e0 = fdlibm_exp(-S.thetas[0]);
   for (int tau0 = lane * U; tau0 < ntask; tau0 += 64 * U) {
     int task[U], kq[U];
     bool open_[U];
@@ -737,7 +740,7 @@ __global__ __launch_bounds__(64 * MS_WAVES) void rank_scan_multi(ScanParams S)
 #pragma unroll
         for (int j = 0; j < KCAP; j++)
           if (j < kq[u]) {
-            w[u][j] = fdlibm_exp(-theta * (ls[u][j] / minDist[u]));
+            w[u][j] = j == 0 && th == 0 ? e0 : fdlibm_exp(-theta * (ls[u][j] / minDist[u]));
             sumw[u] = sumw[u] + w[u][j];
           }
       }
