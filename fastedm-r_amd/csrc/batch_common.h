@@ -126,6 +126,14 @@ struct ScanParams
   int redo_cap;
 };
 
+// The work lists of a batch (direct, rank-table and pruned items: (task, first target)), for the exact
+// re-selection when more pairs were handed over than its list holds.
+struct RedoLists
+{
+  const int2* items[3];
+  int counts[3];
+};
+
 // The simplex weight of a neighbour at exactly the smallest distance is exp(-theta * 1) (src/edm.cpp:
 // 707-735: its distance over the smallest is x / x = 1 exactly), the same for every prediction of a call:
 // a block makes exp(-theta) of the call's first ETH_N thetas once, into LDS (before a barrier), with the

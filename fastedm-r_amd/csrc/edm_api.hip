@@ -1455,7 +1455,7 @@ int edm_predict_noalloc(edm_ctx* c, const edm_manifold_view* M, const edm_manifo
 
 namespace fedm {
 template <int KCAP>
-__global__ void batch_redo(BatchParams B, const int2* items, int num_items);
+__global__ void batch_redo(BatchParams B, RedoLists L);
 
 template <int EMAX>
 __global__ void rank_build(RankParams R);
@@ -1465,6 +1465,8 @@ __global__ void rank_scatter(ScanParams S);
 template <int KCAP, int EMAX>
 __global__ void rank_scan(ScanParams S);
 } // namespace fedm
+
+constexpr int CTR_INTS = (3 + PR_QUEUES) * PR_CSTRIDE;
 
 struct edm_batch
 {
@@ -1488,6 +1490,8 @@ struct edm_batch
   float last_ms = 0.f;
   int (*keep_going)(void) = nullptr; // polled before each task or stage launch
   int redo_last = 0; // targets handed to the exact selection in the last run
+  int* counters = nullptr;       // the per-run counters (CTR_INTS ints, see edm_batch_create)
+  int* handed_host = nullptr;    // pinned copy of the hand-over count, written at the end of each run
   bool outputs_filled = false; // the outputs have held their fill values once (first run)
   bool split_last = false;     // the last run put the direct kernels on the second stream
   // host copies for the task-by-task route
@@ -1531,6 +1535,8 @@ struct edm_batch
       hipGraphDestroy(graph);
     for (void* p : owned)
       hipFree(p);
+    if (handed_host)
+      hipHostFree(handed_host);
   }
 };
 
@@ -2130,11 +2136,17 @@ int edm_batch_create_knobs(edm_ctx* c, const edm_manifold_view* L, const edm_man
     for (int t0 = 0; t0 < pred_count[t]; t0 += 256)
       (is_rank[t] ? ritems : is_pruned[t] ? pitems : items).push_back(make_int2(t, t0));
   }
-  B.counter = b->dev<int>(nullptr, 1);
+  // every per-run counter in one buffer, each on a cache line of its own, so that one fill zeroes them:
+  // the direct kernel's queue head, the hand-over count, the rank scan's queue head, the pruned kernel's
+  // PR_QUEUES queue heads
+  b->counters = b->dev<int>(nullptr, (size_t)CTR_INTS);
+  if (hipHostMalloc((void**)&b->handed_host, sizeof(int)) != hipSuccess)
+    b->handed_host = nullptr;
+  B.counter = b->counters;
   // targets whose squared-distance ranking may differ from the sqrt ranking (rare; see kernels_batch.hip)
   B.redo_cap = (int)std::min<int64_t>(b->total_k, 1 << 16);
   B.redo = b->dev<int2>(nullptr, (size_t)std::max(B.redo_cap, 1));
-  B.redo_count = b->dev<int>(nullptr, 1);
+  B.redo_count = b->counters + PR_CSTRIDE;
   B.thetas = b->dev(o->thetas, (size_t)o->num_thetas);
   B.T = o->num_thetas;
   B.Ea = Ea;
@@ -2196,7 +2208,7 @@ int edm_batch_create_knobs(edm_ctx* c, const edm_manifold_view* L, const edm_man
   }
   B.pitems = b->dev(pitems.data(), pitems.size());
   B.num_pitems = (int)pitems.size();
-  B.pcounter = b->dev<int>(nullptr, (size_t)PR_QUEUES * PR_CSTRIDE);
+  B.pcounter = b->counters + 3 * PR_CSTRIDE;
   {
     // spatial order of every prediction list a pruned task uses: Morton order of the first (up to three)
     // coordinates, quantised to 21 bits over the list's range
@@ -2338,7 +2350,7 @@ int edm_batch_create_knobs(edm_ctx* c, const edm_manifold_view* L, const edm_man
     S.out_begin = B.out_begin;
     S.items = b->dev(ritems.data(), ritems.size());
     S.num_items = (int)ritems.size();
-    S.counter = b->dev<int>(nullptr, 1);
+    S.counter = b->counters + 2 * PR_CSTRIDE;
     S.rank_row = R.rank_row;
     S.rank_s = R.rank_s;
     S.rank_len = R.rank_len;
@@ -2449,9 +2461,7 @@ static int enqueue_device(edm_batch* b, bool timed)
       hipStreamWaitEvent(s, c->join, 0);
     return batch_break(b);
   };
-  dfill(c, B.counter, 1, 0);
-  if (B.num_pitems > 0)
-    dfill(c, B.pcounter, (size_t)PR_QUEUES * PR_CSTRIDE, 0);
+  dfill(c, b->counters, (size_t)CTR_INTS, 0);
   if (!b->outputs_filled) {
     // Every run writes every output of every target (a prediction, or the fill values themselves for a
     // target left unpredicted), so the outputs are filled on the first run only, as a safety net: a
@@ -2461,7 +2471,6 @@ static int enqueue_device(edm_batch* b, bool timed)
     dfill(c, B.k_used, (size_t)b->total_k, (int)FEDM_MISSING_I);
     b->outputs_filled = true;
   }
-  dfill(c, B.redo_count, 1, 0);
   if (timed)
     HIP_TRY(hipEventRecord(c->ev[4], s));
   const void* fn = batch_kernel(b->kcap, b->Ea, b->kn.direct_alt_waves != 0);
@@ -2488,7 +2497,6 @@ static int enqueue_device(edm_batch* b, bool timed)
     HIP_TRY(hipEventRecord(c->join, sd));
   }
   if (b->rank_route) {
-    dfill(c, b->S.counter, 1, 0);
     void* rargs[] = { &b->R };
     HIP_TRY(hipLaunchKernel(rank_build_kernel(b->emax), dim3(b->R.np), dim3(256), rargs, 0, s));
     if (timed)
@@ -2520,20 +2528,13 @@ static int enqueue_device(edm_batch* b, bool timed)
     return r;
   }
   {
-    // targets handed over by the direct kernel or the rank scan: exact selection
-    const int2* none = nullptr;
-    int zero = 0;
-    void* rargs[] = { &B, &none, &zero };
+    // targets handed over by the direct kernel or the rank scan: exact selection; with more hand-overs
+    // than the list holds, every target again (one launch either way, so the host does not wait for the
+    // count mid-step)
+    RedoLists L{ { B.items, b->rank_route ? b->S.items : nullptr, B.pitems },
+                 { B.num_items, b->rank_route ? b->S.num_items : 0, B.num_pitems } };
+    void* rargs[] = { &B, &L };
     HIP_TRY(hipLaunchKernel(batch_redo_kernel(b->kcap), dim3(256), dim3(256), rargs, 0, s));
-    // more hand-overs than the list holds: every target again with the exact selection (these
-    // launches return at once otherwise, so the host does not wait for the count mid-step)
-    const int2* items[3] = { B.items, b->rank_route ? b->S.items : nullptr, B.pitems };
-    int counts[3] = { B.num_items, b->rank_route ? b->S.num_items : 0, B.num_pitems };
-    for (int q = 0; q < 3; q++)
-      if (counts[q] > 0) {
-        void* aargs[] = { &B, &items[q], &counts[q] };
-        HIP_TRY(hipLaunchKernel(batch_redo_kernel(b->kcap), dim3(b->grid), dim3(256), aargs, 0, s));
-      }
   }
   return 0;
 }
@@ -2596,11 +2597,17 @@ int edm_batch_run(edm_batch* b)
     if (r != 0)
       return r;
   }
+  // the hand-over count comes back with the run (pinned memory, so the one wait below covers it)
+  if (b->handed_host)
+    HIP_TRY(hipMemcpyAsync(b->handed_host, b->B.redo_count, sizeof(int), hipMemcpyDeviceToHost, s));
   HIP_TRY(hipEventRecord(c->ev[5], s));
   HIP_TRY(hipEventSynchronize(c->ev[5]));
   {
     int handed = 0;
-    HIP_TRY(hipMemcpy(&handed, b->B.redo_count, sizeof(int), hipMemcpyDeviceToHost));
+    if (b->handed_host)
+      handed = *b->handed_host;
+    else
+      HIP_TRY(hipMemcpy(&handed, b->B.redo_count, sizeof(int), hipMemcpyDeviceToHost));
     b->redo_last = handed;
     c->last_redo = handed;
   }

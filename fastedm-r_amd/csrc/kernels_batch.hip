@@ -964,21 +964,19 @@ __device__ void exact_target_wave(const BatchParams& B, int tk, int t, int lane)
     finish_target<KCAP>(B, tk, t, np, lrows, nl, zeros, k, ms, mi);
 }
 
-// items == null: the (task, target) pairs handed over (at most redo_cap of them), one per wavefront;
-// otherwise every target of the given (task, first target) items, when more pairs were handed over
-// than the list holds (the launch returns at once when they were not).
+// The (task, target) pairs handed over (at most redo_cap of them), one per wavefront; or, when more
+// pairs were handed over than the list holds, every target of every item of the batch's work lists.
 template <int KCAP>
-__global__ __launch_bounds__(256) void batch_redo(BatchParams B, const int2* items, int num_items)
+__global__ __launch_bounds__(256) void batch_redo(BatchParams B, RedoLists L)
 {
-  if (items) {
-    if (*B.redo_count <= B.redo_cap)
-      return;
-    for (int item = blockIdx.x; item < num_items; item += gridDim.x) {
-      const int2 it = items[item];
-      const int t = it.y + threadIdx.x;
-      if (t < B.pred_count[it.x])
-        exact_target<KCAP>(B, it.x, t);
-    }
+  if (*B.redo_count > B.redo_cap) {
+    for (int q = 0; q < 3; q++)
+      for (int item = blockIdx.x; item < L.counts[q]; item += gridDim.x) {
+        const int2 it = L.items[q][item];
+        const int t = it.y + threadIdx.x;
+        if (t < B.pred_count[it.x])
+          exact_target<KCAP>(B, it.x, t);
+      }
   } else {
     const int n = min(*B.redo_count, B.redo_cap);
     const int lane = threadIdx.x & 63;
@@ -995,7 +993,7 @@ __global__ __launch_bounds__(256) void batch_redo(BatchParams B, const int2* ite
   template __global__ void batch_simplex<KC, EM, batch_waves_alt(KC, EM)>(BatchParams);
 #define FEDM_BATCH_E(KC)                                                                                         \
   FEDM_BATCH(KC, 2) FEDM_BATCH(KC, 3) FEDM_BATCH(KC, 4) FEDM_BATCH(KC, 8) FEDM_BATCH(KC, 16) FEDM_BATCH(KC, 32)  \
-  template __global__ void batch_redo<KC>(BatchParams, const int2*, int);
+  template __global__ void batch_redo<KC>(BatchParams, RedoLists);
 FEDM_BATCH_E(4)
 FEDM_BATCH_E(8)
 FEDM_BATCH_E(16)
