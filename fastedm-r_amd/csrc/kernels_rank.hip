@@ -17,7 +17,7 @@
 namespace fedm {
 
 constexpr int RANK_BINS = 2048;   // FP32 bit-pattern bins: 8 exponent + 3 mantissa bits
-constexpr int RANK_TARGET = 600;  // rows wanted below the histogram cut
+constexpr int RANK_TARGET = 500;  // rows wanted below the histogram cut
 constexpr int RANK_SAMPLE = 8;    // the histogram counts one pool row in RANK_SAMPLE (the hash picks 0..7)
 constexpr int RANK_SAMPLE_TARGET = (RANK_TARGET + RANK_SAMPLE - 1) / RANK_SAMPLE;
 constexpr int RANK_CAP = 1024;    // LDS candidate capacity (the stored prefix is at most RANK_STORE)
