@@ -22,6 +22,8 @@ constexpr int RANK_SAMPLE = 8;    // the histogram counts one pool row in RANK_S
 constexpr int RANK_SAMPLE_TARGET = (RANK_TARGET + RANK_SAMPLE - 1) / RANK_SAMPLE;
 constexpr int RANK_CAP = 1024;    // LDS candidate capacity (the stored prefix is at most RANK_STORE)
 constexpr int RANK_SORT = 1024;   // bitonic sort width (power of two >= RANK_CAP)
+constexpr int RANK_NB = 256;      // buckets of the counting sort of the candidates (by s in [0, s_max])
+constexpr int RANK_BMAX = 32;     // a bucket holding more candidates sends the target to the bitonic sort
 static_assert(RANK_STORE <= RANK_CAP, "the stored prefix is cut from the sorted candidates");
 static_assert(RANK_BINS * sizeof(unsigned) <= RANK_SORT * sizeof(double), "the histogram shares the sort keys' LDS");
 __device__ __forceinline__ float up_float(double x)
@@ -36,10 +38,14 @@ template <int EMAX>
 __global__ __launch_bounds__(256) void rank_build(RankParams R)
 {
   // The histogram is done with before pass 2 writes the first candidate, so it shares the sort keys'
-  // storage: about 12 KiB of LDS per block, eight blocks per CU.
+  // storage; with the counting sort's buckets and order, about 18 KiB of LDS per block, eight blocks per
+  // CU.
   __shared__ double cs[RANK_SORT];
   unsigned* hist = reinterpret_cast<unsigned*>(cs);
   __shared__ int cq[RANK_SORT];
+  __shared__ int ord[RANK_CAP];
+  __shared__ unsigned bcnt[RANK_NB], bstart[RANK_NB + 1];
+  __shared__ unsigned bmaxc;
   __shared__ int ncand, nzero, overflow, cut_bin, all_rows;
   __shared__ unsigned wsum[4];
   __shared__ double smax_sh;
@@ -224,82 +230,162 @@ __global__ __launch_bounds__(256) void rank_build(RankParams R)
     return;
   }
   const int n = min(ncand, RANK_CAP);
-  // drop the group tied with the largest kept distance unless every pool row was examined
-  if (!all_rows) {
+  // the largest kept distance; drop the group tied with it unless every pool row was examined
+  {
     double m = 0.0;
     for (int c = threadIdx.x; c < n; c += blockDim.x)
       m = fmax(m, cs[c]);
     if (threadIdx.x == 0)
       smax_sh = 0.0;
+    for (int b = threadIdx.x; b < RANK_NB; b += blockDim.x)
+      bcnt[b] = 0u;
     __syncthreads();
     // block max via LDS (values are non-negative doubles: compare as unsigned 64-bit)
     atomicMax((unsigned long long*)&smax_sh, (unsigned long long)__double_as_longlong(m));
     __syncthreads();
-    const double smax = smax_sh;
+  }
+  const double smax = smax_sh;
+  if (!all_rows)
     for (int c = threadIdx.x; c < n; c += blockDim.x)
       if (cs[c] == smax)
         cs[c] = INF_D;
-  }
-  int width = 64;
-  while (width < n)
-    width <<= 1;
-  for (int c = n + threadIdx.x; c < width; c += blockDim.x) {
-    cs[c] = INF_D;
-    cq[c] = INT_MAX;
+  // Order by (s, row).  A counting sort into RANK_NB buckets of s over [0, s_max] (the bucket index is
+  // monotone in s), then each candidate takes its rank among the members of its bucket; with a bucket
+  // of more than RANK_BMAX candidates (many equal distances) the bitonic sort instead.  Rows are
+  // distinct, so both give the one order.
+  const double scale = (double)RANK_NB / smax;
+  const bool bucketable = smax > 0.0 && scale < INF_D;
+  auto bucket = [&](double s) { return (int)fmin(s * scale, (double)(RANK_NB - 1)); };
+  if (bucketable)
+    for (int c = threadIdx.x; c < n; c += blockDim.x)
+      if (cs[c] < INF_D)
+        atomicAdd(&bcnt[bucket(cs[c])], 1u);
+  __syncthreads();
+  if (threadIdx.x < 64) {
+    // exclusive scan of the bucket counts on wavefront 0 (four buckets a lane), their largest, and the
+    // counts cleared for the placement
+    static_assert(RANK_NB == 256, "four buckets per lane");
+    const int lane = threadIdx.x;
+    unsigned v[4], sum = 0u, mx = 0u;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      v[i] = bcnt[4 * lane + i];
+      sum += v[i];
+      mx = max(mx, v[i]);
+    }
+    unsigned incl = sum;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const unsigned u = __shfl_up(incl, off);
+      if (lane >= off)
+        incl += u;
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+      mx = max(mx, (unsigned)__shfl_xor((int)mx, off));
+    unsigned run = incl - sum;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      bstart[4 * lane + i] = run;
+      run += v[i];
+      bcnt[4 * lane + i] = 0u;
+    }
+    if (lane == 63)
+      bstart[RANK_NB] = incl;
+    if (lane == 0)
+      bmaxc = mx;
   }
   __syncthreads();
-  // Bitonic sort by (s, row) over the next power of two covering the candidates.  Thread t takes the
-  // pairs t, t + 256, ...; for strides up to 64 the pairs of one wavefront stay inside its own
-  // 128-element blocks, so only the stages with a longer stride, and their neighbours, need the
-  // whole block to wait.
-  {
-    bool prev_cross = true;
-    for (int size = 2; size <= width; size <<= 1) {
-      for (int stride = size >> 1; stride > 0; stride >>= 1) {
-        const bool cross = stride > 64;
-        if (cross || prev_cross) {
-          __syncthreads();
-        } else {
-          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-          __builtin_amdgcn_wave_barrier();
-          __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-        }
-        prev_cross = cross;
-        for (int t = threadIdx.x; t < width / 2; t += blockDim.x) {
-          const int i = ((t & ~(stride - 1)) << 1) | (t & (stride - 1));
-          const int j = i + stride;
-          const bool asc = (i & size) == 0;
-          const double si = cs[i], sj = cs[j];
-          const int qi = cq[i], qj = cq[j];
-          const bool gt = si > sj || (si == sj && qi > qj);
-          if (gt == asc) {
-            cs[i] = sj;
-            cs[j] = si;
-            cq[i] = qj;
-            cq[j] = qi;
+  int kept = 0;
+  if (bucketable && bmaxc <= (unsigned)RANK_BMAX) {
+    for (int c = threadIdx.x; c < n; c += blockDim.x)
+      if (cs[c] < INF_D) {
+        const int b = bucket(cs[c]);
+        ord[bstart[b] + atomicAdd(&bcnt[b], 1u)] = c;
+      }
+    __syncthreads();
+    kept = min((int)bstart[RANK_NB], RANK_STORE);
+    for (int c = threadIdx.x; c < n; c += blockDim.x) {
+      const double sc = cs[c];
+      if (!(sc < INF_D))
+        continue;
+      const int qc = cq[c];
+      const int b = bucket(sc);
+      int r = 0;
+      for (int m = (int)bstart[b]; m < (int)bstart[b + 1]; m++) {
+        const int o = ord[m];
+        const double so = cs[o];
+        r += (so < sc || (so == sc && cq[o] < qc)) ? 1 : 0;
+      }
+      const int p = (int)bstart[b] + r;
+      if (p < RANK_STORE) {
+        R.rank_row[(size_t)t * R.tstride + (size_t)p * R.jstride] = qc;
+        R.rank_s[(size_t)t * R.tstride + (size_t)p * R.jstride] = sc;
+      }
+    }
+  } else {
+    int width = 64;
+    while (width < n)
+      width <<= 1;
+    for (int c = n + threadIdx.x; c < width; c += blockDim.x) {
+      cs[c] = INF_D;
+      cq[c] = INT_MAX;
+    }
+    __syncthreads();
+    // Bitonic sort by (s, row) over the next power of two covering the candidates.  Thread t takes the
+    // pairs t, t + 256, ...; for strides up to 64 the pairs of one wavefront stay inside its own
+    // 128-element blocks, so only the stages with a longer stride, and their neighbours, need the
+    // whole block to wait.
+    {
+      bool prev_cross = true;
+      for (int size = 2; size <= width; size <<= 1) {
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+          const bool cross = stride > 64;
+          if (cross || prev_cross) {
+            __syncthreads();
+          } else {
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+          }
+          prev_cross = cross;
+          for (int t = threadIdx.x; t < width / 2; t += blockDim.x) {
+            const int i = ((t & ~(stride - 1)) << 1) | (t & (stride - 1));
+            const int j = i + stride;
+            const bool asc = (i & size) == 0;
+            const double si = cs[i], sj = cs[j];
+            const int qi = cq[i], qj = cq[j];
+            const bool gt = si > sj || (si == sj && qi > qj);
+            if (gt == asc) {
+              cs[i] = sj;
+              cs[j] = si;
+              cq[i] = qj;
+              cq[j] = qi;
+            }
           }
         }
       }
+      __syncthreads();
+    }
+    int kept_b = 0;
+    if (threadIdx.x == 0) {
+      int lo = 0, hi = n; // first INF position
+      while (lo < hi) {
+        const int mid = (lo + hi) / 2;
+        if (cs[mid] != INF_D)
+          lo = mid + 1;
+        else
+          hi = mid;
+      }
+      ncand = min(lo, RANK_STORE);
     }
     __syncthreads();
-  }
-  int kept = 0;
-  if (threadIdx.x == 0) {
-    int lo = 0, hi = n; // first INF position
-    while (lo < hi) {
-      const int mid = (lo + hi) / 2;
-      if (cs[mid] != INF_D)
-        lo = mid + 1;
-      else
-        hi = mid;
+    kept_b = ncand;
+    for (int c = threadIdx.x; c < kept_b; c += blockDim.x) {
+      R.rank_row[(size_t)t * R.tstride + (size_t)c * R.jstride] = cq[c];
+      R.rank_s[(size_t)t * R.tstride + (size_t)c * R.jstride] = cs[c];
     }
-    ncand = min(lo, RANK_STORE);
-  }
-  __syncthreads();
-  kept = ncand;
-  for (int c = threadIdx.x; c < kept; c += blockDim.x) {
-    R.rank_row[(size_t)t * R.tstride + (size_t)c * R.jstride] = cq[c];
-    R.rank_s[(size_t)t * R.tstride + (size_t)c * R.jstride] = cs[c];
+    kept = kept_b;
   }
   if (threadIdx.x == 0) {
     R.rank_len[t] = kept;
