@@ -43,8 +43,8 @@ template <int KCAP, int KP>
 __global__ void knn_mfma(MfmaParams P, int k, int slice_len, double* out_s, int* out_i, int* out_cnt, const int* tl,
                          const int* tn);
 template <int KP>
-__global__ void knn_seed(MfmaParams P, const double* kdm, const int* wst, const int* tord, int win, int k,
-                         double* seed);
+__global__ void knn_seed(MfmaParams P, const double* kdm, const int* wst, const int* tord, const int* sgrp, int ngrp,
+                         int win, int k, double* seed);
 template <int KP>
 __global__ void knn_screen(MfmaParams P, KdScreen K, int split, int* cand, int* ccnt);
 template <int KCAP, int KP>
@@ -161,6 +161,7 @@ enum
   B_KDPOS,
   B_CBOX,
   B_TBOX,
+  B_SGRP,
   B_NUM
 };
 } // namespace fedm
@@ -254,15 +255,15 @@ static void launch_mfma_k(int kp, dim3 g, hipStream_t s, const MfmaParams& P, in
 }
 
 static void launch_seed(int kp, hipStream_t s, const MfmaParams& P, const double* kdp, const int* wst,
-                        const int* tord, int win, int k, double* seed)
+                        const int* tord, const int* sgrp, int ngrp, int win, int k, double* seed)
 {
-  const dim3 g((P.NP + 3) / 4);
+  const dim3 g((ngrp + 3) / 4);
   if (kp == 16)
-    hipLaunchKernelGGL((knn_seed<16>), g, dim3(256), 0, s, P, kdp, wst, tord, win, k, seed);
+    hipLaunchKernelGGL((knn_seed<16>), g, dim3(256), 0, s, P, kdp, wst, tord, sgrp, ngrp, win, k, seed);
   else if (kp == 32)
-    hipLaunchKernelGGL((knn_seed<32>), g, dim3(256), 0, s, P, kdp, wst, tord, win, k, seed);
+    hipLaunchKernelGGL((knn_seed<32>), g, dim3(256), 0, s, P, kdp, wst, tord, sgrp, ngrp, win, k, seed);
   else
-    hipLaunchKernelGGL((knn_seed<64>), g, dim3(256), 0, s, P, kdp, wst, tord, win, k, seed);
+    hipLaunchKernelGGL((knn_seed<64>), g, dim3(256), 0, s, P, kdp, wst, tord, sgrp, ngrp, win, k, seed);
 }
 
 // Phases one and two of the two-phase matrix-core route: the seeded screen of every slice, then the exact
@@ -599,6 +600,7 @@ struct TaskPlan
   std::vector<float> m32, mn, p32, pn;
   std::vector<double> ynorm;
   std::vector<int> kdp, wst, tord; // matrix-core seed: library rows in k-d order, window starts, target order
+  std::vector<int> sgrp;           // seed groups: starts in the target order of runs of up to 4 sharing a window
   std::vector<double> kdm;         // the library rows in that order (FP64)
   std::vector<float> kd32, kdn;    // two-phase screen: the centred FP32 rows and their norms in that order
   std::vector<float> cbox, tbox;   // two-phase screen: boxes of the library chunks and the target blocks
@@ -623,7 +625,7 @@ constexpr int SEED_WIN = 256; // library rows per seed window of the matrix-core
 // a leaf and takes the `win` rows of the order centred on it.  The windows only tighten the screening's
 // starting threshold; any window gives the same results.
 static void kd_seed_windows(const double* L, int NL, const double* P, int NP, int Ea, int win, std::vector<int>& kdp,
-                            std::vector<int>& wst, std::vector<int>& tord)
+                            std::vector<int>& wst, std::vector<int>& tord, std::vector<int>& sgrp)
 {
   constexpr int LEAF = 32;
   struct Node
@@ -703,6 +705,12 @@ static void kd_seed_windows(const double* L, int NL, const double* P, int NP, in
   tord.resize(NP);
   std::iota(tord.begin(), tord.end(), 0);
   std::stable_sort(tord.begin(), tord.end(), [&](int x, int y) { return wst[x] < wst[y]; });
+  // runs of up to 4 consecutive targets with one window (the seed kernel's groups), closed by NP
+  sgrp.clear();
+  for (int i = 0; i < NP; i++)
+    if (i == 0 || wst[tord[i]] != wst[tord[sgrp.back()]] || i - sgrp.back() == 4)
+      sgrp.push_back(i);
+  sgrp.push_back(NP);
 }
 
 // The two-phase screen's operands in the k-d orders (kdp for the library, tord for the targets): the
@@ -929,7 +937,7 @@ static int plan_task(const edm_manifold_view* M, const edm_manifold_view* Mp, co
     by[B_MIX3] = sizeof(float) * tp.pn.size();
     by[B_MIX4] = sizeof(double) * tp.ynorm.size();
     if (knobs.mfma_seed && NL > 0 && NP > 0) {
-      kd_seed_windows(M->data, NL, Mp->data, NP, Ea, SEED_WIN, tp.kdp, tp.wst, tp.tord);
+      kd_seed_windows(M->data, NL, Mp->data, NP, Ea, SEED_WIN, tp.kdp, tp.wst, tp.tord, tp.sgrp);
       tp.seed_win = std::min(NL, SEED_WIN);
       tp.kdm.resize((size_t)NL * Ea);
       for (int i = 0; i < NL; i++)
@@ -937,6 +945,7 @@ static int plan_task(const edm_manifold_view* M, const edm_manifold_view* Mp, co
       by[B_KDP] = sizeof(double) * tp.kdm.size();
       by[B_WST] = sizeof(int) * tp.wst.size();
       by[B_TORD] = sizeof(int) * tp.tord.size();
+      by[B_SGRP] = sizeof(int) * tp.sgrp.size();
       by[B_SEED] = sizeof(double) * (size_t)NP;
       // auto (3): the two-phase form where the seeds are near the final k-th distance, that is for points of
       // up to 16 coordinates and up to 32 neighbours (panel workload, E_a = 16, k = 18: select 39.6 -> 35.5
@@ -1184,6 +1193,8 @@ static int upload_task(const TaskPlan& tp, const edm_manifold_view* M, const edm
     if (e == hipSuccess)
       e = cp(B_TORD, tp.tord.data());
     if (e == hipSuccess)
+      e = cp(B_SGRP, tp.sgrp.data());
+    if (e == hipSuccess)
       e = cp(B_KD32, tp.kd32.data());
     if (e == hipSuccess)
       e = cp(B_KDN, tp.kdn.data());
@@ -1227,7 +1238,8 @@ static int launch_task(TaskPlan& tp, hipStream_t s, hipEvent_t* ev, int (*keep_g
     if (tp.route == R_MFMA) {
       if (tp.MP.seed)
         launch_seed(tp.kp, s, tp.MP, (const double*)tp.ptr[B_KDP], (const int*)tp.ptr[B_WST],
-                    (const int*)tp.ptr[B_TORD], tp.seed_win, tp.k, (double*)tp.ptr[B_SEED]);
+                    (const int*)tp.ptr[B_TORD], (const int*)tp.ptr[B_SGRP], (int)tp.sgrp.size() - 1, tp.seed_win,
+                    tp.k, (double*)tp.ptr[B_SEED]);
       if (tp.MP.seed && tp.two_phase) {
         int* fc = (int*)tp.ptr[B_FCNT];
         int* fl = (int*)tp.ptr[B_FLIST];

@@ -363,65 +363,102 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(mfma_waves(
 // over a window of `win` (<= 256) library rows that lie near the target in a k-d order of the library
 // (rows wst[t] .. wst[t] + win - 1 of that order), or +inf when the window has fewer than k.  The k-th over
 // a subset is never below the k-th over the whole library, so the seed bounds the true k-th distance
-// from above.  Distances are summed in the reference's order (src/distances.cpp:186-216), as in the
-// re-rank.  kdm holds the library rows in the k-d order, so a window is contiguous.  One wavefront per
-// target, four window rows per lane; targets are taken in window order (tord), so consecutive
-// wavefronts read overlapping rows.
+// from above.  Distances are summed in the reference's order (src/distances.cpp:186-216), operation for
+// operation as dist_f64 sums them in the re-rank.  kdm holds the library rows in the k-d order, so a
+// window is contiguous.  One wavefront per group of up to SEED_G consecutive targets of the window order
+// that share a window (groups start at sgrp[g]; sgrp[ngrp] = NP): each lane loads four window rows once and
+// sums their distances to every target of the group (the targets' coordinates in LDS), then each target
+// takes its k-th by k rounds of a wavefront minimum.
+constexpr int SEED_G = 4;
 template <int KP>
 __global__ __launch_bounds__(256) void knn_seed(MfmaParams P, const double* __restrict__ kdm,
-                                               const int* __restrict__ wst, const int* __restrict__ tord, int win,
-                                               int k, double* __restrict__ seed)
+                                               const int* __restrict__ wst, const int* __restrict__ tord,
+                                               const int* __restrict__ sgrp, int ngrp, int win, int k,
+                                               double* __restrict__ seed)
 {
-  const int lane = threadIdx.x & 63;
-  const int wi = blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (wi >= P.NP)
+  __shared__ double ys[4][SEED_G][KP];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int gi = blockIdx.x * 4 + wv;
+  if (gi >= ngrp)
     return;
-  const int t = tord[wi];
-  const int a = wst[t];
-  const double* y = P.Mp + (size_t)t * P.Ea;
-  double v[4];
+  const int s0 = sgrp[gi];
+  const int cnt = min(SEED_G, sgrp[gi + 1] - s0);
+  const int Ea = P.Ea;
+  const int a = wst[tord[s0]];
+  for (int e = lane; e < SEED_G * KP; e += 64) {
+    const int j = e / KP, d = e - j * KP;
+    ys[wv][j][d] = (j < cnt && d < Ea) ? P.Mp[(size_t)tord[s0 + j] * Ea + d] : 0.0;
+  }
+  lds_wave_sync();
+  double v[SEED_G][4];
 #pragma unroll
   for (int r = 0; r < 4; r++) {
-    const int j = lane + 64 * r;
-    double acc = INF_D;
-    if (j < win) {
-      const double d = dist_f64<KP>(kdm + (size_t)(a + j) * P.Ea, y, P.Ea);
-      acc = (d != 0.0 && d != MISSING_D) ? d : INF_D;
+    const int row = lane + 64 * r;
+    double acc[SEED_G];
+#pragma unroll
+    for (int j = 0; j < SEED_G; j++)
+      acc[j] = 0.0;
+    if (row < win) {
+      const double* x = kdm + (size_t)(a + row) * Ea;
+      constexpr int LB = 8;
+#pragma unroll 1
+      for (int d0 = 0; d0 < KP; d0 += LB) {
+        double xv[LB];
+#pragma unroll
+        for (int u = 0; u < LB; u++)
+          xv[u] = d0 + u < Ea ? x[d0 + u] : 0.0;
+#pragma unroll
+        for (int u = 0; u < LB; u++)
+          if (d0 + u < Ea)
+#pragma unroll
+            for (int j = 0; j < SEED_G; j++) {
+              const double dd = xv[u] - ys[wv][j][d0 + u];
+              acc[j] += dd * dd;
+            }
+      }
     }
-    v[r] = acc;
+#pragma unroll
+    for (int j = 0; j < SEED_G; j++)
+      v[j][r] = (row < win && acc[j] != 0.0 && acc[j] != MISSING_D) ? acc[j] : INF_D;
   }
-  // sort the lane's four values, then take k wave-wide minima of the lanes' heads
+  // per target: sort the lane's four values, then take k wave-wide minima of the lanes' heads
   auto cs = [](double& x, double& z) {
     const double lo = fmin(x, z), hi = fmax(x, z);
     x = lo;
     z = hi;
   };
-  cs(v[0], v[1]);
-  cs(v[2], v[3]);
-  cs(v[0], v[2]);
-  cs(v[1], v[3]);
-  cs(v[1], v[2]);
-  double kth = INF_D;
-  int h = 0;
-  for (int i = 0; i < k; i++) {
-    const double head = h == 0 ? v[0] : h == 1 ? v[1] : h == 2 ? v[2] : h == 3 ? v[3] : INF_D;
-    double m = head;
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1)
-      m = fmin(m, __shfl_xor(m, off));
-    kth = m;
-    if (m == INF_D)
+  for (int j = 0; j < SEED_G; j++) {
+    if (j >= cnt)
       break;
-    const unsigned long long b = __ballot(head == m);
-    if (lane == (int)__ffsll((long long)b) - 1)
-      h++;
+    cs(v[j][0], v[j][1]);
+    cs(v[j][2], v[j][3]);
+    cs(v[j][0], v[j][2]);
+    cs(v[j][1], v[j][3]);
+    cs(v[j][1], v[j][2]);
+    double kth = INF_D;
+    int h = 0;
+    for (int i = 0; i < k; i++) {
+      const double head = h == 0 ? v[j][0] : h == 1 ? v[j][1] : h == 2 ? v[j][2] : h == 3 ? v[j][3] : INF_D;
+      double m = head;
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1)
+        m = fmin(m, __shfl_xor(m, off));
+      kth = m;
+      if (m == INF_D)
+        break;
+      const unsigned long long bl = __ballot(head == m);
+      if (lane == (int)__ffsll((long long)bl) - 1)
+        h++;
+    }
+    if (lane == 0)
+      seed[tord[s0 + j]] = kth;
   }
-  if (lane == 0)
-    seed[t] = kth;
 }
 
 #define FEDM_SEED(KPv)                                                                                           \
-  template __global__ void knn_seed<KPv>(MfmaParams, const double*, const int*, const int*, int, int, double*);
+  template __global__ void knn_seed<KPv>(MfmaParams, const double*, const int*, const int*, const int*, int, int, int, \
+                                         double*);
 FEDM_SEED(16)
 FEDM_SEED(32)
 FEDM_SEED(64)
