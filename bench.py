@@ -145,7 +145,7 @@ def roofline(st, libs, reps):
     }
     # HBM bytes per launch from rocprofv3 FETCH_SIZE and WRITE_SIZE passes over a steady-state step,
     # scaled by the same counters on a 1 GiB copy (profiles/ccm_sweep_E3_k4.md)
-    traffic = {"direct": 0.66e9, "direct_pruned": 0.56e9, "rank_build": 0.16e9, "rank_scan": 1.00e9 + 0.98e9}
+    traffic = {"direct": 0.67e9, "direct_pruned": 0.54e9, "rank_build": 0.14e9, "rank_scan": 0.98e9 + 0.98e9}
     times = {"rank_build": st[1], "rank_scan": st[2], "direct": st[4], "direct_pruned": st[5]}
     name = max(times, key=lambda n: times[n])
     bound, amount, unit, peak, kernel, engine = work[name]
@@ -162,12 +162,12 @@ def roofline(st, libs, reps):
 
 # What the profiles show beside each kernel's roofline (profiles/ccm_sweep_E3_k4.md)
 NOTES = {
-    "rank_scan": "the measured HBM traffic (FETCH_SIZE + WRITE_SIZE, 1.98 GB) is below the algorithmic bytes: the "
+    "rank_scan": "the measured HBM traffic (FETCH_SIZE + WRITE_SIZE, 1.96 GB) is below the algorithmic bytes: the "
                  "prefix entries of one target are read once from HBM and reused from L2 by its 2 000 tasks; "
-                 "the scan is bound by VALU issue (83 % active), not by HBM bandwidth",
-    "direct_pruned": "VALU 67 % active; the brute-force pair unit counts pairs the triangle bound skips",
-    "direct": "VALU 81 % active; with libraries of 10..57 rows the list updates, not the distances, are the work",
-    "rank_build": "VALU 81 % active",
+                 "the scan is bound by VALU issue (84 % active), not by HBM bandwidth",
+    "direct_pruned": "VALU 66 % active; the brute-force pair unit counts pairs the triangle bound skips",
+    "direct": "VALU 82 % active; with libraries of 10..57 rows the list updates, not the distances, are the work",
+    "rank_build": "VALU 78 % active",
 }
 
 
