@@ -29,7 +29,7 @@ def ccm_like(seed, n, Ea, sizes, reps, integer=False, offset=0.0):
 
 @pytest.mark.parametrize("precision", [0, 1], ids=["plain", "mixed"])
 @pytest.mark.parametrize("case", ["E3_k4", "E5_k6_integer", "E8_k9_thetas", "tiny_libraries", "E3_k4_long",
-                                  "E3_k4_offset", "E20_k12_chunks"])
+                                  "E3_k4_offset", "E20_k12_chunks", "E3_k4_ten_thetas"])
 def test_batch_matches_oracle(precision, case):
     fe = load_product()
     spec = {"E3_k4": (3, 4, [10, 37, 150, 900], 3, False, (1.0,)),
@@ -41,7 +41,11 @@ def test_batch_matches_oracle(precision, case):
             # more than the k-th distances, so the direct kernel's expanded screen holds only with its bound
             "E3_k4_offset": (3, 4, [10, 37, 150, 900], 2, False, (1.0,)),
             # the 32-wide direct kernel: LDS chunks of 96 rows, so the longer libraries span several
-            "E20_k12_chunks": (20, 12, [14, 30, 150, 400], 2, False, (0.0, 2.0))}[case]
+            "E20_k12_chunks": (20, 12, [14, 30, 150, 400], 2, False, (0.0, 2.0)),
+            # more thetas than the kernels' per-call table of exp(-theta) (eight): the nearest neighbour's
+            # weight of the last two comes from the per-prediction exp (direct, pruned and rank-table
+            # libraries)
+            "E3_k4_ten_thetas": (3, 4, [10, 37, 80, 150, 900], 2, False, tuple(np.linspace(0.0, 4.5, 10)))}[case]
     Ea, k, sizes, reps, integer, thetas = spec
     n = 3000 if case == "E3_k4_long" else 1500
     offset = 3.0e7 if case == "E3_k4_offset" else 0.0
