@@ -2156,7 +2156,7 @@ int edm_batch_create_knobs(edm_ctx* c, const edm_manifold_view* L, const edm_man
     b->handed_host = nullptr;
   B.counter = b->counters;
   // targets whose squared-distance ranking may differ from the sqrt ranking (rare; see kernels_batch.hip)
-  B.redo_cap = (int)std::min<int64_t>(b->total_k, 1 << 16);
+  B.redo_cap = (int)std::min<int64_t>(b->total_k, 1 << 20);
   B.redo = b->dev<int2>(nullptr, (size_t)std::max(B.redo_cap, 1));
   B.redo_count = b->counters + PR_CSTRIDE;
   B.thetas = b->dev(o->thetas, (size_t)o->num_thetas);
