@@ -2546,7 +2546,7 @@ static int enqueue_device(edm_batch* b, bool timed)
     RedoLists L{ { B.items, b->rank_route ? b->S.items : nullptr, B.pitems },
                  { B.num_items, b->rank_route ? b->S.num_items : 0, B.num_pitems } };
     void* rargs[] = { &B, &L };
-    HIP_TRY(hipLaunchKernel(batch_redo_kernel(b->kcap), dim3(256), dim3(256), rargs, 0, s));
+    HIP_TRY(hipLaunchKernel(batch_redo_kernel(b->kcap), dim3(std::max(b->grid, 256)), dim3(256), rargs, 0, s));
   }
   return 0;
 }
