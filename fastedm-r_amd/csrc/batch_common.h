@@ -74,8 +74,10 @@ struct RankParams
   const double* L;   // point table [rows x Ea] FP64
   const float* L32;  // FP32 copy
   const double* P;   // prediction-side table
-  const int* pool;   // pool rows (ascending) [npool]
+  const int* pool;   // pool rows [npool], in a k-d order of their points
   const float* pool32; // FP32 coordinates of the pool rows, in pool order, zero padded to EMAX [npool x EMAX]
+  const float* cbox;   // per 64-row chunk of the pool order: Ea lower then Ea upper bounds of the FP64
+                       // coordinates, rounded outward (null: every chunk is screened)
   int npool;
   const int* pred;   // prediction rows [np]
   int np, Ea;

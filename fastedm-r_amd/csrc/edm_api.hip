@@ -2294,6 +2294,39 @@ int edm_batch_create_knobs(edm_ctx* c, const edm_manifold_view* L, const edm_man
     R.L = B.L;
     R.L32 = B.L32;
     R.P = B.P;
+    {
+      // the pool in the k-d order of the seed windows (its 64-row chunks then cover compact regions), with
+      // each chunk's box: rank_build screens only the chunks that may hold a row within its cut
+      std::vector<double> pts(pool.size() * (size_t)Ea);
+      for (size_t q = 0; q < pool.size(); q++)
+        std::copy(b->Ldata.begin() + (size_t)pool[q] * Ea, b->Ldata.begin() + (size_t)(pool[q] + 1) * Ea,
+                  pts.begin() + q * Ea);
+      std::vector<int> kdp, wst, tord, sgrp;
+      kd_seed_windows(pts.data(), (int)pool.size(), nullptr, 0, Ea, SEED_WIN, kdp, wst, tord, sgrp);
+      std::vector<int> kpool(pool.size());
+      for (size_t q = 0; q < pool.size(); q++)
+        kpool[q] = pool[kdp[q]];
+      pool.swap(kpool);
+      const size_t nch = (pool.size() + 63) / 64;
+      std::vector<float> cbox(nch * 2 * Ea);
+      for (size_t c = 0; c < nch; c++)
+        for (int j = 0; j < Ea; j++) {
+          double l = INFINITY, h = -INFINITY;
+          for (size_t q = 64 * c; q < std::min(pool.size(), 64 * c + 64); q++) {
+            const double v = b->Ldata[(size_t)pool[q] * Ea + j];
+            l = std::min(l, v);
+            h = std::max(h, v);
+          }
+          float fl = (float)l, fh = (float)h;
+          if ((double)fl > l)
+            fl = std::nextafter(fl, -INFINITY);
+          if ((double)fh < h)
+            fh = std::nextafter(fh, INFINITY);
+          cbox[c * 2 * Ea + j] = fl;
+          cbox[c * 2 * Ea + Ea + j] = fh;
+        }
+      R.cbox = b->dev(cbox.data(), cbox.size());
+    }
     R.pool = b->dev(pool.data(), pool.size());
     {
       // the pool rows' FP32 coordinates in pool order, padded to the kernel's width (16-byte rows)

@@ -46,7 +46,7 @@ __global__ __launch_bounds__(256) void rank_build(RankParams R)
   __shared__ int ord[RANK_CAP];
   __shared__ unsigned bcnt[RANK_NB], bstart[RANK_NB + 1];
   __shared__ unsigned bmaxc;
-  __shared__ int ncand, nzero, overflow, cut_bin, all_rows;
+  __shared__ int ncand, nzero, overflow, cut_bin, all_rows, nlist;
   __shared__ unsigned wsum[4];
   __shared__ double smax_sh;
   __shared__ int zr[RANK_ZCAP];
@@ -179,35 +179,69 @@ __global__ __launch_bounds__(256) void rank_build(RankParams R)
       overflow = 0;
     }
     __syncthreads();
-    for (int q0 = threadIdx.x; q0 < R.npool; q0 += UNR * blockDim.x) {
-      float d[UNR];
-#pragma unroll
-      for (int u = 0; u < UNR; u++) {
-        const int q = q0 + u * blockDim.x;
-        const float dq = d32_of(min(q, R.npool - 1)); // unconditional, as in pass 1
-        d[u] = q < R.npool ? dq : __builtin_huge_valf();
+    // The pool is in a k-d order, in 64-row chunks with boxes: the chunks of each window of RANK_CAP whose
+    // box may hold a row within the cut are listed (in ord, free until the sort), and only their rows are
+    // screened.  The squared gap between the target and a chunk's box (outward-rounded FP32 bounds, summed
+    // in FP64 and lowered by 2^-40 of itself) bounds the FP64 distance of every row of the chunk from below,
+    // so a chunk left out holds no candidate and no zero distance.
+    const int nch = (R.npool + 63) / 64;
+    for (int cw = 0; cw < nch; cw += RANK_CAP) {
+      const int ce = min(nch, cw + RANK_CAP);
+      if (threadIdx.x == 0)
+        nlist = 0;
+      __syncthreads();
+      for (int c = cw + (int)threadIdx.x; c < ce; c += blockDim.x) {
+        bool keep = true;
+        if (!all && R.cbox) {
+          const float* bx = R.cbox + (size_t)c * 2 * Ea;
+          double lb = 0.0;
+          for (int j = 0; j < Ea; j++) {
+            const double gp = fmax(0.0, fmax((double)bx[j] - y[j], y[j] - (double)bx[Ea + j]));
+            lb += gp * gp;
+          }
+          keep = lb - lb * 0x1p-40 <= tau;
+        }
+        if (keep)
+          ord[atomicAdd(&nlist, 1)] = c;
       }
+      __syncthreads();
+      const int nl = nlist;
+      // rows of the listed chunks: a wavefront takes one chunk (a row per lane), UNR chunks at a time
+      const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+      for (int i0 = 0; i0 < nl; i0 += UNR * 4) {
+        float d[UNR];
+        int qs[UNR];
 #pragma unroll
-      for (int u = 0; u < UNR; u++) {
-        const int q = q0 + u * blockDim.x;
-        if (q >= R.npool || !(d[u] <= T1))
-          continue;
-        const int row = R.pool[q];
-        const double acc = d64_of(row);
-        if (acc == 0.0) {
-          const int z = atomicAdd(&nzero, 1);
-          if (z < RANK_ZCAP)
-            zr[z] = row;
-        } else if (acc <= tau) {
-          const int c = atomicAdd(&ncand, 1);
-          if (c < RANK_CAP) {
-            cs[c] = sqrt(acc);
-            cq[c] = row;
-          } else {
-            overflow = 1;
+        for (int u = 0; u < UNR; u++) {
+          const int li = i0 + 4 * u + wv;
+          const int q = li < nl ? ord[li] * 64 + lane : R.npool;
+          qs[u] = q;
+          const float dq = d32_of(min(q, R.npool - 1)); // unconditional, as in pass 1
+          d[u] = q < R.npool ? dq : __builtin_huge_valf();
+        }
+#pragma unroll
+        for (int u = 0; u < UNR; u++) {
+          const int q = qs[u];
+          if (q >= R.npool || !(d[u] <= T1))
+            continue;
+          const int row = R.pool[q];
+          const double acc = d64_of(row);
+          if (acc == 0.0) {
+            const int z = atomicAdd(&nzero, 1);
+            if (z < RANK_ZCAP)
+              zr[z] = row;
+          } else if (acc <= tau) {
+            const int c = atomicAdd(&ncand, 1);
+            if (c < RANK_CAP) {
+              cs[c] = sqrt(acc);
+              cq[c] = row;
+            } else {
+              overflow = 1;
+            }
           }
         }
       }
+      __syncthreads(); // ord is listed again for the next window
     }
     __syncthreads();
     if (!overflow)
