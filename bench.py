@@ -164,10 +164,10 @@ def roofline(st, libs, reps):
 NOTES = {
     "rank_scan": "the measured HBM traffic (FETCH_SIZE + WRITE_SIZE, 1.96 GB) is below the algorithmic bytes: the "
                  "prefix entries of one target are read once from HBM and reused from L2 by its 2 000 tasks; "
-                 "the scan is bound by VALU issue (84 % active), not by HBM bandwidth",
+                 "the scan is bound by VALU issue (83 % active), not by HBM bandwidth",
     "direct_pruned": "VALU 66 % active; the brute-force pair unit counts pairs the triangle bound skips",
     "direct": "VALU 82 % active; with libraries of 10..57 rows the list updates, not the distances, are the work",
-    "rank_build": "VALU 78 % active",
+    "rank_build": "VALU 57 % active; the pair unit counts the pairs of the pool chunks its box test skips",
 }
 
 
