@@ -45,6 +45,9 @@ __global__ void knn_mfma(MfmaParams P, int k, int slice_len, double* out_s, int*
 template <int KP>
 __global__ void knn_seed(MfmaParams P, const double* kdm, const int* wst, const int* tord, const int* sgrp, int ngrp,
                          int win, int k, double* seed);
+template <int KP, int R>
+__global__ void knn_seed_wide(MfmaParams P, const double* kdm, const int* wst, int win, const int* fl, const int* fc,
+                              int k, double* seed);
 template <int KP>
 __global__ void knn_screen(MfmaParams P, KdScreen K, int split, int* cand, int* ccnt);
 template <int KCAP, int KP>
@@ -264,6 +267,31 @@ static void launch_seed(int kp, hipStream_t s, const MfmaParams& P, const double
     hipLaunchKernelGGL((knn_seed<32>), g, dim3(256), 0, s, P, kdp, wst, tord, sgrp, ngrp, win, k, seed);
   else
     hipLaunchKernelGGL((knn_seed<64>), g, dim3(256), 0, s, P, kdp, wst, tord, sgrp, ngrp, win, k, seed);
+}
+
+// The wider seed over 64 R rows per target: R = 16 for the few targets the two-phase form hands over, 64
+// before the seeded streaming kernel, where every target takes it.
+template <int R>
+static void launch_seed_wide_r(int kp, hipStream_t s, const MfmaParams& P, const double* kdp, const int* wst, int win,
+                               const int* fl, const int* fc, int k, double* seed)
+{
+  // one wavefront per target for up to 16 384 targets (a listed target count is known on the device only)
+  const dim3 g(std::max(1, std::min(4096, (P.NP + 3) / 4)));
+  if (kp == 16)
+    hipLaunchKernelGGL((knn_seed_wide<16, R>), g, dim3(256), 0, s, P, kdp, wst, win, fl, fc, k, seed);
+  else if (kp == 32)
+    hipLaunchKernelGGL((knn_seed_wide<32, R>), g, dim3(256), 0, s, P, kdp, wst, win, fl, fc, k, seed);
+  else
+    hipLaunchKernelGGL((knn_seed_wide<64, R>), g, dim3(256), 0, s, P, kdp, wst, win, fl, fc, k, seed);
+}
+
+static void launch_seed_wide(int kp, hipStream_t s, const MfmaParams& P, const double* kdp, const int* wst, int win,
+                             const int* fl, const int* fc, int k, double* seed)
+{
+  if (fl)
+    launch_seed_wide_r<16>(kp, s, P, kdp, wst, win, fl, fc, k, seed);
+  else
+    launch_seed_wide_r<64>(kp, s, P, kdp, wst, win, fl, fc, k, seed);
 }
 
 // Phases one and two of the two-phase matrix-core route: the seeded screen of every slice, then the exact
@@ -1250,14 +1278,19 @@ static int launch_task(TaskPlan& tp, hipStream_t s, hipEvent_t* ev, int (*keep_g
         launch_two_phase(tp.kcap, tp.kp, s, tp.MP, tp.KS, tp.split, tp.k, tp.S, tp.sl, (int*)tp.ptr[B_CAND],
                          (int*)tp.ptr[B_CCNT],
                          ps, pi, pc, ff, fl, fc);
-        // the targets that overflowed: the streaming kernel over sf sub-slices of the library, then merged
-        // into the route's per-slice lists
         double* fs = (double*)tp.ptr[B_FPS];
         int* fi = (int*)tp.ptr[B_FPI];
         int* fcn = (int*)tp.ptr[B_FPC];
+        // the targets that overflowed: a wider seed, the streaming kernel over sf sub-slices of the library,
+        // then merged into the route's per-slice lists
+        launch_seed_wide(tp.kp, s, tp.MP, (const double*)tp.ptr[B_KDP], (const int*)tp.ptr[B_WST], tp.seed_win, fl, fc,
+                         tp.k, (double*)tp.ptr[B_SEED]);
         launch_mfma(tp.kcap, tp.kp, dim3(tp.bx, tp.sf), s, tp.MP, tp.k, (tp.NL + tp.sf - 1) / tp.sf, fs, fi, fcn, fl, fc);
         launch_merge_fallback(tp.kcap, s, NP, tp.S, tp.sf, fl, fc, fs, fi, fcn, ps, pi, pc);
       } else {
+        if (tp.MP.seed)
+          launch_seed_wide(tp.kp, s, tp.MP, (const double*)tp.ptr[B_KDP], (const int*)tp.ptr[B_WST], tp.seed_win,
+                           nullptr, nullptr, tp.k, (double*)tp.ptr[B_SEED]);
         launch_mfma(tp.kcap, tp.kp, dim3(tp.bx, tp.S), s, tp.MP, tp.k, tp.sl, ps, pi, pc);
       }
     }

@@ -463,6 +463,95 @@ FEDM_SEED(16)
 FEDM_SEED(32)
 FEDM_SEED(64)
 
+// A wider seed: the k-th smallest nonzero exact squared distance over 64 R rows of the k-d order centred on
+// the target's first window (rows wst[t] .. wst[t] + win - 1), kept when below the target's seed.  Both are
+// k-th smallest values over subsets of the library, so the smaller still bounds the true k-th distance from
+// above, and the streaming kernel's screen starts from a floor nearer to it.  For the targets the two-phase
+// form hands to the streaming kernel (list fl, *fc entries), or for every target (fl null) before the
+// seeded streaming kernel.  Distances are summed as in knn_seed.  One wavefront per target (grid-stride);
+// lane l holds rows l, l + 64, ... of the window, and the k-th smallest is taken by k rounds of a
+// wavefront minimum over the lanes' smallest values not yet taken.
+template <int KP, int R>
+__global__ __launch_bounds__(256) void knn_seed_wide(MfmaParams P, const double* __restrict__ kdm,
+                                                    const int* __restrict__ wst, int win, const int* __restrict__ fl,
+                                                    const int* __restrict__ fc, int k, double* __restrict__ seed)
+{
+  __shared__ double ys[4][KP];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int nlist = fl ? *fc : P.NP;
+  const int Ea = P.Ea;
+  static_assert(R <= 64, "one bit per row of a lane");
+  constexpr int WW = 64 * R;
+  const int ww = min(WW, P.NL);
+  for (int e = blockIdx.x * 4 + wv; e < nlist; e += gridDim.x * 4) {
+    const int t = fl ? fl[e] : e;
+    const int a = max(0, min(P.NL - ww, wst[t] + win / 2 - ww / 2));
+    for (int d = lane; d < KP; d += 64)
+      ys[wv][d] = d < Ea ? P.Mp[(size_t)t * Ea + d] : 0.0;
+    lds_wave_sync();
+    double v[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      const int row = lane + 64 * r;
+      double acc = 0.0;
+      if (row < ww) {
+        const double* x = kdm + (size_t)(a + row) * Ea;
+        constexpr int LB = 8;
+#pragma unroll 1
+        for (int d0 = 0; d0 < KP; d0 += LB) {
+          double xv[LB];
+#pragma unroll
+          for (int u = 0; u < LB; u++)
+            xv[u] = d0 + u < Ea ? x[d0 + u] : 0.0;
+#pragma unroll
+          for (int u = 0; u < LB; u++)
+            if (d0 + u < Ea) {
+              const double dd = xv[u] - ys[wv][d0 + u];
+              acc += dd * dd;
+            }
+        }
+      }
+      v[r] = (row < ww && acc != 0.0 && acc != MISSING_D) ? acc : INF_D;
+    }
+    // k rounds: the lanes' smallest untaken values, their wavefront minimum, taken by the first lane holding it
+    unsigned long long taken = 0ull;
+    double kth = INF_D;
+    for (int i = 0; i < k; i++) {
+      double h = INF_D;
+      int hr = 0;
+#pragma unroll
+      for (int r = 0; r < R; r++)
+        if (!((taken >> r) & 1ull) && v[r] < h) {
+          h = v[r];
+          hr = r;
+        }
+      double m = h;
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1)
+        m = fmin(m, __shfl_xor(m, off));
+      kth = m;
+      if (m == INF_D)
+        break;
+      const unsigned long long bl = __ballot(h == m);
+      if (lane == (int)__ffsll((long long)bl) - 1)
+        taken |= 1ull << hr;
+    }
+    if (lane == 0 && kth < seed[t])
+      seed[t] = kth;
+    lds_wave_sync();
+  }
+}
+
+#define FEDM_SEEDW(KPv, Rv)                                                                                      \
+  template __global__ void knn_seed_wide<KPv, Rv>(MfmaParams, const double*, const int*, int, const int*, const int*,  \
+                                                  int, double*);
+FEDM_SEEDW(16, 16)
+FEDM_SEEDW(32, 16)
+FEDM_SEEDW(64, 16)
+FEDM_SEEDW(16, 64)
+FEDM_SEEDW(32, 64)
+FEDM_SEEDW(64, 64)
+
 // Two-phase form of the seeded matrix-core route.  Phase one (knn_screen) screens the (target, library
 // point) pairs against the target's fixed seeded floor and records the positions that pass, up to SCR_CAP
 // per target; phase two (knn_rerank) recomputes the recorded pairs in FP64 and keeps the exact (sqrt
