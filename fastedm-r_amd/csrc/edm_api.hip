@@ -44,7 +44,7 @@ __global__ void knn_mfma(MfmaParams P, int k, int slice_len, double* out_s, int*
                          const int* tn);
 template <int KP>
 __global__ void knn_seed(MfmaParams P, const double* kdm, const int* wst, const int* tord, const int* sgrp, int ngrp,
-                         int win, int k, double* seed);
+                         int win, int wide, int k, double* seed);
 template <int KP, int R>
 __global__ void knn_seed_wide(MfmaParams P, const double* kdm, const int* wst, int win, const int* fl, const int* fc,
                               int k, double* seed);
@@ -258,40 +258,29 @@ static void launch_mfma_k(int kp, dim3 g, hipStream_t s, const MfmaParams& P, in
 }
 
 static void launch_seed(int kp, hipStream_t s, const MfmaParams& P, const double* kdp, const int* wst,
-                        const int* tord, const int* sgrp, int ngrp, int win, int k, double* seed)
+                        const int* tord, const int* sgrp, int ngrp, int win, int wide, int k, double* seed)
 {
   const dim3 g((ngrp + 3) / 4);
   if (kp == 16)
-    hipLaunchKernelGGL((knn_seed<16>), g, dim3(256), 0, s, P, kdp, wst, tord, sgrp, ngrp, win, k, seed);
+    hipLaunchKernelGGL((knn_seed<16>), g, dim3(256), 0, s, P, kdp, wst, tord, sgrp, ngrp, win, wide, k, seed);
   else if (kp == 32)
-    hipLaunchKernelGGL((knn_seed<32>), g, dim3(256), 0, s, P, kdp, wst, tord, sgrp, ngrp, win, k, seed);
+    hipLaunchKernelGGL((knn_seed<32>), g, dim3(256), 0, s, P, kdp, wst, tord, sgrp, ngrp, win, wide, k, seed);
   else
-    hipLaunchKernelGGL((knn_seed<64>), g, dim3(256), 0, s, P, kdp, wst, tord, sgrp, ngrp, win, k, seed);
+    hipLaunchKernelGGL((knn_seed<64>), g, dim3(256), 0, s, P, kdp, wst, tord, sgrp, ngrp, win, wide, k, seed);
 }
 
-// The wider seed over 64 R rows per target: R = 16 for the few targets the two-phase form hands over, 64
-// before the seeded streaming kernel, where every target takes it.
-template <int R>
-static void launch_seed_wide_r(int kp, hipStream_t s, const MfmaParams& P, const double* kdp, const int* wst, int win,
-                               const int* fl, const int* fc, int k, double* seed)
-{
-  // one wavefront per target for up to 16 384 targets (a listed target count is known on the device only)
-  const dim3 g(std::max(1, std::min(4096, (P.NP + 3) / 4)));
-  if (kp == 16)
-    hipLaunchKernelGGL((knn_seed_wide<16, R>), g, dim3(256), 0, s, P, kdp, wst, win, fl, fc, k, seed);
-  else if (kp == 32)
-    hipLaunchKernelGGL((knn_seed_wide<32, R>), g, dim3(256), 0, s, P, kdp, wst, win, fl, fc, k, seed);
-  else
-    hipLaunchKernelGGL((knn_seed_wide<64, R>), g, dim3(256), 0, s, P, kdp, wst, win, fl, fc, k, seed);
-}
-
+// The wider seed of the targets the two-phase form hands over (list fl, *fc entries on the device): 1 024
+// rows each, one wavefront per target for up to 16 384 targets.
 static void launch_seed_wide(int kp, hipStream_t s, const MfmaParams& P, const double* kdp, const int* wst, int win,
                              const int* fl, const int* fc, int k, double* seed)
 {
-  if (fl)
-    launch_seed_wide_r<16>(kp, s, P, kdp, wst, win, fl, fc, k, seed);
+  const dim3 g(std::max(1, std::min(4096, (P.NP + 3) / 4)));
+  if (kp == 16)
+    hipLaunchKernelGGL((knn_seed_wide<16, 16>), g, dim3(256), 0, s, P, kdp, wst, win, fl, fc, k, seed);
+  else if (kp == 32)
+    hipLaunchKernelGGL((knn_seed_wide<32, 16>), g, dim3(256), 0, s, P, kdp, wst, win, fl, fc, k, seed);
   else
-    launch_seed_wide_r<64>(kp, s, P, kdp, wst, win, fl, fc, k, seed);
+    hipLaunchKernelGGL((knn_seed_wide<64, 16>), g, dim3(256), 0, s, P, kdp, wst, win, fl, fc, k, seed);
 }
 
 // Phases one and two of the two-phase matrix-core route: the seeded screen of every slice, then the exact
@@ -645,6 +634,9 @@ struct TaskPlan
 };
 
 constexpr int SEED_WIN = 256; // library rows per seed window of the matrix-core route (knn_seed: 4 per lane)
+// library rows the seed scans around each window before the seeded streaming kernel (the two-phase form
+// scans the window only: its seeds are already near the k-th distance)
+constexpr int SEED_WIDE = 4096;
 
 // Seed windows of the matrix-core route.  A k-d order of the library rows: a node splits at the median of
 // its coordinate of largest spread, with every row equal to the median on one side (so a coordinate that
@@ -1267,6 +1259,7 @@ static int launch_task(TaskPlan& tp, hipStream_t s, hipEvent_t* ev, int (*keep_g
       if (tp.MP.seed)
         launch_seed(tp.kp, s, tp.MP, (const double*)tp.ptr[B_KDP], (const int*)tp.ptr[B_WST],
                     (const int*)tp.ptr[B_TORD], (const int*)tp.ptr[B_SGRP], (int)tp.sgrp.size() - 1, tp.seed_win,
+                    tp.two_phase ? tp.seed_win : SEED_WIDE,
                     tp.k, (double*)tp.ptr[B_SEED]);
       if (tp.MP.seed && tp.two_phase) {
         int* fc = (int*)tp.ptr[B_FCNT];
@@ -1288,9 +1281,6 @@ static int launch_task(TaskPlan& tp, hipStream_t s, hipEvent_t* ev, int (*keep_g
         launch_mfma(tp.kcap, tp.kp, dim3(tp.bx, tp.sf), s, tp.MP, tp.k, (tp.NL + tp.sf - 1) / tp.sf, fs, fi, fcn, fl, fc);
         launch_merge_fallback(tp.kcap, s, NP, tp.S, tp.sf, fl, fc, fs, fi, fcn, ps, pi, pc);
       } else {
-        if (tp.MP.seed)
-          launch_seed_wide(tp.kp, s, tp.MP, (const double*)tp.ptr[B_KDP], (const int*)tp.ptr[B_WST], tp.seed_win,
-                           nullptr, nullptr, tp.k, (double*)tp.ptr[B_SEED]);
         launch_mfma(tp.kcap, tp.kp, dim3(tp.bx, tp.S), s, tp.MP, tp.k, tp.sl, ps, pi, pc);
       }
     }

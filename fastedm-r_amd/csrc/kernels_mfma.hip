@@ -360,20 +360,23 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(mfma_waves(
 }
 
 // Seed of knn_mfma's screening threshold: for each target, the k-th smallest nonzero exact squared distance
-// over a window of `win` (<= 256) library rows that lie near the target in a k-d order of the library
-// (rows wst[t] .. wst[t] + win - 1 of that order), or +inf when the window has fewer than k.  The k-th over
-// a subset is never below the k-th over the whole library, so the seed bounds the true k-th distance
-// from above.  Distances are summed in the reference's order (src/distances.cpp:186-216), operation for
-// operation as dist_f64 sums them in the re-rank.  kdm holds the library rows in the k-d order, so a
-// window is contiguous.  One wavefront per group of up to SEED_G consecutive targets of the window order
-// that share a window (groups start at sgrp[g]; sgrp[ngrp] = NP): each lane loads four window rows once and
-// sums their distances to every target of the group (the targets' coordinates in LDS), then each target
-// takes its k-th by k rounds of a wavefront minimum.
+// over rows of a k-d order of the library near the target, or +inf when fewer than k are nonzero.  The
+// target's window is rows wst[t] .. wst[t] + win - 1 of that order (win <= 256); with wide > win the rows
+// scanned are `wide` rows centred on that window.  Lane l takes rows l, l + 64, ... and keeps, per target,
+// its four smallest values; the seed is the k-th smallest of what the lanes keep.  That is the k-th
+// smallest over a subset of the library, so it bounds the true k-th distance from above (at wide = win
+// every lane holds at most four rows, so it is the window's k-th).  Distances are summed in the
+// reference's order (src/distances.cpp:186-216), operation for operation as dist_f64 sums them in the
+// re-rank.  kdm holds the library rows in the k-d order, so a window is contiguous.  One wavefront per
+// group of up to SEED_G consecutive targets of the window order that share a window (groups start at
+// sgrp[g]; sgrp[ngrp] = NP): each lane loads its rows once and sums their distances to every target of the
+// group (the targets' coordinates in LDS), then each target takes its k-th by k rounds of a wavefront
+// minimum over the lanes' kept values.
 constexpr int SEED_G = 4;
 template <int KP>
 __global__ __launch_bounds__(256) void knn_seed(MfmaParams P, const double* __restrict__ kdm,
                                                const int* __restrict__ wst, const int* __restrict__ tord,
-                                               const int* __restrict__ sgrp, int ngrp, int win, int k,
+                                               const int* __restrict__ sgrp, int ngrp, int win, int wide, int k,
                                                double* __restrict__ seed)
 {
   __shared__ double ys[4][SEED_G][KP];
@@ -384,58 +387,60 @@ __global__ __launch_bounds__(256) void knn_seed(MfmaParams P, const double* __re
   const int s0 = sgrp[gi];
   const int cnt = min(SEED_G, sgrp[gi + 1] - s0);
   const int Ea = P.Ea;
-  const int a = wst[tord[s0]];
+  const int nr = max(win, min(wide, P.NL));
+  const int a = nr == win ? wst[tord[s0]] : max(0, min(P.NL - nr, wst[tord[s0]] + win / 2 - nr / 2));
   for (int e = lane; e < SEED_G * KP; e += 64) {
     const int j = e / KP, d = e - j * KP;
     ys[wv][j][d] = (j < cnt && d < Ea) ? P.Mp[(size_t)tord[s0 + j] * Ea + d] : 0.0;
   }
   lds_wave_sync();
-  double v[SEED_G][4];
-#pragma unroll
-  for (int r = 0; r < 4; r++) {
-    const int row = lane + 64 * r;
-    double acc[SEED_G];
-#pragma unroll
-    for (int j = 0; j < SEED_G; j++)
-      acc[j] = 0.0;
-    if (row < win) {
-      const double* x = kdm + (size_t)(a + row) * Ea;
-      constexpr int LB = 8;
-#pragma unroll 1
-      for (int d0 = 0; d0 < KP; d0 += LB) {
-        double xv[LB];
-#pragma unroll
-        for (int u = 0; u < LB; u++)
-          xv[u] = d0 + u < Ea ? x[d0 + u] : 0.0;
-#pragma unroll
-        for (int u = 0; u < LB; u++)
-          if (d0 + u < Ea)
-#pragma unroll
-            for (int j = 0; j < SEED_G; j++) {
-              const double dd = xv[u] - ys[wv][j][d0 + u];
-              acc[j] += dd * dd;
-            }
-      }
-    }
-#pragma unroll
-    for (int j = 0; j < SEED_G; j++)
-      v[j][r] = (row < win && acc[j] != 0.0 && acc[j] != MISSING_D) ? acc[j] : INF_D;
-  }
-  // per target: sort the lane's four values, then take k wave-wide minima of the lanes' heads
   auto cs = [](double& x, double& z) {
     const double lo = fmin(x, z), hi = fmax(x, z);
     x = lo;
     z = hi;
   };
+  // per target, the lane's four smallest values in increasing order
+  double v[SEED_G][4];
+#pragma unroll
+  for (int j = 0; j < SEED_G; j++)
+#pragma unroll
+    for (int r = 0; r < 4; r++)
+      v[j][r] = INF_D;
+  for (int row = lane; row < nr; row += 64) {
+    double acc[SEED_G];
+#pragma unroll
+    for (int j = 0; j < SEED_G; j++)
+      acc[j] = 0.0;
+    const double* x = kdm + (size_t)(a + row) * Ea;
+    constexpr int LB = 8;
+#pragma unroll 1
+    for (int d0 = 0; d0 < KP; d0 += LB) {
+      double xv[LB];
+#pragma unroll
+      for (int u = 0; u < LB; u++)
+        xv[u] = d0 + u < Ea ? x[d0 + u] : 0.0;
+#pragma unroll
+      for (int u = 0; u < LB; u++)
+        if (d0 + u < Ea)
+#pragma unroll
+          for (int j = 0; j < SEED_G; j++) {
+            const double dd = xv[u] - ys[wv][j][d0 + u];
+            acc[j] += dd * dd;
+          }
+    }
+#pragma unroll
+    for (int j = 0; j < SEED_G; j++) {
+      v[j][3] = fmin(v[j][3], (acc[j] != 0.0 && acc[j] != MISSING_D) ? acc[j] : INF_D);
+      cs(v[j][2], v[j][3]);
+      cs(v[j][1], v[j][2]);
+      cs(v[j][0], v[j][1]);
+    }
+  }
+  // per target: k wave-wide minima of the lanes' heads
 #pragma unroll
   for (int j = 0; j < SEED_G; j++) {
     if (j >= cnt)
       break;
-    cs(v[j][0], v[j][1]);
-    cs(v[j][2], v[j][3]);
-    cs(v[j][0], v[j][2]);
-    cs(v[j][1], v[j][3]);
-    cs(v[j][1], v[j][2]);
     double kth = INF_D;
     int h = 0;
     for (int i = 0; i < k; i++) {
@@ -458,17 +463,16 @@ __global__ __launch_bounds__(256) void knn_seed(MfmaParams P, const double* __re
 
 #define FEDM_SEED(KPv)                                                                                           \
   template __global__ void knn_seed<KPv>(MfmaParams, const double*, const int*, const int*, const int*, int, int, int, \
-                                         double*);
+                                         int, double*);
 FEDM_SEED(16)
 FEDM_SEED(32)
 FEDM_SEED(64)
 
-// A wider seed: the k-th smallest nonzero exact squared distance over 64 R rows of the k-d order centred on
-// the target's first window (rows wst[t] .. wst[t] + win - 1), kept when below the target's seed.  Both are
-// k-th smallest values over subsets of the library, so the smaller still bounds the true k-th distance from
-// above, and the streaming kernel's screen starts from a floor nearer to it.  For the targets the two-phase
-// form hands to the streaming kernel (list fl, *fc entries), or for every target (fl null) before the
-// seeded streaming kernel.  Distances are summed as in knn_seed.  One wavefront per target (grid-stride);
+// A wider seed for the targets the two-phase form hands to the streaming kernel (list fl, *fc entries): the
+// k-th smallest nonzero exact squared distance over 64 R rows of the k-d order centred on the target's
+// window (rows wst[t] .. wst[t] + win - 1), kept when below the target's seed.  Both are k-th smallest
+// values over subsets of the library, so the smaller still bounds the true k-th distance from above, and
+// the streaming kernel's screen starts from a floor nearer to it.  Distances are summed as in knn_seed.  One wavefront per target (grid-stride);
 // lane l holds rows l, l + 64, ... of the window, and the k-th smallest is taken by k rounds of a
 // wavefront minimum over the lanes' smallest values not yet taken.
 template <int KP, int R>
@@ -478,13 +482,13 @@ __global__ __launch_bounds__(256) void knn_seed_wide(MfmaParams P, const double*
 {
   __shared__ double ys[4][KP];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int nlist = fl ? *fc : P.NP;
+  const int nlist = *fc;
   const int Ea = P.Ea;
   static_assert(R <= 64, "one bit per row of a lane");
   constexpr int WW = 64 * R;
   const int ww = min(WW, P.NL);
   for (int e = blockIdx.x * 4 + wv; e < nlist; e += gridDim.x * 4) {
-    const int t = fl ? fl[e] : e;
+    const int t = fl[e];
     const int a = max(0, min(P.NL - ww, wst[t] + win / 2 - ww / 2));
     for (int d = lane; d < KP; d += 64)
       ys[wv][d] = d < Ea ? P.Mp[(size_t)t * Ea + d] : 0.0;
@@ -548,9 +552,6 @@ __global__ __launch_bounds__(256) void knn_seed_wide(MfmaParams P, const double*
 FEDM_SEEDW(16, 16)
 FEDM_SEEDW(32, 16)
 FEDM_SEEDW(64, 16)
-FEDM_SEEDW(16, 64)
-FEDM_SEEDW(32, 64)
-FEDM_SEEDW(64, 64)
 
 // Two-phase form of the seeded matrix-core route.  Phase one (knn_screen) screens the (target, library
 // point) pairs against the target's fixed seeded floor and records the positions that pass, up to SCR_CAP
