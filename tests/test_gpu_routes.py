@@ -151,6 +151,43 @@ def test_graph_replay_resident_route():
     other_plan.close()
 
 
+@pytest.mark.parametrize("Ea", [12, 20, 40])
+def test_graph_replay_matrix_core_route(Ea):
+    """Mixed-precision S-map batches whose k-NN takes the matrix-core route (E_a >= 8): the two-phase form
+    with its overflow path (E_a = 12, an integer table, so that targets with more candidates than the
+    screen records take the wider seed, the streaming kernel and the merge) and the seeded streaming form
+    (E_a = 20 and 40), replayed as a graph with other GPU work between the replays.  Every replay's
+    predictions, codes and k used equal the eager run's bit for bit, and the eager run equals the plain
+    (FP64) batch's."""
+    fe = load_product()
+    rng = np.random.default_rng(Ea)
+    NL, NP, k = 3000, 500, Ea + 2
+    if Ea == 12:
+        data = 1.0 + rng.integers(0, 4, size=(NL + NP, Ea)).astype(float)
+    else:
+        data = 0.5 + rng.random((NL + NP, Ea))
+    table = {"data": data, "targets": 0.5 + rng.random(NL + NP)}
+    opts = {"algorithm": 1, "k": k, "thetas": [0.0, 2.0]}
+    libs, preds = [np.arange(NL)], [np.arange(NL, NL + NP)]
+    print(f"graph replay, matrix-core route: Ea={Ea} k={k}", flush=True)
+    mixed = fe.Batch(table, table, dict(opts, precision=fe.MIXED), libs, preds, [k])
+    plain = fe.Batch(table, table, dict(opts, precision=fe.PLAIN), libs, preds, [k])
+    mixed.run()
+    ref = [np.copy(a) for a in mixed.fetch()[0]]
+    plain.run()
+    for g, r in zip(plain.fetch()[0], ref):
+        assert np.array_equal(np.asarray(g).view(np.uint8), np.asarray(r).view(np.uint8))
+    assert np.all(ref[1] == 0)
+    assert mixed.set_graph(True) >= 1
+    for i in range(3):
+        mixed.run()
+        for g, r in zip(mixed.fetch()[0], ref):
+            assert np.array_equal(np.asarray(g).view(np.uint8), np.asarray(r).view(np.uint8)), f"replay {i}"
+        plain.run()
+    mixed.close()
+    plain.close()
+
+
 def _stopper(after):
     calls = {"n": 0}
 
