@@ -816,6 +816,44 @@ __global__ __launch_bounds__(256) void knn_rerank(MfmaParams P, int k, int slice
   const int lo = blockIdx.y * slice_len;
   const int hi = min(P.NL, lo + slice_len);
   const double* y = P.Mp + (size_t)t * P.Ea;
+  if (n <= 64) {
+    // one candidate per lane: a bitonic sort of the wavefront by (sqrt distance, position), the positions
+    // distinct (the empty entries, (inf, INT_MAX), alike), then lane o < k holds entry o of the list
+    double ks = INF_D;
+    int ki = INT_MAX, z = 0;
+    if (lane < n) {
+      const int pos = cand[st * SCR_CAP + lane];
+      const double acc = dist_f64<KP>(P.M + (size_t)pos * P.Ea, y, P.Ea);
+      if (acc == 0.0 || acc == MISSING_D) {
+        z = 1;
+      } else {
+        ks = sqrt(acc);
+        ki = pos;
+      }
+    }
+#pragma unroll
+    for (int size = 2; size <= 64; size <<= 1)
+#pragma unroll
+      for (int stride = size / 2; stride > 0; stride >>= 1) {
+        const double os = __shfl_xor(ks, stride);
+        const int oi = __shfl_xor(ki, stride);
+        const bool keep_min = ((lane & stride) == 0) == ((lane & size) == 0 || size == 64);
+        const bool take = keep_min ? key_less(os, oi, ks, ki) : key_less(ks, ki, os, oi);
+        ks = take ? os : ks;
+        ki = take ? oi : ki;
+      }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+      z += __shfl_xor(z, off);
+    const size_t base = st * KCAP;
+    if (lane < KCAP) {
+      out_s[base + lane] = lane < k ? ks : INF_D;
+      out_i[base + lane] = lane < k ? ki : INT_MAX;
+    }
+    if (lane == 0)
+      out_cnt[st] = (hi > lo ? hi - lo : 0) - z;
+    return;
+  }
   double s[4];
   int ix[4];
   int zeros = 0;
