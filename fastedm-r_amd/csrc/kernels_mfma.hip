@@ -788,10 +788,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(KP == 64 ? 
   }
 }
 
-// Phase two: one wavefront per (slice, target) whose candidates all fit.  Each lane recomputes up to four
-// candidates in FP64 in the reference's order (src/distances.cpp:186-216) and sorts them; k rounds of a
-// wavefront minimum by (sqrt distance, position) give the exact list (src/edm.cpp:652-681), written in the
-// layout of knn_mfma's per-slice lists (KCAP entries, +inf / INT_MAX after the k-th).  A target with a
+// Phase two: one wavefront per (slice, target) whose candidates all fit.  The candidates are recomputed in
+// FP64 in the reference's order (src/distances.cpp:186-216) and ordered by (sqrt distance, position), the
+// exact list (src/edm.cpp:652-681): up to 64 candidates one per lane and a bitonic sort of the wavefront;
+// more, up to four per lane, each lane's sorted and k rounds of a wavefront minimum.  The list is written
+// in the layout of knn_mfma's per-slice lists (KCAP entries, +inf / INT_MAX after the k-th).  A target with a
 // (slice, target) that does not fit is appended once (fflag) to the list for the streaming kernel (flist,
 // fcnt).
 template <int KCAP, int KP>
