@@ -634,8 +634,10 @@ struct TaskPlan
 };
 
 constexpr int SEED_WIN = 256; // library rows per seed window of the matrix-core route (knn_seed: 4 per lane)
-// library rows the seed scans around each window before the seeded streaming kernel (the two-phase form
-// scans the window only: its seeds are already near the k-th distance)
+// library rows the seed scans around each window before the seeded streaming kernel for points of more
+// than 16 coordinates, where a window's k-th distance lies far above the library's (Lorenz-96, 40
+// coordinates: select 22.6 -> 11.7 ms); up to 16 coordinates the window's k-th is already near it
+// (panel, 16 coordinates, forced to the streaming form: 94.6 ms from the window, 100.5 from 4 096 rows)
 constexpr int SEED_WIDE = 4096;
 
 // Seed windows of the matrix-core route.  A k-d order of the library rows: a node splits at the median of
@@ -1259,7 +1261,7 @@ static int launch_task(TaskPlan& tp, hipStream_t s, hipEvent_t* ev, int (*keep_g
       if (tp.MP.seed)
         launch_seed(tp.kp, s, tp.MP, (const double*)tp.ptr[B_KDP], (const int*)tp.ptr[B_WST],
                     (const int*)tp.ptr[B_TORD], (const int*)tp.ptr[B_SGRP], (int)tp.sgrp.size() - 1, tp.seed_win,
-                    tp.two_phase ? tp.seed_win : SEED_WIDE,
+                    tp.two_phase || tp.kp == 16 ? tp.seed_win : SEED_WIDE,
                     tp.k, (double*)tp.ptr[B_SEED]);
       if (tp.MP.seed && tp.two_phase) {
         int* fc = (int*)tp.ptr[B_FCNT];

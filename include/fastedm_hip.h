@@ -167,8 +167,8 @@ typedef struct
                                fit (3); (2) for 9..24 unknowns and launches of at least 65 536 systems,
                                else (0) (4).  All give the same bits. (4) */
   int mfma_seed;            /* matrix-core route: each target's screening threshold starts unbounded (0), or
-                               from an exact k-th distance over 4 096 library rows near it in a k-d order of
-                               the library (1); or (2) a screen against the k-th over 256 such rows records
+                               from an exact k-th distance over library rows near it in a k-d order of the
+                               library, 4 096 rows for points of more than 16 coordinates, else 256 (1); or (2) a screen against the k-th over 256 such rows records
                                candidates, one exact FP64 re-rank per target follows, and a target with more
                                than 256 candidates takes a k-th over 1 024 rows and the streaming kernel;
                                (3) (2) for points of up to 16 coordinates and up to 32 neighbours, else (1).

@@ -303,8 +303,8 @@ def test_device_exp_matches_oracle_bitwise():
 @pytest.mark.parametrize("case", ["clustered_dt", "ties", "logistic"])
 def test_mfma_seed_leaves_results_unchanged(case, monkeypatch):
     """The matrix-core route starts each target's FP32 screening threshold from an exact k-th distance
-    over library rows near it in a k-d order (FEDM_MFMA_SEED=1: 4 096 rows around its window, the k-th of
-    each lane's four smallest), or screens every pair against the k-th over its 256-row window, re-ranks
+    over library rows near it in a k-d order (FEDM_MFMA_SEED=1: its 256-row window here; 4 096 rows around
+    it, the k-th of each lane's four smallest, above 16 coordinates), or screens every pair against the k-th over its 256-row window, re-ranks
     the recorded candidates once and hands targets with more than 256 candidates, after a second seed over
     1 024 rows, to the streaming kernel (FEDM_MFMA_SEED=2; 3, the default, chooses); the screen of the
     two-phase form skips the library chunks whose box lies beyond its targets' thresholds unless
